@@ -1,0 +1,114 @@
+/*
+ * ccx.h -- C ABI of the MI355X closed-shell CCSD library (libccx.so, gfx950).
+ *
+ * The drop-in boundary for pycc's spatial RHF-CCSD hot path. Each entry point replaces one piece of
+ * the reference (CrawfordGroup/pycc) that a host binding calls:
+ *
+ *   pycc/device.py:38-86     ContractionBackend.__call__(subscripts, *operands)  -> ccx_contract
+ *   pycc/device.py:89-173    DeviceManager(device, precision)                    -> ccx_create
+ *   pycc/wavefunction.py:165-169, hamiltonian.py:67-70  (ERI/L seeding)          -> ccx_set_integrals_*
+ *   pycc/ccwfn.py:321-372    CCwfn.residuals(F, t1, t2)                          -> ccx_residual
+ *   pycc/ccwfn.py:1122-1162  CCwfn.cc_energy(o, v, F, L, t1, t2)                 -> ccx_energy
+ *   pycc/ccwfn.py:281-284    Jacobi update t += r/D and the 2-norm "rms"         -> ccx_update
+ *   pycc/utils.py:257-361    helper_diis (__init__, add_error_vector, extrapolate) -> ccx_diis_*
+ *
+ * Conventions
+ *   - All amplitude/residual/integral pointers are DEVICE pointers (FP64, C-contiguous):
+ *       t1, r1 [no][nv];  t2, r2 [no][no][nv][nv];  fock [nmo][nmo] with nmo = no + nv (active
+ *       orbitals only, occupied first -- a frozen core is removed by the caller).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream). All work of a call is
+ *     enqueued on it; calls return without synchronising except where noted (scalar readbacks,
+ *     DIIS solves, integral setup).
+ *   - Return value: 0 on success, a negative CCX_ERR_* code on failure; ccx_last_error() returns the
+ *     thread-local message of the last failure.
+ *   - The caller owns amplitude and residual buffers; the context owns its integral copies and
+ *     work space. One context per device and thread.
+ */
+#ifndef CCX_H
+#define CCX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ccx_ctx ccx_ctx;
+
+/* precision modes (pycc/device.py:113-125 validates 'SP'/'DP'; these are the MI355X modes) */
+#define CCX_MODE_DP 0           /* every contraction in FP64 (v_mfma_f64_16x16x4f64)                */
+#define CCX_MODE_MIXED 1        /* ladder + ring GEMMs in FP32 (v_mfma_f32_16x16x4f32, FP32 accum.); */
+                                /* every other term, the update, DIIS, norms and energy in FP64      */
+#define CCX_MODE_MIXED_BF16X3 2 /* as MIXED, with the ladder as BF16x3 split products                */
+
+#define CCX_OK 0
+#define CCX_ERR_ARG (-1)
+#define CCX_ERR_HIP (-2)
+#define CCX_ERR_STATE (-3)
+#define CCX_ERR_NUMERIC (-4)
+#define CCX_ERR_INTERNAL (-5)
+
+/* Create a context for `no` occupied and `nv` virtual orbitals in precision `mode`.
+ * Replaces DeviceManager(device='GPU', precision=...) (pycc/device.py:113-151). */
+int ccx_create(int no, int nv, int mode, ccx_ctx** out);
+int ccx_destroy(ccx_ctx* ctx);
+
+/* Integrals from a dense Dirac ERI <pq|rs> [nmo]^4 (pycc Hamiltonian layout, hamiltonian.py:67-70)
+ * and the Fock matrix. The context builds its own blocks; the inputs may be freed afterwards.
+ * Orbital energies (denominators, ccwfn.py:195-198) are the diagonal of this Fock matrix. */
+int ccx_set_integrals_dense(ccx_ctx* ctx, const double* fock, const double* eri, void* stream);
+
+/* Integrals from density-fitting factors B[naux][nmo][nmo] with (pq|rs) = sum_Q B_Qpq B_Qrs; the
+ * integral blocks are generated on the device (no nmo^4 array is formed). */
+int ccx_set_integrals_df(ccx_ctx* ctx, const double* fock, const double* B, int naux, void* stream);
+
+/* r1, r2 = CCwfn.residuals(F, t1, t2) (pycc/ccwfn.py:321-372): r2 is the full symmetrised
+ * residual (ccwfn.py:790). `fock` may differ from the setup Fock (e.g. a perturbed one). */
+int ccx_residual(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* r1,
+                 double* r2, void* stream);
+
+/* *e_dev = 2 f_ia t_ia + (t2_ijab + t_ia t_jb) L_ijab (pycc/ccwfn.py:1160-1161); e_dev is a device
+ * scalar. */
+int ccx_energy(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
+               void* stream);
+
+/* t1 += r1/Dia, t2 += r2/Dijab; *rms2_dev = sum (r1/Dia)^2 + sum (r2/Dijab)^2 (device scalar;
+ * pycc/ccwfn.py:281-284 takes its square root). */
+int ccx_update(ccx_ctx* ctx, double* t1, double* t2, const double* r1, const double* r2,
+               double* rms2_dev, void* stream);
+
+/* DIIS (pycc/utils.py:257-361).
+ * ccx_diis_reset    = helper_diis(t1, t2, max_diis): seeds the previous amplitudes; `capacity`
+ *                     bounds the stored vectors (>= max_diis + 1, <= 64).
+ * ccx_diis_add      = add_error_vector(t1, t2)   (error e = t - t_previous); synchronises.
+ * ccx_diis_extrapolate = extrapolate(t1, t2): overwrites t1, t2 with sum_k c_k t_k; if coef_out is
+ *                     non-NULL it receives the coefficients (host array, >= capacity entries).
+ *                     Synchronises. A singular Pulay system returns CCX_ERR_NUMERIC. */
+int ccx_diis_reset(ccx_ctx* ctx, const double* t1, const double* t2, int max_diis, int capacity,
+                   void* stream);
+int ccx_diis_add(ccx_ctx* ctx, const double* t1, const double* t2, void* stream);
+int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out, void* stream);
+
+/* Device bytes held by the context (integrals, work space, DIIS history). */
+int64_t ccx_device_bytes(const ccx_ctx* ctx);
+
+/* C = alpha * sum A*B + beta * C for an einsum string "ab,bc->ac" over strided device tensors
+ * (ContractionBackend.__call__, pycc/device.py:64-86; two operands). Shapes and strides are in
+ * elements, one entry per index letter of the corresponding operand. */
+int ccx_contract(const char* subscripts, double alpha, const double* A, const int64_t* a_shape,
+                 const int64_t* a_strides, const double* B, const int64_t* b_shape,
+                 const int64_t* b_strides, double beta, double* C, const int64_t* c_shape,
+                 const int64_t* c_strides, void* stream);
+/* FP32 variant (FP32 MFMA, FP32 accumulation). */
+int ccx_contract_f32(const char* subscripts, float alpha, const float* A, const int64_t* a_shape,
+                     const int64_t* a_strides, const float* B, const int64_t* b_shape,
+                     const int64_t* b_strides, float beta, float* C, const int64_t* c_shape,
+                     const int64_t* c_strides, void* stream);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* ccx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCX_H */

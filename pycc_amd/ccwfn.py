@@ -1,0 +1,263 @@
+"""Closed-shell CCSD amplitude solver on the GPU (counterpart of ``pycc.ccwfn.CCwfn``).
+
+``CCwfn(H, no=..., device='GPU', precision='DP').solve_cc(e_conv, r_conv, maxiter, max_diis,
+start_diis)`` reproduces ``pycc/ccwfn.py:216-319`` for the spin-adapted CCSD model: MP1 guess,
+Jacobi update ``t += r/D``, the 2-norm "rms", convergence when both ``|dE| < e_conv`` and
+``rms < r_conv``, DIIS after the convergence test, and ``None`` when ``maxiter`` is exhausted.
+``residuals(F, t1, t2)`` and ``cc_energy(o, v, F, L, t1, t2)`` keep the reference signatures.
+
+The Hamiltonian ``H`` is either
+
+* a pycc-layout dense Hamiltonian (attributes ``F`` (nmo, nmo) and Dirac ``ERI`` (nmo,)*4, as
+  ``pycc/hamiltonian.py:59-70`` builds them), or
+* density-fitting factors (attributes ``B`` (naux, nmo, nmo) and ``eps`` or ``fock``), in which case
+  the integral blocks are generated on the GPU and no nmo^4 array exists anywhere.
+
+A frozen core is a slice offset (``nfzc``), as in ``pycc/wavefunction.py:304-312``.
+Amplitudes, residuals, denominators, DIIS, norms and energies are FP64 torch tensors on the GPU in
+every precision mode; ``precision`` only changes the arithmetic of the ladder and ring GEMMs.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceManager, current_stream_handle, dptr
+from .exceptions import InvalidKeywordError, PyCCError
+
+
+def _title(text):
+    return "\n" + text
+
+
+def _iteration(niter, energy=None, de=None, rms=None, e_label="Ecorr", note=None):
+    cols = []
+    if energy is not None:
+        cols.append("%s = %.15f" % (e_label, energy))
+    if de is not None:
+        cols.append("dE = % .5E" % de)
+    if rms is not None:
+        cols.append("rms = % .5E" % rms)
+    if note is not None:
+        cols.append(note)
+    return "Iter %3d: %s" % (niter, "  ".join(cols))
+
+
+class _Ctx:
+    """Owns one ``ccx_ctx``."""
+
+    def __init__(self, no, nv, mode):
+        h = ctypes.c_void_p()
+        _lib.call("ccx_create", no, nv, mode, ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().ccx_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+class CCwfn(object):
+    """Spin-adapted closed-shell CCSD on one AMD GPU (see the module docstring)."""
+
+    VALID_MODELS = ['CCSD']
+
+    def __init__(self, H, no=None, nfzc=0, model='CCSD', device='GPU', precision='DP',
+                 verbose=True, **kwargs):
+        time_init = time.time()
+        if kwargs:
+            raise PyCCError("Unexpected keyword argument(s): %s" % sorted(kwargs))   # wavefunction.py:181
+        model = str(model).upper()
+        if model not in self.VALID_MODELS:
+            raise InvalidKeywordError('model', model, self.VALID_MODELS)
+        self.model = model
+        self.verbose = verbose
+        mgr = DeviceManager(device=device, precision=precision)
+        self.device_manager = mgr
+        self.device, self.precision = mgr.device, mgr.precision
+        self.device0, self.device1 = mgr.device0, mgr.device1
+        self.contract = mgr.contract
+        import torch
+        self._torch = torch
+
+        if no is None:
+            no = getattr(H, "no", None)
+        if no is None:
+            raise PyCCError("the number of active occupied orbitals `no` is required")
+        self.nfzc = int(nfzc)
+        self.no = int(no)
+        self.e_conv, self.r_conv, self.maxiter = 1e-7, 1e-7, 100
+
+        dev = self.device1
+        f64 = torch.float64
+        if hasattr(H, "B"):
+            B = H.B
+            fock = H.fock if hasattr(H, "fock") else np.diag(np.asarray(H.eps))
+            fock = np.asarray(fock, dtype=np.float64)
+            nmo_all = fock.shape[0]
+            act = slice(self.nfzc, nmo_all)
+            F = torch.as_tensor(np.ascontiguousarray(fock[act, act]), device=dev)
+            Bt = torch.as_tensor(np.asarray(B, dtype=np.float64), device=dev)[:, act, act].contiguous()
+            self._source = ("df", Bt)
+        elif hasattr(H, "ERI") and hasattr(H, "F"):
+            Fh = H.F.detach().cpu().numpy() if hasattr(H.F, "detach") else np.asarray(H.F)
+            nmo_all = Fh.shape[0]
+            act = slice(self.nfzc, nmo_all)
+            F = torch.as_tensor(np.ascontiguousarray(Fh[act, act], dtype=np.float64), device=dev)
+            E = H.ERI
+            E = E if isinstance(E, torch.Tensor) else torch.as_tensor(np.asarray(E, dtype=np.float64))
+            E = E.to(dev, f64)[act, act, act, act].contiguous()
+            self._source = ("dense", E)
+        else:
+            raise PyCCError("H must provide (F, ERI) or density-fitting factors (B, eps/fock)")
+
+        self.nmo = F.shape[0]
+        self.nv = self.nmo - self.no
+        if self.no < 1 or self.nv < 1:
+            raise PyCCError("need at least one active occupied and one virtual orbital")
+        self.o = slice(0, self.no)
+        self.v = slice(self.no, self.nmo)
+        self.H = _DeviceHamiltonian(F)
+
+        self._ctx = _Ctx(self.no, self.nv, mgr.mode)
+        stream = ctypes.c_void_p(current_stream_handle())
+        kind, src = self._source
+        if kind == "df":
+            _lib.call("ccx_set_integrals_df", self._ctx.h, dptr(F), dptr(src), int(src.shape[0]), stream)
+        else:
+            _lib.call("ccx_set_integrals_dense", self._ctx.h, dptr(F), dptr(src), stream)
+        del src
+        self._source = None
+
+        # denominators (ccwfn.py:195-198) and the MP1 guess (ccwfn.py:210-211)
+        eps = torch.diagonal(F).clone()
+        eps_o, eps_v = eps[self.o], eps[self.v]
+        self.H.eps = eps
+        self.Dia = eps_o.reshape(-1, 1) - eps_v
+        self.t1 = torch.zeros((self.no, self.nv), dtype=f64, device=dev)
+        self.t2 = self._mp1_guess()
+        self._e_dev = torch.zeros(2, dtype=f64, device=dev)
+        self.ecc = None
+        self.niter = 0
+        if self.verbose:
+            print("CCwfn built in %.3f seconds." % (time.time() - time_init))
+
+    # ---- helpers --------------------------------------------------------------------------
+    @property
+    def Dijab(self):
+        """Doubles denominators (ccwfn.py:195-197), formed on request (o^2 v^2 FP64)."""
+        e = self.H.eps
+        eo, ev = e[self.o], e[self.v]
+        return eo.reshape(-1, 1, 1, 1) + eo.reshape(-1, 1, 1) - ev.reshape(-1, 1) - ev
+
+    def _stream(self):
+        return ctypes.c_void_p(current_stream_handle())
+
+    def _mp1_guess(self):
+        # t2 = <ij|ab>/Dijab: r2 at t = 0 is exactly <ij|ab> (every other term is at least linear in
+        # t), so one residual evaluation plus a Jacobi step from zero gives the MP1 amplitudes.
+        torch = self._torch
+        t1 = torch.zeros((self.no, self.nv), dtype=torch.float64, device=self.device1)
+        t2 = torch.zeros((self.no, self.no, self.nv, self.nv), dtype=torch.float64, device=self.device1)
+        r1, r2 = self.residuals(self.H.F, t1, t2)
+        r1.zero_()          # t1 starts at zero (ccwfn.py:210) whatever f_vo is
+        self._update(t1, t2, r1, r2)
+        return t2
+
+    def _update(self, t1, t2, r1, r2):
+        _lib.call("ccx_update", self._ctx.h, dptr(t1), dptr(t2), dptr(r1), dptr(r2),
+                  dptr(self._e_dev[1:]), self._stream())
+        return math.sqrt(float(self._e_dev[1].item()))
+
+    # ---- reference API --------------------------------------------------------------------
+    def residuals(self, F, t1, t2):
+        """r1, r2 = <mu|e^{-T} H e^{T}|0> (ccwfn.py:321-372); r2 symmetrised as ccwfn.py:790."""
+        torch = self._torch
+        F = self._as_dev(F, (self.nmo, self.nmo))
+        t1 = self._as_dev(t1, (self.no, self.nv))
+        t2 = self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        r1 = torch.empty_like(t1)
+        r2 = torch.empty_like(t2)
+        _lib.call("ccx_residual", self._ctx.h, dptr(F), dptr(t1), dptr(t2), dptr(r1), dptr(r2),
+                  self._stream())
+        return r1, r2
+
+    def cc_energy(self, o=None, v=None, F=None, L=None, t1=None, t2=None):
+        """E = 2 f_ia t_ia + tau_ijab L_ijab (ccwfn.py:1160-1161); L is held by the device."""
+        F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
+        t1 = self.t1 if t1 is None else self._as_dev(t1, (self.no, self.nv))
+        t2 = self.t2 if t2 is None else self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        _lib.call("ccx_energy", self._ctx.h, dptr(F), dptr(t1), dptr(t2), dptr(self._e_dev[:1]),
+                  self._stream())
+        return float(self._e_dev[0].item())
+
+    def solve_cc(self, e_conv=1e-7, r_conv=1e-7, maxiter=100, max_diis=8, start_diis=1):
+        """ccwfn.py:216-319 (spatial CCSD branch)."""
+        t0 = time.time()
+        self.e_conv, self.r_conv, self.maxiter = e_conv, r_conv, maxiter
+        F = self.H.F
+        ecc = self.cc_energy()
+        name = "T-amplitudes (%s)" % self.model
+        self.history = [(0, ecc, -ecc, None)]
+        if self.verbose:
+            print(_title(name))
+            print(_iteration(0, energy=ecc, de=-ecc, e_label="CC Ecorr", note="MP2"))
+        cap = max(1, max_diis) + max(1, start_diis)
+        _lib.call("ccx_diis_reset", self._ctx.h, dptr(self.t1), dptr(self.t2), int(max_diis), int(cap),
+                  self._stream())
+        coef = (ctypes.c_double * 64)()
+        for niter in range(1, maxiter + 1):
+            ecc_last = ecc
+            r1, r2 = self.residuals(F, self.t1, self.t2)
+            rms = self._update(self.t1, self.t2, r1, r2)
+            del r1, r2
+            ecc = self.cc_energy()
+            ediff = ecc - ecc_last
+            self.history.append((niter, ecc, ediff, rms))
+            self.niter = niter
+            if self.verbose:
+                print(_iteration(niter, energy=ecc, de=ediff, rms=rms, e_label="CC Ecorr"))
+            if abs(ediff) < e_conv and abs(rms) < r_conv:
+                if self.verbose:
+                    print("%s converged in %.3f seconds." % (name, time.time() - t0))
+                    print("E(%s) = %20.15f" % (self.model, ecc))
+                self.ecc = ecc
+                return ecc
+            _lib.call("ccx_diis_add", self._ctx.h, dptr(self.t1), dptr(self.t2), self._stream())
+            if niter >= start_diis:
+                _lib.call("ccx_diis_extrapolate", self._ctx.h, dptr(self.t1), dptr(self.t2), coef,
+                          self._stream())
+        return None
+
+    def device_bytes(self):
+        return int(_lib.load().ccx_device_bytes(self._ctx.h))
+
+    def _as_dev(self, x, shape):
+        torch = self._torch
+        if not isinstance(x, torch.Tensor):
+            x = torch.as_tensor(np.asarray(x, dtype=np.float64))
+        x = x.to(self.device1, torch.float64)
+        if tuple(x.shape) != tuple(shape):
+            raise PyCCError("expected shape %s, got %s" % (tuple(shape), tuple(x.shape)))
+        return x.contiguous()
+
+
+class _DeviceHamiltonian:
+    """The active-space Fock matrix on the device (the ERIs live inside the library context)."""
+
+    def __init__(self, F):
+        self.F = F
+        self.eps = None
+
+
+def ccwfn(*args, **kwargs):
+    """Lower-case alias, as exported by pycc (``pycc/__init__.py``)."""
+    return CCwfn(*args, **kwargs)

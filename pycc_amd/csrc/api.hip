@@ -1,0 +1,193 @@
+// C ABI of the CCSD device library (declared in include/ccx.h).
+#include "../../include/ccx.h"
+#include "ccsd.h"
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+struct ccx_ctx {
+  ccx::Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return CCX_OK;
+  } catch (const ccx::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return CCX_ERR_INTERNAL;
+  } catch (...) {
+    g_err = "unknown error";
+    return CCX_ERR_INTERNAL;
+  }
+}
+
+inline hipStream_t st(void* s) { return (hipStream_t)s; }
+
+// Parse "ab,bc->ac" into three label strings.
+void parse_subscripts(const char* sub, std::string& a, std::string& b, std::string& c) {
+  CCX_CHECK(sub != nullptr, ccx::ERR_ARG, "null subscripts");
+  std::string s(sub);
+  s.erase(std::remove(s.begin(), s.end(), ' '), s.end());
+  size_t comma = s.find(','), arrow = s.find("->");
+  CCX_CHECK(comma != std::string::npos && arrow != std::string::npos && comma < arrow, ccx::ERR_ARG,
+            "subscripts must look like 'ab,bc->ac'");
+  a = s.substr(0, comma);
+  b = s.substr(comma + 1, arrow - comma - 1);
+  c = s.substr(arrow + 2);
+  CCX_CHECK(!a.empty() && !b.empty() && a.size() <= 8 && b.size() <= 8 && c.size() <= 8, ccx::ERR_ARG,
+            "operand rank must be 1..8");
+}
+
+ccx::TV make_view(const void* p, const std::string& lab, const int64_t* shape, const int64_t* strides) {
+  CCX_CHECK(p && (lab.empty() || (shape && strides)), ccx::ERR_ARG, "null tensor pointer/shape");
+  std::vector<int64_t> e(lab.size()), s(lab.size());
+  for (size_t i = 0; i < lab.size(); ++i) {
+    CCX_CHECK(shape[i] >= 0, ccx::ERR_ARG, "negative extent");
+    e[i] = shape[i];
+    s[i] = strides[i];
+  }
+  return ccx::tvs(p, lab.c_str(), e, s);
+}
+
+template <typename T>
+int contract_any(const char* sub, T alpha, const T* A, const int64_t* as, const int64_t* ast, const T* B,
+                 const int64_t* bs, const int64_t* bst, T beta, T* C, const int64_t* cs, const int64_t* cst,
+                 void* stream) {
+  static thread_local ccx::Workspace ws;
+  return guarded([&] {
+    std::string la, lb, lc;
+    parse_subscripts(sub, la, lb, lc);
+    ccx::contract<T>(alpha, make_view(A, la, as, ast), make_view(B, lb, bs, bst), beta,
+                     make_view(C, lc, cs, cst), ws, st(stream));
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int ccx_create(int no, int nv, int mode, ccx_ctx** out) {
+  return guarded([&] {
+    CCX_CHECK(out != nullptr, ccx::ERR_ARG, "null output handle");
+    CCX_CHECK(no >= 1 && nv >= 1, ccx::ERR_ARG, "need no >= 1 and nv >= 1");
+    CCX_CHECK(mode == CCX_MODE_DP || mode == CCX_MODE_MIXED || mode == CCX_MODE_MIXED_BF16X3, ccx::ERR_ARG,
+              "mode must be CCX_MODE_DP, CCX_MODE_MIXED or CCX_MODE_MIXED_BF16X3");
+    int ndev = 0;
+    CCX_HIP(hipGetDeviceCount(&ndev));
+    CCX_CHECK(ndev > 0, ccx::ERR_HIP, "no HIP device");
+    auto* h = new ccx_ctx;
+    h->c.o = no;
+    h->c.v = nv;
+    h->c.nmo = no + nv;
+    h->c.mode = mode;
+    *out = h;
+  });
+}
+
+int ccx_destroy(ccx_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipDeviceSynchronize();
+    ctx->c.destroy();
+    delete ctx;
+  });
+}
+
+int ccx_set_integrals_dense(ccx_ctx* ctx, const double* fock, const double* eri, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::set_integrals_dense(ctx->c, fock, eri, st(stream));
+    CCX_HIP(hipStreamSynchronize(st(stream)));
+  });
+}
+
+int ccx_set_integrals_df(ccx_ctx* ctx, const double* fock, const double* B, int naux, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::set_integrals_df(ctx->c, fock, B, naux, st(stream));
+    CCX_HIP(hipStreamSynchronize(st(stream)));
+  });
+}
+
+int ccx_residual(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* r1,
+                 double* r2, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::residual(ctx->c, fock, t1, t2, r1, r2, st(stream));
+  });
+}
+
+int ccx_energy(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
+               void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx && fock && t1 && t2 && e_dev, ccx::ERR_ARG, "null argument");
+    auto& c = ctx->c;
+    CCX_CHECK(c.have_ints, ccx::ERR_STATE, "ccx_energy called before the integrals were set");
+    ccx::energy(c.o, c.v, fock, c.nmo, t1, t2, c.Loovv, c.red, e_dev, st(stream));
+  });
+}
+
+int ccx_update(ccx_ctx* ctx, double* t1, double* t2, const double* r1, const double* r2, double* rms2_dev,
+               void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx && t1 && t2 && r1 && r2 && rms2_dev, ccx::ERR_ARG, "null argument");
+    auto& c = ctx->c;
+    CCX_CHECK(c.have_ints, ccx::ERR_STATE, "ccx_update called before the integrals were set");
+    ccx::jacobi_update(c.o, c.v, c.eps, t1, t2, r1, r2, c.red + 1024 * 32, rms2_dev, st(stream));
+  });
+}
+
+int ccx_diis_reset(ccx_ctx* ctx, const double* t1, const double* t2, int max_diis, int capacity,
+                   void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx && t1 && t2, ccx::ERR_ARG, "null argument");
+    ccx::diis_reset(ctx->c, t1, t2, max_diis, capacity, st(stream));
+  });
+}
+
+int ccx_diis_add(ccx_ctx* ctx, const double* t1, const double* t2, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx && t1 && t2, ccx::ERR_ARG, "null argument");
+    ccx::diis_add(ctx->c, t1, t2, st(stream));
+  });
+}
+
+int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx && t1 && t2, ccx::ERR_ARG, "null argument");
+    ccx::diis_extrapolate(ctx->c, t1, t2, coef_out, st(stream));
+  });
+}
+
+int64_t ccx_device_bytes(const ccx_ctx* ctx) {
+  return ctx ? ctx->c.bytes + (int64_t)ctx->c.ws.size : 0;
+}
+
+int ccx_contract(const char* subscripts, double alpha, const double* A, const int64_t* a_shape,
+                 const int64_t* a_strides, const double* B, const int64_t* b_shape, const int64_t* b_strides,
+                 double beta, double* C, const int64_t* c_shape, const int64_t* c_strides, void* stream) {
+  return contract_any<double>(subscripts, alpha, A, a_shape, a_strides, B, b_shape, b_strides, beta, C,
+                              c_shape, c_strides, stream);
+}
+
+int ccx_contract_f32(const char* subscripts, float alpha, const float* A, const int64_t* a_shape,
+                     const int64_t* a_strides, const float* B, const int64_t* b_shape,
+                     const int64_t* b_strides, float beta, float* C, const int64_t* c_shape,
+                     const int64_t* c_strides, void* stream) {
+  return contract_any<float>(subscripts, alpha, A, a_shape, a_strides, B, b_shape, b_strides, beta, C,
+                             c_shape, c_strides, stream);
+}
+
+const char* ccx_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

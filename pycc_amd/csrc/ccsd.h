@@ -1,0 +1,81 @@
+// Device-side closed-shell CCSD context: integral blocks, work buffers, residual, update, DIIS.
+#pragma once
+#include "common.h"
+#include "contract.h"
+#include "gemm.h"
+#include "kernels.h"
+#include <vector>
+
+namespace ccx {
+
+enum Mode : int { MODE_DP = 0, MODE_MIXED = 1, MODE_MIXED_BF16X3 = 2 };
+
+struct Ctx {
+  int64_t o = 0, v = 0, nmo = 0;
+  int mode = MODE_DP;
+  bool have_ints = false;
+  Workspace ws;
+  std::vector<void*> owned;          // every hipMalloc of the context, freed by destroy()
+  int64_t bytes = 0;                 // device bytes held in `owned`
+
+  // ---- integrals (FP64, chemist blocks (pq|rs) stored [p][q][r][s]) ----
+  double* eps = nullptr;             // nmo orbital energies (diagonal of the setup Fock)
+  double* Coooo = nullptr;           // (mi|nj)
+  double* Cooov = nullptr;           // (mi|ne)
+  double* Coovv = nullptr;           // (mn|ef)
+  double* Covov = nullptr;           // (me|nf)  == K[(me),(nf)]
+  double* Covvv = nullptr;           // (me|af)
+  double* Loovv = nullptr;           // L_ijab = 2<ij|ab> - <ij|ba>, [i][j][a][b]
+  double* J = nullptr;               // J[(me),(nf)] = (mf|ne)
+  float* K32 = nullptr;              // FP32 copies for the mixed ring GEMMs
+  float* J32 = nullptr;
+  // packed ladder matrices: rows (e<=f) / (e<f), columns (a<=b)/(a<b) then (m, b)
+  void* LP = nullptr;                // double or float
+  void* LM = nullptr;
+  bf16 *LPh = nullptr, *LPl = nullptr, *LMh = nullptr, *LMl = nullptr;
+
+  // ---- per-residual work buffers (allocated on first use) ----
+  bool have_work = false;
+  double *Fae = nullptr, *Fmi = nullptr, *Fme = nullptr, *G = nullptr, *Hm = nullptr;
+  double *tau = nullptr, *Wmnij = nullptr, *Hh = nullptr, *Zt = nullptr, *Wd = nullptr;
+  void *S1 = nullptr, *A1 = nullptr, *S2 = nullptr, *Tx = nullptr, *W = nullptr;
+  void *Tp = nullptr, *Tm = nullptr, *CP = nullptr, *CM = nullptr;
+  bf16 *Tph = nullptr, *Tpl = nullptr, *Tmh = nullptr, *Tml = nullptr;
+  double* red = nullptr;             // reduction partials
+  double* scal = nullptr;            // small device scalars
+
+  // ---- DIIS (pycc utils.helper_diis) ----
+  int max_diis = 0, cap = 0;
+  std::vector<int> order;            // active slots, oldest first
+  std::vector<std::vector<double>> bdots;   // cached <e_i, e_j> by slot
+  double *dv = nullptr, *de = nullptr, *doldt = nullptr, *dcoef = nullptr;
+
+  int64_t ov() const { return o * v; }
+  int64_t n1() const { return o * v; }
+  int64_t n2() const { return o * o * v * v; }
+  int64_t npP() const { return v * (v + 1) / 2; }
+  int64_t npM() const { return v * (v - 1) / 2; }
+  int64_t noP() const { return o * (o + 1) / 2; }
+  int64_t noM() const { return o * (o - 1) / 2; }
+
+  template <typename T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    CCX_HIP(hipMalloc(&p, n * sizeof(T)));
+    owned.push_back(p);
+    bytes += (int64_t)(n * sizeof(T));
+    return (T*)p;
+  }
+  void destroy();
+};
+
+void set_integrals_dense(Ctx& c, const double* fock, const double* eri, hipStream_t s);
+void set_integrals_df(Ctx& c, const double* fock, const double* B, int64_t naux, hipStream_t s);
+void residual(Ctx& c, const double* fock, const double* t1, const double* t2, double* r1, double* r2,
+              hipStream_t s);
+void diis_reset(Ctx& c, const double* t1, const double* t2, int max_diis, int cap, hipStream_t s);
+void diis_add(Ctx& c, const double* t1, const double* t2, hipStream_t s);
+void diis_extrapolate(Ctx& c, double* t1, double* t2, double* coef_out, hipStream_t s);
+
+}  // namespace ccx

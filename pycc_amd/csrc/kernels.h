@@ -1,0 +1,85 @@
+// Fused elementwise / packing / reduction kernels of the CCSD residual (see kernels.hip).
+#pragma once
+#include "common.h"
+#include <hip/hip_bf16.h>
+
+namespace ccx {
+
+typedef __hip_bfloat16 bf16;
+
+// Pair tables of the packed ladder: index p -> (lo, hi) with lo <= hi (P) or lo < hi (M), ordered
+// by hi then lo: pP(a, b) = b(b+1)/2 + a, pM(a, b) = b(b-1)/2 + a.
+__host__ __device__ inline int64_t pairP(int64_t a, int64_t b) { return b * (b + 1) / 2 + a; }
+__host__ __device__ inline int64_t pairM(int64_t a, int64_t b) { return b * (b - 1) / 2 + a; }
+
+// Chemist block (pq|rs) gathered from a dense Dirac ERI <pq|rs> = (pr|qs):
+// out[p][q][r][s] = ERI[p0+p][r0+r][q0+q][s0+s].
+void gather_chem(const double* eri, int64_t nmo, int64_t p0, int64_t np, int64_t q0, int64_t nq,
+                 int64_t r0, int64_t nr, int64_t s0, int64_t ns, double* out, hipStream_t s);
+
+// tau = f1 * t2 + f2 * t1 (x) t1, layout [i][j][a][b]
+void build_tau(int64_t o, int64_t v, double f1, double f2, const double* t1, const double* t2,
+               double* tau, hipStream_t s);
+
+// Ring-layout operands ([i][a][j][b], i.e. ov x ov matrices), see ring_prep_kernel.
+template <typename T>
+void ring_prep(int64_t o, int64_t v, const double* t1, const double* t2, T* S1, T* A1, T* S2, T* Tx,
+               hipStream_t s);
+
+// Packed ladder amplitudes: Tp[(i<=j)][(e<=f)], Tm[(i<j)][(e<f)] from tau = t2 + t1 t1.
+template <typename T>
+void tau_pack(int64_t o, int64_t v, const double* t1, const double* t2, T* Tp, T* Tm, hipStream_t s);
+// BF16x3 operands are K-contiguous with K padded to a multiple of 32 (padding stays zero).
+__host__ __device__ inline int64_t bf16_kpad(int64_t k) { return (k + 31) / 32 * 32; }
+void tau_pack_bf16x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2,
+                     bf16* Tp_hi, bf16* Tp_lo, bf16* Tm_hi, bf16* Tm_lo, hipStream_t s);
+
+// Source of (ea|fb) for e in [e0, e1): address base + e*se + a*sa + f*sf + b*sb (absolute e).
+struct VVVVSrc {
+  const double* base;
+  int64_t se, sa, sf, sb;
+};
+// Rows (e<=f / e<f) with e0 <= e < e1 of the packed ladder matrices; columns [0, npairs) hold
+// 1/2(<ab|ef> +- <ab|fe>) and columns npairs + m*v + b hold 1/2(<mb|ef> +- <mb|fe>) from Covvv.
+template <typename T>
+void ladder_pack(int64_t o, int64_t v, int64_t e0, int64_t e1, VVVVSrc src, const double* Covvv,
+                 T* LP, T* LM, hipStream_t s);
+// BF16x3 ladder matrices stored [N][Kp]: the (ab) block is symmetric under (ab) <-> (ef), so its
+// rows are written untransposed; the (m, b) rows are written by ladder_pack_o_bf16x3.
+void ladder_pack_bf16x3(int64_t o, int64_t v, int64_t e0, int64_t e1, VVVVSrc src, bf16* LP_hi,
+                        bf16* LP_lo, bf16* LM_hi, bf16* LM_lo, hipStream_t s);
+void ladder_pack_o_bf16x3(int64_t o, int64_t v, const double* Covvv, bf16* LP_hi, bf16* LP_lo,
+                          bf16* LM_hi, bf16* LM_lo, hipStream_t s);
+
+// Zt[m][b][i][j] += Z_mbij from the packed ladder outputs.
+template <typename T>
+void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStream_t s);
+
+// r2[i][j][a][b] = H[ijab] + H[jiba] + ladder[ijab] + P[ia,jb] + P[jb,ia] + Q[ja,ib] + Q[ib,ja]
+template <typename T>
+void assemble_r2(int64_t o, int64_t v, const double* H, const T* CP, const T* CM, const T* P,
+                 const T* Q, double* r2, hipStream_t s);
+
+// E = 2 f_ia t_ia + (t2_ijab + t_ia t_jb) L_ijab, written to e_out (device scalar).
+void energy(int64_t o, int64_t v, const double* fock, int64_t nmo, const double* t1, const double* t2,
+            const double* Loovv, double* partials, double* e_out, hipStream_t s);
+
+// t1 += r1/Dia, t2 += r2/Dijab; rms2_out = sum (r/D)^2 (device scalar).
+void jacobi_update(int64_t o, int64_t v, const double* eps, double* t1, double* t2, const double* r1,
+                   const double* r2, double* partials, double* rms2_out, hipStream_t s);
+
+// out[k] = <x_k, y> for k < nvec, vectors of length n spaced `stride` apart (device array out).
+void dots(int64_t n, int nvec, const double* x0, int64_t stride, const double* y, double* partials,
+          double* out, hipStream_t s);
+
+// out = sum_k c[k] * x_k (c on the device), vectors spaced `stride` apart.
+void lincomb(int64_t n, int nvec, const double* c, const double* x0, int64_t stride, double* out,
+             hipStream_t s);
+
+// out = x - y
+void sub(int64_t n, const double* x, const double* y, double* out, hipStream_t s);
+
+template <typename Tin, typename Tout>
+void cast(int64_t n, const Tin* in, Tout* out, hipStream_t s);
+
+}  // namespace ccx

@@ -1,0 +1,549 @@
+// Fused elementwise, packing and reduction kernels of the CCSD residual (gfx950).
+//
+// These are HBM-bound: each reads its inputs once and writes its outputs once. Transposed reads
+// of t2 blocks go through 32x33 LDS tiles so that global loads and stores stay coalesced.
+#include "kernels.h"
+#include <cmath>
+
+namespace ccx {
+
+namespace {
+
+constexpr int RED_BLOCKS = 1024;   // fixed grid for deterministic two-stage reductions
+
+__device__ __forceinline__ void decodeP(int64_t c, int64_t& a, int64_t& b) {
+  int64_t f = (int64_t)((sqrt(8.0 * (double)c + 1.0) - 1.0) * 0.5);
+  while (f * (f + 1) / 2 > c) --f;
+  while ((f + 1) * (f + 2) / 2 <= c) ++f;
+  b = f;
+  a = c - f * (f + 1) / 2;
+}
+__device__ __forceinline__ void decodeM(int64_t c, int64_t& a, int64_t& b) {
+  int64_t f = (int64_t)((sqrt(8.0 * (double)c + 1.0) + 1.0) * 0.5);
+  while (f * (f - 1) / 2 > c) --f;
+  while ((f + 1) * f / 2 <= c) ++f;
+  b = f;
+  a = c - f * (f - 1) / 2;
+}
+
+__device__ __forceinline__ double block_sum(double x) {
+  __shared__ double sh[256];
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void finish_sum_kernel(const double* __restrict__ partials, int n, double* out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) *out = s;
+}
+
+inline int grid_for(int64_t n, int64_t cap = 65536) {
+  int64_t b = ceil_div(n, 256);
+  return (int)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+__global__ void gather_chem_kernel(const double* __restrict__ eri, int64_t nmo, int64_t p0, int64_t np,
+                                   int64_t q0, int64_t nq, int64_t r0, int64_t nr, int64_t s0, int64_t ns,
+                                   double* __restrict__ out) {
+  int64_t total = np * nq * nr * ns;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t s = e % ns, t = e / ns;
+    int64_t r = t % nr; t /= nr;
+    int64_t q = t % nq, p = t / nq;
+    out[e] = eri[(((p0 + p) * nmo + (r0 + r)) * nmo + (q0 + q)) * nmo + (s0 + s)];
+  }
+}
+
+void gather_chem(const double* eri, int64_t nmo, int64_t p0, int64_t np, int64_t q0, int64_t nq,
+                 int64_t r0, int64_t nr, int64_t s0, int64_t ns, double* out, hipStream_t s) {
+  int64_t total = np * nq * nr * ns;
+  if (!total) return;
+  gather_chem_kernel<<<grid_for(total), 256, 0, s>>>(eri, nmo, p0, np, q0, nq, r0, nr, s0, ns, out);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ void tau_kernel(int64_t o, int64_t v, double f1, double f2, const double* __restrict__ t1,
+                           const double* __restrict__ t2, double* __restrict__ tau) {
+  int64_t total = o * o * v * v;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = e % v, t = e / v;
+    int64_t a = t % v; t /= v;
+    int64_t j = t % o, i = t / o;
+    tau[e] = f1 * t2[e] + f2 * (t1[i * v + a] * t1[j * v + b]);
+  }
+}
+
+void build_tau(int64_t o, int64_t v, double f1, double f2, const double* t1, const double* t2,
+               double* tau, hipStream_t s) {
+  tau_kernel<<<grid_for(o * o * v * v), 256, 0, s>>>(o, v, f1, f2, t1, t2, tau);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+// For every (i, a, j, b), with T = t2_ijab, X = t2_ijba, P = t_ja t_ib, writes at [i][a][j][b]:
+//   S1[0] = -T/2          S1[1] = X/2 + P       (B operands of the J GEMM -> Wej, Wje)
+//   A1    = T - X/2 - P                          (B operand of the K GEMM -> Wej)
+//   S2[0] = 2T - X        S2[1] = T             (A operands of the ring GEMM)
+//   Tx    = X                                    (A operand of the crossed ring GEMM)
+template <typename T>
+__global__ __launch_bounds__(256) void ring_prep_kernel(int64_t o, int64_t v, const double* __restrict__ t1,
+                                                        const double* __restrict__ t2, T* __restrict__ S1,
+                                                        T* __restrict__ A1, T* __restrict__ S2,
+                                                        T* __restrict__ Tx) {
+  __shared__ double tn[32][33];
+  __shared__ double tx[32][33];
+  const int64_t ij = blockIdx.y;
+  const int64_t i = ij / o, j = ij % o;
+  const int64_t tiles_b = (v + 31) / 32;
+  const int64_t a0 = (blockIdx.x / tiles_b) * 32, b0 = (blockIdx.x % tiles_b) * 32;
+  const double* blk = t2 + ij * v * v;
+  const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+  for (int y = y0; y < 32; y += 8) {
+    int64_t a = a0 + y, b = b0 + x;
+    tn[y][x] = (a < v && b < v) ? blk[a * v + b] : 0.0;
+    int64_t bb = b0 + y, aa = a0 + x;
+    tx[y][x] = (bb < v && aa < v) ? blk[bb * v + aa] : 0.0;
+  }
+  __syncthreads();
+  const int64_t n2 = o * v * o * v;
+  for (int y = y0; y < 32; y += 8) {
+    int64_t a = a0 + y, b = b0 + x;
+    if (a >= v || b >= v) continue;
+    double tv = tn[y][x];
+    double xv = tx[x][y];
+    double pv = t1[j * v + a] * t1[i * v + b];
+    int64_t out = ((i * v + a) * o + j) * v + b;
+    S1[out] = (T)(-0.5 * tv);
+    S1[n2 + out] = (T)(0.5 * xv + pv);
+    A1[out] = (T)(tv - 0.5 * xv - pv);
+    S2[out] = (T)(2.0 * tv - xv);
+    S2[n2 + out] = (T)tv;
+    Tx[out] = (T)xv;
+  }
+}
+
+template <typename T>
+void ring_prep(int64_t o, int64_t v, const double* t1, const double* t2, T* S1, T* A1, T* S2, T* Tx,
+               hipStream_t s) {
+  int64_t tiles = ceil_div(v, 32);
+  dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * o));
+  ring_prep_kernel<T><<<grid, 256, 0, s>>>(o, v, t1, t2, S1, A1, S2, Tx);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row (i <= j) of Tp and, when i < j, row pM(i, j) of Tm.
+template <typename T>
+__global__ void tau_pack_kernel(int64_t o, int64_t v, const double* __restrict__ t1,
+                                const double* __restrict__ t2, T* __restrict__ Tp, T* __restrict__ Tm) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
+  int64_t i, j;
+  decodeP(blockIdx.y, i, j);
+  const int64_t rowP = blockIdx.y;
+  const bool hasM = i < j;
+  const int64_t rowM = hasM ? pairM(i, j) : 0;
+  const double* blk = t2 + (i * o + j) * v * v;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npP;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    int64_t e, f;
+    decodeP(c, e, f);
+    double tef = blk[e * v + f] + t1[i * v + e] * t1[j * v + f];
+    double tfe = blk[f * v + e] + t1[i * v + f] * t1[j * v + e];
+    Tp[rowP * npP + c] = (T)(e < f ? tef + tfe : tef);
+    if (hasM && e < f) Tm[rowM * npM + pairM(e, f)] = (T)(tef - tfe);
+  }
+}
+
+template <typename T>
+void tau_pack(int64_t o, int64_t v, const double* t1, const double* t2, T* Tp, T* Tm, hipStream_t s) {
+  int64_t npP = v * (v + 1) / 2;
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(npP, 256), 1024), (unsigned)(o * (o + 1) / 2));
+  tau_pack_kernel<T><<<grid, 256, 0, s>>>(o, v, t1, t2, Tp, Tm);
+  CCX_HIP(hipGetLastError());
+}
+
+__device__ __forceinline__ void split_bf16(double x, bf16& hi, bf16& lo) {
+  hi = __float2bfloat16((float)x);
+  lo = __float2bfloat16((float)(x - (double)__bfloat162float(hi)));
+}
+
+__global__ void tau_pack_bf16x3_kernel(int64_t o, int64_t v, int64_t KpP, int64_t KpM,
+                                       const double* __restrict__ t1, const double* __restrict__ t2,
+                                       bf16* Tp_hi, bf16* Tp_lo, bf16* Tm_hi, bf16* Tm_lo) {
+  const int64_t npP = v * (v + 1) / 2;
+  int64_t i, j;
+  decodeP(blockIdx.y, i, j);
+  const int64_t rowP = blockIdx.y;
+  const bool hasM = i < j;
+  const int64_t rowM = hasM ? pairM(i, j) : 0;
+  const double* blk = t2 + (i * o + j) * v * v;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npP;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    int64_t e, f;
+    decodeP(c, e, f);
+    double tef = blk[e * v + f] + t1[i * v + e] * t1[j * v + f];
+    double tfe = blk[f * v + e] + t1[i * v + f] * t1[j * v + e];
+    split_bf16(e < f ? tef + tfe : tef, Tp_hi[rowP * KpP + c], Tp_lo[rowP * KpP + c]);
+    if (hasM && e < f) {
+      int64_t k = rowM * KpM + pairM(e, f);
+      split_bf16(tef - tfe, Tm_hi[k], Tm_lo[k]);
+    }
+  }
+}
+
+void tau_pack_bf16x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2,
+                     bf16* Tp_hi, bf16* Tp_lo, bf16* Tm_hi, bf16* Tm_lo, hipStream_t s) {
+  int64_t npP = v * (v + 1) / 2;
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(npP, 256), 1024), (unsigned)(o * (o + 1) / 2));
+  tau_pack_bf16x3_kernel<<<grid, 256, 0, s>>>(o, v, KpP, KpM, t1, t2, Tp_hi, Tp_lo, Tm_hi, Tm_lo);
+  CCX_HIP(hipGetLastError());
+}
+
+// (ab) block of the BF16x3 ladder matrices: row (e<=f) = column (e<=f) by symmetry.
+__global__ void ladder_pack_bf16x3_kernel(int64_t v, int64_t e0, VVVVSrc src, bf16* LPh, bf16* LPl,
+                                          bf16* LMh, bf16* LMl, int64_t KpP, int64_t KpM) {
+  const int64_t npP = v * (v + 1) / 2;
+  const int64_t e = e0 + blockIdx.y / v, f = blockIdx.y % v;
+  if (f < e) return;
+  const int64_t rowP = pairP(e, f);
+  const bool hasM = e < f;
+  const int64_t rowM = hasM ? pairM(e, f) : 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npP;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    int64_t a, b;
+    decodeP(c, a, b);
+    double x = src.base[e * src.se + a * src.sa + f * src.sf + b * src.sb];   // <ab|ef> = (ea|fb)
+    double y = src.base[e * src.se + b * src.sa + f * src.sf + a * src.sb];   // <ab|fe> = (eb|fa)
+    split_bf16(0.5 * (x + y), LPh[rowP * KpP + c], LPl[rowP * KpP + c]);
+    if (hasM && a < b) {
+      int64_t k = rowM * KpM + pairM(a, b);
+      split_bf16(0.5 * (x - y), LMh[k], LMl[k]);
+    }
+  }
+}
+
+void ladder_pack_bf16x3(int64_t o, int64_t v, int64_t e0, int64_t e1, VVVVSrc src, bf16* LP_hi,
+                        bf16* LP_lo, bf16* LM_hi, bf16* LM_lo, hipStream_t s) {
+  if (e1 <= e0) return;
+  int64_t npP = v * (v + 1) / 2;
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(npP, 256), 512), (unsigned)((e1 - e0) * v));
+  ladder_pack_bf16x3_kernel<<<grid, 256, 0, s>>>(v, e0, src, LP_hi, LP_lo, LM_hi, LM_lo,
+                                                  bf16_kpad(npP), bf16_kpad(v * (v - 1) / 2));
+  CCX_HIP(hipGetLastError());
+}
+
+// (m, b) rows of the BF16x3 ladder matrices: row npP + mb holds 1/2(<mb|ef> + <mb|fe>) over (e<=f).
+__global__ void ladder_pack_o_bf16x3_kernel(int64_t o, int64_t v, const double* __restrict__ Covvv,
+                                            bf16* LPh, bf16* LPl, bf16* LMh, bf16* LMl) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
+  const int64_t KpP = bf16_kpad(npP), KpM = bf16_kpad(npM);
+  const int64_t mb = blockIdx.y, m = mb / v, b = mb % v;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < npP;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    int64_t e, f;
+    decodeP(k, e, f);
+    double x = Covvv[((m * v + e) * v + b) * v + f];                            // <mb|ef> = (me|bf)
+    double y = Covvv[((m * v + f) * v + b) * v + e];                            // <mb|fe> = (mf|be)
+    split_bf16(0.5 * (x + y), LPh[(npP + mb) * KpP + k], LPl[(npP + mb) * KpP + k]);
+    if (e < f) {
+      int64_t q = (npM + mb) * KpM + pairM(e, f);
+      split_bf16(0.5 * (x - y), LMh[q], LMl[q]);
+    }
+  }
+}
+
+void ladder_pack_o_bf16x3(int64_t o, int64_t v, const double* Covvv, bf16* LP_hi, bf16* LP_lo,
+                          bf16* LM_hi, bf16* LM_lo, hipStream_t s) {
+  int64_t npP = v * (v + 1) / 2;
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(npP, 256), 256), (unsigned)(o * v));
+  ladder_pack_o_bf16x3_kernel<<<grid, 256, 0, s>>>(o, v, Covvv, LP_hi, LP_lo, LM_hi, LM_lo);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename Store>
+__global__ void ladder_pack_kernel(int64_t o, int64_t v, int64_t e0, VVVVSrc src,
+                                   const double* __restrict__ Covvv, Store st) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
+  const int64_t ov = o * v;
+  const int64_t e = e0 + blockIdx.y / v, f = blockIdx.y % v;
+  if (f < e) return;
+  const int64_t rowP = pairP(e, f);
+  const bool hasM = e < f;
+  const int64_t rowM = hasM ? pairM(e, f) : 0;
+  const int64_t ldP = npP + ov, ldM = npM + ov;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npP + ov;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    double x, y;
+    bool mcol;
+    int64_t cm;
+    if (c < npP) {
+      int64_t a, b;
+      decodeP(c, a, b);
+      x = src.base[e * src.se + a * src.sa + f * src.sf + b * src.sb];   // <ab|ef> = (ea|fb)
+      y = src.base[e * src.se + b * src.sa + f * src.sf + a * src.sb];   // <ab|fe> = (eb|fa)
+      mcol = a < b;
+      cm = mcol ? pairM(a, b) : 0;
+    } else {
+      int64_t mb = c - npP, m = mb / v, b = mb - (mb / v) * v;
+      x = Covvv[((m * v + e) * v + b) * v + f];                          // <mb|ef> = (me|bf)
+      y = Covvv[((m * v + f) * v + b) * v + e];                          // <mb|fe> = (mf|be)
+      mcol = true;
+      cm = npM + mb;
+    }
+    st.put(rowP * ldP + c, 0.5 * (x + y), true);
+    if (hasM && mcol) st.put(rowM * ldM + cm, 0.5 * (x - y), false);
+  }
+}
+
+template <typename T>
+struct StorePM {
+  T* P;
+  T* M;
+  __device__ void put(int64_t k, double val, bool plus) const { (plus ? P : M)[k] = (T)val; }
+};
+template <typename T>
+void ladder_pack(int64_t o, int64_t v, int64_t e0, int64_t e1, VVVVSrc src, const double* Covvv, T* LP,
+                 T* LM, hipStream_t s) {
+  if (e1 <= e0) return;
+  int64_t ncol = v * (v + 1) / 2 + o * v;
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(ncol, 256), 512), (unsigned)((e1 - e0) * v));
+  ladder_pack_kernel<StorePM<T>><<<grid, 256, 0, s>>>(o, v, e0, src, Covvv, StorePM<T>{LP, LM});
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void zt_add_kernel(int64_t o, int64_t v, const T* __restrict__ CP, const T* __restrict__ CM,
+                              double* __restrict__ Zt) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
+  const int64_t ldP = npP + o * v, ldM = npM + o * v;
+  const int64_t total = o * v * o * o;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = e % o, t = e / o;
+    int64_t i = t % o, mb = t / o;
+    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    double z = (double)CP[pairP(lo, hi) * ldP + npP + mb];
+    if (i != j) {
+      double zm = (double)CM[pairM(lo, hi) * ldM + npM + mb];
+      z += (i < j) ? zm : -zm;
+    }
+    Zt[e] += z;
+  }
+}
+
+template <typename T>
+void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStream_t s) {
+  zt_add_kernel<T><<<grid_for(o * v * o * o), 256, 0, s>>>(o, v, CP, CM, Zt);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void assemble_r2_kernel(int64_t o, int64_t v, const double* __restrict__ H,
+                                   const T* __restrict__ CP, const T* __restrict__ CM,
+                                   const T* __restrict__ P, const T* __restrict__ Q,
+                                   double* __restrict__ r2) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
+  const int64_t ldP = npP + o * v, ldM = npM + o * v;
+  const int64_t total = o * o * v * v;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = e % v, t = e / v;
+    int64_t a = t % v; t /= v;
+    int64_t j = t % o, i = t / o;
+    double r = H[e] + H[((j * o + i) * v + b) * v + a];
+    int64_t il = i < j ? i : j, ih = i < j ? j : i;
+    int64_t al = a < b ? a : b, ah = a < b ? b : a;
+    r += (double)CP[pairP(il, ih) * ldP + pairP(al, ah)];
+    if (i != j && a != b) {
+      double m = (double)CM[pairM(il, ih) * ldM + pairM(al, ah)];
+      r += ((i < j) == (a < b)) ? m : -m;
+    }
+    r += (double)P[((i * v + a) * o + j) * v + b] + (double)P[((j * v + b) * o + i) * v + a];
+    r += (double)Q[((j * v + a) * o + i) * v + b] + (double)Q[((i * v + b) * o + j) * v + a];
+    r2[e] = r;
+  }
+}
+
+template <typename T>
+void assemble_r2(int64_t o, int64_t v, const double* H, const T* CP, const T* CM, const T* P,
+                 const T* Q, double* r2, hipStream_t s) {
+  assemble_r2_kernel<T><<<grid_for(o * o * v * v), 256, 0, s>>>(o, v, H, CP, CM, P, Q, r2);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ void energy_kernel(int64_t o, int64_t v, const double* __restrict__ fock, int64_t nmo,
+                              const double* __restrict__ t1, const double* __restrict__ t2,
+                              const double* __restrict__ L, double* __restrict__ partials) {
+  const int64_t total = o * o * v * v;
+  double s = 0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = e % v, t = e / v;
+    int64_t a = t % v; t /= v;
+    int64_t j = t % o, i = t / o;
+    s += (t2[e] + t1[i * v + a] * t1[j * v + b]) * L[e];
+  }
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < o * v;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = e / v, a = e % v;
+    s += 2.0 * fock[i * nmo + o + a] * t1[e];
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+void energy(int64_t o, int64_t v, const double* fock, int64_t nmo, const double* t1, const double* t2,
+            const double* Loovv, double* partials, double* e_out, hipStream_t s) {
+  energy_kernel<<<RED_BLOCKS, 256, 0, s>>>(o, v, fock, nmo, t1, t2, Loovv, partials);
+  CCX_HIP(hipGetLastError());
+  finish_sum_kernel<<<1, 256, 0, s>>>(partials, RED_BLOCKS, e_out);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ void jacobi_kernel(int64_t o, int64_t v, const double* __restrict__ eps, double* __restrict__ t1,
+                              double* __restrict__ t2, const double* __restrict__ r1,
+                              const double* __restrict__ r2, double* __restrict__ partials) {
+  const int64_t total = o * o * v * v;
+  double s = 0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b = e % v, t = e / v;
+    int64_t a = t % v; t /= v;
+    int64_t j = t % o, i = t / o;
+    double d = eps[i] + eps[j] - eps[o + a] - eps[o + b];
+    double x = r2[e] / d;
+    t2[e] += x;
+    s += x * x;
+  }
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < o * v;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = e / v, a = e % v;
+    double x = r1[e] / (eps[i] - eps[o + a]);
+    t1[e] += x;
+    s += x * x;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+void jacobi_update(int64_t o, int64_t v, const double* eps, double* t1, double* t2, const double* r1,
+                   const double* r2, double* partials, double* rms2_out, hipStream_t s) {
+  jacobi_kernel<<<RED_BLOCKS, 256, 0, s>>>(o, v, eps, t1, t2, r1, r2, partials);
+  CCX_HIP(hipGetLastError());
+  finish_sum_kernel<<<1, 256, 0, s>>>(partials, RED_BLOCKS, rms2_out);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+__global__ void dots_kernel(int64_t n, const double* __restrict__ x0, int64_t stride,
+                            const double* __restrict__ y, double* __restrict__ partials) {
+  const double* x = x0 + blockIdx.y * stride;
+  double s = 0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    s += x[e] * y[e];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void dots_finish_kernel(const double* __restrict__ partials, int nb, double* out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[blockIdx.x * nb + i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+void dots(int64_t n, int nvec, const double* x0, int64_t stride, const double* y, double* partials,
+          double* out, hipStream_t s) {
+  if (nvec <= 0) return;
+  const int nb = 256;
+  dots_kernel<<<dim3(nb, nvec), 256, 0, s>>>(n, x0, stride, y, partials);
+  CCX_HIP(hipGetLastError());
+  dots_finish_kernel<<<nvec, 256, 0, s>>>(partials, nb, out);
+  CCX_HIP(hipGetLastError());
+}
+
+__global__ void lincomb_kernel(int64_t n, int nvec, const double* __restrict__ c,
+                               const double* __restrict__ x0, int64_t stride, double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int k = 0; k < nvec; ++k)
+      if (c[k] != 0.0) s += c[k] * x0[k * stride + e];   // unused slots carry c = 0
+    out[e] = s;
+  }
+}
+
+void lincomb(int64_t n, int nvec, const double* c, const double* x0, int64_t stride, double* out,
+             hipStream_t s) {
+  lincomb_kernel<<<grid_for(n), 256, 0, s>>>(n, nvec, c, x0, stride, out);
+  CCX_HIP(hipGetLastError());
+}
+
+__global__ void sub_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                           double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = x[e] - y[e];
+}
+
+void sub(int64_t n, const double* x, const double* y, double* out, hipStream_t s) {
+  sub_kernel<<<grid_for(n), 256, 0, s>>>(n, x, y, out);
+  CCX_HIP(hipGetLastError());
+}
+
+template <typename Tin, typename Tout>
+__global__ void cast_kernel(int64_t n, const Tin* __restrict__ in, Tout* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = (Tout)in[e];
+}
+
+template <typename Tin, typename Tout>
+void cast(int64_t n, const Tin* in, Tout* out, hipStream_t s) {
+  if (!n) return;
+  cast_kernel<Tin, Tout><<<grid_for(n), 256, 0, s>>>(n, in, out);
+  CCX_HIP(hipGetLastError());
+}
+
+// explicit instantiations
+template void ring_prep<double>(int64_t, int64_t, const double*, const double*, double*, double*, double*,
+                                double*, hipStream_t);
+template void ring_prep<float>(int64_t, int64_t, const double*, const double*, float*, float*, float*,
+                               float*, hipStream_t);
+template void tau_pack<double>(int64_t, int64_t, const double*, const double*, double*, double*, hipStream_t);
+template void tau_pack<float>(int64_t, int64_t, const double*, const double*, float*, float*, hipStream_t);
+template void ladder_pack<double>(int64_t, int64_t, int64_t, int64_t, VVVVSrc, const double*, double*,
+                                  double*, hipStream_t);
+template void ladder_pack<float>(int64_t, int64_t, int64_t, int64_t, VVVVSrc, const double*, float*,
+                                 float*, hipStream_t);
+template void zt_add<double>(int64_t, int64_t, const double*, const double*, double*, hipStream_t);
+template void zt_add<float>(int64_t, int64_t, const float*, const float*, double*, hipStream_t);
+template void assemble_r2<double>(int64_t, int64_t, const double*, const double*, const double*,
+                                  const double*, const double*, double*, hipStream_t);
+template void assemble_r2<float>(int64_t, int64_t, const double*, const float*, const float*, const float*,
+                                 const float*, double*, hipStream_t);
+template void cast<double, float>(int64_t, const double*, float*, hipStream_t);
+template void cast<float, double>(int64_t, const float*, double*, hipStream_t);
+
+}  // namespace ccx
