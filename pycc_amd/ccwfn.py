@@ -142,9 +142,9 @@ class CCwfn(object):
         eps_o, eps_v = eps[self.o], eps[self.v]
         self.H.eps = eps
         self.Dia = eps_o.reshape(-1, 1) - eps_v
+        self._e_dev = torch.zeros(2, dtype=f64, device=dev)     # device scalars: energy, rms^2
         self.t1 = torch.zeros((self.no, self.nv), dtype=f64, device=dev)
         self.t2 = self._mp1_guess()
-        self._e_dev = torch.zeros(2, dtype=f64, device=dev)
         self.ecc = None
         self.niter = 0
         if self.verbose:

@@ -5,10 +5,12 @@
 // e.g. 'jnfb,mnef->mbej' runs without materialising permuted copies. Operands are staged through
 // LDS in k-major tiles; each of the 4 waves owns a (BM/2) x (BN/2) block of 16x16 MFMA tiles.
 //
-// Lane layout of the 16x16x4 MFMAs used here (f64 and f32 variants are identical):
+// Lane layout of the 16x16x4 MFMAs used here (measured on gfx950):
 //   A (16x4):  lane l holds A[l % 16][l / 16]
 //   B (4x16):  lane l holds B[l / 16][l % 16]
-//   C (16x16): lane l, register r holds C[4 * (l / 16) + r][l % 16]
+//   C (16x16): lane l, register r holds C[row][l % 16] with
+//              row = 4 * (l / 16) + r   for v_mfma_f32_16x16x4f32
+//              row = 4 * r + (l / 16)   for v_mfma_f64_16x16x4f64
 #include "common.h"
 #include "gemm.h"
 
@@ -180,6 +182,8 @@ __global__ __launch_bounds__(256) void tc_gemm_kernel(GemmParams<T> p, T* __rest
 
   const int fr = lane >> 4;   // k row within an MFMA step
   const int fc = lane & 15;   // m / n column within a 16-wide block
+  // output row of register r within a 16x16 block (see the layout note at the top)
+  auto orow = [&](int r) { return sizeof(T) == 8 ? 4 * r + fr : 4 * fr + r; };
 
   if (k_begin < k_end) {
     load_a(k_begin);
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(256) void tc_gemm_kernel(GemmParams<T> p, T* __rest
       for (int i = 0; i < NBI; ++i) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int64_t m = tm * BM + wm * WM + i * 16 + fr * 4 + r;
+          int64_t m = tm * BM + wm * WM + i * 16 + orow(r);
           if (m >= p.M) continue;
           T* dst = Cb + off2(p.cm, m) + no;
           T v = p.alpha * acc[i][j][r];
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(256) void tc_gemm_kernel(GemmParams<T> p, T* __rest
       for (int i = 0; i < NBI; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int64_t m = tm * BM + wm * WM + i * 16 + fr * 4 + r;
+          int64_t m = tm * BM + wm * WM + i * 16 + orow(r);
           if (m < p.M) Wb[m * p.N + n] = acc[i][j][r];
         }
     }

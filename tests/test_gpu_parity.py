@@ -138,8 +138,10 @@ def test_solve_returns_none_when_not_converged(pycc_amd):
     assert cc.solve_cc(1e-14, 1e-14, maxiter=2) is None
 
 
-@pytest.mark.parametrize("precision,tol_r,tol_e", [("MIXED", 1e-5, 1e-7), ("MIXED_BF16X3", 3e-4, 3e-6)])
-def test_mixed_precision_residual_and_energy(pycc_amd, precision, tol_r, tol_e):
+# Low-precision contractions leave a noise floor in the Jacobi step norm, so the mixed solves use
+# r_conv above that floor; the energy bar is the north star's 1 microhartree.
+@pytest.mark.parametrize("precision,tol_r,r_conv", [("MIXED", 1e-5, 1e-7), ("MIXED_BF16X3", 3e-4, 1e-6)])
+def test_mixed_precision_residual_and_energy(pycc_amd, precision, tol_r, r_conv):
     no, nv = 6, 30
     sd, H = _ham(no, nv, 11, t2max=0.03)
     ref = OracleCCSD(H, no)
@@ -151,7 +153,11 @@ def test_mixed_precision_residual_and_energy(pycc_amd, precision, tol_r, tol_e):
     rel1 = np.abs(r1.cpu().numpy() - r1_ref).max() / np.abs(r1_ref).max()
     print("%s residual rel err r1 %.2e r2 %.2e" % (precision, rel1, rel2))
     assert rel2 < tol_r and rel1 < tol_r
-    e_ref = OracleCCSD(H, no).solve_cc(1e-10, 1e-10)
-    e = cc.solve_cc(1e-10, 1e-10)
-    print("%s energy error %.3e Eh" % (precision, e - e_ref))
-    assert e is not None and abs(e - e_ref) < tol_e
+    ref = OracleCCSD(H, no)
+    e_ref = ref.solve_cc(1e-10, 1e-10)
+    e = cc.solve_cc(1e-9, r_conv)
+    assert e is not None
+    print("%s energy error %.3e Eh, max|dT1| %.2e max|dT2| %.2e, iterations %d (FP64 oracle %d)" % (
+        precision, e - e_ref, np.abs(cc.t1.cpu().numpy() - ref.t1).max(),
+        np.abs(cc.t2.cpu().numpy() - ref.t2).max(), cc.niter, ref.niter))
+    assert abs(e - e_ref) < 1e-6
