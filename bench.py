@@ -1,0 +1,235 @@
+#!/usr/bin/env python
+"""CCSD iterations/s and TFLOP/s on one MI355X: mixed precision vs plain FP64 at <= 1 uEh.
+
+Workload (default): seeded synthetic closed-shell Hamiltonian, o = 48, v = 384 (nmo = 432,
+naux = 864, seed 3), integrals generated on the GPU from density-fitting factors. One "step" is
+one iteration of the CCSD solve loop (residual, Jacobi update, energy, DIIS), exactly as
+``solve_cc`` runs it.
+
+Legs (each in its own context, run one after the other):
+  mixed  -- precision ``--precision`` (default MIXED_BF16X3): W warmup + K timed iterations,
+            then iterated (untimed) to convergence;
+  fp64   -- precision DP, the same; its converged energy is the FP64 reference at this size.
+``value`` is the mixed leg's iterations/s; the FP64 leg, the speed-up and the energy / amplitude
+errors of the mixed solution against the FP64 one are reported beside it.
+
+The roofline object describes the dominant kernel family of the mixed leg (the ring GEMMs), timed
+with HIP events on the residual's stream by the library over the timed iterations; ``traffic``
+comes from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/ (null if absent).
+The CPU baseline times the numpy oracle (a restatement of pycc's numpy path) on a bounded sample:
+iterations at o = 16, v = 128, scaled to this workload by the ratio of reference flop counts.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK = {"DP": 78.6e12, "MIXED": 157.3e12, "MIXED_BF16X3": 157.3e12}   # ring GEMM precision peak
+PEAK_FP64, PEAK_FP32, PEAK_BF16 = 78.6e12, 157.3e12, 2.5e15
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv):
+    import torch
+    import pycc_amd
+    t0 = time.time()
+    cc = pycc_amd.CCwfn(sd, no=no, precision=precision, verbose=False)
+    torch.cuda.synchronize()
+    setup = time.time() - t0
+    log("[%s] setup %.1f s, device memory %.1f GB" % (precision, setup, cc.device_bytes() / 1e9))
+    it = cc.iterate(e_conv, r_conv, maxiter, 8, 1)
+    last = None
+    done = False
+    for _ in range(warmup):
+        last = next(it, None)
+        if last is None or last["converged"]:
+            done = True
+            break
+    torch.cuda.synchronize()
+    cc.timing_start()
+    t1 = time.perf_counter()
+    n = 0
+    while n < steps and not done:
+        last = next(it, None)
+        if last is None:
+            done = True
+            break
+        n += 1
+        if last["converged"]:
+            done = True
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    timing = cc.timing_read()
+    log("[%s] %d timed iterations: %.3f s/iter (E = %.12f, rms %.2e)" % (
+        precision, n, wall / max(n, 1), last["ecc"] if last else float("nan"), last["rms"] if last else 0))
+    if converge:
+        while not done:
+            last = next(it, None)
+            if last is None:
+                break
+            if last["niter"] % 5 == 0:
+                log("[%s] iter %d E = %.12f dE = %.2e rms = %.2e" % (
+                    precision, last["niter"], last["ecc"], last["ediff"], last["rms"]))
+            if last["converged"]:
+                done = True
+    out = {
+        "precision": precision, "setup_s": setup, "timed_steps": n, "wall_s": wall,
+        "ms_per_step": 1e3 * wall / max(n, 1), "iters_per_s": n / wall if wall > 0 else 0.0,
+        "timing": timing, "converged": bool(last and last["converged"]),
+        "iterations": cc.niter, "energy": cc.ecc if (last and last["converged"]) else (last["ecc"] if last else None),
+        "device_GB": cc.device_bytes() / 1e9,
+    }
+    t1c, t2c = cc.t1.clone(), cc.t2.clone()
+    del cc, it
+    torch.cuda.empty_cache()
+    return out, t1c, t2c
+
+
+def cpu_baseline(threads):
+    """numpy oracle iterations at o=16, v=128 (seed 0), scaled by F_ref(workload)/F_ref(sample)."""
+    from threadpoolctl import threadpool_limits
+    from oracle.ccsd_ref import OracleCCSD, OracleDIIS
+    from pycc_amd.synth import synthetic_df, DenseHamiltonian, dirac_eri
+    so, sv = 16, 128
+    sd = synthetic_df(so, sv, seed=0)
+    H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
+    with threadpool_limits(threads):
+        cc = OracleCCSD(H, so)
+        diis = OracleDIIS(cc.t1, cc.t2, 8)
+        times = []
+        for k in range(4):
+            t0 = time.perf_counter()
+            r1, r2 = cc.residuals(H.F, cc.t1, cc.t2)
+            cc.t1 = cc.t1 + r1 / cc.Dia
+            cc.t2 = cc.t2 + r2 / cc.Dijab
+            cc.energy()
+            diis.add_error_vector(cc.t1, cc.t2)
+            cc.t1, cc.t2 = diis.extrapolate(cc.t1, cc.t2)
+            times.append(time.perf_counter() - t0)
+    s_per_iter = sorted(times[1:])[len(times[1:]) // 2]      # median after one warm-up
+    return so, sv, s_per_iter
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no", type=int, default=48)
+    ap.add_argument("--nv", type=int, default=384)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--precision", default="MIXED_BF16X3", choices=["MIXED", "MIXED_BF16X3", "DP"])
+    ap.add_argument("--maxiter", type=int, default=60)
+    ap.add_argument("--e-conv", type=float, default=1e-9)
+    ap.add_argument("--r-conv-dp", type=float, default=1e-8)
+    ap.add_argument("--r-conv-mixed", type=float, default=1e-6)
+    ap.add_argument("--no-converge", action="store_true", help="skip the untimed convergence legs")
+    ap.add_argument("--no-dp", action="store_true", help="skip the FP64 leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the sharding design "
+              "for more GPUs is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from pycc_amd.flops import f_ref, f_ladder_kernel, f_ring_kernel
+    from pycc_amd.synth import synthetic_df
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    o, v = args.no, args.nv
+    t0 = time.time()
+    sd = synthetic_df(o, v, seed=args.seed)
+    log("synthetic Hamiltonian o=%d v=%d naux=%d seed=%d generated in %.1f s" % (o, v, sd.naux, args.seed, time.time() - t0))
+
+    converge = not args.no_converge
+    mixed, mt1, mt2 = run_leg(sd, o, args.precision, args.warmup, args.steps, converge, args.maxiter,
+                              args.e_conv, args.r_conv_mixed if args.precision != "DP" else args.r_conv_dp)
+    mt1, mt2 = mt1.cpu(), mt2.cpu()
+    dp = None
+    if not args.no_dp and args.precision != "DP":
+        torch.cuda.empty_cache()
+        dp, dt1, dt2 = run_leg(sd, o, "DP", args.warmup, args.steps, converge, args.maxiter,
+                               args.e_conv, args.r_conv_dp)
+        dt1, dt2 = dt1.cpu(), dt2.cpu()
+
+    F = f_ref(o, v)
+    res = {}
+    res["metric"] = "CCSD iters/sec + TFLOP/s, mixed vs FP64 plain at <=1 uEh error (o=48,v=384)"
+    res["value"] = mixed["iters_per_s"]
+    res["unit"] = "iters/s"
+    res["n_gpus"] = 1
+    res["steps"] = mixed["timed_steps"]
+    res["warmup"] = args.warmup
+    res["ms_per_step"] = mixed["ms_per_step"]
+    res["higher_is_better"] = True
+    res["scaling"] = "weak"
+    res["vs_baseline"] = None
+    res["dtype"] = {"MIXED": "f32 (ladder+rings), f64 (rest)", "MIXED_BF16X3": "bf16x3 ladder, f32 rings, f64 rest",
+                    "DP": "f64"}[args.precision]
+    res["data"] = "synthetic (seeded DF Hamiltonian, no external data)"
+    res["config"] = {"workload": "closed-shell CCSD iteration, synthetic DF Hamiltonian",
+                     "no": o, "nv": v, "nmo": o + v, "naux": sd.naux, "seed": args.seed,
+                     "precision": args.precision, "diis": 8}
+    res["tflops"] = F / (mixed["ms_per_step"] * 1e-3) / 1e12
+    res["f_ref_per_iter"] = F
+    if dp:
+        res["fp64"] = {"iters_per_s": dp["iters_per_s"], "ms_per_step": dp["ms_per_step"],
+                       "tflops": F / (dp["ms_per_step"] * 1e-3) / 1e12,
+                       "iterations": dp["iterations"], "energy": dp["energy"], "converged": dp["converged"]}
+        res["speedup_vs_fp64"] = mixed["iters_per_s"] / dp["iters_per_s"] if dp["iters_per_s"] else None
+        if mixed["energy"] is not None and dp["energy"] is not None:
+            res["energy_error_uEh"] = 1e6 * (mixed["energy"] - dp["energy"])
+            res["max_abs_dT1"] = float((mt1 - dt1).abs().max())
+            res["max_abs_dT2"] = float((mt2 - dt2).abs().max())
+    res["mixed"] = {"iterations": mixed["iterations"], "energy": mixed["energy"], "converged": mixed["converged"],
+                    "setup_s": mixed["setup_s"], "device_GB": mixed["device_GB"]}
+
+    # roofline of the dominant kernel family (ring GEMMs) over the timed iterations
+    n_r, ms_r, fl_r = mixed["timing"]["ring"]
+    n_l, ms_l, fl_l = mixed["timing"]["ladder"]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ring_gemm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(args.precision)
+    ring_prec = "MIXED" if args.precision != "DP" else "DP"
+    achieved = fl_r / (ms_r * 1e-3) if ms_r > 0 else 0.0
+    res["roofline"] = {"kernel": "plain_gemm_kernel (ring GEMMs, %s)" % ("f32" if ring_prec == "MIXED" else "f64"),
+                       "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[ring_prec] / 1e12,
+                       "unit": "TFLOP/s", "frac": achieved / PEAK[ring_prec], "traffic": traffic,
+                       "launches": n_r, "avg_launch_ms": ms_r / max(n_r, 1),
+                       "flops_per_launch": fl_r / max(n_r, 1)}
+    lad_peak = PEAK_BF16 / 3 if args.precision == "MIXED_BF16X3" else PEAK[args.precision]
+    res["ladder"] = {"achieved_tflops": (fl_l / (ms_l * 1e-3) / 1e12) if ms_l > 0 else 0.0,
+                     "peak_tflops": lad_peak / 1e12, "ms_per_step": ms_l / max(mixed["timed_steps"], 1),
+                     "flops_per_step": f_ladder_kernel(o, v)}
+    res["ring_ms_per_step"] = ms_r / max(mixed["timed_steps"], 1)
+
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        so, sv, s_it = cpu_baseline(threads)
+        scale = f_ref(o, v) / f_ref(so, sv)
+        res["cpu_baseline"] = {"value": 1.0 / (s_it * scale), "unit": "iters/s", "cores": threads,
+                               "kind": "port",
+                               "sample": "numpy oracle (restated pycc numpy path), median of 3 iterations at "
+                                         "o=16,v=128 seed 0 = %.3f s/iter, scaled by F_ref ratio %.1f" % (s_it, scale)}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -89,6 +89,15 @@ int ccx_diis_reset(ccx_ctx* ctx, const double* t1, const double* t2, int max_dii
 int ccx_diis_add(ccx_ctx* ctx, const double* t1, const double* t2, void* stream);
 int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out, void* stream);
 
+/* Measurement hooks (no counterpart in the reference, which has only wall-clock prints,
+ * pycc/utils.py:229-238): HIP events recorded on the residual's stream around the ladder GEMMs,
+ * the ring GEMMs and each whole residual.
+ * ccx_timing_start clears and starts recording; ccx_timing_read synchronises, writes
+ * out[3*p + {0,1,2}] = {launches, milliseconds, algorithmic flops} for p = 0 (ladder GEMMs),
+ * 1 (ring GEMMs), 2 (whole residuals), n <= 9 entries, and stops recording. */
+int ccx_timing_start(ccx_ctx* ctx);
+int ccx_timing_read(ccx_ctx* ctx, double* out, int n);
+
 /* Device bytes held by the context (integrals, work space, DIIS history). */
 int64_t ccx_device_bytes(const ccx_ctx* ctx);
 

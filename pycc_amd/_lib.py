@@ -37,6 +37,8 @@ SIGNATURES = {
     "ccx_diis_reset": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "ccx_diis_add": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "ccx_diis_extrapolate": (ctypes.c_int, [_vp, _vp, _vp, c_dp, _vp]),
+    "ccx_timing_start": (ctypes.c_int, [_vp]),
+    "ccx_timing_read": (ctypes.c_int, [_vp, c_dp, ctypes.c_int]),
     "ccx_device_bytes": (ctypes.c_int64, [_vp]),
     "ccx_contract": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, _vp, c_i64p, c_i64p, _vp, c_i64p, c_i64p,
                                     ctypes.c_double, _vp, c_i64p, c_i64p, _vp]),

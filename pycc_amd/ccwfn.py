@@ -200,11 +200,25 @@ class CCwfn(object):
         return float(self._e_dev[0].item())
 
     def solve_cc(self, e_conv=1e-7, r_conv=1e-7, maxiter=100, max_diis=8, start_diis=1):
-        """ccwfn.py:216-319 (spatial CCSD branch)."""
+        """ccwfn.py:216-319 (spatial CCSD branch): returns the correlation energy, or None when
+        ``maxiter`` iterations do not converge."""
         t0 = time.time()
+        name = "T-amplitudes (%s)" % self.model
+        for step in self.iterate(e_conv, r_conv, maxiter, max_diis, start_diis):
+            if step["converged"]:
+                if self.verbose:
+                    print("%s converged in %.3f seconds." % (name, time.time() - t0))
+                    print("E(%s) = %20.15f" % (self.model, step["ecc"]))
+                return step["ecc"]
+        return None
+
+    def iterate(self, e_conv=1e-7, r_conv=1e-7, maxiter=100, max_diis=8, start_diis=1):
+        """Generator over the iterations of ``solve_cc`` (ccwfn.py:260-319); yields one dict per
+        iteration (niter, ecc, ediff, rms, converged) after its convergence test, so a caller can
+        time or stop the solve between iterations. Stops after the converged iteration."""
         self.e_conv, self.r_conv, self.maxiter = e_conv, r_conv, maxiter
         F = self.H.F
-        ecc = self.cc_energy()
+        ecc = self.cc_energy()                                              # ccwfn.py:260
         name = "T-amplitudes (%s)" % self.model
         self.history = [(0, ecc, -ecc, None)]
         if self.verbose:
@@ -212,30 +226,41 @@ class CCwfn(object):
             print(_iteration(0, energy=ecc, de=-ecc, e_label="CC Ecorr", note="MP2"))
         cap = max(1, max_diis) + max(1, start_diis)
         _lib.call("ccx_diis_reset", self._ctx.h, dptr(self.t1), dptr(self.t2), int(max_diis), int(cap),
-                  self._stream())
+                  self._stream())                                           # ccwfn.py:266
         coef = (ctypes.c_double * 64)()
         for niter in range(1, maxiter + 1):
             ecc_last = ecc
-            r1, r2 = self.residuals(F, self.t1, self.t2)
-            rms = self._update(self.t1, self.t2, r1, r2)
+            r1, r2 = self.residuals(F, self.t1, self.t2)                    # ccwfn.py:272
+            rms = self._update(self.t1, self.t2, r1, r2)                    # ccwfn.py:281-284
             del r1, r2
-            ecc = self.cc_energy()
+            ecc = self.cc_energy()                                          # ccwfn.py:286
             ediff = ecc - ecc_last
             self.history.append((niter, ecc, ediff, rms))
             self.niter = niter
             if self.verbose:
                 print(_iteration(niter, energy=ecc, de=ediff, rms=rms, e_label="CC Ecorr"))
-            if abs(ediff) < e_conv and abs(rms) < r_conv:
-                if self.verbose:
-                    print("%s converged in %.3f seconds." % (name, time.time() - t0))
-                    print("E(%s) = %20.15f" % (self.model, ecc))
+            conv = abs(ediff) < e_conv and abs(rms) < r_conv                # ccwfn.py:295
+            if conv:
                 self.ecc = ecc
-                return ecc
+            yield {"niter": niter, "ecc": ecc, "ediff": ediff, "rms": rms, "converged": conv}
+            if conv:
+                return
             _lib.call("ccx_diis_add", self._ctx.h, dptr(self.t1), dptr(self.t2), self._stream())
-            if niter >= start_diis:
+            if niter >= start_diis:                                         # ccwfn.py:317-319
                 _lib.call("ccx_diis_extrapolate", self._ctx.h, dptr(self.t1), dptr(self.t2), coef,
                           self._stream())
-        return None
+
+    # ---- measurement hooks -----------------------------------------------------------------
+    def timing_start(self):
+        """Start HIP-event timing of the ladder / ring GEMMs and whole residuals (ccx_timing_*)."""
+        _lib.call("ccx_timing_start", self._ctx.h)
+
+    def timing_read(self):
+        """{'ladder'|'ring'|'residual': (launches, ms, algorithmic flops)} since timing_start."""
+        out = (ctypes.c_double * 9)()
+        _lib.call("ccx_timing_read", self._ctx.h, out, 9)
+        return {k: (int(out[3 * i]), out[3 * i + 1], out[3 * i + 2])
+                for i, k in enumerate(("ladder", "ring", "residual"))}
 
     def device_bytes(self):
         return int(_lib.load().ccx_device_bytes(self._ctx.h))

@@ -169,6 +169,20 @@ int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out,
   });
 }
 
+int ccx_timing_start(ccx_ctx* ctx) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::timing_start(ctx->c);
+  });
+}
+
+int ccx_timing_read(ccx_ctx* ctx, double* out, int n) {
+  return guarded([&] {
+    CCX_CHECK(ctx && out && n >= 0, ccx::ERR_ARG, "bad timing arguments");
+    ccx::timing_read(ctx->c, out, n);
+  });
+}
+
 int64_t ccx_device_bytes(const ccx_ctx* ctx) {
   return ctx ? ctx->c.bytes + (int64_t)ctx->c.ws.size : 0;
 }

@@ -26,6 +26,7 @@ struct Ctx {
   double* Covov = nullptr;           // (me|nf)  == K[(me),(nf)]
   double* Covvv = nullptr;           // (me|af)
   double* Loovv = nullptr;           // L_ijab = 2<ij|ab> - <ij|ba>, [i][j][a][b]
+  double* Looov = nullptr;           // Lp[i][m][n][e] = 2(mi|ne) - (ni|me) = L_mnie = L_nmei
   double* J = nullptr;               // J[(me),(nf)] = (mf|ne)
   float* K32 = nullptr;              // FP32 copies for the mixed ring GEMMs
   float* J32 = nullptr;
@@ -37,12 +38,44 @@ struct Ctx {
   // ---- per-residual work buffers (allocated on first use) ----
   bool have_work = false;
   double *Fae = nullptr, *Fmi = nullptr, *Fme = nullptr, *G = nullptr, *Hm = nullptr;
-  double *tau = nullptr, *Wmnij = nullptr, *Hh = nullptr, *Zt = nullptr, *Wd = nullptr;
+  double *tau = nullptr, *Wmnij = nullptr, *Hh = nullptr, *Zt = nullptr, *Wd = nullptr, *Ud = nullptr;
   void *S1 = nullptr, *A1 = nullptr, *S2 = nullptr, *Tx = nullptr, *W = nullptr;
   void *Tp = nullptr, *Tm = nullptr, *CP = nullptr, *CM = nullptr;
   bf16 *Tph = nullptr, *Tpl = nullptr, *Tmh = nullptr, *Tml = nullptr;
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
+
+  // ---- optional HIP-event timing of the GEMM phases (ccx_timing_start / ccx_timing_read) ----
+  enum Phase : int { PH_LADDER = 0, PH_RING = 1, PH_RESIDUAL = 2, PH_COUNT = 3 };
+  struct Span {
+    hipEvent_t a, b;
+    int phase;
+    double flops;
+  };
+  bool timing = false;
+  std::vector<Span> spans;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t take_event() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      CCX_HIP(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  // record a start event; returns its span index (or -1 when timing is off)
+  int tbegin(hipStream_t s, int phase, double flops) {
+    if (!timing) return -1;
+    Span sp{take_event(), take_event(), phase, flops};
+    CCX_HIP(hipEventRecord(sp.a, s));
+    spans.push_back(sp);
+    return (int)spans.size() - 1;
+  }
+  void tend(hipStream_t s, int k) {
+    if (k >= 0) CCX_HIP(hipEventRecord(spans[k].b, s));
+  }
 
   // ---- DIIS (pycc utils.helper_diis) ----
   int max_diis = 0, cap = 0;
@@ -74,6 +107,8 @@ void set_integrals_dense(Ctx& c, const double* fock, const double* eri, hipStrea
 void set_integrals_df(Ctx& c, const double* fock, const double* B, int64_t naux, hipStream_t s);
 void residual(Ctx& c, const double* fock, const double* t1, const double* t2, double* r1, double* r2,
               hipStream_t s);
+void timing_start(Ctx& c);
+void timing_read(Ctx& c, double* out, int n);
 void diis_reset(Ctx& c, const double* t1, const double* t2, int max_diis, int cap, hipStream_t s);
 void diis_add(Ctx& c, const double* t1, const double* t2, hipStream_t s);
 void diis_extrapolate(Ctx& c, double* t1, double* t2, double* coef_out, hipStream_t s);

@@ -18,6 +18,13 @@
 namespace ccx {
 
 void Ctx::destroy() {
+  for (auto& sp : spans) {
+    (void)hipEventDestroy(sp.a);
+    (void)hipEventDestroy(sp.b);
+  }
+  for (auto e : pool) (void)hipEventDestroy(e);
+  spans.clear();
+  pool.clear();
   for (void* p : owned) (void)hipFree(p);
   owned.clear();
   ws.release();
@@ -56,6 +63,7 @@ static void alloc_integrals(Ctx& c) {
   c.Covov = c.alloc<double>(ov * ov);
   c.Covvv = c.alloc<double>(ov * v * v);
   c.Loovv = c.alloc<double>(ov * ov);
+  c.Looov = c.alloc<double>(o * o * o * v);
   c.J = c.alloc<double>(ov * ov);
   const int64_t szP = c.npP() * (c.npP() + ov), szM = c.npM() * (c.npM() + ov);
   if (c.mode == MODE_DP) {
@@ -98,6 +106,9 @@ static void finish_integrals(Ctx& c, const double* fock, hipStream_t s) {
   // L_ijab = 2<ij|ab> - <ij|ba> = 2(ia|jb) - (ib|ja)   (hamiltonian.py:70)
   axpby<double>(2.0, tv(c.Covov, "iajb", {o, v, o, v}), 0.0, tv(c.Loovv, "ijab", {o, o, v, v}), s);
   axpby<double>(-1.0, tv(c.Covov, "ibja", {o, v, o, v}), 1.0, tv(c.Loovv, "ijab", {o, o, v, v}), s);
+  // Lp[i][m][n][e] = 2(mi|ne) - (ni|me): L_mnie (Fmi, ccwfn.py:532) and L_nmei (r_T1, :760)
+  axpby<double>(2.0, tv(c.Cooov, "mine", {o, o, o, v}), 0.0, tv(c.Looov, "imne", {o, o, o, v}), s);
+  axpby<double>(-1.0, tv(c.Cooov, "nime", {o, o, o, v}), 1.0, tv(c.Looov, "imne", {o, o, o, v}), s);
   // J[(me),(nf)] = (mf|ne)
   axpby<double>(1.0, tv(c.Covov, "mfne", {o, v, o, v}), 0.0, tv(c.J, "menf", {o, v, o, v}), s);
   if (c.mode != MODE_DP) {
@@ -170,6 +181,7 @@ static void alloc_work(Ctx& c) {
   c.Hh = c.alloc<double>(ov * ov);
   c.Zt = c.alloc<double>(ov * o * o);
   c.Wd = c.alloc<double>(2 * ov * ov);
+  c.Ud = c.alloc<double>(ov * ov);
   const size_t es = c.mode == MODE_DP ? sizeof(double) : sizeof(float);
   auto talloc = [&](size_t n) -> void* { return es == 8 ? (void*)c.alloc<double>(n) : (void*)c.alloc<float>(n); };
   c.S1 = talloc(2 * ov * ov);
@@ -235,8 +247,7 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   const TV Fmi = tv(c.Fmi, "mi", OO);
   axpby<double>(1.0, Foo, 0.0, Fmi, s);
   cd(0.5, T1("ie"), Fov, 1.0, Fmi);
-  cd(2.0, T1("ne"), tv(c.Cooov, "mine", OOOV), 1.0, Fmi);     // L_mnie = 2(mi|ne) - (ni|me)
-  cd(-1.0, T1("ne"), tv(c.Cooov, "nime", OOOV), 1.0, Fmi);
+  cd(1.0, T1("ne"), tv(c.Looov, "imne", OOOV), 1.0, Fmi);     // L_mnie = 2(mi|ne) - (ni|me)
   cd(1.0, tv(c.tau, "inef", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fmi);
 
   // ---- Wmnij: ccwfn.py:596-603 (tau = t2 + t1 t1) -------------------------------------------
@@ -254,8 +265,10 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   T* S2 = (T*)c.S2;
   T* Tx = (T*)c.Tx;
   T* W = (T*)c.W;
-  ring_prep<T>(o, v, t1, t2, S1, A1, S2, Tx, s);
-  const TV Wej = tv(c.Wd, "mejb", OVOV), Wje = tv(c.Wd + ov * ov, "mejb", OVOV);
+  ring_prep<T>(o, v, t1, t2, S1, A1, S2, Tx, c.Ud, s);
+  // W[(me)][x][(jb)]: x = 0 holds Wmbej, x = 1 holds Wmbje
+  const std::vector<int64_t> WSTR{v * 2 * ov, 2 * ov, v, 1};
+  const TV Wej = tvs(c.Wd, "mejb", OVOV, WSTR), Wje = tvs(c.Wd + ov, "mejb", OVOV, WSTR);
   axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, Wej, s);      // <mb|ej> = (me|jb)        :641
   cd(1.0, T1("jf"), tv(c.Covvv, "mebf", OVVV), 1.0, Wej);          // t_jf <mb|ef>, (me|bf)     :642
   cd(-1.0, T1("nb"), tv(c.Cooov, "njme", OOOV), 1.0, Wej);         // t_nb <mn|ej>, (nj|me)     :643
@@ -265,23 +278,34 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   if (!std::is_same<T, double>::value) cast<double, T>(2 * ov * ov, c.Wd, W, s);
   const T* Kt = std::is_same<T, double>::value ? (const T*)c.Covov : (const T*)c.K32;
   const T* Jt = std::is_same<T, double>::value ? (const T*)c.J : (const T*)c.J32;
-  const std::vector<int64_t> XOVOV{2, o, v, o, v};
+  const std::vector<int64_t> OVXOV{o, v, 2, o, v};
   // :644 -tau'_jnfb <mn|ef> (tau' = t2/2 + t1 t1), :645 +1/2 t_njfb L_mnef, :683 +tau'_jnfb <mn|fe>:
   //   Wej += K.(T - Tx/2 - P1) - 1/2 J.T ;  Wje += J.tau'
-  contract<T>(T(1), tv(Jt, "menf", OVOV), tv(S1, "xnfjb", XOVOV), T(1), tv(W, "xmejb", XOVOV), ws, s);
-  contract<T>(T(1), tv(Kt, "menf", OVOV), tv(A1, "nfjb", OVOV), T(1), tv(W, "mejb", OVOV), ws, s);
+  const double unit = 2.0 * (double)ov * ov * ov;   // one ov x ov x ov GEMM
+  int kr = c.tbegin(s, Ctx::PH_RING, 2 * unit);
+  contract<T>(T(1), tv(Jt, "menf", OVOV), tv(S1, "nfxjb", OVXOV), T(1), tv(W, "mexjb", OVXOV), ws, s);
+  c.tend(s, kr);
+  kr = c.tbegin(s, Ctx::PH_RING, unit);
+  contract<T>(T(1), tv(Kt, "menf", OVOV), tv(A1, "nfjb", OVOV), T(1), tvs(W, "mejb", OVOV, WSTR), ws, s);
+  c.tend(s, kr);
 
   // ---- ladder + Z: ccwfn.py:931 (1/2 tau_ijef <ab|ef>) and :715 (Z_mbij = <mb|ef> tau_ijef) ----
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
+  const double f_lad_P = 2.0 * noP * (npP + ov) * npP, f_lad_M = 2.0 * noM * (npM + ov) * npM;
   if (c.mode == MODE_MIXED_BF16X3) {
     ladder_bf16x3(c, t1, t2, s);
   } else {
     tau_pack<T>(o, v, t1, t2, (T*)c.Tp, (T*)c.Tm, s);
+    int k = c.tbegin(s, Ctx::PH_LADDER, f_lad_P);
     contract<T>(T(1), tv(c.Tp, "xk", {noP, npP}), tv(c.LP, "ky", {npP, npP + ov}), T(0),
                 tv(c.CP, "xy", {noP, npP + ov}), ws, s);
-    if (noM > 0)
+    c.tend(s, k);
+    if (noM > 0) {
+      k = c.tbegin(s, Ctx::PH_LADDER, f_lad_M);
       contract<T>(T(1), tv(c.Tm, "xk", {noM, npM}), tv(c.LM, "ky", {npM, npM + ov}), T(0),
                   tv(c.CM, "xy", {noM, npM + ov}), ws, s);
+      c.tend(s, k);
+    }
   }
 
   // ---- ring GEMMs: ccwfn.py:933-935 ------------------------------------------------------------
@@ -289,8 +313,12 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // Q[(ja),(ib)] = t2_mjae Wmbie                                (:935)
   T* P = A1;   // A1 is dead after the K GEMM
   T* Q = S1;   // S1 is dead after the J GEMM
-  contract<T>(T(1), tv(S2, "xiame", XOVOV), tv(W, "xmejb", XOVOV), T(0), tv(P, "iajb", OVOV), ws, s);
-  contract<T>(T(1), tv(Tx, "jame", OVOV), tv(W + ov * ov, "meib", OVOV), T(0), tv(Q, "jaib", OVOV), ws, s);
+  kr = c.tbegin(s, Ctx::PH_RING, 2 * unit);
+  contract<T>(T(1), tv(S2, "iamex", {o, v, o, v, 2}), tv(W, "mexjb", OVXOV), T(0), tv(P, "iajb", OVOV), ws, s);
+  c.tend(s, kr);
+  kr = c.tbegin(s, Ctx::PH_RING, unit);
+  contract<T>(T(1), tv(Tx, "jame", OVOV), tvs(W + ov, "meib", OVOV, WSTR), T(0), tv(Q, "jaib", OVOV), ws, s);
+  c.tend(s, kr);
 
   // ---- Zt_mbij = Z_mbij + <mb|ij> + t_ie <mb|ej> + t_je <mb|ie> -----------------------------------
   // :932 -t_ma Z_mbij, :940 -t_ma <mb|ij>, :937 -t_ie t_ma <mb|ej>, and :938 -t_ie t_mb <ma|je>
@@ -335,14 +363,12 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   axpby<double>(1.0, Fvo, 0.0, R1, s);                               // f_ai                 :754
   cd(1.0, T1("ie"), Fae, 1.0, R1);                                   //                      :755
   cd(-1.0, Fmi, T1("ma"), 1.0, R1);                                  //                      :756
-  cd(2.0, T2("imae"), Fme, 1.0, R1);                                 // (2t2 - t2^T) Fme     :757
-  cd(-1.0, T2("imea"), Fme, 1.0, R1);
+  // U = 2 t2 - t2^T in ring layout: Ud[i][a][m][e] = 2 t2_imae - t2_imea
+  cd(1.0, tv(c.Ud, "iame", OVOV), Fme, 1.0, R1);                     // (2t2 - t2^T) Fme     :757
   cd(2.0, T1("nf"), tv(c.Covov, "nfia", OVOV), 1.0, R1);             // L_nafi = 2(nf|ia)    :758
   cd(-1.0, T1("nf"), tv(c.Coovv, "niaf", OOVV), 1.0, R1);            //        - (ni|af)
-  cd(2.0, T2("mief"), tv(c.Covvv, "meaf", OVVV), 1.0, R1);           // <ma|ef> = (me|af)    :759
-  cd(-1.0, T2("mife"), tv(c.Covvv, "meaf", OVVV), 1.0, R1);
-  cd(-2.0, T2("mnae"), tv(c.Cooov, "mine", OOOV), 1.0, R1);          // L_nmei = 2(mi|ne)    :760
-  cd(1.0, T2("mnae"), tv(c.Cooov, "nime", OOOV), 1.0, R1);           //        - (ni|me)
+  cd(1.0, tv(c.Ud, "meif", OVOV), tv(c.Covvv, "meaf", OVVV), 1.0, R1); // <ma|ef> = (me|af)    :759
+  cd(-1.0, T2("mnae"), tv(c.Looov, "imne", OOOV), 1.0, R1);         // L_nmei = 2(mi|ne) - (ni|me) :760
 }
 
 void residual(Ctx& c, const double* fock, const double* t1, const double* t2, double* r1, double* r2,
@@ -350,8 +376,35 @@ void residual(Ctx& c, const double* fock, const double* t1, const double* t2, do
   CCX_CHECK(c.have_ints, ERR_STATE, "ccx_residual called before the integrals were set");
   CCX_CHECK(fock && t1 && t2 && r1 && r2, ERR_ARG, "null amplitude/residual pointer");
   alloc_work(c);
+  int k = c.tbegin(s, Ctx::PH_RESIDUAL, 0.0);
   if (c.mode == MODE_DP) residual_t<double>(c, fock, t1, t2, r1, r2, s);
   else residual_t<float>(c, fock, t1, t2, r1, r2, s);
+  c.tend(s, k);
+}
+
+void timing_start(Ctx& c) {
+  for (auto& sp : c.spans) {
+    c.pool.push_back(sp.a);
+    c.pool.push_back(sp.b);
+  }
+  c.spans.clear();
+  c.timing = true;
+}
+
+// out[3*p + 0..2] = launches, milliseconds, algorithmic flops of phase p; stops recording.
+void timing_read(Ctx& c, double* out, int n) {
+  std::vector<double> acc(3 * Ctx::PH_COUNT, 0.0);
+  for (auto& sp : c.spans) {
+    CCX_HIP(hipEventSynchronize(sp.b));
+    float ms = 0.f;
+    CCX_HIP(hipEventElapsedTime(&ms, sp.a, sp.b));
+    acc[3 * sp.phase] += 1;
+    acc[3 * sp.phase + 1] += ms;
+    acc[3 * sp.phase + 2] += sp.flops;
+  }
+  for (int i = 0; i < n && i < (int)acc.size(); ++i) out[i] = acc[i];
+  timing_start(c);
+  c.timing = false;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -1,6 +1,8 @@
 // Einsum-style contraction planner and strided elementwise kernels (see contract.h).
 #include "contract.h"
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace ccx {
@@ -28,6 +30,16 @@ TV tvs(const void* p, const char* lab, std::vector<int64_t> ext, std::vector<int
 }
 
 namespace {
+
+// CCX_TRACE=1 prints every planned contraction to stderr (diagnostics only).
+bool trace_on() {
+  static int on = -1;
+  if (on < 0) {
+    const char* e = std::getenv("CCX_TRACE");
+    on = (e && e[0] == '1') ? 1 : 0;
+  }
+  return on == 1;
+}
 
 // One index of a GEMM class: its extent and its stride in each of the (up to 3) operands.
 struct Lab {
@@ -163,6 +175,38 @@ void contract(T alpha, const TV& A_, const TV& B_, T beta, const TV& C, Workspac
     for (int64_t i = 0; i < e; ++i)
       contract<T>(alpha, slice<T>(A, c, i), slice<T>(B, c, i), beta, slice<T>(C, c, i), ws, s);
   };
+  // An M (N) class with a third index and no batch index: that index becomes the batch, with
+  // stride 0 in B (A), which lacks it. Pick the index whose removal leaves a fusable class.
+  auto try_batch = [&](std::vector<Lab>& g, const std::vector<Lab>& orig, int ops, int absent) -> bool {
+    if (!gb.empty() || g.size() <= 2) return false;
+    for (size_t x = 0; x < orig.size(); ++x) {
+      if (orig[x].ext == 1) continue;
+      std::vector<Lab> rest;
+      for (size_t y = 0; y < orig.size(); ++y)
+        if (y != x) rest.push_back(orig[y]);
+      fuse(rest, ops);
+      if (rest.size() <= 2) {
+        Lab bl = orig[x];
+        bl.s[absent] = 0;
+        gb.push_back(bl);
+        g = rest;
+        return true;
+      }
+    }
+    return false;
+  };
+  try_batch(gm, om, 1 | 4, 1);
+  try_batch(gn, on, 2 | 4, 0);
+  if (trace_on()) {
+    auto str = [](const std::vector<Lab>& g) {
+      std::string r;
+      for (auto& l : g) r += l.c;
+      return r.empty() ? std::string("-") : r;
+    };
+    std::fprintf(stderr, "[ccx] %s,%s->%s  M{%s} N{%s} K{%s} B{%s} M=%lld N=%lld K=%lld%s\n", A_.lab.c_str(), B_.lab.c_str(),
+                 C.lab.c_str(), str(gm).c_str(), str(gn).c_str(), str(gk).c_str(), str(gb).c_str(), (long long)extent(gm), (long long)extent(gn), (long long)extent(gk),
+                 (gb.size() > 1 || gk.size() > 2 || gm.size() > 2 || gn.size() > 2) ? "  HOST-LOOP" : "");
+  }
   if (gb.size() > 1) { loop_over(smallest(ob)); return; }
   if (gk.size() > 2) {
     // a K index is sliced by accumulating: the first slice applies beta, the rest add

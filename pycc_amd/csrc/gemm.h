@@ -30,4 +30,22 @@ struct Workspace {
 template <typename T>
 void tc_gemm(const GemmParams<T>& p, Workspace& ws, hipStream_t s);
 
+// Plain strided matrices (gemm_plain.hip): C[b][m * ldc + n] = alpha op(A) op(B) + beta C.
+template <typename T>
+struct PlainGemm {
+  int64_t M = 0, N = 0, K = 0, batch = 1;
+  const T* A = nullptr;
+  const T* B = nullptr;
+  T* C = nullptr;
+  int64_t lda = 0, ldb = 0, ldc = 0;
+  int64_t a_b = 0, b_b = 0, c_b = 0;
+  bool a_kc = true;   // A[m * lda + k] (else A[k * lda + m])
+  bool b_nc = true;   // B[k * ldb + n] (else B[n * ldb + k])
+  T alpha = 1, beta = 0;
+};
+template <typename T>
+bool plain_gemm_ok(const PlainGemm<T>& p);
+template <typename T>
+void plain_gemm(const PlainGemm<T>& p, hipStream_t s);
+
 }  // namespace ccx

@@ -13,6 +13,8 @@
 //              row = 4 * r + (l / 16)   for v_mfma_f64_16x16x4f64
 #include "common.h"
 #include "gemm.h"
+#include <cstdio>
+#include <cstdlib>
 
 namespace ccx {
 
@@ -46,20 +48,91 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t w, int64_t ntiles) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// LDS tile of one operand: stored [rows][BK + 1] when the operand is loaded along k (k-fast) and
+// [BK][rows + 1] when it is loaded along m / n. The +1 shifts consecutive rows by two 4-byte banks,
+// so neither the stores nor the MFMA fragment reads conflict.
+template <typename T, int R, int BK, bool KFAST>
+struct LdsTile {
+  T d[KFAST ? R : BK][(KFAST ? BK : R) + 1];
+  __device__ __forceinline__ T& at(int r, int k) { return KFAST ? d[r][k] : d[k][r]; }
+};
+
+// Loader of one operand tile (R rows x BK k's) from a 2-level-indexed global array. Each thread
+// owns NE = R * BK / 256 elements: k-fast -> one k column and NE rows; otherwise one row, NE k's.
+template <typename T, int R, int BK, bool KFAST>
+struct TileLoader {
+  static constexpr int NT = 256;
+  static constexpr int NE = R * BK / NT;
+  int kl, rl;                      // local coordinates of the thread's first element
+  int64_t roff[KFAST ? NE : 1];    // global offsets of the thread's rows
+  bool rok[KFAST ? NE : 1];
+  T v[NE];
+
+  __device__ __forceinline__ void init(const Idx2& rix, int64_t row0, int64_t nrows, int tid) {
+    if (KFAST) {
+      kl = tid % BK; rl = tid / BK;
+#pragma unroll
+      for (int i = 0; i < NE; ++i) {
+        int64_t r = row0 + rl + i * (NT / BK);
+        rok[i] = r < nrows;
+        roff[i] = rok[i] ? off2(rix, r) : 0;
+      }
+    } else {
+      rl = tid % R; kl = tid / R;
+      int64_t r = row0 + rl;
+      rok[0] = r < nrows;
+      roff[0] = rok[0] ? off2(rix, r) : 0;
+    }
+  }
+  // Unguarded load of a k-tile that lies inside one k1 block (kix.n2 % BK == 0) and whose rows are
+  // all valid: one offset computation per tile, then constant strides.
+  __device__ __forceinline__ void load_fast(const T* __restrict__ base, const Idx2& kix, int64_t k0) {
+    const int64_t kb = off2(kix, k0);
+    if (KFAST) {
+      const int64_t ko = kb + (int64_t)kl * kix.s2;
+#pragma unroll
+      for (int i = 0; i < NE; ++i) v[i] = base[roff[i] + ko];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) v[i] = base[roff[0] + kb + (int64_t)(kl + i * (NT / R)) * kix.s2];
+    }
+  }
+  // Guarded load for edge tiles / k-tiles that straddle a k1 block.
+  __device__ __forceinline__ void load_safe(const T* __restrict__ base, const Idx2& kix, int64_t k0, int64_t kend) {
+    if (KFAST) {
+      int64_t k = k0 + kl;
+      bool kok = k < kend;
+      int64_t ko = kok ? off2(kix, k) : 0;
+#pragma unroll
+      for (int i = 0; i < NE; ++i) v[i] = (kok && rok[i]) ? base[roff[i] + ko] : T(0);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) {
+        int64_t k = k0 + kl + i * (NT / R);
+        v[i] = (rok[0] && k < kend) ? base[roff[0] + off2(kix, k)] : T(0);
+      }
+    }
+  }
+  __device__ __forceinline__ void store(LdsTile<T, R, BK, KFAST>& t) {
+    if (KFAST) {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) t.at(rl + i * (NT / BK), kl) = v[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NE; ++i) t.at(rl, kl + i * (NT / R)) = v[i];
+    }
+  }
+};
+
 template <typename T, int BM, int BN, int BK, bool A_KFAST, bool B_KFAST>
-__global__ __launch_bounds__(256) void tc_gemm_kernel(GemmParams<T> p, T* __restrict__ work,
-                                                      int splits, int64_t kchunk) {
-  constexpr int NT = 256;
+__global__ __launch_bounds__(256, 2) void tc_gemm_kernel(GemmParams<T> p, T* __restrict__ work,
+                                                         int splits, int64_t kchunk) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int NBI = WM / 16, NBJ = WN / 16;
-  constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
-  constexpr int NA = BM * BK / NT, NB = BN * BK / NT;
-  static_assert(NA >= 1 && NB >= 1, "tile too small for 256 threads");
-  static_assert(NT % BK == 0 && NT % BM == 0 && NT % BN == 0, "tile shape");
   typedef typename AccT<T>::type acc_t;
 
-  __shared__ T As[2][BK][BM + PAD];
-  __shared__ T Bs[2][BK][BN + PAD];
+  __shared__ LdsTile<T, BM, BK, A_KFAST> As[2];
+  __shared__ LdsTile<T, BN, BK, B_KFAST> Bs[2];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -86,93 +159,13 @@ __global__ __launch_bounds__(256) void tc_gemm_kernel(GemmParams<T> p, T* __rest
   const T* __restrict__ Ab = p.A + bidx * p.a_b;
   const T* __restrict__ Bb = p.B + bidx * p.b_b;
 
-  // ---- per-thread load geometry ----
-  // A tile: BM x BK. K-fast: thread owns one k column and NA rows; M-fast: one row, NA k's.
-  int a_kl, a_ml;            // local coordinates of the first element
-  int64_t a_moff[A_KFAST ? NA : 1];
-  bool a_mok[A_KFAST ? NA : 1];
-  if (A_KFAST) {
-    a_kl = tid % BK; a_ml = tid / BK;
-#pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      int64_t m = tm * BM + a_ml + i * (NT / BK);
-      a_mok[i] = m < p.M;
-      a_moff[i] = a_mok[i] ? off2(p.am, m) : 0;
-    }
-  } else {
-    a_ml = tid % BM; a_kl = tid / BM;
-    int64_t m = tm * BM + a_ml;
-    a_mok[0] = m < p.M;
-    a_moff[0] = a_mok[0] ? off2(p.am, m) : 0;
-  }
-  int b_kl, b_nl;
-  int64_t b_noff[B_KFAST ? NB : 1];
-  bool b_nok[B_KFAST ? NB : 1];
-  if (B_KFAST) {
-    b_kl = tid % BK; b_nl = tid / BK;
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      int64_t n = tn * BN + b_nl + i * (NT / BK);
-      b_nok[i] = n < p.N;
-      b_noff[i] = b_nok[i] ? off2(p.bn, n) : 0;
-    }
-  } else {
-    b_nl = tid % BN; b_kl = tid / BN;
-    int64_t n = tn * BN + b_nl;
-    b_nok[0] = n < p.N;
-    b_noff[0] = b_nok[0] ? off2(p.bn, n) : 0;
-  }
-
-  T ra[NA], rb[NB];
-
-  auto load_a = [&](int64_t k0) {
-    if (A_KFAST) {
-      int64_t k = k0 + a_kl;
-      bool kok = k < k_end;
-      int64_t ko = kok ? off2(p.ak, k) : 0;
-#pragma unroll
-      for (int i = 0; i < NA; ++i) ra[i] = (kok && a_mok[i]) ? Ab[a_moff[i] + ko] : T(0);
-    } else {
-#pragma unroll
-      for (int i = 0; i < NA; ++i) {
-        int64_t k = k0 + a_kl + i * (NT / BM);
-        bool ok = a_mok[0] && k < k_end;
-        ra[i] = ok ? Ab[a_moff[0] + off2(p.ak, k)] : T(0);
-      }
-    }
-  };
-  auto load_b = [&](int64_t k0) {
-    if (B_KFAST) {
-      int64_t k = k0 + b_kl;
-      bool kok = k < k_end;
-      int64_t ko = kok ? off2(p.bk, k) : 0;
-#pragma unroll
-      for (int i = 0; i < NB; ++i) rb[i] = (kok && b_nok[i]) ? Bb[b_noff[i] + ko] : T(0);
-    } else {
-#pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        int64_t k = k0 + b_kl + i * (NT / BN);
-        bool ok = b_nok[0] && k < k_end;
-        rb[i] = ok ? Bb[b_noff[0] + off2(p.bk, k)] : T(0);
-      }
-    }
-  };
-  auto store = [&](int buf) {
-    if (A_KFAST) {
-#pragma unroll
-      for (int i = 0; i < NA; ++i) As[buf][a_kl][a_ml + i * (NT / BK)] = ra[i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < NA; ++i) As[buf][a_kl + i * (NT / BM)][a_ml] = ra[i];
-    }
-    if (B_KFAST) {
-#pragma unroll
-      for (int i = 0; i < NB; ++i) Bs[buf][b_kl][b_nl + i * (NT / BK)] = rb[i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < NB; ++i) Bs[buf][b_kl + i * (NT / BN)][b_nl] = rb[i];
-    }
-  };
+  TileLoader<T, BM, BK, A_KFAST> la;
+  TileLoader<T, BN, BK, B_KFAST> lb;
+  la.init(p.am, tm * BM, p.M, tid);
+  lb.init(p.bn, tn * BN, p.N, tid);
+  // interior tile whose k-tiles never straddle a k1 block: unguarded loads
+  const bool fast = (tm + 1) * BM <= p.M && (tn + 1) * BN <= p.N && (p.ak.n2 % BK == 0) &&
+                    (k_begin % BK == 0);
 
   acc_t acc[NBI][NBJ];
 #pragma unroll
@@ -185,28 +178,41 @@ __global__ __launch_bounds__(256) void tc_gemm_kernel(GemmParams<T> p, T* __rest
   // output row of register r within a 16x16 block (see the layout note at the top)
   auto orow = [&](int r) { return sizeof(T) == 8 ? 4 * r + fr : 4 * fr + r; };
 
+  auto load = [&](int64_t k0) {
+    if (fast && k0 + BK <= k_end) {
+      la.load_fast(Ab, p.ak, k0);
+      lb.load_fast(Bb, p.bk, k0);
+    } else {
+      la.load_safe(Ab, p.ak, k0, k_end);
+      lb.load_safe(Bb, p.bk, k0, k_end);
+    }
+  };
+
   if (k_begin < k_end) {
-    load_a(k_begin);
-    load_b(k_begin);
-    store(0);
+    load(k_begin);
+    la.store(As[0]);
+    lb.store(Bs[0]);
     __syncthreads();
     int buf = 0;
     for (int64_t k0 = k_begin; k0 < k_end; k0 += BK) {
       const bool has_next = k0 + BK < k_end;
-      if (has_next) { load_a(k0 + BK); load_b(k0 + BK); }
+      if (has_next) load(k0 + BK);
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
         T a[NBI], b[NBJ];
 #pragma unroll
-        for (int i = 0; i < NBI; ++i) a[i] = As[buf][ks + fr][wm * WM + i * 16 + fc];
+        for (int i = 0; i < NBI; ++i) a[i] = As[buf].at(wm * WM + i * 16 + fc, ks + fr);
 #pragma unroll
-        for (int j = 0; j < NBJ; ++j) b[j] = Bs[buf][ks + fr][wn * WN + j * 16 + fc];
+        for (int j = 0; j < NBJ; ++j) b[j] = Bs[buf].at(wn * WN + j * 16 + fc, ks + fr);
 #pragma unroll
         for (int i = 0; i < NBI; ++i)
 #pragma unroll
           for (int j = 0; j < NBJ; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
       }
-      if (has_next) store(buf ^ 1);
+      if (has_next) {
+        la.store(As[buf ^ 1]);
+        lb.store(Bs[buf ^ 1]);
+      }
       __syncthreads();
       buf ^= 1;
     }
@@ -281,9 +287,58 @@ static void launch_tiles(const GemmParams<T>& p, bool akf, bool bkf, T* work, in
   CCX_HIP(hipGetLastError());
 }
 
+// Single stride of a 2-level index over `extent` values, or -1 when it is not a plain stride.
+static int64_t plain_stride(const Idx2& d, int64_t extent) {
+  if (extent <= 1) return 0;
+  if (extent <= d.n2) return d.s2;
+  if (d.s1 == d.n2 * d.s2) return d.s2;
+  return -1;
+}
+
+// Express the contraction as a plain strided GEMM when every operand is an ordinary matrix.
+template <typename T>
+static bool as_plain(const GemmParams<T>& p, PlainGemm<T>& q) {
+  const int64_t am = plain_stride(p.am, p.M), ak = plain_stride(p.ak, p.K);
+  const int64_t bk = plain_stride(p.bk, p.K), bn = plain_stride(p.bn, p.N);
+  const int64_t cm = plain_stride(p.cm, p.M), cn = plain_stride(p.cn, p.N);
+  if (am < 0 || ak < 0 || bk < 0 || bn < 0 || cm < 0 || cn < 0 || p.K < 1) return false;
+  q = PlainGemm<T>{};
+  q.batch = p.batch;
+  q.alpha = p.alpha;
+  q.beta = p.beta;
+  if (cn == 1 || p.N == 1) {          // C row-major
+    q.M = p.M; q.N = p.N; q.K = p.K;
+    q.A = p.A; q.B = p.B; q.C = p.C;
+    q.a_b = p.a_b; q.b_b = p.b_b; q.c_b = p.c_b;
+    q.ldc = cm;
+    if (ak == 1) { q.a_kc = true; q.lda = am; } else if (am == 1) { q.a_kc = false; q.lda = ak; } else return false;
+    if (bn == 1) { q.b_nc = true; q.ldb = bk; } else if (bk == 1) { q.b_nc = false; q.ldb = bn; } else return false;
+  } else if (cm == 1) {               // C column-major: C^T = B^T A^T
+    q.M = p.N; q.N = p.M; q.K = p.K;
+    q.A = p.B; q.B = p.A; q.C = p.C;
+    q.a_b = p.b_b; q.b_b = p.a_b; q.c_b = p.c_b;
+    q.ldc = cn;
+    if (bk == 1) { q.a_kc = true; q.lda = bn; } else if (bn == 1) { q.a_kc = false; q.lda = bk; } else return false;
+    if (am == 1) { q.b_nc = true; q.ldb = ak; } else if (ak == 1) { q.b_nc = false; q.ldb = am; } else return false;
+  } else {
+    return false;
+  }
+  if (q.lda < 1 || q.ldb < 1 || q.ldc < 1) return false;
+  return plain_gemm_ok(q);
+}
+
 template <typename T>
 void tc_gemm(const GemmParams<T>& p, Workspace& ws, hipStream_t s) {
   if (p.M == 0 || p.N == 0 || p.batch == 0) return;
+  {
+    // enough 128 x 128 tiles to fill the 256 CUs: the plain-matrix kernel
+    PlainGemm<T> q;
+    if (ceil_div(p.M, 128) * ceil_div(p.N, 128) * p.batch >= 240 && as_plain(p, q)) {
+      if (std::getenv("CCX_TRACE")) std::fprintf(stderr, "[ccx]   -> plain %s%s batch %lld\n", q.a_kc ? "N" : "T", q.b_nc ? "N" : "T", (long long)q.batch);
+      plain_gemm<T>(q, s);
+      return;
+    }
+  }
   CCX_CHECK(p.M < (1ll << 31) && p.N < (1ll << 31) && p.K < (1ll << 31), ERR_ARG, "gemm extent too large");
   CCX_CHECK(p.batch < 65536, ERR_ARG, "gemm batch too large");
   // A is "k-fast" when its inner k index is contiguous (coalesced loads along k); otherwise the
@@ -317,6 +372,9 @@ void tc_gemm(const GemmParams<T>& p, Workspace& ws, hipStream_t s) {
     else kchunk = p.K;
   }
   if (p.K == 0) kchunk = 0;
+  if (std::getenv("CCX_TRACE"))
+    std::fprintf(stderr, "[ccx]   -> generic %s tile%s%s batch %lld splits %d\n", small ? "64" : "128", akf ? " Ak" : " Am",
+                 bkf ? " Bk" : " Bn", (long long)p.batch, splits);
   if (small) launch_tiles<T, 64, 64, 16>(p, akf, bkf, work, splits, kchunk, s);
   else launch_tiles<T, 128, 128, 16>(p, akf, bkf, work, splits, kchunk, s);
   if (splits > 1) {

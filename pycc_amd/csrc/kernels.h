@@ -24,7 +24,7 @@ void build_tau(int64_t o, int64_t v, double f1, double f2, const double* t1, con
 // Ring-layout operands ([i][a][j][b], i.e. ov x ov matrices), see ring_prep_kernel.
 template <typename T>
 void ring_prep(int64_t o, int64_t v, const double* t1, const double* t2, T* S1, T* A1, T* S2, T* Tx,
-               hipStream_t s);
+               double* Ud, hipStream_t s);
 
 // Packed ladder amplitudes: Tp[(i<=j)][(e<=f)], Tm[(i<j)][(e<f)] from tau = t2 + t1 t1.
 template <typename T>

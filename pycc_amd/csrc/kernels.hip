@@ -95,16 +95,19 @@ void build_tau(int64_t o, int64_t v, double f1, double f2, const double* t1, con
 }
 
 // ---------------------------------------------------------------------------------------------
-// For every (i, a, j, b), with T = t2_ijab, X = t2_ijba, P = t_ja t_ib, writes at [i][a][j][b]:
-//   S1[0] = -T/2          S1[1] = X/2 + P       (B operands of the J GEMM -> Wej, Wje)
-//   A1    = T - X/2 - P                          (B operand of the K GEMM -> Wej)
-//   S2[0] = 2T - X        S2[1] = T             (A operands of the ring GEMM)
-//   Tx    = X                                    (A operand of the crossed ring GEMM)
+// For every (i, a, j, b), with T = t2_ijab, X = t2_ijba, P = t_ja t_ib and the compound indices
+// (ia), (jb) of ov x ov matrices, writes
+//   S1[(ia)][0][(jb)] = -T/2      S1[(ia)][1][(jb)] = X/2 + P   (B of the J GEMM -> Wej | Wje)
+//   A1[(ia)][(jb)]    = T - X/2 - P                            (B of the K GEMM -> Wej)
+//   S2[(ia)][(jb)][0] = 2T - X    S2[(ia)][(jb)][1] = T         (A of the ring GEMM, K = (me, x))
+//   Tx[(ia)][(jb)]    = X                                      (A of the crossed ring GEMM)
+//   Ud[(ia)][(jb)]    = 2T - X  in FP64                        (r_T1 terms with 2 t2 - t2^T)
+// The interleavings make every ring GEMM a plain row-major matrix product.
 template <typename T>
 __global__ __launch_bounds__(256) void ring_prep_kernel(int64_t o, int64_t v, const double* __restrict__ t1,
                                                         const double* __restrict__ t2, T* __restrict__ S1,
                                                         T* __restrict__ A1, T* __restrict__ S2,
-                                                        T* __restrict__ Tx) {
+                                                        T* __restrict__ Tx, double* __restrict__ Ud) {
   __shared__ double tn[32][33];
   __shared__ double tx[32][33];
   const int64_t ij = blockIdx.y;
@@ -120,29 +123,31 @@ __global__ __launch_bounds__(256) void ring_prep_kernel(int64_t o, int64_t v, co
     tx[y][x] = (bb < v && aa < v) ? blk[bb * v + aa] : 0.0;
   }
   __syncthreads();
-  const int64_t n2 = o * v * o * v;
+  const int64_t ov = o * v;
   for (int y = y0; y < 32; y += 8) {
     int64_t a = a0 + y, b = b0 + x;
     if (a >= v || b >= v) continue;
     double tv = tn[y][x];
     double xv = tx[x][y];
     double pv = t1[j * v + a] * t1[i * v + b];
-    int64_t out = ((i * v + a) * o + j) * v + b;
-    S1[out] = (T)(-0.5 * tv);
-    S1[n2 + out] = (T)(0.5 * xv + pv);
+    const int64_t ia = i * v + a, jb = j * v + b;
+    const int64_t out = ia * ov + jb;
+    S1[2 * ia * ov + jb] = (T)(-0.5 * tv);
+    S1[2 * ia * ov + ov + jb] = (T)(0.5 * xv + pv);
     A1[out] = (T)(tv - 0.5 * xv - pv);
-    S2[out] = (T)(2.0 * tv - xv);
-    S2[n2 + out] = (T)tv;
+    S2[2 * out] = (T)(2.0 * tv - xv);
+    S2[2 * out + 1] = (T)tv;
     Tx[out] = (T)xv;
+    Ud[out] = 2.0 * tv - xv;
   }
 }
 
 template <typename T>
 void ring_prep(int64_t o, int64_t v, const double* t1, const double* t2, T* S1, T* A1, T* S2, T* Tx,
-               hipStream_t s) {
+               double* Ud, hipStream_t s) {
   int64_t tiles = ceil_div(v, 32);
   dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * o));
-  ring_prep_kernel<T><<<grid, 256, 0, s>>>(o, v, t1, t2, S1, A1, S2, Tx);
+  ring_prep_kernel<T><<<grid, 256, 0, s>>>(o, v, t1, t2, S1, A1, S2, Tx, Ud);
   CCX_HIP(hipGetLastError());
 }
 
@@ -528,9 +533,9 @@ void cast(int64_t n, const Tin* in, Tout* out, hipStream_t s) {
 
 // explicit instantiations
 template void ring_prep<double>(int64_t, int64_t, const double*, const double*, double*, double*, double*,
-                                double*, hipStream_t);
+                                double*, double*, hipStream_t);
 template void ring_prep<float>(int64_t, int64_t, const double*, const double*, float*, float*, float*,
-                               float*, hipStream_t);
+                               float*, double*, hipStream_t);
 template void tau_pack<double>(int64_t, int64_t, const double*, const double*, double*, double*, hipStream_t);
 template void tau_pack<float>(int64_t, int64_t, const double*, const double*, float*, float*, hipStream_t);
 template void ladder_pack<double>(int64_t, int64_t, int64_t, int64_t, VVVVSrc, const double*, double*,

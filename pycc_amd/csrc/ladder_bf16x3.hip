@@ -20,11 +20,6 @@ namespace {
 
 constexpr int BM = 128, BN = 128, BK = 32, KPAD = 8;   // LDS row = 40 bf16 = 80 bytes
 
-union V16 {
-  uint4 u;
-  bf16x8 v;
-};
-
 __device__ __forceinline__ int64_t xcd_remap_l(int64_t w, int64_t ntiles) {
   const int64_t nx = 8;
   int64_t q = ntiles / nx, r = ntiles % nx;
@@ -36,8 +31,8 @@ __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(int64_t M, int64_t N, 
                                                          const bf16* __restrict__ Ah, const bf16* __restrict__ Al,
                                                          const bf16* __restrict__ Bh, const bf16* __restrict__ Bl,
                                                          float* __restrict__ C, int64_t ldc) {
-  __shared__ __attribute__((aligned(16))) bf16 sA[2][2][BM][BK + KPAD];   // [buf][hi/lo]
-  __shared__ __attribute__((aligned(16))) bf16 sB[2][2][BN][BK + KPAD];
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][2][BM][BK + KPAD];   // [buf][hi/lo]
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][2][BN][BK + KPAD];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -48,29 +43,42 @@ __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(int64_t M, int64_t N, 
 
   // each operand tile (128 rows x 32 k) = 512 vectors of 16 B; thread loads rows r and r + 64
   const int lr = tid >> 2, lk = (tid & 3) * 8;
-  uint4 ra[2][2], rb[2][2];   // [row half][hi/lo]
+  const int64_t mA0 = tm * BM + lr, mA1 = mA0 + 64, nB0 = tn * BN + lr, nB1 = nB0 + 64;
+  const bool okA0 = mA0 < M, okA1 = mA1 < M, okB0 = nB0 < N, okB1 = nB1 < N;
+  // rows beyond the matrix read row 0 (valid memory) and are zeroed before the LDS store
+  const bf16* pAh0 = Ah + (okA0 ? mA0 : 0) * Kp + lk;
+  const bf16* pAh1 = Ah + (okA1 ? mA1 : 0) * Kp + lk;
+  const bf16* pAl0 = Al + (okA0 ? mA0 : 0) * Kp + lk;
+  const bf16* pAl1 = Al + (okA1 ? mA1 : 0) * Kp + lk;
+  const bf16* pBh0 = Bh + (okB0 ? nB0 : 0) * Kp + lk;
+  const bf16* pBh1 = Bh + (okB1 ? nB1 : 0) * Kp + lk;
+  const bf16* pBl0 = Bl + (okB0 ? nB0 : 0) * Kp + lk;
+  const bf16* pBl1 = Bl + (okB1 ? nB1 : 0) * Kp + lk;
+  const bf16x8 zero8 = bf16x8{};
+  bf16x8 rAh0, rAh1, rAl0, rAl1, rBh0, rBh1, rBl0, rBl1;
 
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      int64_t m = tm * BM + lr + 64 * h;
-      int64_t n = tn * BN + lr + 64 * h;
-      uint4 z = make_uint4(0, 0, 0, 0);
-      ra[h][0] = m < M ? *(const uint4*)(Ah + m * Kp + k0 + lk) : z;
-      ra[h][1] = m < M ? *(const uint4*)(Al + m * Kp + k0 + lk) : z;
-      rb[h][0] = n < N ? *(const uint4*)(Bh + n * Kp + k0 + lk) : z;
-      rb[h][1] = n < N ? *(const uint4*)(Bl + n * Kp + k0 + lk) : z;
-    }
-  };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        *(uint4*)&sA[buf][p][lr + 64 * h][lk] = ra[h][p];
-        *(uint4*)&sB[buf][p][lr + 64 * h][lk] = rb[h][p];
-      }
-  };
+#define CCX_GLOAD(k0)                                        \
+  do {                                                       \
+    rAh0 = *(const bf16x8*)(pAh0 + (k0));                    \
+    rAh1 = *(const bf16x8*)(pAh1 + (k0));                    \
+    rAl0 = *(const bf16x8*)(pAl0 + (k0));                    \
+    rAl1 = *(const bf16x8*)(pAl1 + (k0));                    \
+    rBh0 = *(const bf16x8*)(pBh0 + (k0));                    \
+    rBh1 = *(const bf16x8*)(pBh1 + (k0));                    \
+    rBl0 = *(const bf16x8*)(pBl0 + (k0));                    \
+    rBl1 = *(const bf16x8*)(pBl1 + (k0));                    \
+  } while (0)
+#define CCX_SSTORE(b)                                                    \
+  do {                                                                   \
+    *(bf16x8*)&sA[b][0][lr][lk] = okA0 ? rAh0 : zero8;                   \
+    *(bf16x8*)&sA[b][0][lr + 64][lk] = okA1 ? rAh1 : zero8;              \
+    *(bf16x8*)&sA[b][1][lr][lk] = okA0 ? rAl0 : zero8;                   \
+    *(bf16x8*)&sA[b][1][lr + 64][lk] = okA1 ? rAl1 : zero8;              \
+    *(bf16x8*)&sB[b][0][lr][lk] = okB0 ? rBh0 : zero8;                   \
+    *(bf16x8*)&sB[b][0][lr + 64][lk] = okB1 ? rBh1 : zero8;              \
+    *(bf16x8*)&sB[b][1][lr][lk] = okB0 ? rBl0 : zero8;                   \
+    *(bf16x8*)&sB[b][1][lr + 64][lk] = okB1 ? rBl1 : zero8;              \
+  } while (0)
 
   f4 acc[4][4];
 #pragma unroll
@@ -79,33 +87,43 @@ __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(int64_t M, int64_t N, 
     for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
   const int fr = lane >> 4, fc = lane & 15;
-  gload(0);
-  sstore(0);
+  CCX_GLOAD(0);
+  CCX_SSTORE(0);
   __syncthreads();
   int buf = 0;
   for (int64_t k0 = 0; k0 < Kp; k0 += BK) {
     const bool has_next = k0 + BK < Kp;
-    if (has_next) gload(k0 + BK);
-    V16 ah[4], al[4], bh[4], bl[4];
+    if (has_next) CCX_GLOAD(k0 + BK);
+    bf16x8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      ah[i].u = *(const uint4*)&sA[buf][0][wm * 64 + i * 16 + fc][fr * 8];
-      al[i].u = *(const uint4*)&sA[buf][1][wm * 64 + i * 16 + fc][fr * 8];
-      bh[i].u = *(const uint4*)&sB[buf][0][wn * 64 + i * 16 + fc][fr * 8];
-      bl[i].u = *(const uint4*)&sB[buf][1][wn * 64 + i * 16 + fc][fr * 8];
+      ah[i] = *(const bf16x8*)&sA[buf][0][wm * 64 + i * 16 + fc][fr * 8];
+      al[i] = *(const bf16x8*)&sA[buf][1][wm * 64 + i * 16 + fc][fr * 8];
+      bh[i] = *(const bf16x8*)&sB[buf][0][wn * 64 + i * 16 + fc][fr * 8];
+      bl[i] = *(const bf16x8*)&sB[buf][1][wn * 64 + i * 16 + fc][fr * 8];
     }
+    // the small products first; 16 independent accumulators between dependent MFMAs
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i].v, bh[j].v, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i].v, bl[j].v, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i].v, bh[j].v, acc[i][j], 0, 0, 0);
-      }
-    if (has_next) sstore(buf ^ 1);
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    if (has_next) CCX_SSTORE(buf ^ 1);
     __syncthreads();
     buf ^= 1;
   }
+#undef CCX_GLOAD
+#undef CCX_SSTORE
 
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -143,8 +161,15 @@ void ladder_bf16x3(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
   const int64_t KpP = bf16_kpad(npP), KpM = bf16_kpad(npM);
   tau_pack_bf16x3(o, v, KpP, KpM, t1, t2, c.Tph, c.Tpl, c.Tmh, c.Tml, s);
+  // algorithmic flops counted as one GEMM (the three split products are the method's cost)
+  int k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noP * (npP + ov) * npP);
   bf16x3_gemm(noP, npP + ov, KpP, c.Tph, c.Tpl, c.LPh, c.LPl, (float*)c.CP, npP + ov, s);
-  if (noM > 0) bf16x3_gemm(noM, npM + ov, KpM, c.Tmh, c.Tml, c.LMh, c.LMl, (float*)c.CM, npM + ov, s);
+  c.tend(s, k);
+  if (noM > 0) {
+    k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noM * (npM + ov) * npM);
+    bf16x3_gemm(noM, npM + ov, KpM, c.Tmh, c.Tml, c.LMh, c.LMl, (float*)c.CM, npM + ov, s);
+    c.tend(s, k);
+  }
 }
 
 }  // namespace ccx
