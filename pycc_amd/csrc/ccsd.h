@@ -24,6 +24,7 @@ struct Ctx {
   double* Cooov = nullptr;           // (mi|ne)
   double* Coovv = nullptr;           // (mn|ef)
   double* Covov = nullptr;           // (me|nf)  == K[(me),(nf)]
+  double* Ioovv = nullptr;           // <mn|ef> = (me|nf) in Dirac order [m][n][e][f]
   double* Covvv = nullptr;           // (me|af)
   double* Loovv = nullptr;           // L_ijab = 2<ij|ab> - <ij|ba>, [i][j][a][b]
   double* Looov = nullptr;           // Lp[i][m][n][e] = 2(mi|ne) - (ni|me) = L_mnie = L_nmei

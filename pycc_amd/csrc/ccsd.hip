@@ -61,6 +61,7 @@ static void alloc_integrals(Ctx& c) {
   c.Cooov = c.alloc<double>(o * o * o * v);
   c.Coovv = c.alloc<double>(o * o * v * v);
   c.Covov = c.alloc<double>(ov * ov);
+  c.Ioovv = c.alloc<double>(ov * ov);
   c.Covvv = c.alloc<double>(ov * v * v);
   c.Loovv = c.alloc<double>(ov * ov);
   c.Looov = c.alloc<double>(o * o * o * v);
@@ -103,6 +104,8 @@ static void finish_integrals(Ctx& c, const double* fock, hipStream_t s) {
   const int64_t o = c.o, v = c.v, ov = c.ov(), nmo = c.nmo;
   // eps = diag(F) (hamiltonian.py:63), copied with a strided view
   axpby<double>(1.0, tvs(fock, "p", {nmo}, {nmo + 1}), 0.0, tv(c.eps, "p", {nmo}), s);
+  // <ij|ab> = (ia|jb) in Dirac order
+  axpby<double>(1.0, tv(c.Covov, "iajb", {o, v, o, v}), 0.0, tv(c.Ioovv, "ijab", {o, o, v, v}), s);
   // L_ijab = 2<ij|ab> - <ij|ba> = 2(ia|jb) - (ib|ja)   (hamiltonian.py:70)
   axpby<double>(2.0, tv(c.Covov, "iajb", {o, v, o, v}), 0.0, tv(c.Loovv, "ijab", {o, o, v, v}), s);
   axpby<double>(-1.0, tv(c.Covov, "ibja", {o, v, o, v}), 1.0, tv(c.Loovv, "ijab", {o, o, v, v}), s);
@@ -256,7 +259,7 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   axpby<double>(1.0, tv(c.Coooo, "minj", OOOO), 0.0, Wmnij, s);   // <mn|ij> = (mi|nj)
   cd(1.0, T1("je"), tv(c.Cooov, "mine", OOOV), 1.0, Wmnij);        // <mn|ie> = (mi|ne)
   cd(1.0, T1("ie"), tv(c.Cooov, "njme", OOOV), 1.0, Wmnij);        // <mn|ej> = (nj|me)
-  cd(1.0, tv(c.tau, "ijef", OOVV), tv(c.Covov, "menf", OVOV), 1.0, Wmnij);  // <mn|ef> = (me|nf)
+  cd(1.0, tv(c.tau, "ijef", OOVV), tv(c.Ioovv, "mnef", OOVV), 1.0, Wmnij);  // <mn|ef>
 
   // ---- ring operands and the W intermediates: ccwfn.py:641-645 (Wmbej), :680-683 (Wmbje) ----
   // Wd[0] = Wmbej and Wd[1] = Wmbje, both stored [m][e][j][b].
@@ -323,7 +326,7 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // ---- Zt_mbij = Z_mbij + <mb|ij> + t_ie <mb|ej> + t_je <mb|ie> -----------------------------------
   // :932 -t_ma Z_mbij, :940 -t_ma <mb|ij>, :937 -t_ie t_ma <mb|ej>, and :938 -t_ie t_mb <ma|je>
   // taken in its (ij)(ab)-swapped form -t_ma (t_je <mb|ie>) (same symmetrised r2, :790).
-  const TV Zt = tv(c.Zt, "mbij", {o, v, o, o});
+  const TV Zt = tv(c.Zt, "mijb", {o, o, o, v});                     // Z-like terms, [m][i][j][b]
   axpby<double>(1.0, tv(c.Cooov, "mijb", OOOV), 0.0, Zt, s);        // <mb|ij> = (mi|jb)
   cd(1.0, T1("ie"), tv(c.Covov, "mejb", OVOV), 1.0, Zt);            // <mb|ej> = (me|jb)
   cd(1.0, T1("je"), tv(c.Coovv, "mibe", OOVV), 1.0, Zt);            // <mb|ie> = (mi|be)
@@ -332,7 +335,7 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
 
   // ---- half residual Hh (before P(ij,ab)): ccwfn.py:922-940 ------------------------------------
   const TV Hh = tv(c.Hh, "ijab", OOVV);
-  axpby<double>(0.5, tv(c.Covov, "iajb", OVOV), 0.0, Hh, s);        // 1/2 <ab|ij> = 1/2 (ia|jb) :922
+  axpby<double>(0.5, tv(c.Ioovv, "ijab", OOVV), 0.0, Hh, s);        // 1/2 <ab|ij> = 1/2 <ij|ab> :922
   // :923-925  t2_ijae (Fae_be - 1/2 t_mb Fme_me)
   const TV G = tv(c.G, "be", VV);
   axpby<double>(1.0, tv(c.Fae, "be", VV), 0.0, G, s);

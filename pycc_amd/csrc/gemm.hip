@@ -351,10 +351,11 @@ void tc_gemm(const GemmParams<T>& p, Workspace& ws, hipStream_t s) {
   CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "gemm grid too large");
   // split K when the tile grid cannot fill the 256 CUs and K is long
   int splits = 1;
-  if (p.K > 0 && tiles < 512) {
-    int64_t want = ceil_div(512, tiles);
+  if (p.K > 0 && tiles < 1024) {
+    // fill ~4 workgroups per CU; every split keeps at least 8 k-tiles
+    int64_t want = ceil_div(1024, tiles);
     int64_t maxs = p.K / (BKt * 8);
-    splits = (int)std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(maxs, 64)));
+    splits = (int)std::max<int64_t>(1, std::min<int64_t>(want, std::min<int64_t>(maxs, 256)));
     splits = (int)std::max<int64_t>(1, std::min<int64_t>(splits, 65535 / p.batch));   // grid.y limit
   }
   int64_t kchunk = p.K;

@@ -51,7 +51,7 @@ void ladder_pack_bf16x3(int64_t o, int64_t v, int64_t e0, int64_t e1, VVVVSrc sr
 void ladder_pack_o_bf16x3(int64_t o, int64_t v, const double* Covvv, bf16* LP_hi, bf16* LP_lo,
                           bf16* LM_hi, bf16* LM_lo, hipStream_t s);
 
-// Zt[m][b][i][j] += Z_mbij from the packed ladder outputs.
+// Zt[m][i][j][b] += Z_mbij from the packed ladder outputs.
 template <typename T>
 void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStream_t s);
 

@@ -338,11 +338,14 @@ __global__ void zt_add_kernel(int64_t o, int64_t v, const T* __restrict__ CP, co
                               double* __restrict__ Zt) {
   const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
   const int64_t ldP = npP + o * v, ldM = npM + o * v;
-  const int64_t total = o * v * o * o;
+  const int64_t total = o * o * o * v;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t j = e % o, t = e / o;
-    int64_t i = t % o, mb = t / o;
+    // Zt layout [m][i][j][b]
+    int64_t b = e % v, t = e / v;
+    int64_t j = t % o; t /= o;
+    int64_t i = t % o, m = t / o;
+    int64_t mb = m * v + b;
     int64_t lo = i < j ? i : j, hi = i < j ? j : i;
     double z = (double)CP[pairP(lo, hi) * ldP + npP + mb];
     if (i != j) {
@@ -360,29 +363,50 @@ void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStrea
 }
 
 // ---------------------------------------------------------------------------------------------
+// One block per (i, j) pair and 32 x 32 tile of (a, b). The three transposed operands
+// (H[j][i][b][a], P[(jb),(ia)], Q[(ib),(ja)]) are read along their contiguous index and transposed
+// through LDS, so every global access is coalesced.
 template <typename T>
-__global__ void assemble_r2_kernel(int64_t o, int64_t v, const double* __restrict__ H,
-                                   const T* __restrict__ CP, const T* __restrict__ CM,
-                                   const T* __restrict__ P, const T* __restrict__ Q,
-                                   double* __restrict__ r2) {
+__global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, const double* __restrict__ H,
+                                                          const T* __restrict__ CP, const T* __restrict__ CM,
+                                                          const T* __restrict__ P, const T* __restrict__ Q,
+                                                          double* __restrict__ r2) {
+  __shared__ double sh[32][33];
+  __shared__ T sp[32][33];
+  __shared__ T sq[32][33];
   const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
-  const int64_t ldP = npP + o * v, ldM = npM + o * v;
-  const int64_t total = o * o * v * v;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t b = e % v, t = e / v;
-    int64_t a = t % v; t /= v;
-    int64_t j = t % o, i = t / o;
-    double r = H[e] + H[((j * o + i) * v + b) * v + a];
-    int64_t il = i < j ? i : j, ih = i < j ? j : i;
-    int64_t al = a < b ? a : b, ah = a < b ? b : a;
-    r += (double)CP[pairP(il, ih) * ldP + pairP(al, ah)];
-    if (i != j && a != b) {
-      double m = (double)CM[pairM(il, ih) * ldM + pairM(al, ah)];
-      r += ((i < j) == (a < b)) ? m : -m;
+  const int64_t ldP = npP + o * v, ldM = npM + o * v, ov = o * v;
+  const int64_t ij = blockIdx.y;
+  const int64_t i = ij / o, j = ij % o;
+  const int64_t tiles_b = (v + 31) / 32;
+  const int64_t a0 = (blockIdx.x / tiles_b) * 32, b0 = (blockIdx.x % tiles_b) * 32;
+  const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+  // transposed operands: element (a = a0 + x, b = b0 + y) read with x (a) fastest
+  for (int y = y0; y < 32; y += 8) {
+    const int64_t a = a0 + x, b = b0 + y;
+    const bool ok = a < v && b < v;
+    sh[y][x] = ok ? H[((j * o + i) * v + b) * v + a] : 0.0;
+    sp[y][x] = ok ? P[(j * v + b) * ov + i * v + a] : T(0);
+    sq[y][x] = ok ? Q[(i * v + b) * ov + j * v + a] : T(0);
+  }
+  __syncthreads();
+  const int64_t il = i < j ? i : j, ih = i < j ? j : i;
+  const T* cpr = CP + pairP(il, ih) * ldP;
+  const T* cmr = (i != j) ? CM + pairM(il, ih) * ldM : nullptr;
+  const bool sij = i < j;
+  for (int y = y0; y < 32; y += 8) {
+    const int64_t a = a0 + y, b = b0 + x;
+    if (a >= v || b >= v) continue;
+    const int64_t e = ((i * o + j) * v + a) * v + b;
+    double r = H[e] + sh[x][y];
+    const int64_t al = a < b ? a : b, ah = a < b ? b : a;
+    r += (double)cpr[pairP(al, ah)];
+    if (cmr && a != b) {
+      double m = (double)cmr[pairM(al, ah)];
+      r += (sij == (a < b)) ? m : -m;
     }
-    r += (double)P[((i * v + a) * o + j) * v + b] + (double)P[((j * v + b) * o + i) * v + a];
-    r += (double)Q[((j * v + a) * o + i) * v + b] + (double)Q[((i * v + b) * o + j) * v + a];
+    r += (double)P[(i * v + a) * ov + j * v + b] + (double)sp[x][y];
+    r += (double)Q[(j * v + a) * ov + i * v + b] + (double)sq[x][y];
     r2[e] = r;
   }
 }
@@ -390,7 +414,9 @@ __global__ void assemble_r2_kernel(int64_t o, int64_t v, const double* __restric
 template <typename T>
 void assemble_r2(int64_t o, int64_t v, const double* H, const T* CP, const T* CM, const T* P,
                  const T* Q, double* r2, hipStream_t s) {
-  assemble_r2_kernel<T><<<grid_for(o * o * v * v), 256, 0, s>>>(o, v, H, CP, CM, P, Q, r2);
+  int64_t tiles = ceil_div(v, 32);
+  dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * o));
+  assemble_r2_kernel<T><<<grid, 256, 0, s>>>(o, v, H, CP, CM, P, Q, r2);
   CCX_HIP(hipGetLastError());
 }
 
