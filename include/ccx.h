@@ -89,6 +89,43 @@ int ccx_diis_reset(ccx_ctx* ctx, const double* t1, const double* t2, int max_dii
 int ccx_diis_add(ccx_ctx* ctx, const double* t1, const double* t2, void* stream);
 int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out, void* stream);
 
+/* ---- backward: HBAR, the Lambda residual and the vector-Jacobian product (FP64, DP contexts) ----
+ *
+ * ccx_hbar_build = cchbar(ccwfn) (pycc/cchbar.py:54-98, spatial CCSD builders :128-823) at the given
+ *   fock, t1, t2 (copied; the inputs may change afterwards). flags = 0 holds the blocks of size
+ *   o^2 v^2 and below and evaluates every Lambda term that needs Hvvvv, Hvovv or Hvvvo in factored
+ *   form from the packed <ab|ef> of the context (no v^4 block is formed); flags = CCX_HBAR_STORED
+ *   also stores Hvvvv (v^4), Hvovv and Hvvvo as the reference does. */
+#define CCX_HBAR_STORED 1
+#define CCX_VJP_REUSE_HBAR 2
+int ccx_hbar_build(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, int flags,
+                   void* stream);
+
+/* Copy HBAR block `block` into `out` (device, reference layout and order of cchbar.py:86-96):
+ *   0 Hov[o][v]  1 Hvv[v][v]  2 Hoo[o][o]  3 Hoooo[o][o][o][o]  4 Hvvvv[v][v][v][v]  5 Hvovv[v][o][v][v]
+ *   6 Hooov[o][o][o][v]  7 Hovvo[o][v][v][o]  8 Hovov[o][v][o][v]  9 Hvvvo[v][v][v][o]  10 Hovoo[o][v][o][o]
+ * Blocks 4, 5 and 9 exist only after a CCX_HBAR_STORED build. */
+int ccx_hbar_block(ccx_ctx* ctx, int block, double* out, void* stream);
+
+/* r1, r2 = the Lambda residuals r_L1, r_L2 at (l1, l2) on the current HBAR
+ * (cclambda.residuals, pycc/cclambda.py:202-256: build_Goo :281, build_Gvv :306, r_L1 :345-369,
+ * r_L2 :468-496). with_source = 0 leaves out the l-independent terms 2 Hov and L. l2 must be
+ * pair-symmetric (l2_ijab = l2_jiba) unless the HBAR is stored. Outputs must not alias inputs. */
+int ccx_lambda_residual(ccx_ctx* ctx, const double* l1, const double* l2, double* r1, double* r2,
+                        int with_source, void* stream);
+
+/* *e_dev = 1/2 <ij|ab> l2_ijab (cclambda.pseudoenergy, pycc/cclambda.py:547-570). */
+int ccx_pseudoenergy(ccx_ctx* ctx, const double* l2, double* e_dev, void* stream);
+
+/* Vector-Jacobian product of ccx_residual at (fock, t1, t2) with cotangent (c1, c2):
+ *   g1 = sum c1 . dr1/dt1 + sum c2 . dr2/dt1,   g2 = the same for t2, pair-symmetrised,
+ * computed as the Lambda residual without source terms at l1 = c1, l2 = c2 + c2^T (the backward of
+ * the residual; cclambda.py:345-369, :468-496 are its contractions). The fock matrix is held
+ * constant. flags: CCX_HBAR_STORED as for ccx_hbar_build; CCX_VJP_REUSE_HBAR skips the HBAR build
+ * and uses the last one (built from the same fock, t1, t2). Outputs must not alias inputs. */
+int ccx_vjp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, const double* c1,
+            const double* c2, double* g1, double* g2, int flags, void* stream);
+
 /* Measurement hooks (no counterpart in the reference, which has only wall-clock prints,
  * pycc/utils.py:229-238): HIP events recorded on the residual's stream around the ladder GEMMs,
  * the ring GEMMs and each whole residual.

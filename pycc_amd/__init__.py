@@ -3,6 +3,9 @@
 Public API (names as exported by ``pycc/__init__.py``):
 
 * ``CCwfn`` / ``ccwfn`` -- amplitude solver: ``residuals``, ``cc_energy``, ``solve_cc``;
+* ``cchbar``, ``cclambda`` -- HBAR and the Lambda equations (``solve_lambda``, ``residuals``,
+  ``pseudoenergy``); ``autograd.residual`` -- the residual as a ``torch.autograd.Function`` whose
+  backward is the Lambda-equation vector-Jacobian product;
 * ``DeviceManager``, ``ContractionBackend`` -- device/precision policy and the GPU einsum;
 * ``PyCCError``, ``PyCCWarning``, ``InvalidKeywordError``, ``DeviceError``.
 
@@ -11,7 +14,9 @@ The arithmetic runs in ``libccx.so`` (HIP, gfx950), declared in ``include/ccx.h`
 from .exceptions import PyCCError, PyCCWarning, InvalidKeywordError, DeviceError
 from .device import DeviceManager, ContractionBackend
 from .ccwfn import CCwfn, ccwfn
+from .cchbar import cchbar
+from .cclambda import cclambda
 from . import synth
 
-__all__ = ["CCwfn", "ccwfn", "DeviceManager", "ContractionBackend", "PyCCError", "PyCCWarning",
+__all__ = ["CCwfn", "ccwfn", "cchbar", "cclambda", "DeviceManager", "ContractionBackend", "PyCCError", "PyCCWarning",
            "InvalidKeywordError", "DeviceError", "synth"]

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libccx.so")
 MODE_DP = 0
 MODE_MIXED = 1
 MODE_MIXED_BF16X3 = 2
+HBAR_STORED = 1
+VJP_REUSE_HBAR = 2
 
 _lock = threading.Lock()
 _lib = None
@@ -37,6 +39,11 @@ SIGNATURES = {
     "ccx_diis_reset": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     "ccx_diis_add": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "ccx_diis_extrapolate": (ctypes.c_int, [_vp, _vp, _vp, c_dp, _vp]),
+    "ccx_hbar_build": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "ccx_hbar_block": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "ccx_lambda_residual": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "ccx_pseudoenergy": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "ccx_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "ccx_timing_start": (ctypes.c_int, [_vp]),
     "ccx_timing_read": (ctypes.c_int, [_vp, c_dp, ctypes.c_int]),
     "ccx_device_bytes": (ctypes.c_int64, [_vp]),

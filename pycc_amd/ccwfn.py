@@ -250,6 +250,76 @@ class CCwfn(object):
                 _lib.call("ccx_diis_extrapolate", self._ctx.h, dptr(self.t1), dptr(self.t2), coef,
                           self._stream())
 
+    # ---- backward: HBAR, Lambda residual, vector-Jacobian product ----------------------------
+    def _require_dp(self, what):
+        if self.precision != 'DP':
+            raise PyCCError("%s runs in FP64: build the CCwfn with precision='DP' (this one is %s)"
+                            % (what, self.precision))
+
+    def _hbar_for(self, F, t1, t2, stored=False, owner=None):
+        """Make the context's HBAR the one at (F, t1, t2), building it unless it already is.
+
+        The context holds one HBAR. It is identified by the tensors it was built from (held here, so
+        their memory cannot be reused) and their version counters, so an in-place change of t1, t2
+        or F forces a rebuild. A stored HBAR also serves factored requests."""
+        self._require_dp("HBAR")
+        key = tuple((x.data_ptr(), x._version, tuple(x.shape)) for x in (F, t1, t2))
+        cur = getattr(self, "_hbar_src", None)
+        if cur is not None and cur["key"] == key and (cur["stored"] or not stored) and cur["owner"] is owner:
+            return
+        self._hbar_src = None
+        _lib.call("ccx_hbar_build", self._ctx.h, dptr(F), dptr(t1), dptr(t2),
+                  _lib.HBAR_STORED if stored else 0, self._stream())
+        self._hbar_src = {"key": key, "stored": bool(stored), "owner": owner, "refs": (F, t1, t2)}
+
+    def _lambda_residual(self, l1, l2, with_source=True):
+        """r_L1, r_L2 (cclambda.py:345-369, :468-496) on the context's current HBAR."""
+        torch = self._torch
+        l1 = self._as_dev(l1, (self.no, self.nv))
+        l2 = self._as_dev(l2, (self.no, self.no, self.nv, self.nv))
+        r1 = torch.empty_like(l1)
+        r2 = torch.empty_like(l2)
+        _lib.call("ccx_lambda_residual", self._ctx.h, dptr(l1), dptr(l2), dptr(r1), dptr(r2),
+                  1 if with_source else 0, self._stream())
+        return r1, r2
+
+    def vjp(self, t1, t2, c1, c2, F=None, store_vvvv=False):
+        """(g1, g2) = the vector-Jacobian product of ``residuals(F, t1, t2)`` with cotangent
+        (c1, c2): g = sum_k c_k . d r_k / dt, with g2 pair-symmetrised (the t2 parameterisation is
+        pair-symmetric). It is the Lambda residual without source terms at l1 = c1, l2 = c2 + c2^T
+        (cclambda.py:345-369, :468-496); F is held constant."""
+        torch = self._torch
+        F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
+        t1 = self._as_dev(t1, (self.no, self.nv))
+        t2 = self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        c1 = self._as_dev(c1, (self.no, self.nv))
+        c2 = self._as_dev(c2, (self.no, self.no, self.nv, self.nv))
+        self._hbar_for(F, t1, t2, store_vvvv)
+        g1 = torch.empty_like(c1)
+        g2 = torch.empty_like(c2)
+        _lib.call("ccx_vjp", self._ctx.h, None, None, None, dptr(c1), dptr(c2), dptr(g1), dptr(g2),
+                  _lib.VJP_REUSE_HBAR, self._stream())
+        return g1, g2
+
+    def energy_gradient(self, t1=None, t2=None, F=None):
+        """(dE/dt1, dE/dt2) of ``cc_energy`` = (2 Hov, L): the source terms of the Lambda equations
+        (cclambda.py:348, :468)."""
+        torch = self._torch
+        F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
+        t1 = self.t1 if t1 is None else self._as_dev(t1, (self.no, self.nv))
+        t2 = self.t2 if t2 is None else self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        self._hbar_for(F, t1, t2)
+        z1 = torch.zeros_like(t1)
+        z2 = torch.zeros_like(t2)
+        r1, r2 = self._lambda_residual(z1, z2, with_source=True)
+        return r1, r2.mul_(0.5)
+
+    def pseudoenergy(self, l2):
+        """1/2 <ij|ab> l2_ijab (cclambda.py:570)."""
+        l2 = self._as_dev(l2, (self.no, self.no, self.nv, self.nv))
+        _lib.call("ccx_pseudoenergy", self._ctx.h, dptr(l2), dptr(self._e_dev[:1]), self._stream())
+        return float(self._e_dev[0].item())
+
     # ---- measurement hooks -----------------------------------------------------------------
     def timing_start(self):
         """Start HIP-event timing of the ladder / ring GEMMs and whole residuals (ccx_timing_*)."""

@@ -169,6 +169,54 @@ int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out,
   });
 }
 
+int ccx_hbar_build(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, int flags,
+                   void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    CCX_CHECK((flags & ~CCX_HBAR_STORED) == 0, ccx::ERR_ARG, "unknown ccx_hbar_build flag");
+    ccx::hbar_build(ctx->c, fock, t1, t2, (flags & CCX_HBAR_STORED) != 0, st(stream));
+  });
+}
+
+int ccx_hbar_block(ccx_ctx* ctx, int block, double* out, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::hbar_block(ctx->c, block, out, st(stream));
+  });
+}
+
+int ccx_lambda_residual(ccx_ctx* ctx, const double* l1, const double* l2, double* r1, double* r2,
+                        int with_source, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::lambda_residual(ctx->c, l1, l2, r1, r2, with_source != 0, 1.0, st(stream));
+  });
+}
+
+int ccx_pseudoenergy(ccx_ctx* ctx, const double* l2, double* e_dev, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::pseudoenergy(ctx->c, l2, e_dev, st(stream));
+  });
+}
+
+int ccx_vjp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, const double* c1,
+            const double* c2, double* g1, double* g2, int flags, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx && c1 && c2 && g1 && g2, ccx::ERR_ARG, "null argument");
+    CCX_CHECK((flags & ~(CCX_HBAR_STORED | CCX_VJP_REUSE_HBAR)) == 0, ccx::ERR_ARG, "unknown ccx_vjp flag");
+    auto& c = ctx->c;
+    hipStream_t s = st(stream);
+    if (flags & CCX_VJP_REUSE_HBAR)
+      CCX_CHECK(c.have_hbar, ccx::ERR_STATE, "CCX_VJP_REUSE_HBAR without a built HBAR");
+    else
+      ccx::hbar_build(c, fock, t1, t2, (flags & CCX_HBAR_STORED) != 0, s);
+    // l1 = c1, l2 = c2 + c2^T (r2 is pair-symmetric, so only the symmetric part of c2 acts)
+    ccx::pair_symmetrize(c.o, c.v, 1.0, c2, c.Lsym, s);
+    ccx::lambda_residual(c, c1, c.Lsym, g1, g2, false, 0.5, s);
+  });
+}
+
 int ccx_timing_start(ccx_ctx* ctx) {
   return guarded([&] {
     CCX_CHECK(ctx, ccx::ERR_ARG, "null context");

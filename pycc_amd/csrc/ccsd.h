@@ -78,6 +78,15 @@ struct Ctx {
     if (k >= 0) CCX_HIP(hipEventRecord(spans[k].b, s));
   }
 
+  // ---- HBAR (cchbar) and Lambda work buffers (allocated on first use, FP64) ----
+  bool have_hbar_bufs = false, have_hbar = false, hbar_stored = false;
+  double *Ht1 = nullptr, *Ht2 = nullptr;               // amplitudes the HBAR was built from
+  double *Hov = nullptr, *Hvv = nullptr, *Hoo = nullptr, *Hoooo = nullptr, *Hooov = nullptr;
+  double *Hovvo = nullptr, *Hovov = nullptr, *Hovoo = nullptr;
+  double *Hvvvv = nullptr, *Hvovv = nullptr, *Hvvvo = nullptr;   // stored HBAR only
+  double *Ltmp = nullptr, *Goo = nullptr, *Gvv = nullptr, *Lh = nullptr, *Lsym = nullptr;
+  double *LY = nullptr, *LVf = nullptr, *LVg = nullptr, *LU = nullptr, *LQ = nullptr, *Lsm = nullptr;
+
   // ---- DIIS (pycc utils.helper_diis) ----
   int max_diis = 0, cap = 0;
   std::vector<int> order;            // active slots, oldest first
@@ -108,6 +117,17 @@ void set_integrals_dense(Ctx& c, const double* fock, const double* eri, hipStrea
 void set_integrals_df(Ctx& c, const double* fock, const double* B, int64_t naux, hipStream_t s);
 void residual(Ctx& c, const double* fock, const double* t1, const double* t2, double* r1, double* r2,
               hipStream_t s);
+void alloc_work(Ctx& c);
+// HBAR at (F, t1, t2): stored = all 11 blocks, else the o^2 v^2-sized ones (hbar.hip).
+void hbar_build(Ctx& c, const double* F, const double* t1, const double* t2, bool stored, hipStream_t s);
+void hbar_block(Ctx& c, int block, double* out, hipStream_t s);
+// r_L1 and r2 = alpha * r_L2 at (l1, l2) on the current HBAR; without the source terms it is the
+// vector-Jacobian product of the CCSD residual.
+void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, double* r2, bool with_source,
+                     double alpha, hipStream_t s);
+// out[i][j][a][b] = alpha * (h[i][j][a][b] + h[j][i][b][a])
+void pair_symmetrize(int64_t o, int64_t v, double alpha, const double* h, double* out, hipStream_t s);
+void pseudoenergy(Ctx& c, const double* l2, double* e_dev, hipStream_t s);
 void timing_start(Ctx& c);
 void timing_read(Ctx& c, double* out, int n);
 void diis_reset(Ctx& c, const double* t1, const double* t2, int max_diis, int cap, hipStream_t s);

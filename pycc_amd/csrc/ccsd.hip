@@ -171,7 +171,7 @@ void set_integrals_df(Ctx& c, const double* fock, const double* B, int64_t naux,
 }
 
 // ---------------------------------------------------------------------------------------------
-static void alloc_work(Ctx& c) {
+void alloc_work(Ctx& c) {
   if (c.have_work) return;
   const int64_t o = c.o, v = c.v, ov = c.ov();
   c.Fae = c.alloc<double>(v * v);

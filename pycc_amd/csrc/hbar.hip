@@ -1,0 +1,464 @@
+// CCSD similarity-transformed Hamiltonian (HBAR) and the Lambda residual on gfx950, FP64.
+//
+// Reference: pycc/cchbar.py (spatial CCSD branches of the block builders, :128-823) and
+// pycc/cclambda.py (build_Goo :281, build_Gvv :306, r_L1 :345-369, r_L2 :468-496,
+// pseudoenergy :570). Every contraction cites its reference line; the Dirac integral of each term
+// is read from a chemist block, <pq|rs> = (pr|qs).
+//
+// Two ways to hold HBAR:
+//   stored    all 11 blocks, as the reference builds them (Hvvvv is v^4, Hvvvo and Hvovv ov^3);
+//   factored  the blocks of size o^2 v^2 and below only. Every Lambda term that reads Hvvvv, Hvvvo
+//             or Hvovv is expanded into the packed ladder GEMM of the forward residual (<ab|ef> in
+//             its symmetric/antisymmetric pair form), o^3 v^3 contractions and smaller ones, so no
+//             v^4 or ov^3 HBAR block is formed and the ladder costs o^2 v^4 / 2 flops per Lambda
+//             residual instead of 2 o^2 v^4. The factored form needs l2 with pair symmetry
+//             (l2_ijab = l2_jiba), which every Lambda amplitude has.
+//
+// The Lambda residual with the source terms (2 Hov in r_L1, L in r_L2) left out is the
+// vector-Jacobian product of the CCSD residual: with the Lagrangian E + l1.r1 + 1/2 l2.r2,
+// r_L1 = dL/dt1 and r_L2 = dL/dt2 + its (ij)(ab) transpose at any t and l, and the source terms
+// are the energy gradient.
+#include "ccsd.h"
+
+namespace ccx {
+
+namespace {
+
+// <ab|ef> for every a, b, e, f from the packed FP64 ladder matrices (rows (e<=f), columns (a<=b)):
+// <ab|ef> = Vp[(ef)][(ab)] + s(a,b) s(e,f) Vm[(ef)][(ab)], s(p,q) = +1 (p < q), -1 (p > q).
+__global__ void unpack_vvvv_kernel(int64_t o, int64_t v, const double* __restrict__ LP,
+                                   const double* __restrict__ LM, double* __restrict__ out) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2, ov = o * v;
+  const int64_t ldP = npP + ov, ldM = npM + ov;
+  const int64_t total = v * v * v * v;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    int64_t f = x % v, t = x / v;
+    int64_t e = t % v;
+    t /= v;
+    int64_t b = t % v, a = t / v;
+    int64_t el = e < f ? e : f, eh = e < f ? f : e, al = a < b ? a : b, ah = a < b ? b : a;
+    double r = LP[pairP(el, eh) * ldP + pairP(al, ah)];
+    if (e != f && a != b) {
+      double m = LM[pairM(el, eh) * ldM + pairM(al, ah)];
+      r += ((e < f) == (a < b)) ? m : -m;
+    }
+    out[x] = r;
+  }
+}
+
+// out[i][j][a][b] = alpha * (h[i][j][a][b] + h[j][i][b][a]), through 32 x 33 LDS tiles.
+__global__ __launch_bounds__(256) void pair_sym_kernel(int64_t o, int64_t v, double alpha,
+                                                       const double* __restrict__ h, double* __restrict__ out) {
+  __shared__ double sh[32][33];
+  const int64_t ij = blockIdx.y, i = ij / o, j = ij % o;
+  const int64_t tiles_b = (v + 31) / 32;
+  const int64_t a0 = (blockIdx.x / tiles_b) * 32, b0 = (blockIdx.x % tiles_b) * 32;
+  const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+  for (int y = y0; y < 32; y += 8) {
+    const int64_t a = a0 + x, b = b0 + y;
+    sh[y][x] = (a < v && b < v) ? h[((j * o + i) * v + b) * v + a] : 0.0;
+  }
+  __syncthreads();
+  for (int y = y0; y < 32; y += 8) {
+    const int64_t a = a0 + y, b = b0 + x;
+    if (a >= v || b >= v) continue;
+    const int64_t e = ((i * o + j) * v + a) * v + b;
+    out[e] = alpha * (h[e] + sh[x][y]);
+  }
+}
+
+// Y[i][j][a][b] = CP[(ij)][(ab)] + s(i,j) s(a,b) CM[(ij)][(ab)]: the full ladder product
+// sum_ef X_ijef <ab|ef> of a pair-symmetric X from its packed GEMM outputs.
+__global__ void ladder_unpack_kernel(int64_t o, int64_t v, const double* __restrict__ CP,
+                                     const double* __restrict__ CM, double* __restrict__ Y) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2, ov = o * v;
+  const int64_t ldP = npP + ov, ldM = npM + ov;
+  const int64_t ij = blockIdx.y, i = ij / o, j = ij % o;
+  const int64_t il = i < j ? i : j, ih = i < j ? j : i;
+  const double* cpr = CP + pairP(il, ih) * ldP;
+  const double* cmr = i != j ? CM + pairM(il, ih) * ldM : nullptr;
+  const bool sij = i < j;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < v * v;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = x / v, b = x % v;
+    const int64_t al = a < b ? a : b, ah = a < b ? b : a;
+    double r = cpr[pairP(al, ah)];
+    if (cmr && a != b) {
+      double m = cmr[pairM(al, ah)];
+      r += (sij == (a < b)) ? m : -m;
+    }
+    Y[ij * v * v + x] = r;
+  }
+}
+
+}  // namespace
+
+void pair_symmetrize(int64_t o, int64_t v, double alpha, const double* h, double* out, hipStream_t s) {
+  CCX_CHECK(o * o < 65536, ERR_ARG, "too many occupied orbitals for the pair grid");
+  int64_t tiles = ceil_div(v, 32);
+  pair_sym_kernel<<<dim3((unsigned)(tiles * tiles), (unsigned)(o * o)), 256, 0, s>>>(o, v, alpha, h, out);
+  CCX_HIP(hipGetLastError());
+}
+
+static void alloc_hbar(Ctx& c, bool stored) {
+  const int64_t o = c.o, v = c.v, ov = o * v;
+  if (!c.have_hbar_bufs) {
+    c.Ht1 = c.alloc<double>(ov);
+    c.Ht2 = c.alloc<double>(ov * ov);
+    c.Hov = c.alloc<double>(ov);
+    c.Hvv = c.alloc<double>(v * v);
+    c.Hoo = c.alloc<double>(o * o);
+    c.Hoooo = c.alloc<double>(o * o * o * o);
+    c.Hooov = c.alloc<double>(o * o * ov);
+    c.Hovvo = c.alloc<double>(ov * ov);
+    c.Hovov = c.alloc<double>(ov * ov);
+    c.Hovoo = c.alloc<double>(o * o * ov);
+    c.Ltmp = c.alloc<double>(ov * ov);
+    c.Goo = c.alloc<double>(o * o);
+    c.Gvv = c.alloc<double>(v * v);
+    c.Lh = c.alloc<double>(ov * ov);
+    c.Lsym = c.alloc<double>(ov * ov);
+    c.LY = c.alloc<double>(ov * ov);
+    c.LVf = c.alloc<double>(o * o * ov);
+    c.LVg = c.alloc<double>(o * o * ov);
+    c.LU = c.alloc<double>(o * o * o * o);
+    c.LQ = c.alloc<double>(o * o * ov);
+    c.Lsm = c.alloc<double>(2 * o * o + 2 * ov);
+    c.have_hbar_bufs = true;
+  }
+  if (stored && !c.Hvvvv) {
+    c.Hvvvv = c.alloc<double>(v * v * v * v);
+    c.Hvovv = c.alloc<double>(ov * v * v);
+    c.Hvvvo = c.alloc<double>(ov * v * v);
+  }
+}
+
+void hbar_build(Ctx& c, const double* F, const double* t1, const double* t2, bool stored, hipStream_t s) {
+  CCX_CHECK(c.have_ints, ERR_STATE, "HBAR requested before the integrals were set");
+  CCX_CHECK(c.mode == MODE_DP, ERR_STATE, "HBAR and the Lambda equations run in a DP context");
+  CCX_CHECK(F && t1 && t2, ERR_ARG, "null argument");
+  CCX_CHECK(c.o * c.o < 65536, ERR_ARG, "too many occupied orbitals");
+  alloc_work(c);
+  alloc_hbar(c, stored);
+  c.have_hbar = false;
+  const int64_t o = c.o, v = c.v, nmo = c.nmo, ov = o * v;
+  Workspace& ws = c.ws;
+  const std::vector<int64_t> OV{o, v}, OO{o, o}, VV{v, v}, OOVV{o, o, v, v}, OVOV{o, v, o, v};
+  const std::vector<int64_t> OOOV{o, o, o, v}, OVVV{o, v, v, v}, OOOO{o, o, o, o}, VVVV{v, v, v, v};
+  const std::vector<int64_t> VOVV{v, o, v, v}, OVVO{o, v, v, o}, VVVO{v, v, v, o}, OVOO{o, v, o, o};
+  auto cd = [&](double a, const TV& A, const TV& B, double b, const TV& C) {
+    contract<double>(a, A, B, b, C, ws, s);
+  };
+  CCX_HIP(hipMemcpyAsync(c.Ht1, t1, ov * sizeof(double), hipMemcpyDeviceToDevice, s));
+  CCX_HIP(hipMemcpyAsync(c.Ht2, t2, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, s));
+  auto T1 = [&](const char* l) { return tv(c.Ht1, l, OV); };
+  auto T2 = [&](const char* l) { return tv(c.Ht2, l, OOVV); };
+  const TV Fov = tvs(F + o, "me", OV, {nmo, 1});
+  const TV Fvv = tvs(F + o * nmo + o, "ae", VV, {nmo, 1});
+  const TV Foo = tvs(F, "mi", OO, {nmo, 1});
+  build_tau(o, v, 1.0, 1.0, c.Ht1, c.Ht2, c.tau, s);      // tau = t2 + t1 t1 (ccwfn.py:455)
+  auto TAU = [&](const char* l) { return tv(c.tau, l, OOVV); };
+  auto CVVV = [&](const char* l) { return tv(c.Covvv, l, OVVV); };   // (me|af) stored [m][e][a][f]
+  auto COOV = [&](const char* l) { return tv(c.Cooov, l, OOOV); };   // (mi|ne) stored [m][i][n][e]
+  auto COVV = [&](const char* l) { return tv(c.Coovv, l, OOVV); };   // (mn|ef) stored [m][n][e][f]
+  auto COVO = [&](const char* l) { return tv(c.Covov, l, OVOV); };   // (me|nf) stored [m][e][n][f]
+  auto IOVV = [&](const char* l) { return tv(c.Ioovv, l, OOVV); };   // <mn|ef> stored [m][n][e][f]
+  auto LOVV = [&](const char* l) { return tv(c.Loovv, l, OOVV); };   // L_mnef
+  auto LOOV = [&](const char* l) { return tv(c.Looov, l, OOOV); };   // L_mnie stored [i][m][n][e]
+
+  // ---- Hov: cchbar.py:152-153 ----
+  const TV Hov = tv(c.Hov, "me", OV);
+  axpby<double>(1.0, Fov, 0.0, Hov, s);
+  cd(1.0, T1("nf"), LOVV("mnef"), 1.0, Hov);
+
+  // ---- Hvv: :206-209 (L_amef = 2(mf|ae) - (me|af)) ----
+  const TV Hvv = tv(c.Hvv, "ae", VV);
+  axpby<double>(1.0, Fvv, 0.0, Hvv, s);
+  cd(-1.0, Fov, T1("ma"), 1.0, Hvv);
+  cd(2.0, T1("mf"), CVVV("mfae"), 1.0, Hvv);
+  cd(-1.0, T1("mf"), CVVV("meaf"), 1.0, Hvv);
+  cd(-1.0, TAU("mnfa"), LOVV("mnfe"), 1.0, Hvv);
+
+  // ---- Hoo: :269-272 ----
+  const TV Hoo = tv(c.Hoo, "mi", OO);
+  axpby<double>(1.0, Foo, 0.0, Hoo, s);
+  cd(1.0, T1("ie"), Fov, 1.0, Hoo);
+  cd(1.0, T1("ne"), LOOV("imne"), 1.0, Hoo);
+  cd(1.0, TAU("inef"), LOVV("mnef"), 1.0, Hoo);
+
+  // ---- Hoooo: :331-337 -- <mn|ij> + t_je <mn|ie> + t_ie <nm|je> + tau_ijef <mn|ef> ----
+  const TV Hoooo = tv(c.Hoooo, "mnij", OOOO);
+  axpby<double>(1.0, tv(c.Coooo, "minj", OOOO), 0.0, Hoooo, s);     // (mi|nj)
+  cd(1.0, T1("je"), COOV("mine"), 1.0, Hoooo);                       // (mi|ne)
+  cd(1.0, T1("ie"), COOV("njme"), 1.0, Hoooo);                       // (nj|me)
+  cd(1.0, TAU("ijef"), IOVV("mnef"), 1.0, Hoooo);
+
+  // ---- Hooov: :505-506 -- <mn|ie> + t_if <nm|ef> ----
+  const TV Hooov = tv(c.Hooov, "mnie", OOOV);
+  axpby<double>(1.0, COOV("mine"), 0.0, Hooov, s);
+  cd(1.0, T1("if"), IOVV("nmef"), 1.0, Hooov);
+
+  // ---- Hovvo: :563-568 -- <mb|ej> + t_jf <mb|ef> - t_nb <mn|ej> - tau_jnfb <mn|ef> + t_njfb L_mnef ----
+  const TV Hovvo = tv(c.Hovvo, "mbej", OVVO);
+  axpby<double>(1.0, COVO("mejb"), 0.0, Hovvo, s);                  // (me|jb)
+  cd(1.0, T1("jf"), CVVV("mebf"), 1.0, Hovvo);                       // (me|bf)
+  cd(-1.0, T1("nb"), COOV("njme"), 1.0, Hovvo);                      // (nj|me)
+  cd(-1.0, TAU("jnfb"), IOVV("mnef"), 1.0, Hovvo);
+  cd(1.0, T2("njfb"), LOVV("mnef"), 1.0, Hovvo);
+
+  // ---- Hovov: :625-629 -- <mb|je> + t_jf <bm|ef> - t_nb <mn|je> - tau_jnfb <nm|ef> ----
+  const TV Hovov = tv(c.Hovov, "mbje", OVOV);
+  axpby<double>(1.0, COVV("mjbe"), 0.0, Hovov, s);                  // (mj|be)
+  cd(1.0, T1("jf"), CVVV("mfbe"), 1.0, Hovov);                       // <bm|ef> = (mf|be)
+  cd(-1.0, T1("nb"), COOV("mjne"), 1.0, Hovov);                      // <mn|je> = (mj|ne)
+  cd(-1.0, TAU("jnfb"), IOVV("nmef"), 1.0, Hovov);
+
+  // ---- Hovoo: :806-819 ----
+  const TV Hovoo = tv(c.Hovoo, "mbij", OVOO);
+  axpby<double>(1.0, COOV("mijb"), 0.0, Hovoo, s);                  // <mb|ij> = (mi|jb)    :806
+  cd(1.0, Hov, T2("ijeb"), 1.0, Hovoo);                              //                      :807
+  cd(-1.0, T1("nb"), Hoooo, 1.0, Hovoo);                             //                      :808
+  cd(1.0, TAU("ijef"), CVVV("mebf"), 1.0, Hovoo);                    // <mb|ef> = (me|bf)    :809
+  cd(-1.0, T2("ineb"), COOV("njme"), 1.0, Hovoo);                    // <nm|je> = (nj|me)    :810
+  cd(-1.0, T2("jneb"), COOV("mine"), 1.0, Hovoo);                    // <mn|ie> = (mi|ne)    :811
+  cd(1.0, T2("njeb"), LOOV("imne"), 1.0, Hovoo);                     // L_mnie               :812
+  {
+    const TV tmp = tv(c.Ltmp, "mbie", OVOV);                        //                      :813-815
+    axpby<double>(1.0, COVV("mibe"), 0.0, tmp, s);                   // <mb|ie> = (mi|be)
+    cd(-1.0, T2("infb"), IOVV("mnfe"), 1.0, tmp);
+    cd(1.0, T1("je"), tmp, 1.0, Hovoo);
+  }
+  {
+    const TV tmp = tv(c.Ltmp, "bmje", {v, o, o, v});                //                      :816-819
+    axpby<double>(1.0, COVO("mejb"), 0.0, tmp, s);                   // <bm|je> = (me|jb)
+    cd(-1.0, T2("jnfb"), IOVV("mnef"), 1.0, tmp);
+    cd(1.0, T2("njfb"), LOVV("mnef"), 1.0, tmp);
+    cd(1.0, T1("ie"), tmp, 1.0, Hovoo);
+  }
+
+  c.hbar_stored = stored;
+  if (stored) {
+    // ---- Hvvvv: :395-401 -- <ab|ef> - t_mb <am|ef> - t_ma <bm|fe> + tau_mnab <mn|ef> ----
+    const TV Hvvvv = tv(c.Hvvvv, "abef", VVVV);
+    const int64_t total = v * v * v * v;
+    unpack_vvvv_kernel<<<(unsigned)std::min<int64_t>(ceil_div(total, 256), 65536), 256, 0, s>>>(
+        o, v, (const double*)c.LP, (const double*)c.LM, c.Hvvvv);
+    CCX_HIP(hipGetLastError());
+    cd(-1.0, T1("mb"), CVVV("mfae"), 1.0, Hvvvv);                    // <am|ef> = (mf|ae)
+    cd(-1.0, T1("ma"), CVVV("mebf"), 1.0, Hvvvv);                    // <bm|fe> = (me|bf)
+    cd(1.0, TAU("mnab"), IOVV("mnef"), 1.0, Hvvvv);
+
+    // ---- Hvovv: :455-456 -- <am|ef> - t_na <nm|ef> ----
+    const TV Hvovv = tv(c.Hvovv, "amef", VOVV);
+    axpby<double>(1.0, CVVV("mfae"), 0.0, Hvovv, s);
+    cd(-1.0, T1("na"), IOVV("nmef"), 1.0, Hvovv);
+
+    // ---- Hvvvo: :682-695 ----
+    const TV Hvvvo = tv(c.Hvvvo, "abei", VVVO);
+    axpby<double>(1.0, CVVV("ibae"), 0.0, Hvvvo, s);                // <ab|ei> = (ib|ae)    :682
+    cd(-1.0, Hov, T2("miab"), 1.0, Hvvvo);                           //                      :683
+    cd(1.0, T1("if"), Hvvvv, 1.0, Hvvvo);                            //                      :684
+    cd(1.0, TAU("mnab"), COOV("nime"), 1.0, Hvvvo);                  // <mn|ei> = (ni|me)    :685
+    cd(-1.0, T2("imfa"), CVVV("mebf"), 1.0, Hvvvo);                  // <bm|fe> = (me|bf)    :686
+    cd(-1.0, T2("imfb"), CVVV("mfae"), 1.0, Hvvvo);                  // <am|ef> = (mf|ae)    :687
+    cd(2.0, T2("mifb"), CVVV("mfae"), 1.0, Hvvvo);                   // L_amef = 2(mf|ae)    :688
+    cd(-1.0, T2("mifb"), CVVV("meaf"), 1.0, Hvvvo);                  //        - (me|af)
+    {
+      const TV tmp = tv(c.Ltmp, "amei", {v, o, v, o});              //                      :689-691
+      axpby<double>(1.0, COVV("miae"), 0.0, tmp, s);                 // <am|ei> = (mi|ae)
+      cd(-1.0, T2("infa"), IOVV("mnfe"), 1.0, tmp);
+      cd(-1.0, T1("mb"), tmp, 1.0, Hvvvo);
+    }
+    {
+      const TV tmp = tv(c.Ltmp, "bmie", {v, o, o, v});              //                      :692-695
+      axpby<double>(1.0, COVO("meib"), 0.0, tmp, s);                 // <bm|ie> = (me|ib)
+      cd(-1.0, T2("infb"), IOVV("mnef"), 1.0, tmp);
+      cd(1.0, T2("nifb"), LOVV("mnef"), 1.0, tmp);
+      cd(-1.0, T1("ma"), tmp, 1.0, Hvvvo);
+    }
+  }
+  c.have_hbar = true;
+}
+
+void hbar_block(Ctx& c, int block, double* out, hipStream_t s) {
+  CCX_CHECK(c.have_hbar, ERR_STATE, "no HBAR has been built");
+  CCX_CHECK(out, ERR_ARG, "null output");
+  const int64_t o = c.o, v = c.v;
+  const double* src = nullptr;
+  int64_t n = 0;
+  bool big = false;
+  switch (block) {
+    case 0: src = c.Hov; n = o * v; break;
+    case 1: src = c.Hvv; n = v * v; break;
+    case 2: src = c.Hoo; n = o * o; break;
+    case 3: src = c.Hoooo; n = o * o * o * o; break;
+    case 4: src = c.Hvvvv; n = v * v * v * v; big = true; break;
+    case 5: src = c.Hvovv; n = v * o * v * v; big = true; break;
+    case 6: src = c.Hooov; n = o * o * o * v; break;
+    case 7: src = c.Hovvo; n = o * v * v * o; break;
+    case 8: src = c.Hovov; n = o * v * o * v; break;
+    case 9: src = c.Hvvvo; n = v * v * v * o; big = true; break;
+    case 10: src = c.Hovoo; n = o * v * o * o; break;
+    default: CCX_CHECK(false, ERR_ARG, "HBAR block index out of range");
+  }
+  CCX_CHECK(!big || c.hbar_stored, ERR_STATE, "Hvvvv, Hvovv and Hvvvo are held by a stored HBAR only");
+  CCX_HIP(hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+
+// r_L1 / r_L2 (cclambda.py:345-369, :468-496), r2 written as alpha * (H + H^T) (:496). With
+// with_source = false the terms that do not depend on l (2 Hov and L) are left out.
+void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, double* r2, bool with_source,
+                     double alpha, hipStream_t s) {
+  CCX_CHECK(c.have_hbar, ERR_STATE, "the Lambda residual needs an HBAR (ccx_hbar_build)");
+  CCX_CHECK(l1 && l2 && r1 && r2, ERR_ARG, "null argument");
+  const int64_t o = c.o, v = c.v, ov = o * v;
+  Workspace& ws = c.ws;
+  const std::vector<int64_t> OV{o, v}, OO{o, o}, VV{v, v}, OOVV{o, o, v, v}, OVOV{o, v, o, v};
+  const std::vector<int64_t> OOOV{o, o, o, v}, OOOO{o, o, o, o}, OVVV{o, v, v, v};
+  const std::vector<int64_t> VOVV{v, o, v, v}, OVVO{o, v, v, o}, VVVO{v, v, v, o}, OVOO{o, v, o, o};
+  const std::vector<int64_t> OOVO{o, o, v, o};
+  auto cd = [&](double a, const TV& A, const TV& B, double b, const TV& C) {
+    contract<double>(a, A, B, b, C, ws, s);
+  };
+  auto L1 = [&](const char* l) { return tv(l1, l, OV); };
+  auto L2 = [&](const char* l) { return tv(l2, l, OOVV); };
+  auto T1 = [&](const char* l) { return tv(c.Ht1, l, OV); };
+  auto T2 = [&](const char* l) { return tv(c.Ht2, l, OOVV); };
+  auto CVVV = [&](const char* l) { return tv(c.Covvv, l, OVVV); };
+  auto COOV = [&](const char* l) { return tv(c.Cooov, l, OOOV); };
+  auto COVV = [&](const char* l) { return tv(c.Coovv, l, OOVV); };
+  auto COVO = [&](const char* l) { return tv(c.Covov, l, OVOV); };
+  auto IOVV = [&](const char* l) { return tv(c.Ioovv, l, OOVV); };
+  auto LOVV = [&](const char* l) { return tv(c.Loovv, l, OOVV); };
+
+  // ---- Goo, Gvv: cclambda.py:281, :306 ----
+  const TV Goo = tv(c.Goo, "mi", OO), Gvv = tv(c.Gvv, "ae", VV);
+  cd(1.0, T2("mjab"), L2("ijab"), 0.0, Goo);
+  cd(-1.0, T2("ijeb"), L2("ijab"), 0.0, Gvv);
+
+  // ---- r_L1: :348-369 ----
+  const TV R1 = tv(r1, "ia", OV);
+  if (with_source) axpby<double>(2.0, tv(c.Hov, "ia", OV), 0.0, R1, s);          // :348
+  else scale<double>(0.0, R1, s);
+  cd(1.0, L1("ie"), tv(c.Hvv, "ea", VV), 1.0, R1);                               // :354
+  cd(-1.0, L1("ma"), tv(c.Hoo, "im", OO), 1.0, R1);                              // :355
+  cd(-1.0, L2("mnae"), tv(c.Hovoo, "iemn", OVOO), 1.0, R1);                      // :357
+  cd(2.0, L1("me"), tv(c.Hovvo, "ieam", OVVO), 1.0, R1);                         // :358
+  cd(-1.0, L1("me"), tv(c.Hovov, "iema", OVOV), 1.0, R1);
+  cd(-2.0, relabel(Goo, "mn"), tv(c.Hooov, "mina", OOOV), 1.0, R1);              // :368
+  cd(1.0, relabel(Goo, "mn"), tv(c.Hooov, "imna", OOOV), 1.0, R1);               // :369
+
+  // ---- r_L2 before P(ij,ab): :468-494 ----
+  const TV H = tv(c.Lh, "ijab", OOVV);
+  if (with_source) axpby<double>(1.0, LOVV("ijab"), 0.0, H, s);                  // :468
+  else scale<double>(0.0, H, s);
+  cd(2.0, L1("ia"), tv(c.Hov, "jb", OV), 1.0, H);                                // :476
+  cd(-1.0, L1("ja"), tv(c.Hov, "ib", OV), 1.0, H);                               // :477
+  cd(-2.0, L1("mb"), tv(c.Hooov, "jima", OOOV), 1.0, H);                         // :480
+  cd(1.0, L1("mb"), tv(c.Hooov, "ijma", OOOV), 1.0, H);                          // :481
+  cd(1.0, L2("ijeb"), tv(c.Hvv, "ea", VV), 1.0, H);                              // :486
+  cd(-1.0, L2("mjab"), tv(c.Hoo, "im", OO), 1.0, H);                             // :487
+  cd(0.5, L2("mnab"), tv(c.Hoooo, "ijmn", OOOO), 1.0, H);                        // :488
+  cd(2.0, L2("mjeb"), tv(c.Hovvo, "ieam", OVVO), 1.0, H);                        // :490
+  cd(-1.0, L2("mjeb"), tv(c.Hovov, "iema", OVOV), 1.0, H);
+  cd(-1.0, L2("mibe"), tv(c.Hovov, "jema", OVOV), 1.0, H);                       // :491
+  cd(-1.0, L2("mieb"), tv(c.Hovvo, "jeam", OVVO), 1.0, H);                       // :492
+  cd(1.0, Gvv, LOVV("ijeb"), 1.0, H);                                            // :493
+  cd(-1.0, Goo, LOVV("mjab"), 1.0, H);                                           // :494
+
+  if (c.hbar_stored) {
+    cd(1.0, L2("imef"), tv(c.Hvvvo, "efam", VVVO), 1.0, R1);                     // :356
+    cd(-2.0, relabel(Gvv, "ef"), tv(c.Hvovv, "eifa", VOVV), 1.0, R1);            // :366
+    cd(1.0, relabel(Gvv, "ef"), tv(c.Hvovv, "eiaf", VOVV), 1.0, R1);             // :367
+    cd(2.0, L1("ie"), tv(c.Hvovv, "ejab", VOVV), 1.0, H);                        // :478
+    cd(-1.0, L1("ie"), tv(c.Hvovv, "ejba", VOVV), 1.0, H);                       // :479
+    cd(0.5, L2("ijef"), tv(c.Hvvvv, "efab", {v, v, v, v}), 1.0, H);              // :489
+  } else {
+    // Factored form of the terms that read Hvvvv, Hvovv and Hvvvo, with the intermediates
+    //   Vf_ijem = l2_ijef t_mf,  Vg_ijfm = l2_ijef t_me,  U_ijmn = l2_ijef tau_mnef,
+    //   Y_ijab  = l2_ijef Hvvvv_efab
+    //           = ladder(l2)_ijab - Vf_ijem <em|ab> - Vg_ijfm <fm|ba> + U_ijmn <mn|ab>   (:395-401)
+    build_tau(o, v, 1.0, 1.0, c.Ht1, c.Ht2, c.tau, s);
+    auto TAU = [&](const char* l) { return tv(c.tau, l, OOVV); };
+    const TV Vf = tv(c.LVf, "ijem", OOVO), Vg = tv(c.LVg, "ijfm", OOVO), U = tv(c.LU, "ijmn", OOOO);
+    const TV Y = tv(c.LY, "ijab", OOVV);
+    cd(1.0, L2("ijef"), T1("mf"), 0.0, Vf);
+    cd(1.0, L2("ijef"), T1("me"), 0.0, Vg);
+    cd(1.0, L2("ijef"), TAU("mnef"), 0.0, U);
+    {
+      // the packed ladder GEMM of the forward residual applied to l2, (ab) columns only
+      const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
+      tau_pack<double>(o, v, nullptr, l2, (double*)c.Tp, (double*)c.Tm, s);
+      cd(1.0, tv(c.Tp, "xk", {noP, npP}), tvs(c.LP, "ky", {npP, npP}, {npP + ov, 1}), 0.0,
+         tvs(c.CP, "xy", {noP, npP}, {npP + ov, 1}));
+      if (noM > 0)
+        cd(1.0, tv(c.Tm, "xk", {noM, npM}), tvs(c.LM, "ky", {npM, npM}, {npM + ov, 1}), 0.0,
+           tvs(c.CM, "xy", {noM, npM}, {npM + ov, 1}));
+      dim3 grid((unsigned)std::min<int64_t>(ceil_div(v * v, 256), 64), (unsigned)(o * o));
+      ladder_unpack_kernel<<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
+      CCX_HIP(hipGetLastError());
+    }
+    cd(-1.0, Vf, CVVV("mbea"), 1.0, Y);                        // <em|ab> = (mb|ea)
+    cd(-1.0, Vg, CVVV("mafb"), 1.0, Y);                        // <fm|ba> = (ma|fb)
+    cd(1.0, U, IOVV("mnab"), 1.0, Y);                          // <mn|ab>
+    axpby<double>(0.5, Y, 1.0, H, s);                                            // :489
+
+    // :356  l2_imef Hvvvo_efam, term by term over :682-695 (p, q occupied and g virtual dummies)
+    cd(1.0, L2("imef"), CVVV("mfea"), 1.0, R1);                // :682 <ef|am> = (mf|ea)
+    const TV Xoo = tv(c.Lsm, "ip", OO);                        // :683 -Hov_pa l2_imef t_pmef
+    cd(1.0, L2("imef"), T2("pmef"), 0.0, Xoo);
+    cd(-1.0, Xoo, tv(c.Hov, "pa", OV), 1.0, R1);
+    cd(1.0, T1("mg"), relabel(Y, "imag"), 1.0, R1);            // :684 t_mg Hvvvv_efag -> t_mg Y_imag
+    cd(1.0, relabel(U, "impq"), COOV("qmpa"), 1.0, R1);        // :685 <pq|am> = (qm|pa)
+    const TV W = tv(c.Ltmp, "ifpg", OVOV);                     // :686 -t_mpge <fp|ga>, (pa|fg)
+    cd(1.0, L2("imef"), T2("mpge"), 0.0, W);
+    cd(-1.0, W, CVVV("pafg"), 1.0, R1);
+    const TV W6 = tv(c.Ltmp, "iepg", OVOV);                    // :687 -t_mpgf <ep|ag>, (pg|ea)
+    cd(1.0, L2("imef"), T2("mpgf"), 0.0, W6);
+    cd(-1.0, W6, CVVV("pgea"), 1.0, R1);
+    cd(1.0, L2("imef"), T2("pmgf"), 0.0, W6);                  // :688 t_pmgf L_epag
+    cd(2.0, W6, CVVV("pgea"), 1.0, R1);                        //   L_epag = 2(pg|ea) - (pa|eg)
+    cd(-1.0, W6, CVVV("paeg"), 1.0, R1);
+    // :689-691  -t_pf A_epam,  A_epam = <ep|am> - t_mqge <pq|ga>;  Vf_imep = l2_imef t_pf
+    cd(-1.0, relabel(Vf, "imep"), COVV("pmea"), 1.0, R1);      //   <ep|am> = (pm|ea)
+    const TV Q = tv(c.LQ, "ipqg", OOOV);
+    cd(1.0, relabel(Vf, "imep"), T2("mqge"), 0.0, Q);
+    cd(1.0, Q, IOVV("pqga"), 1.0, R1);
+    // :692-695  -t_pe B_fpma,  B_fpma = <fp|ma> - t_mqgf <pq|ag> + t_qmgf L_pqag;  Vg_imfp = l2_imef t_pe
+    cd(-1.0, relabel(Vg, "imfp"), COVO("mfpa"), 1.0, R1);      //   <fp|ma> = (mf|pa)
+    cd(1.0, relabel(Vg, "imfp"), T2("mqgf"), 0.0, Q);
+    cd(1.0, Q, IOVV("pqag"), 1.0, R1);
+    cd(1.0, relabel(Vg, "imfp"), T2("qmgf"), 0.0, Q);
+    cd(-1.0, Q, LOVV("pqag"), 1.0, R1);
+
+    // :366-367  Gvv_ef Hvovv, Hvovv_eifa = <ei|fa> - t_ne <ni|fa>
+    const TV Gef = relabel(Gvv, "ef");
+    cd(-2.0, Gef, CVVV("iaef"), 1.0, R1);                      // <ei|fa> = (ia|ef)
+    cd(1.0, Gef, CVVV("ifea"), 1.0, R1);                       // <ei|af> = (if|ea)
+    const TV gt = tv(c.Lsm + o * o, "nf", OV);                 // gt_nf = Gvv_ef t_ne
+    cd(1.0, Gef, T1("ne"), 0.0, gt);
+    cd(2.0, gt, IOVV("nifa"), 1.0, R1);
+    cd(-1.0, gt, IOVV("niaf"), 1.0, R1);
+
+    // :478-479  l1_ie Hvovv_ejab, Hvovv_ejab = <ej|ab> - t_ne <nj|ab>
+    cd(2.0, L1("ie"), CVVV("jbea"), 1.0, H);                   // <ej|ab> = (jb|ea)
+    cd(-1.0, L1("ie"), CVVV("jaeb"), 1.0, H);                  // <ej|ba> = (ja|eb)
+    const TV lt = tv(c.Lsm + o * o + ov, "in", OO);            // lt_in = l1_ie t_ne
+    cd(1.0, L1("ie"), T1("ne"), 0.0, lt);
+    cd(-2.0, lt, IOVV("njab"), 1.0, H);
+    cd(1.0, lt, IOVV("njba"), 1.0, H);
+  }
+
+  pair_symmetrize(o, v, alpha, c.Lh, r2, s);                                     // :496
+}
+
+// 1/2 <ij|ab> l2_ijab (cclambda.py:570) into a device scalar.
+void pseudoenergy(Ctx& c, const double* l2, double* e_dev, hipStream_t s) {
+  CCX_CHECK(c.have_ints, ERR_STATE, "pseudoenergy requested before the integrals were set");
+  CCX_CHECK(l2 && e_dev, ERR_ARG, "null argument");
+  dots(c.n2(), 1, c.Ioovv, 0, l2, c.red, e_dev, s);
+  scale<double>(0.5, tv(e_dev, "x", {1}), s);
+}
+
+}  // namespace ccx

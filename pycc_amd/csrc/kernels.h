@@ -26,7 +26,8 @@ template <typename T>
 void ring_prep(int64_t o, int64_t v, const double* t1, const double* t2, T* S1, T* A1, T* S2, T* Tx,
                double* Ud, hipStream_t s);
 
-// Packed ladder amplitudes: Tp[(i<=j)][(e<=f)], Tm[(i<j)][(e<f)] from tau = t2 + t1 t1.
+// Packed ladder amplitudes: Tp[(i<=j)][(e<=f)], Tm[(i<j)][(e<f)] from tau = t2 + t1 t1 (t1 may be
+// null: tau = t2).
 template <typename T>
 void tau_pack(int64_t o, int64_t v, const double* t1, const double* t2, T* Tp, T* Tm, hipStream_t s);
 // BF16x3 operands are K-contiguous with K padded to a multiple of 32 (padding stays zero).

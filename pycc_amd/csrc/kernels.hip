@@ -167,8 +167,11 @@ __global__ void tau_pack_kernel(int64_t o, int64_t v, const double* __restrict__
        c += (int64_t)gridDim.x * blockDim.x) {
     int64_t e, f;
     decodeP(c, e, f);
-    double tef = blk[e * v + f] + t1[i * v + e] * t1[j * v + f];
-    double tfe = blk[f * v + e] + t1[i * v + f] * t1[j * v + e];
+    double tef = blk[e * v + f], tfe = blk[f * v + e];
+    if (t1) {
+      tef += t1[i * v + e] * t1[j * v + f];
+      tfe += t1[i * v + f] * t1[j * v + e];
+    }
     Tp[rowP * npP + c] = (T)(e < f ? tef + tfe : tef);
     if (hasM && e < f) Tm[rowM * npM + pairM(e, f)] = (T)(tef - tfe);
   }

@@ -1,0 +1,264 @@
+"""GPU parity of the backward: HBAR, the Lambda residual, the vector-Jacobian product and the Lambda
+solve, against the CPU oracle (``oracle/lambda_ref.py``, pinned by the reference's water STO-3G
+pseudoenergy and by central differences of the Lagrangian).
+
+Bars: HBAR blocks and Lambda residuals agree with the oracle to 1e-11 (relative to the largest
+element); the vJp matches central finite differences (step 1e-5) to 1e-6 relative and the oracle's
+Lambda residual to 1e-11; a converged Lambda solve matches the oracle's l1/l2 to 1e-8 max-abs and its
+pseudoenergy to 1e-10 Eh.
+"""
+import numpy as np
+import pytest
+
+from conftest import random_amplitudes
+from oracle.ccsd_ref import OracleCCSD
+from oracle.lambda_ref import OracleHBAR, OracleLambda
+from pycc_amd.synth import synthetic_df, DenseHamiltonian, dirac_eri
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+BLOCKS = ("Hov", "Hvv", "Hoo", "Hoooo", "Hvvvv", "Hvovv", "Hooov", "Hovvo", "Hovov", "Hvvvo", "Hovoo")
+LARGE = ("Hvvvv", "Hvovv", "Hvvvo")
+
+
+def _ham(no, nv, seed, t2max=0.05, offdiag=0.0):
+    sd = synthetic_df(no, nv, seed=seed, t2max=t2max)
+    H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
+    if offdiag:
+        X = np.random.default_rng(seed + 100).standard_normal(H.F.shape) * offdiag
+        H.F = H.F + X + X.T
+    return H
+
+
+def _sym2(x):
+    return 0.5 * (x + x.transpose(1, 0, 3, 2))
+
+
+def _dev(x):
+    return torch.tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda")
+
+
+def _np(x):
+    return x.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(1.0, np.abs(b).max()))
+
+
+@pytest.fixture(scope="module")
+def pycc_amd():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import pycc_amd
+    return pycc_amd
+
+
+def _setup(pycc_amd, no, nv, seed, offdiag=0.03):
+    H = _ham(no, nv, seed, offdiag=offdiag)
+    cc = pycc_amd.CCwfn(H, no=no, precision='DP', verbose=False)
+    ref = OracleCCSD(H, no)
+    t1, t2 = random_amplitudes(no, nv, seed + 1, scale=0.05)
+    t2 = _sym2(t2)
+    return H, cc, ref, t1, t2
+
+
+@pytest.mark.parametrize("no,nv,seed", [(3, 7, 1), (4, 12, 2), (5, 19, 3)])
+@pytest.mark.parametrize("stored", [True, False])
+def test_hbar_blocks_match_oracle(pycc_amd, no, nv, seed, stored):
+    print("hbar blocks", no, nv, seed, "stored" if stored else "factored", flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    cc.t1.copy_(_dev(t1))
+    cc.t2.copy_(_dev(t2))
+    hbar = pycc_amd.cchbar(cc, store_vvvv=stored)
+    oh = OracleHBAR(ref, t1, t2)
+    for name in BLOCKS:
+        if name in LARGE and not stored:
+            with pytest.raises(pycc_amd.PyCCError):
+                getattr(hbar, name)
+            continue
+        err = _rel(_np(getattr(hbar, name)), getattr(oh, name))
+        assert err < 1e-11, (name, err)
+
+
+@pytest.mark.parametrize("no,nv,seed", [(1, 5, 4), (3, 7, 5), (4, 12, 6), (5, 19, 7)])
+@pytest.mark.parametrize("stored", [True, False])
+@pytest.mark.parametrize("with_source", [True, False])
+def test_lambda_residual_matches_oracle(pycc_amd, no, nv, seed, stored, with_source):
+    print("lambda residual", no, nv, seed, stored, with_source, flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    l1, l2 = random_amplitudes(no, nv, seed + 2, scale=0.5)
+    l2 = _sym2(l2)
+    oh = OracleHBAR(ref, t1, t2)
+    lam = OracleLambda(ref, oh)
+    r1, r2 = lam.residuals_with(oh, t2, l1, l2)
+    if not with_source:
+        r1 = r1 - 2.0 * oh.Hov
+        r2 = r2 - 2.0 * H.L[:no, :no, no:, no:]
+    cc._hbar_for(cc.H.F, _dev(t1), _dev(t2), stored)
+    g1, g2 = cc._lambda_residual(_dev(l1), _dev(l2), with_source=with_source)
+    assert _rel(_np(g1), r1) < 1e-11
+    assert _rel(_np(g2), r2) < 1e-11
+
+
+@pytest.mark.parametrize("stored", [True, False])
+def test_vjp_is_sourceless_lambda_residual(pycc_amd, stored):
+    no, nv, seed = 4, 11, 8
+    print("vjp vs oracle", stored, flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    rng = np.random.default_rng(seed)
+    c1 = rng.standard_normal((no, nv))
+    c2 = rng.standard_normal((no, no, nv, nv))          # not pair-symmetric: only its symmetric part acts
+    oh = OracleHBAR(ref, t1, t2)
+    lam = OracleLambda(ref, oh)
+    r1, r2 = lam.residuals_with(oh, t2, c1, c2 + c2.transpose(1, 0, 3, 2))
+    r1 = r1 - 2.0 * oh.Hov
+    r2 = r2 - 2.0 * H.L[:no, :no, no:, no:]
+    g1, g2 = cc.vjp(_dev(t1), _dev(t2), _dev(c1), _dev(c2), store_vvvv=stored)
+    assert _rel(_np(g1), r1) < 1e-11
+    assert _rel(_np(g2), 0.5 * r2) < 1e-11
+
+
+def test_vjp_central_differences(pycc_amd):
+    """Random N(0,1) cotangent: the vJp against central differences (step 1e-5) of c . r(t) along
+    random pair-symmetric directions and for 32 seeded t2 elements and all of t1."""
+    no, nv, seed = 4, 12, 9
+    print("vjp finite differences", flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    rng = np.random.default_rng(seed)
+    c1 = _dev(rng.standard_normal((no, nv)))
+    c2 = _dev(rng.standard_normal((no, no, nv, nv)))
+    T1, T2 = _dev(t1), _dev(t2)
+    g1, g2 = cc.vjp(T1, T2, c1, c2)
+    g1, g2 = _np(g1), _np(g2)
+
+    def f(a1, a2):
+        r1, r2 = cc.residuals(cc.H.F, a1, a2)
+        return float((c1 * r1).sum() + (c2 * r2).sum())
+
+    h = 1e-5
+    for k in range(3):
+        d1 = rng.standard_normal((no, nv))
+        d2 = _sym2(rng.standard_normal((no, no, nv, nv)))
+        fd = (f(T1 + h * _dev(d1), T2 + h * _dev(d2)) - f(T1 - h * _dev(d1), T2 - h * _dev(d2))) / (2 * h)
+        an = float((g1 * d1).sum() + (g2 * d2).sum())
+        assert abs(fd - an) < 1e-6 * max(1.0, abs(an)), (k, fd, an)
+    idx = [tuple(int(x) for x in rng.integers(0, [no, no, nv, nv])) for _ in range(32)]
+    for (i, j, a, b) in idx:
+        d2 = np.zeros((no, no, nv, nv))
+        d2[i, j, a, b] = 1.0
+        d2[j, i, b, a] = 1.0                      # one parameter of the pair-symmetric t2
+        mult = 1.0 if (i, a) == (j, b) else 2.0
+        fd = (f(T1, T2 + h * _dev(d2)) - f(T1, T2 - h * _dev(d2))) / (2 * h)
+        assert abs(fd - mult * g2[i, j, a, b]) < 1e-6 * max(1.0, abs(fd)), ((i, j, a, b), fd, g2[i, j, a, b])
+    for (i, a) in np.ndindex(no, nv):
+        d1 = np.zeros((no, nv))
+        d1[i, a] = 1.0
+        fd = (f(T1 + h * _dev(d1), T2) - f(T1 - h * _dev(d1), T2)) / (2 * h)
+        assert abs(fd - g1[i, a]) < 1e-6 * max(1.0, abs(fd)), ((i, a), fd, g1[i, a])
+
+
+def test_autograd_function_backward(pycc_amd):
+    """torch.autograd through the residual Function equals the vJp; the energy Function's gradient is
+    (2 Hov, L); the Lagrangian's gradient is the Lambda residual."""
+    from pycc_amd import autograd as ccad
+    no, nv, seed = 3, 9, 10
+    print("autograd", flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    rng = np.random.default_rng(seed)
+    T1 = _dev(t1).requires_grad_(True)
+    T2 = _dev(t2).requires_grad_(True)
+    c1, c2 = _dev(rng.standard_normal((no, nv))), _dev(rng.standard_normal((no, no, nv, nv)))
+    r1, r2 = ccad.residual(cc, T1, T2)
+    a1, a2 = torch.autograd.grad((r1, r2), (T1, T2), (c1, c2))
+    g1, g2 = cc.vjp(T1.detach(), T2.detach(), c1, c2)
+    assert _rel(_np(a1), _np(g1)) < 1e-14 and _rel(_np(a2), _np(g2)) < 1e-14
+    oh = OracleHBAR(ref, t1, t2)
+    e = ccad.energy(cc, T1, T2)
+    assert abs(float(e.detach()) - ref.energy(t1, t2)) < 1e-12
+    e1, e2 = torch.autograd.grad(e, (T1, T2))
+    assert _rel(_np(e1), 2.0 * oh.Hov) < 1e-11
+    assert _rel(_np(e2), H.L[:no, :no, no:, no:]) < 1e-11
+    l1, l2 = random_amplitudes(no, nv, seed + 3, scale=0.5)
+    l2 = _sym2(l2)
+    lag = ccad.lagrangian(cc, T1, T2, _dev(l1), _dev(l2))
+    d1, d2 = torch.autograd.grad(lag, (T1, T2))
+    rl1, rl2 = OracleLambda(ref, oh).residuals_with(oh, t2, l1, l2)
+    assert _rel(_np(d1), rl1) < 1e-11
+    assert _rel(_np(2.0 * d2), rl2) < 1e-11
+
+
+@pytest.mark.parametrize("no,nv,seed,stored", [(4, 12, 11, True), (4, 12, 11, False), (5, 19, 12, False)])
+def test_solve_lambda_matches_oracle(pycc_amd, no, nv, seed, stored):
+    print("solve_lambda", no, nv, seed, stored, flush=True)
+    H = _ham(no, nv, seed, t2max=0.05)
+    cc = pycc_amd.CCwfn(H, no=no, precision='DP', verbose=False)
+    cc.solve_cc(1e-12, 1e-12, 100)
+    lam = pycc_amd.cclambda(cc, pycc_amd.cchbar(cc, store_vvvv=stored))
+    lecc = lam.solve_lambda(1e-12, 1e-12, 100)
+    ref = OracleCCSD(H, no)
+    ref.solve_cc(1e-12, 1e-12, 100)
+    olam = OracleLambda(ref, OracleHBAR(ref))
+    olecc = olam.solve_lambda(1e-12, 1e-12, 100)
+    print("lecc", lecc, olecc, len(lam.history), len(olam.history), flush=True)
+    assert lecc is not None and olecc is not None
+    assert abs(lecc - olecc) < 1e-10
+    assert np.abs(_np(lam.l1) - olam.l1).max() < 1e-8
+    assert np.abs(_np(lam.l2) - olam.l2).max() < 1e-8
+
+
+def test_lambda_by_iterating_the_vjp(pycc_amd):
+    """The Lambda equations solved with nothing but the autograd backward of the residual and energy
+    (Jacobi on dL/dt) reproduce the oracle's solve_lambda l1/l2 and pseudoenergy."""
+    from pycc_amd import autograd as ccad
+    no, nv, seed = 4, 12, 13
+    print("lambda from vjp", flush=True)
+    H = _ham(no, nv, seed, t2max=0.05)
+    cc = pycc_amd.CCwfn(H, no=no, precision='DP', verbose=False)
+    cc.solve_cc(1e-12, 1e-12, 100)
+    ref = OracleCCSD(H, no)
+    ref.solve_cc(1e-12, 1e-12, 100)
+    olam = OracleLambda(ref, OracleHBAR(ref))
+    olecc = olam.solve_lambda(1e-12, 1e-12, 100)
+    T1 = cc.t1.clone().requires_grad_(True)
+    T2 = cc.t2.clone().requires_grad_(True)
+    l1 = 2.0 * cc.t1
+    l2 = (2.0 * (2.0 * cc.t2 - cc.t2.swapaxes(2, 3))).contiguous()
+    Dia, Dijab = cc.Dia, cc.Dijab
+    for it in range(200):
+        lag = ccad.lagrangian(cc, T1, T2, l1, l2)
+        d1, d2 = torch.autograd.grad(lag, (T1, T2))
+        r1, r2 = d1, 2.0 * d2                      # r_L1 = dL/dt1, r_L2 = dL/dt2 + transpose
+        l1 = l1 + r1 / Dia
+        l2 = l2 + r2 / Dijab
+        rms = float(torch.sqrt((r1 / Dia).pow(2).sum() + (r2 / Dijab).pow(2).sum()))
+        if rms < 1e-11:
+            break
+    print("iterations", it + 1, "rms", rms, flush=True)
+    assert rms < 1e-11
+    assert abs(cc.pseudoenergy(l2) - olecc) < 1e-10
+    assert np.abs(_np(l1) - olam.l1).max() < 1e-8
+    assert np.abs(_np(l2) - olam.l2).max() < 1e-8
+
+
+def test_water_sto3g_lambda_constant_on_gpu(pycc_amd):
+    """pycc tests/test_003_ccsd_lambda.py:50 (H2O/STO-3G, frozen core): -0.068826452648939."""
+    from oracle.gaussian import water_sto3g_hamiltonian
+    F, ERI, escf, nocc = water_sto3g_hamiltonian()
+    H = DenseHamiltonian(F, ERI)
+    cc = pycc_amd.CCwfn(H, no=nocc - 1, nfzc=1, precision='DP', verbose=False)
+    cc.solve_cc(1e-12, 1e-12, 75)
+    for stored in (True, False):
+        lam = pycc_amd.cclambda(cc, pycc_amd.cchbar(cc, store_vvvv=stored))
+        lecc = lam.solve_lambda(1e-12, 1e-12)
+        assert abs(lecc - (-0.068826452648939)) < 1e-11, (stored, lecc)
+
+
+def test_backward_requires_fp64_context(pycc_amd):
+    H = _ham(2, 6, 14)
+    cc = pycc_amd.CCwfn(H, no=2, precision='MIXED', verbose=False)
+    with pytest.raises(pycc_amd.PyCCError):
+        pycc_amd.cchbar(cc)
+    with pytest.raises(pycc_amd.PyCCError):
+        cc.vjp(cc.t1, cc.t2, cc.t1, cc.t2)
