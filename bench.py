@@ -41,7 +41,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv):
+def _ms(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0), out
+
+
+def backward_pass(cc, store_vvvv=False):
+    """One FP64 backward of the converged solve: HBAR at (F, t1, t2), then the gradient of the CCSD
+    Lagrangian E + l1.r1 + 1/2 l2.r2 at the guess l1 = 2 t1, l2 = 2(2 t2 - t2^T) (the Lambda
+    residual), and a second Lambda residual on the same HBAR (the cost of one Lambda iteration)."""
+    import torch
+    import pycc_amd
+    l1 = (2.0 * cc.t1).contiguous()
+    l2 = (2.0 * (2.0 * cc.t2 - cc.t2.swapaxes(2, 3))).contiguous()
+    hbar_ms, hbar = _ms(lambda: pycc_amd.cchbar(cc, store_vvvv=store_vvvv))
+    res_ms, (r1, r2) = _ms(lambda: cc._lambda_residual(l1, l2, with_source=True))
+    res2_ms, _ = _ms(lambda: cc._lambda_residual(l1, l2, with_source=True))
+    out = {"store_vvvv": bool(store_vvvv), "hbar_build_ms": hbar_ms, "lambda_residual_ms": res2_ms,
+           "first_residual_ms": res_ms, "total_ms": hbar_ms + res_ms, "device_GB": cc.device_bytes() / 1e9,
+           "gradient_norm": float(torch.sqrt((r1 * r1).sum() + (r2 * r2).sum()))}
+    del hbar, r1, r2, l1, l2
+    return out
+
+
+def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv, backward=False):
     import torch
     import pycc_amd
     t0 = time.time()
@@ -91,6 +118,11 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv)
         "iterations": cc.niter, "energy": cc.ecc if (last and last["converged"]) else (last["ecc"] if last else None),
         "device_GB": cc.device_bytes() / 1e9,
     }
+    if backward and out["converged"]:
+        out["backward"] = backward_pass(cc)
+        log("[%s] FP64 backward: HBAR %.0f ms, Lambda residual %.0f ms, %.1f GB" % (
+            precision, out["backward"]["hbar_build_ms"], out["backward"]["lambda_residual_ms"],
+            out["backward"]["device_GB"]))
     t1c, t2c = cc.t1.clone(), cc.t2.clone()
     del cc, it
     torch.cuda.empty_cache()
@@ -138,12 +170,18 @@ def main():
     ap.add_argument("--no-converge", action="store_true", help="skip the untimed convergence legs")
     ap.add_argument("--no-dp", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-backward", action="store_true", help="skip the FP64 backward pass")
+    ap.add_argument("--backward", action="store_true",
+                    help="measure the backward instead: o=32 v=256 seed 2 FP64 (override with --no/--nv/"
+                         "--seed), HBAR stored vs factored, finite-difference checks, Lambda solve")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (the sharding design "
-              "for more GPUs is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: the library runs on one GPU; --gpus %d is not supported (multi-GPU sharding "
+              "is designed, not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.backward:
+        return backward_bench(args)
 
     import numpy as np
     import torch
@@ -164,7 +202,7 @@ def main():
     if not args.no_dp and args.precision != "DP":
         torch.cuda.empty_cache()
         dp, dt1, dt2 = run_leg(sd, o, "DP", args.warmup, args.steps, converge, args.maxiter,
-                               args.e_conv, args.r_conv_dp)
+                               args.e_conv, args.r_conv_dp, backward=not args.no_backward)
         dt1, dt2 = dt1.cpu(), dt2.cpu()
 
     F = f_ref(o, v)
@@ -196,6 +234,11 @@ def main():
             res["energy_error_uEh"] = 1e6 * (mixed["energy"] - dp["energy"])
             res["max_abs_dT1"] = float((mt1 - dt1).abs().max())
             res["max_abs_dT2"] = float((mt2 - dt2).abs().max())
+    if dp and "backward" in dp:
+        b = dp["backward"]
+        b["workload"] = ("FP64 Lagrangian backward (factored HBAR + Lambda residual = vJp with the energy "
+                         "source) at the converged FP64 amplitudes, o=%d v=%d" % (o, v))
+        res["backward"] = b
     res["mixed"] = {"iterations": mixed["iterations"], "energy": mixed["energy"], "converged": mixed["converged"],
                     "setup_s": mixed["setup_s"], "device_GB": mixed["device_GB"]}
 
@@ -228,6 +271,72 @@ def main():
                                "kind": "port",
                                "sample": "numpy oracle (restated pycc numpy path), median of 3 iterations at "
                                          "o=16,v=128 seed 0 = %.3f s/iter, scaled by F_ref ratio %.1f" % (s_it, scale)}
+    print(json.dumps(res), flush=True)
+
+
+def backward_bench(args):
+    """Differentiable CCSD at o=32, v=256 (seed 2, FP64): converge, then the backward through the
+    torch.autograd.Function with two cotangents (the energy Lagrangian and a seeded N(0,1) one),
+    32 seeded t2 elements against central differences (step 1e-5), timings and device memory with
+    the v^4 HBAR block stored and factored, and the Lambda solve."""
+    import numpy as np
+    import torch
+    import pycc_amd
+    from pycc_amd import autograd as ccad
+    from pycc_amd.synth import synthetic_df
+    o = 32 if args.no == 48 else args.no
+    v = 256 if args.nv == 384 else args.nv
+    seed = 2 if args.seed == 3 else args.seed
+    sd = synthetic_df(o, v, seed=seed)
+    cc = pycc_amd.CCwfn(sd, no=o, precision="DP", verbose=False)
+    solve_ms, ecc = _ms(lambda: cc.solve_cc(1e-10, 1e-10, 100))
+    log("[backward] FP64 CCSD o=%d v=%d: E = %.12f in %d iterations (%.1f s)" % (o, v, ecc, cc.niter, solve_ms / 1e3))
+    res = {"metric": "CCSD backward (vJp) time and HBM, FP64, v^4 stored vs factored", "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic (seeded DF Hamiltonian)",
+           "config": {"workload": "differentiable CCSD backward", "no": o, "nv": v, "seed": seed},
+           "forward": {"energy": ecc, "iterations": cc.niter, "solve_ms": solve_ms}}
+    base_GB = cc.device_bytes() / 1e9
+    for stored in (False, True):
+        b = backward_pass(cc, store_vvvv=stored)
+        b["device_GB_before"] = base_GB
+        res["stored" if stored else "factored"] = b
+        log("[backward] %s: HBAR %.0f ms, Lambda residual %.0f ms, device %.1f GB" % (
+            "stored" if stored else "factored", b["hbar_build_ms"], b["lambda_residual_ms"], b["device_GB"]))
+    # autograd with a seeded N(0,1) cotangent vs central differences on 32 seeded t2 elements
+    rng = np.random.default_rng(seed)
+    c1 = torch.tensor(rng.standard_normal((o, v)), device="cuda")
+    c2 = torch.tensor(rng.standard_normal((o, o, v, v)), device="cuda")
+    T1 = cc.t1.clone().requires_grad_(True)
+    T2 = cc.t2.clone().requires_grad_(True)
+    vjp_ms, grads = _ms(lambda: torch.autograd.grad(ccad.residual(cc, T1, T2), (T1, T2), (c1, c2)))
+    g1, g2 = grads
+    F = cc.H.F
+
+    def f(a1, a2):
+        r1, r2 = cc.residuals(F, a1, a2)
+        return float((c1 * r1).sum() + (c2 * r2).sum())
+
+    h, worst = 1e-5, 0.0
+    t1d, t2d = T1.detach(), T2.detach()
+    for _ in range(32):
+        i, j, a, b = (int(x) for x in rng.integers(0, [o, o, v, v]))
+        d2 = torch.zeros_like(t2d)
+        d2[i, j, a, b] = 1.0
+        d2[j, i, b, a] = 1.0
+        mult = 1.0 if (i, a) == (j, b) else 2.0
+        fd = (f(t1d, t2d + h * d2) - f(t1d, t2d - h * d2)) / (2 * h)
+        an = mult * float(g2[i, j, a, b])
+        worst = max(worst, abs(fd - an) / max(1.0, abs(an)))
+    res["random_cotangent"] = {"vjp_ms_incl_forward_and_hbar": vjp_ms, "fd_t2_elements": 32,
+                               "fd_step": h, "fd_max_rel_error": worst}
+    log("[backward] random cotangent: %.0f ms, FD max rel error %.2e over 32 t2 elements" % (vjp_ms, worst))
+    # Lambda solve (factored HBAR)
+    lam = pycc_amd.cclambda(cc, pycc_amd.cchbar(cc))
+    lsolve_ms, lecc = _ms(lambda: lam.solve_lambda(1e-10, 1e-10, 100))
+    res["lambda_solve"] = {"pseudoenergy": lecc, "iterations": len(lam.history) - 1, "ms": lsolve_ms,
+                           "ms_per_iter": lsolve_ms / max(1, len(lam.history) - 1)}
+    log("[backward] Lambda solve: %s in %d iterations (%.1f s)" % (lecc, len(lam.history) - 1, lsolve_ms / 1e3))
+    res["value"] = res["factored"]["total_ms"]
     print(json.dumps(res), flush=True)
 
 
