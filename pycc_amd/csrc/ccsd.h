@@ -31,6 +31,7 @@ struct Ctx {
   double* J = nullptr;               // J[(me),(nf)] = (mf|ne)
   float* K32 = nullptr;              // FP32 copies for the mixed ring GEMMs
   float* J32 = nullptr;
+  float* Io32 = nullptr;             // FP32 <mn|ef> for the mixed hole-hole ladder terms
   // packed ladder matrices: rows (e<=f) / (e<f), columns (a<=b)/(a<b) then (m, b)
   void* LP = nullptr;                // double or float
   void* LM = nullptr;
@@ -43,6 +44,7 @@ struct Ctx {
   void *S1 = nullptr, *A1 = nullptr, *S2 = nullptr, *Tx = nullptr, *W = nullptr;
   void *Tp = nullptr, *Tm = nullptr, *CP = nullptr, *CM = nullptr;
   bf16 *Tph = nullptr, *Tpl = nullptr, *Tmh = nullptr, *Tml = nullptr;
+  float *tau32 = nullptr, *W32 = nullptr;   // mixed modes: FP32 tau and Wmnij (hole-hole terms)
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
 

@@ -73,6 +73,7 @@ static void alloc_integrals(Ctx& c) {
   } else {
     c.K32 = c.alloc<float>(ov * ov);
     c.J32 = c.alloc<float>(ov * ov);
+    c.Io32 = c.alloc<float>(ov * ov);
     if (c.mode == MODE_MIXED) {
       c.LP = c.alloc<float>(szP);
       c.LM = c.alloc<float>(szM);
@@ -117,6 +118,7 @@ static void finish_integrals(Ctx& c, const double* fock, hipStream_t s) {
   if (c.mode != MODE_DP) {
     cast<double, float>(ov * ov, c.Covov, c.K32, s);
     cast<double, float>(ov * ov, c.J, c.J32, s);
+    cast<double, float>(ov * ov, c.Ioovv, c.Io32, s);
   }
   if (c.mode == MODE_MIXED_BF16X3) ladder_pack_o_bf16x3(o, v, c.Covvv, c.LPh, c.LPl, c.LMh, c.LMl, s);
   c.have_ints = true;
@@ -205,6 +207,10 @@ void alloc_work(Ctx& c) {
     c.Tp = talloc(noP * npP);
     c.Tm = talloc(noM * npM);
   }
+  if (c.mode != MODE_DP) {
+    c.tau32 = c.alloc<float>(ov * ov);
+    c.W32 = c.alloc<float>(o * o * o * o);
+  }
   c.CP = talloc(noP * (npP + ov));
   c.CM = talloc(noM * (npM + ov));
   c.have_work = true;
@@ -259,7 +265,15 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   axpby<double>(1.0, tv(c.Coooo, "minj", OOOO), 0.0, Wmnij, s);   // <mn|ij> = (mi|nj)
   cd(1.0, T1("je"), tv(c.Cooov, "mine", OOOV), 1.0, Wmnij);        // <mn|ie> = (mi|ne)
   cd(1.0, T1("ie"), tv(c.Cooov, "njme", OOOV), 1.0, Wmnij);        // <mn|ej> = (nj|me)
-  cd(1.0, tv(c.tau, "ijef", OOVV), tv(c.Ioovv, "mnef", OOVV), 1.0, Wmnij);  // <mn|ef>
+  if (std::is_same<T, double>::value) {
+    cd(1.0, tv(c.tau, "ijef", OOVV), tv(c.Ioovv, "mnef", OOVV), 1.0, Wmnij);  // <mn|ef>
+  } else {
+    // mixed modes: the two o^4 v^2 hole-hole ladder contractions (:603 and :930) run in FP32
+    cast<double, float>(ov * ov, c.tau, c.tau32, s);
+    contract<float>(1.0f, tv(c.tau32, "ijef", OOVV), tv(c.Io32, "mnef", OOVV), 0.0f, tv(c.W32, "mnij", OOOO),
+                    ws, s);
+    accumulate(o * o * o * o, 1.0, c.W32, c.Wmnij, s);
+  }
 
   // ---- ring operands and the W intermediates: ccwfn.py:641-645 (Wmbej), :680-683 (Wmbje) ----
   // Wd[0] = Wmbej and Wd[1] = Wmbje, both stored [m][e][j][b].
@@ -347,7 +361,14 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   cd(0.5, T1("je"), Fme, 1.0, Hm);
   cd(-1.0, T2("imab"), Hm, 1.0, Hh);
   // :930  1/2 tau_mnab Wmnij
-  cd(0.5, tv(c.tau, "mnab", OOVV), Wmnij, 1.0, Hh);
+  if (std::is_same<T, double>::value) {
+    cd(0.5, tv(c.tau, "mnab", OOVV), Wmnij, 1.0, Hh);
+  } else {
+    float* X32 = (float*)c.S2;   // S2 is dead after the P ring GEMM
+    cast<double, float>(o * o * o * o, c.Wmnij, c.W32, s);
+    contract<float>(0.5f, tv(c.tau32, "mnab", OOVV), tv(c.W32, "mnij", OOOO), 0.0f, tv(X32, "ijab", OOVV), ws, s);
+    accumulate(ov * ov, 1.0, X32, c.Hh, s);
+  }
   // :932, :937, :938, :940 through Zt
   cd(-1.0, T1("ma"), Zt, 1.0, Hh);
   // :939  t_ie <ab|ej> = t_ie (jb|ae)

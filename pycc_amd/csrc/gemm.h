@@ -42,6 +42,9 @@ struct PlainGemm {
   bool a_kc = true;   // A[m * lda + k] (else A[k * lda + m])
   bool b_nc = true;   // B[k * ldb + n] (else B[n * ldb + k])
   T alpha = 1, beta = 0;
+  // split-K: when ktot > 0, grid.y runs over K chunks of length K (the last one shorter), each
+  // chunk is a "batch" entry (a_b / b_b advance by one chunk) and ktot is the total K.
+  int64_t ktot = 0;
 };
 template <typename T>
 bool plain_gemm_ok(const PlainGemm<T>& p);

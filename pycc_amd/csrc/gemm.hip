@@ -275,6 +275,64 @@ __global__ void splitk_reduce_kernel(GemmParams<T> p, const T* __restrict__ work
   }
 }
 
+// C[m * ldc + n] = alpha * sum_s work[s][m][n] + beta * C, fixed summation order (deterministic).
+template <typename T>
+__global__ void plain_reduce_kernel(int64_t M, int64_t N, int splits, const T* __restrict__ work,
+                                    T* __restrict__ C, int64_t ldc, T alpha, T beta) {
+  const int64_t total = M * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    T s = 0;
+    for (int k = 0; k < splits; ++k) s += work[k * total + e];
+    const int64_t m = e / N, n = e - m * N;
+    T* dst = C + m * ldc + n;
+    T v = alpha * s;
+    if (beta != T(0)) v += beta * *dst;
+    *dst = v;
+  }
+}
+
+// Plain GEMM, split along K into workspace partials when its tile grid cannot fill the chip.
+template <typename T>
+static void plain_dispatch(const PlainGemm<T>& q, Workspace& ws, hipStream_t s) {
+  const int64_t BKp = sizeof(T) == 8 ? 16 : 32;
+  const int64_t tiles = ceil_div(q.M, 128) * ceil_div(q.N, 128);
+  int64_t splits = 1;
+  if (q.batch == 1 && tiles < 768) {
+    splits = std::min<int64_t>(ceil_div(1024, tiles), std::min<int64_t>(q.K / (BKp * 16), 64));
+    const size_t one = (size_t)q.M * q.N * sizeof(T);
+    splits = std::min<int64_t>(splits, (int64_t)(ws.limit / one));
+  }
+  if (splits > 1) {
+    const int64_t kchunk = ceil_div(ceil_div(q.K, splits), BKp) * BKp;
+    splits = ceil_div(q.K, kchunk);
+  }
+  if (std::getenv("CCX_TRACE"))
+    std::fprintf(stderr, "[ccx]   -> plain %s%s batch %lld splits %lld\n", q.a_kc ? "N" : "T", q.b_nc ? "N" : "T",
+                 (long long)q.batch, (long long)splits);
+  if (splits <= 1) {
+    plain_gemm<T>(q, s);
+    return;
+  }
+  const int64_t kchunk = ceil_div(ceil_div(q.K, splits), BKp) * BKp;
+  T* work = (T*)ws.get((size_t)splits * q.M * q.N * sizeof(T));
+  PlainGemm<T> w = q;
+  w.batch = splits;
+  w.ktot = q.K;
+  w.K = kchunk;
+  w.a_b = q.a_kc ? kchunk : kchunk * q.lda;
+  w.b_b = q.b_nc ? kchunk * q.ldb : kchunk;
+  w.C = work;
+  w.ldc = q.N;
+  w.c_b = q.M * q.N;
+  w.alpha = T(1);
+  w.beta = T(0);
+  plain_gemm<T>(w, s);
+  const int blocks = (int)std::min<int64_t>(ceil_div(q.M * q.N, 256), 65536);
+  plain_reduce_kernel<T><<<blocks, 256, 0, s>>>(q.M, q.N, (int)splits, work, q.C, q.ldc, q.alpha, q.beta);
+  CCX_HIP(hipGetLastError());
+}
+
 template <typename T, int BM, int BN, int BK>
 static void launch_tiles(const GemmParams<T>& p, bool akf, bool bkf, T* work, int splits,
                          int64_t kchunk, hipStream_t s) {
@@ -334,8 +392,7 @@ void tc_gemm(const GemmParams<T>& p, Workspace& ws, hipStream_t s) {
     // enough 128 x 128 tiles to fill the 256 CUs: the plain-matrix kernel
     PlainGemm<T> q;
     if (ceil_div(p.M, 128) * ceil_div(p.N, 128) * p.batch >= 240 && as_plain(p, q)) {
-      if (std::getenv("CCX_TRACE")) std::fprintf(stderr, "[ccx]   -> plain %s%s batch %lld\n", q.a_kc ? "N" : "T", q.b_nc ? "N" : "T", (long long)q.batch);
-      plain_gemm<T>(q, s);
+      plain_dispatch<T>(q, ws, s);
       return;
     }
   }

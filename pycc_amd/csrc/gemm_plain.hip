@@ -114,6 +114,9 @@ __global__ __launch_bounds__(256, 2) void plain_gemm_kernel(PlainGemm<T> p) {
   const int64_t tm = first_m + within % gsize, tn = within / gsize;
   const int64_t m0 = tm * BM, n0 = tn * BN;
 
+  // K of this grid row: a split-K chunk may be shorter than p.K
+  const int64_t Kb = p.ktot > 0 ? (p.ktot - (int64_t)blockIdx.y * p.K < p.K ? p.ktot - (int64_t)blockIdx.y * p.K : p.K)
+                                : p.K;
   const T* A = p.A + blockIdx.y * p.a_b;
   const T* B = p.B + blockIdx.y * p.b_b;
   T* C = p.C + blockIdx.y * p.c_b;
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(256, 2) void plain_gemm_kernel(PlainGemm<T> p) {
   lb.init(tid);
 
   auto load = [&](int64_t k0) {
-    const int64_t nk = p.K - k0;
+    const int64_t nk = Kb - k0;
     if (interior && nk >= BK) {
       la.load_fast(a_at(k0), p.lda);
       lb.load_fast(b_at(k0), p.ldb);
@@ -146,14 +149,14 @@ __global__ __launch_bounds__(256, 2) void plain_gemm_kernel(PlainGemm<T> p) {
 #pragma unroll
     for (int j = 0; j < NBJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  if (p.K > 0) {
+  if (Kb > 0) {
     load(0);
     la.store(As[0]);
     lb.store(Bs[0]);
     __syncthreads();
     int buf = 0;
-    for (int64_t k0 = 0; k0 < p.K; k0 += BK) {
-      const bool has_next = k0 + BK < p.K;
+    for (int64_t k0 = 0; k0 < Kb; k0 += BK) {
+      const bool has_next = k0 + BK < Kb;
       if (has_next) load(k0 + BK);
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {

@@ -83,4 +83,7 @@ void sub(int64_t n, const double* x, const double* y, double* out, hipStream_t s
 template <typename Tin, typename Tout>
 void cast(int64_t n, const Tin* in, Tout* out, hipStream_t s);
 
+// out += alpha * in (FP32 partial results folded into FP64 sums)
+void accumulate(int64_t n, double alpha, const float* in, double* out, hipStream_t s);
+
 }  // namespace ccx

@@ -560,6 +560,18 @@ void cast(int64_t n, const Tin* in, Tout* out, hipStream_t s) {
   CCX_HIP(hipGetLastError());
 }
 
+__global__ void accumulate_kernel(int64_t n, double alpha, const float* __restrict__ in, double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] += alpha * (double)in[e];
+}
+
+void accumulate(int64_t n, double alpha, const float* in, double* out, hipStream_t s) {
+  if (!n) return;
+  accumulate_kernel<<<grid_for(n), 256, 0, s>>>(n, alpha, in, out);
+  CCX_HIP(hipGetLastError());
+}
+
 // explicit instantiations
 template void ring_prep<double>(int64_t, int64_t, const double*, const double*, double*, double*, double*,
                                 double*, double*, hipStream_t);

@@ -139,6 +139,120 @@ __global__ __launch_bounds__(256) void bf16x3_gemm_kernel(int64_t M, int64_t N, 
   }
 }
 
+// 256 x 256 tiles, 4 waves (one per SIMD), each computing 128 x 128 (8 x 8 MFMA blocks of 16 x 16).
+// Per launch the operand panels are read (M/256)(N/256)(256 + 256) K times instead of
+// (M/128)(N/128)(128 + 128) K times: half the L2 -> HBM traffic of the 128 x 128 kernel, which is
+// what bounds this GEMM (K ~ 7e4, so each panel is tens of MB). LDS: 2 buffers x (A, B) x (hi, lo)
+// x 256 rows x 40 bf16 = 160 KiB, one workgroup per CU.
+constexpr int BT = 256;
+
+__global__ __launch_bounds__(256, 1) void bf16x3_gemm256_kernel(int64_t M, int64_t N, int64_t Kp,
+                                                               const bf16* __restrict__ Ah,
+                                                               const bf16* __restrict__ Al,
+                                                               const bf16* __restrict__ Bh,
+                                                               const bf16* __restrict__ Bl,
+                                                               float* __restrict__ C, int64_t ldc) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][2][BT][BK + KPAD];   // [buf][hi/lo]
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][2][BT][BK + KPAD];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t tiles_m = (M + BT - 1) / BT, tiles_n = (N + BT - 1) / BT;
+  const int64_t t = xcd_remap_l(blockIdx.x, tiles_m * tiles_n);
+  const int64_t tm = t % tiles_m, tn = t / tiles_m;   // the M tiles of one N panel run together
+
+  // each operand tile (256 rows x 32 k) = 1024 vectors of 16 B; thread loads rows lr + 64 q, q < 4
+  const int lr = tid >> 2, lk = (tid & 3) * 8;
+  int64_t offA[4], offB[4];   // element offsets of this thread's rows at k = 0
+  bool okA[4], okB[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t m = tm * BT + lr + 64 * q, n = tn * BT + lr + 64 * q;
+    okA[q] = m < M;
+    okB[q] = n < N;
+    // rows beyond the matrix read row 0 (valid memory) and are zeroed before the LDS store
+    offA[q] = (okA[q] ? m : 0) * Kp + lk;
+    offB[q] = (okB[q] ? n : 0) * Kp + lk;
+  }
+  const bf16x8 zero8 = bf16x8{};
+  bf16x8 rA[2][4], rB[2][4];
+
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      rA[0][q] = *(const bf16x8*)(Ah + offA[q] + k0);
+      rA[1][q] = *(const bf16x8*)(Al + offA[q] + k0);
+      rB[0][q] = *(const bf16x8*)(Bh + offB[q] + k0);
+      rB[1][q] = *(const bf16x8*)(Bl + offB[q] + k0);
+    }
+  };
+  auto sstore = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        *(bf16x8*)&sA[b][h][lr + 64 * q][lk] = okA[q] ? rA[h][q] : zero8;
+        *(bf16x8*)&sB[b][h][lr + 64 * q][lk] = okB[q] ? rB[h][q] : zero8;
+      }
+  };
+
+  f4 acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  const int fr = lane >> 4, fc = lane & 15;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < Kp; k0 += BK) {
+    const bool has_next = k0 + BK < Kp;
+    if (has_next) gload(k0 + BK);
+    bf16x8 a[8], b[8];
+    // lo(A) x hi(B)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = *(const bf16x8*)&sA[buf][1][wm * 128 + i * 16 + fc][fr * 8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = *(const bf16x8*)&sB[buf][0][wn * 128 + j * 16 + fc][fr * 8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    // hi(A) x hi(B)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = *(const bf16x8*)&sA[buf][0][wm * 128 + i * 16 + fc][fr * 8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    // hi(A) x lo(B)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = *(const bf16x8*)&sB[buf][1][wn * 128 + j * 16 + fc][fr * 8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (has_next) sstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int64_t n = tn * BT + wn * 128 + j * 16 + fc;
+    if (n >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = tm * BT + wm * 128 + i * 16 + fr * 4 + r;
+        if (m < M) C[m * ldc + n] = acc[i][j][r];
+      }
+  }
+}
+
 void bf16x3_gemm(int64_t M, int64_t N, int64_t Kp, const bf16* Ah, const bf16* Al, const bf16* Bh,
                  const bf16* Bl, float* C, int64_t ldc, hipStream_t s) {
   if (M == 0 || N == 0) return;
@@ -149,7 +263,12 @@ void bf16x3_gemm(int64_t M, int64_t N, int64_t Kp, const bf16* Ah, const bf16* A
     scale<float>(0.f, tvs(C, "mn", {M, N}, {ldc, 1}), s);
     return;
   }
-  bf16x3_gemm_kernel<<<dim3((unsigned)tiles), 256, 0, s>>>(M, N, Kp, Ah, Al, Bh, Bl, C, ldc);
+  if (M >= BT) {
+    const int64_t tiles256 = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
+    bf16x3_gemm256_kernel<<<dim3((unsigned)tiles256), 256, 0, s>>>(M, N, Kp, Ah, Al, Bh, Bl, C, ldc);
+  } else {
+    bf16x3_gemm_kernel<<<dim3((unsigned)tiles), 256, 0, s>>>(M, N, Kp, Ah, Al, Bh, Bl, C, ldc);
+  }
   CCX_HIP(hipGetLastError());
 }
 

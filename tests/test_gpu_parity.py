@@ -161,3 +161,21 @@ def test_mixed_precision_residual_and_energy(pycc_amd, precision, tol_r, r_conv)
         precision, e - e_ref, np.abs(cc.t1.cpu().numpy() - ref.t1).max(),
         np.abs(cc.t2.cpu().numpy() - ref.t2).max(), cc.niter, ref.niter))
     assert abs(e - e_ref) < 1e-6
+
+
+@pytest.mark.parametrize("precision,tol_r", [("MIXED", 1e-5), ("MIXED_BF16X3", 3e-4)])
+def test_mixed_residual_large_pair_count(pycc_amd, precision, tol_r):
+    """o = 24 gives 300 occupied pairs: the ladder takes the 256 x 256 BF16x3 tiles (with partial edge
+    tiles in M and N) and the hole-hole terms the FP32 split-K path."""
+    no, nv = 24, 40
+    print("large pair count", precision, no, nv, flush=True)
+    sd, H = _ham(no, nv, 17, t2max=0.03)
+    ref = OracleCCSD(H, no)
+    t1, t2 = random_amplitudes(no, nv, 17, scale=0.02)
+    r1_ref, r2_ref = ref.residuals(H.F, t1, t2)
+    cc = pycc_amd.CCwfn(H, no=no, precision=precision, verbose=False)
+    r1, r2 = cc.residuals(H.F, t1, t2)
+    rel2 = np.abs(r2.cpu().numpy() - r2_ref).max() / np.abs(r2_ref).max()
+    rel1 = np.abs(r1.cpu().numpy() - r1_ref).max() / np.abs(r1_ref).max()
+    print("%s residual rel err r1 %.2e r2 %.2e" % (precision, rel1, rel2))
+    assert rel2 < tol_r and rel1 < tol_r
