@@ -31,3 +31,12 @@ def random_amplitudes(no, nv, seed, scale=0.05):
 @pytest.fixture
 def amplitudes():
     return random_amplitudes
+
+
+def reference_constant(system, quantity):
+    """(value, tolerance) of a known answer held by the reference's tests (tests/golden)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_constants.json")) as fh:
+        d = json.load(fh)[system][quantity]
+    return d["value"], d["tolerance"]

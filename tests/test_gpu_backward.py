@@ -10,7 +10,7 @@ pseudoenergy to 1e-10 Eh.
 import numpy as np
 import pytest
 
-from conftest import random_amplitudes
+from conftest import random_amplitudes, reference_constant
 from oracle.ccsd_ref import OracleCCSD
 from oracle.lambda_ref import OracleHBAR, OracleLambda
 from pycc_amd.synth import synthetic_df, DenseHamiltonian, dirac_eri
@@ -243,7 +243,7 @@ def test_lambda_by_iterating_the_vjp(pycc_amd):
 
 
 def test_water_sto3g_lambda_constant_on_gpu(pycc_amd):
-    """pycc tests/test_003_ccsd_lambda.py:50 (H2O/STO-3G, frozen core): -0.068826452648939."""
+    """pycc tests/test_003_ccsd_lambda.py:49 (H2O/STO-3G, frozen core): -0.068826452648939."""
     from oracle.gaussian import water_sto3g_hamiltonian
     F, ERI, escf, nocc = water_sto3g_hamiltonian()
     H = DenseHamiltonian(F, ERI)
@@ -252,7 +252,8 @@ def test_water_sto3g_lambda_constant_on_gpu(pycc_amd):
     for stored in (True, False):
         lam = pycc_amd.cclambda(cc, pycc_amd.cchbar(cc, store_vvvv=stored))
         lecc = lam.solve_lambda(1e-12, 1e-12)
-        assert abs(lecc - (-0.068826452648939)) < 1e-11, (stored, lecc)
+        ref, tol = reference_constant("water_sto3g_frozen_core", "lambda_pseudoenergy")
+        assert abs(lecc - ref) < tol, (stored, lecc)
 
 
 def test_backward_requires_fp64_context(pycc_amd):

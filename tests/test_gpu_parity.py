@@ -8,7 +8,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import random_amplitudes
+from conftest import random_amplitudes, reference_constant
 from oracle.ccsd_ref import OracleCCSD
 from pycc_amd.synth import synthetic_df, DenseHamiltonian, dirac_eri
 
@@ -129,7 +129,8 @@ def test_water_sto3g_reference_constant_on_gpu(pycc_amd):
     H = DenseHamiltonian(F, ERI)
     cc = pycc_amd.CCwfn(H, no=nocc - 1, nfzc=1, verbose=False)
     e = cc.solve_cc(1e-12, 1e-12, 75)
-    assert abs(e - (-0.070616830152761)) < 1e-11      # pycc tests/test_002_ccsd_energy.py:30
+    ref, tol = reference_constant("water_sto3g_frozen_core", "ccsd_correlation_energy")
+    assert abs(e - ref) < tol      # pycc tests/test_002_ccsd_energy.py:31
 
 
 def test_solve_returns_none_when_not_converged(pycc_amd):
