@@ -13,6 +13,8 @@
 // is loaded into registers while the current one is consumed from LDS.
 #include "common.h"
 #include "gemm.h"
+#include <cstdlib>
+#include <type_traits>
 
 namespace ccx {
 
@@ -20,6 +22,16 @@ typedef double d4p __attribute__((ext_vector_type(4)));
 typedef float f4p __attribute__((ext_vector_type(4)));
 
 namespace {
+
+// CCX_PLAIN_TILE=128 or 256 forces the FP32 tile size (measurement of the two variants).
+int plain_tile_override() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("CCX_PLAIN_TILE");
+    v = e ? std::atoi(e) : 0;
+  }
+  return v;
+}
 
 template <typename T> struct PAcc;
 template <> struct PAcc<double> { typedef d4p type; };
@@ -44,7 +56,10 @@ __device__ __forceinline__ int64_t pxcd_remap(int64_t w, int64_t ntiles) {
 template <typename T, int R, int BK, bool KC>
 struct alignas(16) PTile {
   static constexpr int VEC = 16 / sizeof(T);
-  static constexpr int PAD = VEC;                       // keeps every LDS row 16-byte aligned
+  // Row padding (keeps every LDS row 16-byte aligned). A k-contiguous tile is read along its rows
+  // (16 lanes on 16 rows) and VEC suffices; a row-index-contiguous tile is read by 4 lane groups on
+  // 4 consecutive k rows, which must then start 16 banks apart: R + 16 elements per row.
+  static constexpr int PAD = KC ? VEC : 16;
   static constexpr int NV = R * BK / VEC / 256;         // 16-byte vectors per thread
   T d[KC ? R : BK][(KC ? BK : R) + PAD];
 };
@@ -93,9 +108,11 @@ struct PLoader {
   }
 };
 
-template <typename T, int BK, bool A_KC, bool B_NC>
-__global__ __launch_bounds__(256, 2) void plain_gemm_kernel(PlainGemm<T> p) {
-  constexpr int BM = 128, BN = 128, WM = 64, WN = 64, NBI = 4, NBJ = 4;
+// BT = 128: 4 waves of 64 x 64, two workgroups per CU. BT = 256 (FP32 only): 4 waves of 128 x 128
+// (256 accumulator registers each), one workgroup per CU, 4x the MFMAs per LDS read and per barrier.
+template <typename T, int BK, bool A_KC, bool B_NC, int BT>
+__global__ __launch_bounds__(256, BT == 128 ? 2 : 1) void plain_gemm_kernel(PlainGemm<T> p) {
+  constexpr int BM = BT, BN = BT, WM = BT / 2, WN = BT / 2, NBI = WM / 16, NBJ = WN / 16;
   typedef typename PAcc<T>::type acc_t;
   __shared__ PTile<T, BM, BK, A_KC> As[2];
   __shared__ PTile<T, BN, BK, !B_NC> Bs[2];
@@ -132,24 +149,28 @@ __global__ __launch_bounds__(256, 2) void plain_gemm_kernel(PlainGemm<T> p) {
   la.init(tid);
   lb.init(tid);
 
-  auto load = [&](int64_t k0) {
-    const int64_t nk = Kb - k0;
-    if (interior && nk >= BK) {
-      la.load_fast(a_at(k0), p.lda);
-      lb.load_fast(b_at(k0), p.ldb);
-    } else {
-      la.load_safe(a_at(k0), p.lda, a_rows, nk);
-      lb.load_safe(b_at(k0), p.ldb, b_rows, nk);
-    }
-  };
-
   acc_t acc[NBI][NBJ];
 #pragma unroll
   for (int i = 0; i < NBI; ++i)
 #pragma unroll
     for (int j = 0; j < NBJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  if (Kb > 0) {
+  // The k loop, compiled twice: FAST (an interior tile and K a multiple of BK, so every load is a
+  // full in-bounds vector) and checked. Keeping the checked loads out of the FAST loop lets the
+  // prefetch of tile k + 1 stay in flight while tile k is multiplied (a shared loop forces a full
+  // wait on the loads at the join of the two load paths).
+  auto kloop = [&](auto fast_tag) {
+    constexpr bool FAST = decltype(fast_tag)::value;
+    auto load = [&](int64_t k0) {
+      if constexpr (FAST) {
+        la.load_fast(a_at(k0), p.lda);
+        lb.load_fast(b_at(k0), p.ldb);
+      } else {
+        const int64_t nk = Kb - k0;
+        la.load_safe(a_at(k0), p.lda, a_rows, nk);
+        lb.load_safe(b_at(k0), p.ldb, b_rows, nk);
+      }
+    };
     load(0);
     la.store(As[0]);
     lb.store(Bs[0]);
@@ -179,6 +200,10 @@ __global__ __launch_bounds__(256, 2) void plain_gemm_kernel(PlainGemm<T> p) {
       __syncthreads();
       buf ^= 1;
     }
+  };
+  if (Kb > 0) {
+    if (interior && Kb % BK == 0) kloop(std::true_type{});
+    else kloop(std::false_type{});
   }
 
 #pragma unroll
@@ -200,11 +225,28 @@ __global__ __launch_bounds__(256, 2) void plain_gemm_kernel(PlainGemm<T> p) {
   }
 }
 
+// 256 x 256 tiles for large FP32 GEMMs (enough tiles to fill the 256 CUs several times over).
+template <typename T>
+bool plain_big_tiles(const PlainGemm<T>& p) {
+  if (sizeof(T) != 4 || plain_tile_override() == 128) return false;
+  return plain_tile_override() == 256 || ceil_div(p.M, 256) * ceil_div(p.N, 256) * p.batch >= 1024;
+}
+
 template <typename T, bool A_KC, bool B_NC>
 void launch_plain(const PlainGemm<T>& p, hipStream_t s) {
   constexpr int BK = sizeof(T) == 8 ? 16 : 32;
-  int64_t tiles = ceil_div(p.M, 128) * ceil_div(p.N, 128);
-  plain_gemm_kernel<T, BK, A_KC, B_NC><<<dim3((unsigned)tiles, (unsigned)p.batch), 256, 0, s>>>(p);
+  if constexpr (sizeof(T) == 4) {
+    if (plain_big_tiles(p)) {
+      int64_t tiles = ceil_div(p.M, 256) * ceil_div(p.N, 256);
+      plain_gemm_kernel<T, BK, A_KC, B_NC, 256><<<dim3((unsigned)tiles, (unsigned)p.batch), 256, 0, s>>>(p);
+      CCX_HIP(hipGetLastError());
+      return;
+    }
+  }
+  {
+    int64_t tiles = ceil_div(p.M, 128) * ceil_div(p.N, 128);
+    plain_gemm_kernel<T, BK, A_KC, B_NC, 128><<<dim3((unsigned)tiles, (unsigned)p.batch), 256, 0, s>>>(p);
+  }
   CCX_HIP(hipGetLastError());
 }
 

@@ -180,3 +180,18 @@ def test_mixed_residual_large_pair_count(pycc_amd, precision, tol_r):
     rel1 = np.abs(r1.cpu().numpy() - r1_ref).max() / np.abs(r1_ref).max()
     print("%s residual rel err r1 %.2e r2 %.2e" % (precision, rel1, rel2))
     assert rel2 < tol_r and rel1 < tol_r
+
+
+def test_contract_f32_large_tiles(pycc_amd):
+    """8200 x 8200 output: the FP32 plain GEMM takes its 256 x 256 tiles (1089 tiles, partial edge
+    tiles in M and N), with a K that is not a multiple of the 32-wide k tile."""
+    print("f32 256-tile GEMM", flush=True)
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((8200, 72))
+    B = rng.standard_normal((72, 8200))
+    mgr = pycc_amd.DeviceManager('GPU', 'DP')
+    for sub, a, b in (("ij,jk->ik", A, B), ("ji,jk->ik", A.T.copy(), B)):
+        out = mgr.contract(sub, torch.tensor(a, dtype=torch.float32, device="cuda"),
+                           torch.tensor(b, dtype=torch.float32, device="cuda"))
+        ref = np.einsum(sub, a, b)
+        assert np.abs(out.cpu().double().numpy() - ref).max() < 2e-4 * np.abs(ref).max()
