@@ -126,6 +126,13 @@ int ccx_pseudoenergy(ccx_ctx* ctx, const double* l2, double* e_dev, void* stream
 int ccx_vjp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, const double* c1,
             const double* c2, double* g1, double* g2, int flags, void* stream);
 
+/* Jacobian-vector product of ccx_residual at (fock, t1, t2) along (x1, x2): j1, j2 = the derivative of
+ * (r1, r2) in that direction, the CCSD Jacobian applied on HBAR (CCderiv._ccsd_jacobian,
+ * pycc/ccderiv.py:149-180, with its doubles symmetrised as ccderiv.py:257-258 does). Only the
+ * pair-symmetric part of x2 acts. flags as for ccx_vjp. Outputs must not alias inputs. */
+int ccx_jvp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, const double* x1,
+            const double* x2, double* j1, double* j2, int flags, void* stream);
+
 /* Measurement hooks (no counterpart in the reference, which has only wall-clock prints,
  * pycc/utils.py:229-238): HIP events recorded on the residual's stream around the ladder GEMMs,
  * the ring GEMMs and each whole residual.

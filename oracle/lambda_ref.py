@@ -11,6 +11,7 @@ Follows, term by term:
 * ``pycc/cclambda.py:202-256`` ``residuals`` (HBAR rebuilt from arbitrary t1, t2)
 * ``pycc/cclambda.py:258-306`` ``build_Goo``, ``build_Gvv``
 * ``pycc/cclambda.py:345-369`` ``r_L1``; ``:468-496`` ``r_L2``; ``:570`` ``pseudoenergy``
+* ``pycc/ccderiv.py:149-180`` ``_ccsd_jacobian`` (the Jacobian-vector product on HBAR)
 """
 from __future__ import annotations
 
@@ -231,3 +232,28 @@ class OracleLambda:
             if niter >= start_diis:
                 self.l1, self.l2 = diis.extrapolate(self.l1, self.l2)
         return None
+
+
+def ccsd_jacobian(cc: OracleCCSD, h: OracleHBAR, X1, X2, t2=None):
+    """``CCderiv._ccsd_jacobian`` (pycc/ccderiv.py:149-180): (HBAR . X) on singles and (un-symmetrised)
+    doubles. The derivative of the residual along X is (j1, j2 + j2^T)."""
+    o, v = cc.o, cc.v
+    t2 = cc.t2 if t2 is None else t2
+    L = cc.H.L
+    Hvvmvo = 2.0 * h.Hvovv - h.Hvovv.swapaxes(2, 3)                                   # :163
+    Hooomv = 2.0 * h.Hooov - h.Hooov.swapaxes(0, 1)                                   # :164
+    r1 = contract('ie,ae->ia', X1, h.Hvv) - contract('ma,mi->ia', X1, h.Hoo)          # :165
+    r1 = r1 + 2.0 * contract('me,maei->ia', X1, h.Hovvo) - contract('me,maie->ia', X1, h.Hovov)   # :166
+    r1 = r1 + contract('me,miea->ia', h.Hov, (2.0 * X2 - X2.swapaxes(0, 1)))         # :167
+    r1 = r1 + contract('imef,amef->ia', X2, Hvvmvo)                                   # :168
+    r1 = r1 - contract('mnae,mnie->ia', X2, Hooomv)                                   # :169
+    Zvv = contract('amef,mf->ae', Hvvmvo, X1) - contract('mnef,mnaf->ae', L[o, o, v, v], X2)     # :170
+    Zoo = -1.0 * contract('mnie,ne->mi', Hooomv, X1) - contract('mnef,inef->mi', L[o, o, v, v], X2)  # :171
+    r2 = contract('ie,abej->ijab', X1, h.Hvvvo) - contract('ma,mbij->ijab', X1, h.Hovoo)          # :172
+    r2 = r2 + contract('mi,mjab->ijab', Zoo, t2) + contract('ae,ijeb->ijab', Zvv, t2)             # :173
+    r2 = r2 + contract('ijeb,ae->ijab', X2, h.Hvv) - contract('mjab,mi->ijab', X2, h.Hoo)         # :174
+    r2 = r2 + 0.5 * contract('mnab,mnij->ijab', X2, h.Hoooo) + 0.5 * contract('ijef,abef->ijab', X2, h.Hvvvv)  # :175
+    r2 = r2 - contract('imeb,maje->ijab', X2, h.Hovov) - contract('imea,mbej->ijab', X2, h.Hovvo)  # :176
+    r2 = r2 + 2.0 * contract('miea,mbej->ijab', X2, h.Hovvo) - contract('miea,mbje->ijab', X2, h.Hovov)  # :177
+    return r1, r2
+

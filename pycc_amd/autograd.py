@@ -1,7 +1,8 @@
 """Differentiable CCSD residual and energy (``torch.autograd.Function`` wrappers).
 
 ``residual(cc, t1, t2, F=None)`` returns ``(r1, r2) = cc.residuals(F, t1, t2)`` as autograd outputs.
-Its backward is the vector-Jacobian product kernel set of the library: the Lambda-equation
+Its forward-mode derivative (``jvp``) is the CCSD Jacobian on HBAR (``pycc/ccderiv.py:149-180``). Its
+backward is the vector-Jacobian product kernel set of the library: the Lambda-equation
 contractions of ``pycc/cclambda.py:345-369`` (r_L1) and ``:468-496`` (r_L2) without their source
 terms, evaluated at l1 = c1, l2 = c2 + c2^T for the cotangent (c1, c2), on an HBAR
 (``pycc/cchbar.py:128-823``) built at the forward's amplitudes. The t2 gradient is pair-symmetrised:
@@ -27,7 +28,15 @@ class CCSDResidual(torch.autograd.Function):
         ctx.F = F
         ctx.store_vvvv = store_vvvv
         ctx.save_for_backward(t1, t2)
+        ctx.save_for_forward(t1, t2)
         return r1, r2
+
+    @staticmethod
+    def jvp(ctx, dt1, dt2, _cc, _F, _store):
+        t1, t2 = ctx.saved_tensors
+        dt1 = torch.zeros_like(t1) if dt1 is None else dt1.contiguous()
+        dt2 = torch.zeros_like(t2) if dt2 is None else dt2.contiguous()
+        return ctx.cc.jvp(t1.detach(), t2.detach(), dt1, dt2, F=ctx.F, store_vvvv=ctx.store_vvvv)
 
     @staticmethod
     def backward(ctx, c1, c2):

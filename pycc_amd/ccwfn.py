@@ -301,6 +301,23 @@ class CCwfn(object):
                   _lib.VJP_REUSE_HBAR, self._stream())
         return g1, g2
 
+    def jvp(self, t1, t2, x1, x2, F=None, store_vvvv=False):
+        """(j1, j2) = the derivative of ``residuals(F, t1, t2)`` along (x1, x2): the CCSD Jacobian
+        applied on HBAR (``CCderiv._ccsd_jacobian``, ccderiv.py:149-180, doubles symmetrised). Only the
+        pair-symmetric part of x2 acts; F is held constant."""
+        torch = self._torch
+        F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
+        t1 = self._as_dev(t1, (self.no, self.nv))
+        t2 = self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        x1 = self._as_dev(x1, (self.no, self.nv))
+        x2 = self._as_dev(x2, (self.no, self.no, self.nv, self.nv))
+        self._hbar_for(F, t1, t2, store_vvvv)
+        j1 = torch.empty_like(x1)
+        j2 = torch.empty_like(x2)
+        _lib.call("ccx_jvp", self._ctx.h, None, None, None, dptr(x1), dptr(x2), dptr(j1), dptr(j2),
+                  _lib.VJP_REUSE_HBAR, self._stream())
+        return j1, j2
+
     def energy_gradient(self, t1=None, t2=None, F=None):
         """(dE/dt1, dE/dt2) of ``cc_energy`` = (2 Hov, L): the source terms of the Lambda equations
         (cclambda.py:348, :468)."""

@@ -217,6 +217,23 @@ int ccx_vjp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2
   });
 }
 
+int ccx_jvp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, const double* x1,
+            const double* x2, double* j1, double* j2, int flags, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx && x1 && x2 && j1 && j2, ccx::ERR_ARG, "null argument");
+    CCX_CHECK((flags & ~(CCX_HBAR_STORED | CCX_VJP_REUSE_HBAR)) == 0, ccx::ERR_ARG, "unknown ccx_jvp flag");
+    auto& c = ctx->c;
+    hipStream_t s = st(stream);
+    if (flags & CCX_VJP_REUSE_HBAR)
+      CCX_CHECK(c.have_hbar, ccx::ERR_STATE, "CCX_VJP_REUSE_HBAR without a built HBAR");
+    else
+      ccx::hbar_build(c, fock, t1, t2, (flags & CCX_HBAR_STORED) != 0, s);
+    // the pair-symmetric part of x2 (t2_ijab and t2_jiba are one parameter)
+    ccx::pair_symmetrize(c.o, c.v, 0.5, x2, c.Lsym, s);
+    ccx::jacobian(c, x1, c.Lsym, j1, j2, s);
+  });
+}
+
 int ccx_timing_start(ccx_ctx* ctx) {
   return guarded([&] {
     CCX_CHECK(ctx, ccx::ERR_ARG, "null context");

@@ -88,6 +88,7 @@ struct Ctx {
   double *Hvvvv = nullptr, *Hvovv = nullptr, *Hvvvo = nullptr;   // stored HBAR only
   double *Ltmp = nullptr, *Goo = nullptr, *Gvv = nullptr, *Lh = nullptr, *Lsym = nullptr;
   double *LY = nullptr, *LVf = nullptr, *LVg = nullptr, *LU = nullptr, *LQ = nullptr, *Lsm = nullptr;
+  double* Jsm = nullptr;
 
   // ---- DIIS (pycc utils.helper_diis) ----
   int max_diis = 0, cap = 0;
@@ -130,6 +131,8 @@ void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, dou
 // out[i][j][a][b] = alpha * (h[i][j][a][b] + h[j][i][b][a])
 void pair_symmetrize(int64_t o, int64_t v, double alpha, const double* h, double* out, hipStream_t s);
 void pseudoenergy(Ctx& c, const double* l2, double* e_dev, hipStream_t s);
+// j1, j2 = the CCSD Jacobian applied to (x1, x2) on the current HBAR (x2 pair-symmetric)
+void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2, hipStream_t s);
 void timing_start(Ctx& c);
 void timing_read(Ctx& c, double* out, int n);
 void diis_reset(Ctx& c, const double* t1, const double* t2, int max_diis, int cap, hipStream_t s);

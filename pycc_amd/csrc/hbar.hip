@@ -92,6 +92,20 @@ __global__ void ladder_unpack_kernel(int64_t o, int64_t v, const double* __restr
   }
 }
 
+// Z[i][j][e][f] += x_ie t_jf + t_ie x_jf (the t1 part of a Jacobian direction, pair-symmetric)
+__global__ void pair_outer_kernel(int64_t o, int64_t v, const double* __restrict__ x, const double* __restrict__ t,
+                                  double* __restrict__ Z) {
+  const int64_t total = o * o * v * v;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t f = e % v, r = e / v;
+    int64_t a = r % v;
+    r /= v;
+    int64_t j = r % o, i = r / o;
+    Z[e] += x[i * v + a] * t[j * v + f] + t[i * v + a] * x[j * v + f];
+  }
+}
+
 }  // namespace
 
 void pair_symmetrize(int64_t o, int64_t v, double alpha, const double* h, double* out, hipStream_t s) {
@@ -125,6 +139,7 @@ static void alloc_hbar(Ctx& c, bool stored) {
     c.LU = c.alloc<double>(o * o * o * o);
     c.LQ = c.alloc<double>(o * o * ov);
     c.Lsm = c.alloc<double>(2 * o * o + 2 * ov);
+    c.Jsm = c.alloc<double>(v * v + 4 * o * o + 2 * ov);
     c.have_hbar_bufs = true;
   }
   if (stored && !c.Hvvvv) {
@@ -451,6 +466,148 @@ void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, dou
   }
 
   pair_symmetrize(o, v, alpha, c.Lh, r2, s);                                     // :496
+}
+
+// The CCSD Jacobian applied to (x1, x2) on the current HBAR (CCderiv._ccsd_jacobian,
+// ccderiv.py:149-180): j1 = sigma1, j2 = sigma2 + sigma2^T, the derivative of the residual along x.
+// x2 must be pair-symmetric.
+void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2, hipStream_t s) {
+  CCX_CHECK(c.have_hbar, ERR_STATE, "the Jacobian needs an HBAR (ccx_hbar_build)");
+  CCX_CHECK(x1 && x2 && j1 && j2, ERR_ARG, "null argument");
+  const int64_t o = c.o, v = c.v, ov = o * v;
+  Workspace& ws = c.ws;
+  const std::vector<int64_t> OV{o, v}, OO{o, o}, VV{v, v}, OOVV{o, o, v, v}, OVOV{o, v, o, v};
+  const std::vector<int64_t> OOOV{o, o, o, v}, OOOO{o, o, o, o}, OVVV{o, v, v, v};
+  const std::vector<int64_t> VOVV{v, o, v, v}, OVVO{o, v, v, o}, VVVO{v, v, v, o}, OVOO{o, v, o, o};
+  auto cd = [&](double a, const TV& A, const TV& B, double b, const TV& C) {
+    contract<double>(a, A, B, b, C, ws, s);
+  };
+  auto X1 = [&](const char* l) { return tv(x1, l, OV); };
+  auto X2 = [&](const char* l) { return tv(x2, l, OOVV); };
+  auto T1 = [&](const char* l) { return tv(c.Ht1, l, OV); };
+  auto T2 = [&](const char* l) { return tv(c.Ht2, l, OOVV); };
+  auto CVVV = [&](const char* l) { return tv(c.Covvv, l, OVVV); };
+  auto COOV = [&](const char* l) { return tv(c.Cooov, l, OOOV); };
+  auto COVV = [&](const char* l) { return tv(c.Coovv, l, OOVV); };
+  auto COVO = [&](const char* l) { return tv(c.Covov, l, OVOV); };
+  auto IOVV = [&](const char* l) { return tv(c.Ioovv, l, OOVV); };
+  auto LOVV = [&](const char* l) { return tv(c.Loovv, l, OOVV); };
+  auto HOOV = [&](const char* l) { return tv(c.Hooov, l, OOOV); };
+  const TV Hov = tv(c.Hov, "me", OV);
+  double* sm = c.Jsm;
+  const TV Zvv = tv(sm, "ae", VV), Zoo = tv(sm + v * v, "mi", OO);
+
+  // ---- sigma1: ccderiv.py:165-169 ----
+  const TV R1 = tv(j1, "ia", OV);
+  cd(1.0, X1("ie"), tv(c.Hvv, "ae", VV), 0.0, R1);                                    // :165
+  cd(-1.0, X1("ma"), tv(c.Hoo, "mi", OO), 1.0, R1);
+  cd(2.0, X1("me"), tv(c.Hovvo, "maei", OVVO), 1.0, R1);                              // :166
+  cd(-1.0, X1("me"), tv(c.Hovov, "maie", OVOV), 1.0, R1);
+  cd(2.0, Hov, X2("miea"), 1.0, R1);                                                  // :167
+  cd(-1.0, Hov, X2("imea"), 1.0, R1);
+  cd(-2.0, X2("mnae"), HOOV("mnie"), 1.0, R1);                                        // :169
+  cd(1.0, X2("mnae"), HOOV("nmie"), 1.0, R1);
+  // Zvv, Zoo: :170-171 (the Hvvmvo parts below, by HBAR form)
+  cd(-1.0, LOVV("mnef"), X2("mnaf"), 0.0, Zvv);
+  cd(-2.0, HOOV("mnie"), X1("ne"), 0.0, Zoo);
+  cd(1.0, HOOV("nmie"), X1("ne"), 1.0, Zoo);
+  cd(-1.0, LOVV("mnef"), X2("inef"), 1.0, Zoo);
+
+  // ---- sigma2 before the final P(ij,ab): ccderiv.py:172-177 ----
+  const TV H = tv(c.Lh, "ijab", OOVV);
+  cd(-1.0, X1("ma"), tv(c.Hovoo, "mbij", OVOO), 0.0, H);                              // :172
+  cd(1.0, X2("ijeb"), tv(c.Hvv, "ae", VV), 1.0, H);                                   // :174
+  cd(-1.0, X2("mjab"), tv(c.Hoo, "mi", OO), 1.0, H);
+  cd(0.5, X2("mnab"), tv(c.Hoooo, "mnij", OOOO), 1.0, H);                             // :175
+  cd(-1.0, X2("imeb"), tv(c.Hovov, "maje", OVOV), 1.0, H);                            // :176
+  cd(-1.0, X2("imea"), tv(c.Hovvo, "mbej", OVVO), 1.0, H);
+  cd(2.0, X2("miea"), tv(c.Hovvo, "mbej", OVVO), 1.0, H);                             // :177
+  cd(-1.0, X2("miea"), tv(c.Hovov, "mbje", OVOV), 1.0, H);
+
+  if (c.hbar_stored) {
+    // Hvvmvo = 2 Hvovv - Hvovv^(ef) (:163)
+    cd(2.0, X2("imef"), tv(c.Hvovv, "amef", VOVV), 1.0, R1);                          // :168
+    cd(-1.0, X2("imef"), tv(c.Hvovv, "amfe", VOVV), 1.0, R1);
+    cd(2.0, tv(c.Hvovv, "amef", VOVV), X1("mf"), 1.0, Zvv);                           // :170
+    cd(-1.0, tv(c.Hvovv, "amfe", VOVV), X1("mf"), 1.0, Zvv);
+    cd(1.0, X1("ie"), tv(c.Hvvvo, "abej", VVVO), 1.0, H);                             // :172
+    cd(0.5, X2("ijef"), tv(c.Hvvvv, "abef", {v, v, v, v}), 1.0, H);                   // :175
+  } else {
+    // Hvvmvo_amef = L_amef - t_na L_nmef with L_amef = 2 (mf|ae) - (me|af)
+    const TV Win = tv(sm + v * v + o * o, "in", OO), q = tv(sm + v * v + 2 * o * o, "ne", OV);
+    cd(2.0, X2("imef"), CVVV("mfae"), 1.0, R1);                                       // :168
+    cd(-1.0, X2("imef"), CVVV("meaf"), 1.0, R1);
+    cd(1.0, X2("imef"), LOVV("nmef"), 0.0, Win);
+    cd(-1.0, T1("na"), Win, 1.0, R1);
+    cd(2.0, CVVV("mfae"), X1("mf"), 1.0, Zvv);                                        // :170
+    cd(-1.0, CVVV("meaf"), X1("mf"), 1.0, Zvv);
+    cd(1.0, X1("mf"), LOVV("nmef"), 0.0, q);
+    cd(-1.0, T1("na"), q, 1.0, Zvv);
+    // x_ie Hvvvo_abej over cchbar.py:682-695 (its Hvvvv part joins the x2 ladder below)
+    build_tau(o, v, 1.0, 1.0, c.Ht1, c.Ht2, c.tau, s);
+    auto TAU = [&](const char* l) { return tv(c.tau, l, OOVV); };
+    cd(1.0, X1("ie"), CVVV("jbae"), 1.0, H);                                          // <ab|ej> = (jb|ae)
+    const TV u = tv(sm + v * v + 3 * o * o, "im", OO);                                // -Hov_me t_mjab
+    cd(1.0, X1("ie"), Hov, 0.0, u);
+    cd(-1.0, u, T2("mjab"), 1.0, H);
+    const TV V = tv(c.LU, "mnij", OOOO);                                              // tau_mnab <mn|ej>
+    cd(1.0, X1("ie"), COOV("njme"), 0.0, V);
+    cd(1.0, TAU("mnab"), V, 1.0, H);
+    const TV G = tv(c.Ltmp, "imbf", OOVV);                                            // -t_jmfa <bm|fe>
+    cd(1.0, X1("ie"), CVVV("mebf"), 0.0, G);
+    cd(-1.0, T2("jmfa"), G, 1.0, H);
+    const TV G2 = tv(c.Ltmp, "imfa", OOVV), G4 = tv(c.LY, "imfa", OOVV);
+    cd(1.0, X1("ie"), CVVV("mfae"), 0.0, G2);                                         // x_ie <am|ef>
+    cd(-1.0, T2("jmfb"), G2, 1.0, H);                                                 // -t_jmfb <am|ef>
+    cd(2.0, T2("mjfb"), G2, 1.0, H);                                                  // +t_mjfb L_amef
+    cd(1.0, X1("ie"), CVVV("meaf"), 0.0, G4);
+    cd(-1.0, T2("mjfb"), G4, 1.0, H);
+    const TV K1 = tv(c.LVf, "imja", OOOV), K2 = tv(c.LVg, "imnf", OOOV), K3 = tv(c.LQ, "ijma", OOOV);
+    cd(1.0, X1("ie"), COVV("mjae"), 0.0, K1);                                         // -t_mb A_amej
+    cd(-1.0, T1("mb"), K1, 1.0, H);
+    cd(1.0, X1("ie"), IOVV("mnfe"), 0.0, K2);
+    cd(1.0, T2("jnfa"), K2, 0.0, K3);
+    cd(1.0, T1("mb"), K3, 1.0, H);
+    const TV K4 = tv(c.LVf, "imjb", OOOV), K5 = tv(c.LVg, "imnf", OOOV), K6 = tv(c.LQ, "ijmb", OOOV);
+    cd(1.0, X1("ie"), COVO("mejb"), 0.0, K4);                                         // -t_ma B_bmje
+    cd(-1.0, T1("ma"), K4, 1.0, H);
+    cd(1.0, X1("ie"), IOVV("mnef"), 0.0, K5);
+    cd(1.0, T2("jnfb"), K5, 0.0, K6);
+    cd(1.0, T1("ma"), K6, 1.0, H);
+    const TV K7 = tv(c.LVg, "imnf", OOOV), K8 = tv(c.LQ, "ijmb", OOOV);
+    cd(1.0, X1("ie"), LOVV("mnef"), 0.0, K7);
+    cd(1.0, T2("njfb"), K7, 0.0, K8);
+    cd(-1.0, T1("ma"), K8, 1.0, H);
+    // 1/2 x2 Hvvvv (:175) and x_ie t_jf Hvvvv_abef (:684 in :172) are, after P(ij,ab), the one
+    // pair-symmetric contraction 1/2 Y(Z) with Z = x2 + x1 t1 + t1 x1 and Y(Z)_ijab = Z_ijef Hvvvv_abef
+    //   = ladder(Z) - t_mb Z_ijef <am|ef> - t_ma Z_ijef <bm|fe> + tau_mnab Z_ijef <mn|ef>
+    CCX_HIP(hipMemcpyAsync(c.LY, x2, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, s));
+    pair_outer_kernel<<<(unsigned)std::min<int64_t>(ceil_div(ov * ov, 256), 65536), 256, 0, s>>>(o, v, x1, c.Ht1,
+                                                                                                 c.LY);
+    CCX_HIP(hipGetLastError());
+    const TV Ze = tv(c.LU, "ijmn", OOOO);
+    cd(1.0, tv(c.LY, "ijef", OOVV), IOVV("mnef"), 0.0, Ze);
+    const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
+    tau_pack<double>(o, v, nullptr, c.LY, (double*)c.Tp, (double*)c.Tm, s);
+    cd(1.0, tv(c.Tp, "xk", {noP, npP}), tv(c.LP, "ky", {npP, npP + ov}), 0.0, tv(c.CP, "xy", {noP, npP + ov}));
+    if (noM > 0)
+      cd(1.0, tv(c.Tm, "xk", {noM, npM}), tv(c.LM, "ky", {npM, npM + ov}), 0.0, tv(c.CM, "xy", {noM, npM + ov}));
+    dim3 grid((unsigned)std::min<int64_t>(ceil_div(v * v, 256), 64), (unsigned)(o * o));
+    ladder_unpack_kernel<<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
+    CCX_HIP(hipGetLastError());
+    // Zf[m][i][j][b] = Z_ijef <mb|ef> from the (m, b) columns of the same GEMM
+    const TV Zf = tv(c.LQ, "mijb", OOOV);
+    scale<double>(0.0, Zf, s);
+    zt_add<double>(o, v, (const double*)c.CP, (const double*)c.CM, c.LQ, s);
+    const TV Y = tv(c.LY, "ijab", OOVV);
+    cd(-1.0, T1("mb"), tv(c.LQ, "mjia", OOOV), 1.0, Y);                               // Z_ijef <am|ef> = Zf[m][j][i][a]
+    cd(-1.0, T1("ma"), Zf, 1.0, Y);                                                   // Z_ijef <bm|fe> = Zf[m][i][j][b]
+    cd(1.0, TAU("mnab"), Ze, 1.0, Y);
+    axpby<double>(0.5, Y, 1.0, H, s);
+  }
+  cd(1.0, Zoo, T2("mjab"), 1.0, H);                                                   // :173
+  cd(1.0, Zvv, T2("ijeb"), 1.0, H);
+  pair_symmetrize(o, v, 1.0, c.Lh, j2, s);
 }
 
 // 1/2 <ij|ab> l2_ijab (cclambda.py:570) into a device scalar.

@@ -263,3 +263,60 @@ def test_backward_requires_fp64_context(pycc_amd):
         pycc_amd.cchbar(cc)
     with pytest.raises(pycc_amd.PyCCError):
         cc.vjp(cc.t1, cc.t2, cc.t1, cc.t2)
+
+
+# ---- forward mode: the Jacobian-vector product (ccderiv.py:149-180) -------------------------------
+@pytest.mark.parametrize("no,nv,seed", [(1, 5, 31), (3, 7, 32), (4, 12, 33), (5, 19, 34)])
+@pytest.mark.parametrize("stored", [True, False])
+def test_jvp_matches_oracle_jacobian(pycc_amd, no, nv, seed, stored):
+    from oracle.lambda_ref import ccsd_jacobian
+    print("jvp vs oracle", no, nv, seed, stored, flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    x1, x2 = random_amplitudes(no, nv, seed + 5, scale=1.0)
+    j1, j2 = ccsd_jacobian(ref, OracleHBAR(ref, t1, t2), x1, x2, t2=t2)
+    g1, g2 = cc.jvp(_dev(t1), _dev(t2), _dev(x1), _dev(x2), store_vvvv=stored)
+    assert _rel(_np(g1), j1) < 1e-11
+    assert _rel(_np(g2), j2 + j2.transpose(1, 0, 3, 2)) < 1e-11
+
+
+def test_jvp_central_differences_and_adjoint(pycc_amd):
+    """The JVP against central differences of the GPU residual, and <c, J x> = <J^T c, x> with the
+    GPU vJp (the two derivative kernel sets are adjoint)."""
+    no, nv, seed = 4, 12, 35
+    print("jvp finite differences", flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    rng = np.random.default_rng(seed)
+    T1, T2 = _dev(t1), _dev(t2)
+    x1 = _dev(rng.standard_normal((no, nv)))
+    x2 = _dev(_sym2(rng.standard_normal((no, no, nv, nv))))
+    j1, j2 = cc.jvp(T1, T2, x1, x2)
+    h = 1e-5
+    p1, p2 = cc.residuals(cc.H.F, T1 + h * x1, T2 + h * x2)
+    m1, m2 = cc.residuals(cc.H.F, T1 - h * x1, T2 - h * x2)
+    assert _rel(_np((p1 - m1) / (2 * h)), _np(j1)) < 1e-7
+    assert _rel(_np((p2 - m2) / (2 * h)), _np(j2)) < 1e-7
+    c1 = _dev(rng.standard_normal((no, nv)))
+    c2 = _dev(rng.standard_normal((no, no, nv, nv)))
+    g1, g2 = cc.vjp(T1, T2, c1, c2)
+    lhs = float((c1 * j1).sum() + (c2 * j2).sum())
+    rhs = float((g1 * x1).sum() + (g2 * x2).sum())
+    assert abs(lhs - rhs) < 1e-10 * max(1.0, abs(lhs))
+
+
+def test_forward_mode_autograd(pycc_amd):
+    """torch forward-mode AD through the residual Function gives the JVP."""
+    import torch.autograd.forward_ad as fwAD
+    from pycc_amd import autograd as ccad
+    no, nv, seed = 3, 9, 36
+    print("forward-mode autograd", flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    rng = np.random.default_rng(seed)
+    x1 = _dev(rng.standard_normal((no, nv)))
+    x2 = _dev(_sym2(rng.standard_normal((no, no, nv, nv))))
+    with fwAD.dual_level():
+        d1 = fwAD.make_dual(_dev(t1), x1)
+        d2 = fwAD.make_dual(_dev(t2), x2)
+        r1, r2 = ccad.residual(cc, d1, d2)
+        tan1, tan2 = fwAD.unpack_dual(r1).tangent, fwAD.unpack_dual(r2).tangent
+    j1, j2 = cc.jvp(_dev(t1), _dev(t2), x1, x2)
+    assert _rel(_np(tan1), _np(j1)) < 1e-14 and _rel(_np(tan2), _np(j2)) < 1e-14

@@ -163,3 +163,42 @@ def test_lambda_residual_is_lagrangian_gradient():
         g2[idx] = (lag(t1, p) - lag(t1, m)) / (2 * h)
     assert np.abs(g1 - rl1).max() < 1e-8
     assert np.abs(g2 + g2.transpose(1, 0, 3, 2) - rl2).max() < 1e-8
+
+
+def test_jacobian_is_residual_derivative():
+    """ccsd_jacobian (ccderiv.py:149-180) = the directional derivative of the pinned residual."""
+    from oracle.lambda_ref import ccsd_jacobian
+    no, nv = 2, 4
+    H = _ham(no, nv, 21)
+    cc = OracleCCSD(H, no)
+    t1, t2 = random_amplitudes(no, nv, 5, scale=0.1)
+    x1, x2 = random_amplitudes(no, nv, 6, scale=1.0)
+    j1, j2 = ccsd_jacobian(cc, OracleHBAR(cc, t1, t2), x1, x2, t2=t2)
+    h = 1e-5
+    p1, p2 = cc.residuals(H.F, t1 + h * x1, t2 + h * x2)
+    m1, m2 = cc.residuals(H.F, t1 - h * x1, t2 - h * x2)
+    fd1, fd2 = (p1 - m1) / (2 * h), (p2 - m2) / (2 * h)
+    assert np.abs(fd1 - j1).max() < 1e-8 * max(1.0, np.abs(j1).max())
+    assert np.abs(fd2 - (j2 + j2.transpose(1, 0, 3, 2))).max() < 1e-8 * max(1.0, np.abs(fd2).max())
+
+
+def test_jacobian_transpose_is_the_lambda_vjp():
+    """<c, J x> = <J^T c, x> with J^T c the source-free Lambda residual at l1 = c1, l2 = c2 + c2^T
+    (t2 part halved): the JVP and the vJp are adjoint."""
+    from oracle.lambda_ref import ccsd_jacobian
+    no, nv = 3, 5
+    H = _ham(no, nv, 22)
+    cc = OracleCCSD(H, no)
+    t1, t2 = random_amplitudes(no, nv, 7, scale=0.1)
+    x1, x2 = random_amplitudes(no, nv, 8, scale=1.0)
+    rng = np.random.default_rng(9)
+    c1, c2 = rng.standard_normal((no, nv)), rng.standard_normal((no, no, nv, nv))
+    h = OracleHBAR(cc, t1, t2)
+    j1, j2 = ccsd_jacobian(cc, h, x1, x2, t2=t2)
+    lhs = np.sum(c1 * j1) + np.sum(c2 * (j2 + j2.transpose(1, 0, 3, 2)))
+    lam = OracleLambda(cc, h)
+    r1, r2 = lam.residuals_with(h, t2, c1, c2 + c2.transpose(1, 0, 3, 2))
+    g1 = r1 - 2.0 * h.Hov
+    g2 = 0.5 * (r2 - 2.0 * H.L[:no, :no, no:, no:])
+    rhs = np.sum(g1 * x1) + np.sum(g2 * x2)
+    assert abs(lhs - rhs) < 1e-10 * max(1.0, abs(lhs))
