@@ -2,7 +2,7 @@
 
 This package restates, in plain numpy FP64, the reference algorithm of CrawfordGroup/pycc for the
 spin-adapted RHF-CCSD residual, its Jacobi + DIIS iteration, the HBAR blocks, the Lambda
-equations (the backward) and the Jacobian-vector product. Every function cites the reference
+equations (the backward), the densities and the Jacobian-vector product. Every function cites the reference
 ``file:line`` it follows.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may import
@@ -23,6 +23,10 @@ transposed operand fails:
   Pinned by central differences of the pinned forward residual (the Lambda residual is the gradient
   of the CC Lagrangian at any t and l) and by the reference's H2O/STO-3G pseudoenergy
   (``tests/test_003_ccsd_lambda.py:49``).
+* ``density_ref`` -- the one- and two-particle densities (``ccdensity.py:335-931``, the integral
+  half of the vJp). Pinned by central differences of the pinned Lagrangian with respect to Fock
+  elements and 8-fold symmetric ERI perturbations at arbitrary t and l, by L = <D, F> + <Gamma, ERI>
+  (L is linear in H), and by compute_energy = E_CCSD at convergence for a canonical reference.
 * ``lambda_ref.ccsd_jacobian`` -- the Jacobian-vector product (``ccderiv.py:149-180``).
   Pinned by central differences of the pinned forward residual along random directions, and by
   the transpose identity <c, J x> = <J^T c, x> against the pinned Lambda (vector-Jacobian) residual.

@@ -202,3 +202,55 @@ def test_jacobian_transpose_is_the_lambda_vjp():
     g2 = 0.5 * (r2 - 2.0 * H.L[:no, :no, no:, no:])
     rhs = np.sum(g1 * x1) + np.sum(g2 * x2)
     assert abs(lhs - rhs) < 1e-10 * max(1.0, abs(lhs))
+
+
+def _lagrangian(H, no, t1, t2, l1, l2):
+    cc = OracleCCSD(H, no)
+    r1, r2 = cc.residuals(H.F, t1, t2)
+    return cc.energy(t1, t2) + np.sum(l1 * r1) + 0.5 * np.sum(l2 * r2)
+
+
+def test_densities_are_lagrangian_derivatives():
+    """ccdensity's spatial builders (ccdensity.py:335-931) are dL/dF and dL/d<pq|rs> of the
+    Lagrangian at any t and l (central differences; the ERI perturbation keeps the 8-fold symmetry),
+    and L = <D, F> + <Gamma, ERI> (L is linear in H)."""
+    from oracle.density_ref import OracleDensity
+    no, nv = 2, 4
+    H = _ham(no, nv, 23, offdiag=0.05)
+    nmo = no + nv
+    t1, t2 = random_amplitudes(no, nv, 11, scale=0.1)
+    l1, l2 = random_amplitudes(no, nv, 12, scale=0.3)
+    D, G = OracleDensity(OracleCCSD(H, no), t1, t2, l1, l2).gradient_densities()
+    L0 = _lagrangian(H, no, t1, t2, l1, l2)
+    assert abs(L0 - (np.sum(D * H.F) + np.sum(G * H.ERI))) < 1e-12 * max(1.0, abs(L0))
+    h = 1e-5
+    rng = np.random.default_rng(13)
+    for _ in range(12):
+        p, q = rng.integers(0, nmo, 2)
+        dF = np.zeros_like(H.F)
+        dF[p, q] = 1.0
+        Hp, Hm = DenseHamiltonian(H.F + h * dF, H.ERI), DenseHamiltonian(H.F - h * dF, H.ERI)
+        fd = (_lagrangian(Hp, no, t1, t2, l1, l2) - _lagrangian(Hm, no, t1, t2, l1, l2)) / (2 * h)
+        assert abs(fd - D[p, q]) < 1e-8
+    for _ in range(24):
+        p, q, r, s = rng.integers(0, nmo, 4)
+        dE = np.zeros((nmo,) * 4)
+        for idx in [(p, q, r, s), (r, q, p, s), (p, s, r, q), (r, s, p, q),
+                    (q, p, s, r), (s, p, q, r), (q, r, s, p), (s, r, q, p)]:
+            dE[idx] = 1.0
+        Hp, Hm = DenseHamiltonian(H.F, H.ERI + h * dE), DenseHamiltonian(H.F, H.ERI - h * dE)
+        fd = (_lagrangian(Hp, no, t1, t2, l1, l2) - _lagrangian(Hm, no, t1, t2, l1, l2)) / (2 * h)
+        assert abs(fd - np.sum(G * dE)) < 1e-8
+
+
+def test_density_energy_at_convergence():
+    """compute_energy (ccdensity.py:113-211) reproduces E_CCSD for a canonical reference (f_ov = 0)
+    once t and l are converged (r = 0, so L = E)."""
+    from oracle.density_ref import OracleDensity
+    no, nv = 3, 6
+    H = _ham(no, nv, 24, offdiag=0.0, t2max=0.05)
+    cc = OracleCCSD(H, no)
+    e = cc.solve_cc(1e-12, 1e-12)
+    lam = OracleLambda(cc, OracleHBAR(cc))
+    lam.solve_lambda(1e-12, 1e-12)
+    assert abs(OracleDensity(cc, cc.t1, cc.t2, lam.l1, lam.l2).energy() - e) < 1e-10
