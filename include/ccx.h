@@ -133,6 +133,26 @@ int ccx_vjp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2
 int ccx_jvp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, const double* x1,
             const double* x2, double* j1, double* j2, int flags, void* stream);
 
+/* One- and two-particle densities of the CC Lagrangian E + l1.r1 + 1/2 l2.r2 at (t1, t2, l1, l2):
+ * the spatial CCSD builders of ccdensity.py (build_Doo :335, Dvv :373, Dvo :410, Dov :415,
+ * Doooo :563, Dvvvv :587, Dooov :611, Dvvvo :682, Dovov :753, Doovv :784). They are dL/dF and
+ * dL/d<pq|rs> at fixed amplitudes, so with l = (c1, c2 + c2^T) and the energy part (l = 0)
+ * subtracted they are the integral half of the vector-Jacobian product. Outputs (device, FP64):
+ *   D [nmo][nmo] (Doo, Dov, Dvo, Dvv blocks), Doooo [o][o][o][o], Dooov [o][o][o][v],
+ *   Dovov [o][v][o][v], Doovv [o][o][v][v], Dvvvo [v][v][v][o] (may be NULL), Dvvvv [v]^4 (may be NULL).
+ * DP contexts. */
+int ccx_densities(ccx_ctx* ctx, const double* t1, const double* t2, const double* l1, const double* l2,
+                  double* D, double* Doooo, double* Dooov, double* Dovov, double* Doovv, double* Dvvvo,
+                  double* Dvvvv, void* stream);
+
+/* *e_dev = ccdensity.compute_energy (pycc/ccdensity.py:113-211) from the densities above:
+ * F_ij D_ij + F_ab D_ab + 1/2 <ij|kl> Doooo + <ij|ka> Dooov + <ab|ci> Dvvvo + <ia|jb> Dovov
+ * + 1/2 <ij|ab> Doovv + 1/2 <ab|cd> Dvvvv, with the Dvvvv term evaluated from (t1, t2, l2) through
+ * the packed ladder (no v^4 array). Like the reference it omits the f_ov terms (canonical orbitals). */
+int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const double* l2, const double* fock,
+                       const double* D, const double* Doooo, const double* Dooov, const double* Dovov,
+                       const double* Doovv, const double* Dvvvo, double* e_dev, void* stream);
+
 /* Measurement hooks (no counterpart in the reference, which has only wall-clock prints,
  * pycc/utils.py:229-238): HIP events recorded on the residual's stream around the ladder GEMMs,
  * the ring GEMMs and each whole residual.

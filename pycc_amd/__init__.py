@@ -5,7 +5,8 @@ Public API (names as exported by ``pycc/__init__.py``):
 * ``CCwfn`` / ``ccwfn`` -- amplitude solver: ``residuals``, ``cc_energy``, ``solve_cc``;
 * ``cchbar``, ``cclambda`` -- HBAR and the Lambda equations (``solve_lambda``, ``residuals``,
   ``pseudoenergy``); ``autograd.residual`` -- the residual as a ``torch.autograd.Function`` whose
-  backward is the Lambda-equation vector-Jacobian product;
+  backward is the Lambda-equation vector-Jacobian product; ``ccdensity`` -- the one- and
+  two-particle densities (the integral half of the vJp);
 * ``DeviceManager``, ``ContractionBackend`` -- device/precision policy and the GPU einsum;
 * ``PyCCError``, ``PyCCWarning``, ``InvalidKeywordError``, ``DeviceError``.
 
@@ -16,7 +17,8 @@ from .device import DeviceManager, ContractionBackend
 from .ccwfn import CCwfn, ccwfn
 from .cchbar import cchbar
 from .cclambda import cclambda
+from .ccdensity import ccdensity
 from . import synth
 
-__all__ = ["CCwfn", "ccwfn", "cchbar", "cclambda", "DeviceManager", "ContractionBackend", "PyCCError", "PyCCWarning",
+__all__ = ["CCwfn", "ccwfn", "cchbar", "cclambda", "ccdensity", "DeviceManager", "ContractionBackend", "PyCCError", "PyCCWarning",
            "InvalidKeywordError", "DeviceError", "synth"]

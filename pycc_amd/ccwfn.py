@@ -318,6 +318,24 @@ class CCwfn(object):
                   _lib.VJP_REUSE_HBAR, self._stream())
         return j1, j2
 
+    def integral_vjp(self, t1, t2, c1, c2, vvvv=False):
+        """The integral half of the vector-Jacobian product of ``residuals(F, t1, t2)`` with cotangent
+        (c1, c2): d(c1.r1 + c2.r2)/dF and d(...)/d<pq|rs> at fixed amplitudes, as the density blocks of
+        ``ccdensity.py:335-931`` at l = (c1, c2 + c2^T) with the energy part (l = 0) removed. Returns a
+        dict of device tensors: ``D`` (nmo x nmo, Fock gradient) and ``Doooo``, ``Dooov``, ``Dovov``,
+        ``Doovv``, ``Dvvvo`` (and ``Dvvvv`` with ``vvvv=True``) in the reference's density layout."""
+        from .ccdensity import _blocks
+        self._require_dp("the integral vJp")
+        t1 = self._as_dev(t1, (self.no, self.nv))
+        t2 = self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        c1 = self._as_dev(c1, (self.no, self.nv))
+        c2 = self._as_dev(c2, (self.no, self.no, self.nv, self.nv))
+        l2 = (c2 + c2.permute(1, 0, 3, 2)).contiguous()
+        with_l = _blocks(self, t1, t2, c1, l2, big=True, vvvv=vvvv)
+        z1, z2 = self._torch.zeros_like(c1), self._torch.zeros_like(c2)
+        energy_part = _blocks(self, t1, t2, z1, z2, big=True, vvvv=vvvv)
+        return {k: (x - energy_part[k]) for k, x in with_l.items() if x is not None}
+
     def energy_gradient(self, t1=None, t2=None, F=None):
         """(dE/dt1, dE/dt2) of ``cc_energy`` = (2 Hov, L): the source terms of the Lambda equations
         (cclambda.py:348, :468)."""

@@ -234,6 +234,24 @@ int ccx_jvp(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2
   });
 }
 
+int ccx_densities(ccx_ctx* ctx, const double* t1, const double* t2, const double* l1, const double* l2,
+                  double* D, double* Doooo, double* Dooov, double* Dovov, double* Doovv, double* Dvvvo,
+                  double* Dvvvv, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::densities(ctx->c, t1, t2, l1, l2, D, Doooo, Dooov, Dovov, Doovv, Dvvvo, Dvvvv, st(stream));
+  });
+}
+
+int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const double* l2, const double* fock,
+                       const double* D, const double* Doooo, const double* Dooov, const double* Dovov,
+                       const double* Doovv, const double* Dvvvo, double* e_dev, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::density_energy(ctx->c, t1, t2, l2, fock, D, Doooo, Dooov, Dovov, Doovv, Dvvvo, e_dev, st(stream));
+  });
+}
+
 int ccx_timing_start(ccx_ctx* ctx) {
   return guarded([&] {
     CCX_CHECK(ctx, ccx::ERR_ARG, "null context");

@@ -89,6 +89,7 @@ struct Ctx {
   double *Ltmp = nullptr, *Goo = nullptr, *Gvv = nullptr, *Lh = nullptr, *Lsym = nullptr;
   double *LY = nullptr, *LVf = nullptr, *LVg = nullptr, *LU = nullptr, *LQ = nullptr, *Lsm = nullptr;
   double* Jsm = nullptr;
+  double* Jv3 = nullptr;   // density energy: two v^3 slices and the term values
 
   // ---- DIIS (pycc utils.helper_diis) ----
   int max_diis = 0, cap = 0;
@@ -131,6 +132,14 @@ void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, dou
 // out[i][j][a][b] = alpha * (h[i][j][a][b] + h[j][i][b][a])
 void pair_symmetrize(int64_t o, int64_t v, double alpha, const double* h, double* out, hipStream_t s);
 void pseudoenergy(Ctx& c, const double* l2, double* e_dev, hipStream_t s);
+// dL/dF (full [nmo][nmo] D) and the two-particle densities of the Lagrangian at (t, l)
+void densities(Ctx& c, const double* t1, const double* t2, const double* l1, const double* l2, double* D,
+               double* Doooo, double* Dooov, double* Dovov, double* Doovv, double* Dvvvo, double* Dvvvv,
+               hipStream_t s);
+// ccdensity.compute_energy from the densities (Dvvvv term through the packed ladder of l2)
+void density_energy(Ctx& c, const double* t1, const double* t2, const double* l2, const double* F,
+                    const double* D, const double* Doooo, const double* Dooov, const double* Dovov,
+                    const double* Doovv, const double* Dvvvo, double* e_dev, hipStream_t s);
 // j1, j2 = the CCSD Jacobian applied to (x1, x2) on the current HBAR (x2 pair-symmetric)
 void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2, hipStream_t s);
 void timing_start(Ctx& c);

@@ -106,6 +106,20 @@ __global__ void pair_outer_kernel(int64_t o, int64_t v, const double* __restrict
   }
 }
 
+// out[0] = sum of x[0..n) in a fixed order (one block).
+__global__ void sum_terms_kernel(int n, const double* __restrict__ x, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) a += x[k];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
 }  // namespace
 
 void pair_symmetrize(int64_t o, int64_t v, double alpha, const double* h, double* out, hipStream_t s) {
@@ -608,6 +622,249 @@ void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2
   cd(1.0, Zoo, T2("mjab"), 1.0, H);                                                   // :173
   cd(1.0, Zvv, T2("ijeb"), 1.0, H);
   pair_symmetrize(o, v, 1.0, c.Lh, j2, s);
+}
+
+// One- and two-particle densities of the CC Lagrangian E + l1.r1 + 1/2 l2.r2 at (t, l)
+// (ccdensity.py, spatial CCSD builders :335-931): dL/dF and dL/d<pq|rs> at fixed t and l.
+// D is written as the full [nmo][nmo] matrix (Doo, Dov, Dvo, Dvv blocks); Dvvvo and Dvvvv may be
+// null (they are ov^3 and v^4).
+void densities(Ctx& c, const double* t1, const double* t2, const double* l1, const double* l2, double* D,
+               double* Doooo, double* Dooov, double* Dovov, double* Doovv, double* Dvvvo, double* Dvvvv,
+               hipStream_t s) {
+  CCX_CHECK(c.have_ints, ERR_STATE, "densities requested before the integrals were set");
+  CCX_CHECK(c.mode == MODE_DP, ERR_STATE, "the densities run in a DP context");
+  CCX_CHECK(t1 && t2 && l1 && l2 && D && Doooo && Dooov && Dovov && Doovv, ERR_ARG, "null argument");
+  alloc_work(c);
+  alloc_hbar(c, false);
+  const int64_t o = c.o, v = c.v, nmo = c.nmo;
+  Workspace& ws = c.ws;
+  const std::vector<int64_t> OV{o, v}, OO{o, o}, VV{v, v}, OOVV{o, o, v, v}, OVOV{o, v, o, v};
+  const std::vector<int64_t> OOOV{o, o, o, v}, OOOO{o, o, o, o}, VVVO{v, v, v, o}, VOVO{v, o, v, o};
+  auto cd = [&](double a, const TV& A, const TV& B, double b, const TV& C) {
+    contract<double>(a, A, B, b, C, ws, s);
+  };
+  auto T1 = [&](const char* l) { return tv(t1, l, OV); };
+  auto T2 = [&](const char* l) { return tv(t2, l, OOVV); };
+  auto L1 = [&](const char* l) { return tv(l1, l, OV); };
+  auto L2 = [&](const char* l) { return tv(l2, l, OOVV); };
+  build_tau(o, v, 1.0, 1.0, t1, t2, c.tau, s);
+  auto TAU = [&](const char* l) { return tv(c.tau, l, OOVV); };
+  const TV Goo = tv(c.Goo, "mi", OO), Gvv = tv(c.Gvv, "ae", VV);
+  cd(1.0, T2("mjab"), L2("ijab"), 0.0, Goo);                                        // cclambda.py:281
+  cd(-1.0, T2("ijeb"), L2("ijab"), 0.0, Gvv);                                       // cclambda.py:306
+
+  // ---- one-particle: :364-365, :401-402, :413, :454-461 ----
+  const TV Doo = tvs(D, "ij", OO, {nmo, 1}), Dvv = tvs(D + o * nmo + o, "ab", VV, {nmo, 1});
+  const TV Dov = tvs(D + o, "ia", OV, {nmo, 1}), Dvo = tvs(D + o * nmo, "ai", {v, o}, {nmo, 1});
+  cd(-1.0, T1("ie"), L1("je"), 0.0, Doo);
+  cd(-1.0, T2("imef"), L2("jmef"), 1.0, Doo);
+  cd(1.0, T1("mb"), L1("ma"), 0.0, Dvv);
+  cd(1.0, T2("mnbe"), L2("mnae"), 1.0, Dvv);
+  axpby<double>(1.0, L1("ia"), 0.0, Dvo, s);
+  axpby<double>(2.0, T1("ia"), 0.0, Dov, s);
+  cd(2.0, L1("me"), T2("imae"), 1.0, Dov);
+  cd(-1.0, L1("me"), TAU("miae"), 1.0, Dov);
+  const TV soo = tv(c.Lsm, "mi", OO), svv = tv(c.LY, "ea", VV);
+  cd(1.0, L2("mnef"), T2("inef"), 0.0, soo);
+  cd(-1.0, soo, T1("ma"), 1.0, Dov);
+  cd(1.0, L2("mnef"), T2("mnaf"), 0.0, svv);
+  cd(-1.0, svv, T1("ie"), 1.0, Dov);
+
+  // ---- Doooo :585, Dvvvv :609 ----
+  cd(1.0, TAU("ijef"), L2("klef"), 0.0, tv(Doooo, "ijkl", OOOO));
+  if (Dvvvv) cd(1.0, TAU("mnab"), L2("mncd"), 0.0, tv(Dvvvv, "abcd", {v, v, v, v}));
+
+  // ---- Dooov :645-670 ----
+  {
+    const TV Dq = tv(Dooov, "ijka", OOOV);
+    const TV tv4 = tv(c.Ltmp, "jake", OVOV), to4 = tv(c.LU, "ijkm", OOOO);
+    cd(-2.0, L1("ke"), TAU("ijea"), 0.0, Dq);                                        // :646 (2 tau - tau^T)
+    cd(1.0, L1("ke"), TAU("ijae"), 1.0, Dq);
+    cd(-1.0, T1("ie"), L2("jkae"), 1.0, Dq);                                         // :647
+    cd(-2.0, relabel(Goo, "ik"), T1("ja"), 1.0, Dq);                                 // :652
+    cd(1.0, relabel(Goo, "jk"), T1("ia"), 1.0, Dq);                                  // :653
+    cd(1.0, T2("jmaf"), L2("kmef"), 0.0, tv4);                                       // :654
+    cd(-2.0, tv4, T1("ie"), 1.0, Dq);                                                // :655
+    cd(1.0, relabel(tv4, "iake"), T1("je"), 1.0, Dq);                                // :656
+    cd(1.0, T2("ijef"), L2("kmef"), 0.0, to4);                                       // :658
+    cd(1.0, to4, T1("ma"), 1.0, Dq);                                                 // :659
+    cd(1.0, T2("mjaf"), L2("kmef"), 0.0, tv4);                                       // :660
+    cd(1.0, tv4, T1("ie"), 1.0, Dq);                                                 // :661
+    const TV tk = tv(c.Ltmp, "iakf", OVOV);
+    cd(1.0, T2("imea"), L2("kmef"), 0.0, tk);                                        // :662
+    cd(1.0, tk, T1("jf"), 1.0, Dq);                                                  // :663
+    const TV t3 = tv(c.LVf, "kmej", {o, o, v, o}), t4 = tv(c.LU, "kmij", OOOO);
+    cd(1.0, L2("kmef"), T1("jf"), 0.0, t3);                                          // :668
+    cd(1.0, t3, T1("ie"), 0.0, t4);                                                  // :669
+    cd(1.0, t4, T1("ma"), 1.0, Dq);                                                  // :670
+  }
+
+  // ---- Dvvvo :716-741 (the v^4 intermediate of :729-730 taken as t_nmab (l_nmce t_ie)) ----
+  if (Dvvvo) {
+    const TV Dq = tv(Dvvvo, "abci", VVVO);
+    cd(2.0, L1("mc"), TAU("miab"), 0.0, Dq);                                         // :717 (2 tau - tau^T)
+    cd(-1.0, L1("mc"), TAU("miba"), 1.0, Dq);
+    cd(1.0, T1("ma"), L2("imbc"), 1.0, Dq);                                          // :718
+    cd(-2.0, relabel(Gvv, "ca"), T1("ib"), 1.0, Dq);                                 // :723
+    cd(1.0, relabel(Gvv, "cb"), T1("ia"), 1.0, Dq);                                  // :724
+    const TV tb = tv(c.Ltmp, "ibnc", OVOV);
+    cd(1.0, T2("imbe"), L2("nmce"), 0.0, tb);                                        // :725
+    cd(2.0, tb, T1("na"), 1.0, Dq);                                                  // :726
+    cd(-1.0, relabel(tb, "ianc"), T1("nb"), 1.0, Dq);                                // :727
+    const TV q = tv(c.LVf, "nmci", {o, o, v, o});
+    cd(1.0, L2("nmce"), T1("ie"), 0.0, q);                                           // :729-730 as
+    cd(-1.0, T2("nmab"), q, 1.0, Dq);                                                //   t_nmab (l_nmce t_ie)
+    const TV ta = tv(c.Ltmp, "iamc", OVOV);
+    cd(1.0, T2("niae"), L2("nmce"), 0.0, ta);                                        // :731
+    cd(-1.0, ta, T1("mb"), 1.0, Dq);                                                 // :732
+    cd(1.0, T2("mibe"), L2("nmce"), 0.0, tb);                                        // :733
+    cd(-1.0, tb, T1("na"), 1.0, Dq);                                                 // :734
+    const TV am = tv(c.Ltmp, "amci", VOVO);
+    cd(1.0, q, T1("na"), 0.0, am);                                                   // :739-740
+    cd(-1.0, am, T1("mb"), 1.0, Dq);                                                 // :741
+  }
+
+  // ---- Dovov :775-780 ----
+  {
+    const TV Dq = tv(Dovov, "iajb", OVOV);
+    cd(-1.0, T1("ia"), L1("jb"), 0.0, Dq);
+    cd(-1.0, TAU("mibe"), L2("jmea"), 1.0, Dq);
+    cd(-1.0, T2("imbe"), L2("mjea"), 1.0, Dq);
+  }
+
+  // ---- Doovv :830-922 ----
+  {
+    const TV Dq = tv(Doovv, "ijab", OOVV);
+    cd(4.0, T1("ia"), L1("jb"), 0.0, Dq);                                            // :860
+    axpby<double>(4.0, TAU("ijab"), 1.0, Dq, s);                                     // :861  2 (2 tau - tau^T)
+    axpby<double>(-2.0, TAU("ijba"), 1.0, Dq, s);
+    axpby<double>(1.0, L2("ijab"), 1.0, Dq, s);                                      // :862
+    const TV x2 = tv(c.Lsm, "jb", OV), x3 = tv(c.LVf, "ijmb", OOOV);
+    cd(4.0, L1("me"), T2("jmbe"), 0.0, x2);                                          // :864-865  2 l_me (2 t - t^T)
+    cd(-2.0, L1("me"), T2("jmeb"), 1.0, x2);
+    cd(2.0, x2, T1("ia"), 1.0, Dq);                                                  // :866
+    cd(-1.0, relabel(x2, "ja"), T1("ib"), 1.0, Dq);                                  // :867
+    cd(4.0, T2("ijeb"), L1("me"), 0.0, x3);                                          // :868
+    cd(-2.0, T2("ijbe"), L1("me"), 1.0, x3);
+    cd(-1.0, x3, T1("ma"), 1.0, Dq);                                                 // :869
+    // :870-871  -(2 l_me (2 tau_jmba - tau_jmab)) t_ie = -2 l_me t_ie (2 tau_jmba - tau_jmab)
+    const TV li = tv(c.Lsm + o * v, "mi", OO);
+    cd(1.0, L1("me"), T1("ie"), 0.0, li);
+    cd(-4.0, li, TAU("jmba"), 1.0, Dq);
+    cd(2.0, li, TAU("jmab"), 1.0, Dq);
+    cd(4.0, T2("imae"), L2("mjeb"), 1.0, Dq);                                        // :876
+    cd(-2.0, TAU("mjbe"), L2("imae"), 1.0, Dq);                                      // :877
+    const TV too = tv(c.LU, "ijmn", OOOO);
+    cd(1.0, T2("ijef"), L2("mnef"), 0.0, too);                                       // :879
+    cd(1.0, too, T2("mnab"), 1.0, Dq);                                               // :880
+    const TV ta = tv(c.Ltmp, "jbme", OVOV);
+    cd(1.0, T2("njbf"), L2("mnef"), 0.0, ta);                                        // :881
+    cd(1.0, ta, T2("miae"), 1.0, Dq);                                                // :882
+    const TV tb = tv(c.Ltmp, "ibne", OVOV);
+    cd(1.0, T2("imfb"), L2("mnef"), 0.0, tb);                                        // :883
+    cd(1.0, tb, T2("njae"), 1.0, Dq);                                                // :884
+    cd(4.0, relabel(Gvv, "eb"), TAU("ijae"), 1.0, Dq);                               // :886
+    cd(-2.0, relabel(Gvv, "ea"), TAU("ijbe"), 1.0, Dq);                              // :887
+    cd(-4.0, relabel(Goo, "jm"), TAU("imab"), 1.0, Dq);                              // :889
+    cd(2.0, relabel(Goo, "jm"), TAU("imba"), 1.0, Dq);                               // :890
+    const TV tc = tv(c.Ltmp, "iame", OVOV);
+    cd(1.0, T2("inaf"), L2("mnef"), 0.0, tc);                                        // :891
+    cd(-4.0, tc, TAU("mjbe"), 1.0, Dq);                                              // :892
+    cd(2.0, relabel(tc, "ibme"), TAU("mjae"), 1.0, Dq);                              // :893
+    cd(4.0, relabel(tc, "jbme"), T2("imae"), 1.0, Dq);                               // :894
+    cd(-2.0, relabel(tc, "jame"), T2("imbe"), 1.0, Dq);                              // :895
+    const TV tm = tv(c.LVf, "ijmb", OOOV);
+    cd(1.0, T1("nb"), too, 0.0, tm);                                                 // :898
+    cd(1.0, T1("ma"), tm, 1.0, Dq);                                                  // :899
+    const TV y1 = tv(c.LVg, "mnif", OOOV), y2 = tv(c.LU, "mnij", OOOO);
+    cd(1.0, T1("ie"), L2("mnef"), 0.0, y1);                                          // :900
+    cd(1.0, T1("jf"), y1, 0.0, y2);                                                  // :901
+    cd(1.0, y2, T2("mnab"), 1.0, Dq);                                                // :902
+    const TV y3 = tv(c.LQ, "mijb", OOOV);
+    cd(1.0, y1, T2("njbf"), 0.0, y3);                                                // :903-904
+    cd(1.0, T1("ma"), y3, 1.0, Dq);                                                  // :905
+    const TV y4 = tv(c.LVg, "mnej", {o, o, v, o}), y5 = tv(c.LQ, "njia", OOOV);
+    cd(1.0, T1("jf"), L2("mnef"), 0.0, y4);                                          // :906
+    cd(1.0, y4, T2("miae"), 0.0, y5);                                                // :907
+    cd(1.0, T1("nb"), y5, 1.0, Dq);                                                  // :908
+    const TV y6 = tv(c.LVg, "mnjf", OOOV), y7 = tv(c.LQ, "njib", OOOV);
+    cd(1.0, T1("je"), L2("mnef"), 0.0, y6);                                          // :909
+    cd(1.0, y6, T2("imfb"), 0.0, y7);                                                // :910
+    cd(1.0, T1("na"), y7, 1.0, Dq);                                                  // :911
+    const TV y8 = tv(c.LVg, "mnei", {o, o, v, o}), y9 = tv(c.LQ, "mija", OOOV);
+    cd(1.0, T1("if"), L2("mnef"), 0.0, y8);                                          // :912
+    cd(1.0, y8, T2("njae"), 0.0, y9);                                                // :913
+    cd(1.0, T1("mb"), y9, 1.0, Dq);                                                  // :914
+    const TV z1 = tv(c.LVg, "mnej", {o, o, v, o}), z2 = tv(c.LU, "mnij", OOOO);
+    const TV z3 = tv(c.LQ, "mbij", {o, v, o, o});
+    cd(1.0, T1("jf"), L2("mnef"), 0.0, z1);                                          // :919
+    cd(1.0, T1("ie"), z1, 0.0, z2);                                                  // :920
+    cd(1.0, T1("nb"), z2, 0.0, z3);                                                  // :921
+    cd(1.0, T1("ma"), z3, 1.0, Dq);                                                  // :922
+  }
+}
+
+// compute_energy (ccdensity.py:113-211): F_ij D_ij + F_ab D_ab + 1/2 <ij|kl> Doooo + <ij|ka> Dooov
+// + <ab|ci> Dvvvo + <ia|jb> Dovov + 1/2 <ij|ab> Doovv + 1/2 <ab|cd> Dvvvv, the last one taken as
+// 1/2 tau_mnab (l2_mncd <ab|cd>) through the packed ladder GEMM (no v^4 array). e_dev: device scalar.
+void density_energy(Ctx& c, const double* t1, const double* t2, const double* l2, const double* F,
+                    const double* D, const double* Doooo, const double* Dooov, const double* Dovov,
+                    const double* Doovv, const double* Dvvvo, double* e_dev, hipStream_t s) {
+  CCX_CHECK(c.have_ints && c.mode == MODE_DP, ERR_STATE, "density energy needs a DP context with integrals");
+  CCX_CHECK(t1 && t2 && l2 && F && D && Doooo && Dooov && Dovov && Doovv && Dvvvo && e_dev, ERR_ARG,
+            "null argument");
+  alloc_work(c);
+  alloc_hbar(c, false);
+  const int64_t o = c.o, v = c.v, nmo = c.nmo, ov = o * v;
+  const int64_t slab = std::max<int64_t>(v * v * v, o * o + v * v);   // one v^3 slice, or the Fock blocks
+  if (!c.Jv3) c.Jv3 = c.alloc<double>(2 * slab + o + 16);
+  Workspace& ws = c.ws;
+  const std::vector<int64_t> OO{o, o}, VV{v, v}, OOVV{o, o, v, v}, OVOV{o, v, o, v};
+  const std::vector<int64_t> OOOV{o, o, o, v}, OOOO{o, o, o, o}, VVV{v, v, v};
+  double* part = c.Jv3 + 2 * slab;        // one value per term (o + 7 of them)
+  double* buf = c.Ltmp;                   // the o^2 v^2 integral block permuted to the density layout
+  int nt = 0;
+  auto dot = [&](int64_t n, const double* x, const double* y, double w) {
+    dots(n, 1, x, 0, y, c.red, part + nt, s);
+    if (w != 1.0) scale<double>(w, tv(part + nt, "x", {1}), s);
+    ++nt;
+  };
+  // Fock terms (blocks copied contiguous)
+  axpby<double>(1.0, tvs(F, "ij", OO, {nmo, 1}), 0.0, tv(c.Jv3, "ij", OO), s);
+  axpby<double>(1.0, tvs(D, "ij", OO, {nmo, 1}), 0.0, tv(c.Jv3 + slab, "ij", OO), s);
+  dot(o * o, c.Jv3, c.Jv3 + slab, 1.0);
+  axpby<double>(1.0, tvs(F + o * nmo + o, "ab", VV, {nmo, 1}), 0.0, tv(c.Jv3, "ab", VV), s);
+  axpby<double>(1.0, tvs(D + o * nmo + o, "ab", VV, {nmo, 1}), 0.0, tv(c.Jv3 + slab, "ab", VV), s);
+  dot(v * v, c.Jv3, c.Jv3 + slab, 1.0);
+  axpby<double>(1.0, tv(c.Coooo, "ikjl", OOOO), 0.0, tv(c.LU, "ijkl", OOOO), s);  // <ij|kl> = (ik|jl)
+  dot(o * o * o * o, c.LU, Doooo, 0.5);
+  axpby<double>(1.0, tv(c.Cooov, "ikja", OOOV), 0.0, tv(c.LVf, "ijka", OOOV), s); // <ij|ka> = (ik|ja)
+  dot(o * o * ov, c.LVf, Dooov, 1.0);
+  axpby<double>(1.0, tv(c.Coovv, "ijab", OOVV), 0.0, tv(buf, "iajb", OVOV), s);   // <ia|jb> = (ij|ab)
+  dot(ov * ov, buf, Dovov, 1.0);
+  dot(ov * ov, c.Ioovv, Doovv, 0.5);                                              // <ij|ab>
+  // <ab|ci> = (ac|bi) = (ib|ac) = Covvv[i][b][a][c], one i at a time against Dvvvo[a][b][c][i]
+  for (int64_t i = 0; i < o; ++i) {
+    axpby<double>(1.0, tv(c.Covvv + i * v * v * v, "bac", VVV), 0.0, tv(c.Jv3, "abc", VVV), s);
+    axpby<double>(1.0, tvs(Dvvvo + i, "abc", VVV, {v * v * o, v * o, o}), 0.0, tv(c.Jv3 + slab, "abc", VVV), s);
+    dot(v * v * v, c.Jv3, c.Jv3 + slab, 1.0);
+  }
+  {   // 1/2 <ab|cd> Dvvvv_abcd = 1/2 tau_mnab ladder(l2)_mnab
+    const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
+    tau_pack<double>(o, v, nullptr, l2, (double*)c.Tp, (double*)c.Tm, s);
+    contract<double>(1.0, tv(c.Tp, "xk", {noP, npP}), tvs(c.LP, "ky", {npP, npP}, {npP + ov, 1}), 0.0,
+                     tvs(c.CP, "xy", {noP, npP}, {npP + ov, 1}), ws, s);
+    if (noM > 0)
+      contract<double>(1.0, tv(c.Tm, "xk", {noM, npM}), tvs(c.LM, "ky", {npM, npM}, {npM + ov, 1}), 0.0,
+                       tvs(c.CM, "xy", {noM, npM}, {npM + ov, 1}), ws, s);
+    dim3 grid((unsigned)std::min<int64_t>(ceil_div(v * v, 256), 64), (unsigned)(o * o));
+    ladder_unpack_kernel<<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
+    CCX_HIP(hipGetLastError());
+    build_tau(o, v, 1.0, 1.0, t1, t2, c.tau, s);
+    dot(ov * ov, c.tau, c.LY, 0.5);
+  }
+  sum_terms_kernel<<<1, 256, 0, s>>>(nt, part, e_dev);
+  CCX_HIP(hipGetLastError());
 }
 
 // 1/2 <ij|ab> l2_ijab (cclambda.py:570) into a device scalar.

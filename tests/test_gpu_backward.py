@@ -320,3 +320,93 @@ def test_forward_mode_autograd(pycc_amd):
         tan1, tan2 = fwAD.unpack_dual(r1).tangent, fwAD.unpack_dual(r2).tangent
     j1, j2 = cc.jvp(_dev(t1), _dev(t2), x1, x2)
     assert _rel(_np(tan1), _np(j1)) < 1e-14 and _rel(_np(tan2), _np(j2)) < 1e-14
+
+
+# ---- densities: the integral half of the vJp (ccdensity.py:335-931) -------------------------------
+def _gamma_full(b, no, nv):
+    """gradient_densities' four-fold symmetrised Gamma (ccdensity.py:315-334) from density blocks."""
+    nmo = no + nv
+    o, v = slice(0, no), slice(no, nmo)
+    G = np.zeros((nmo,) * 4)
+    G[o, o, o, o] = 0.5 * b["Doooo"]
+    G[v, v, v, v] = 0.5 * b["Dvvvv"]
+    G[o, v, o, v] = b["Dovov"]
+    G[o, o, v, v] = 0.25 * b["Doovv"]
+    G[v, v, o, o] = 0.25 * b["Doovv"].transpose(2, 3, 0, 1)
+    G[o, o, o, v] = 0.5 * b["Dooov"]
+    G[o, v, o, o] = 0.5 * b["Dooov"].transpose(2, 3, 0, 1)
+    G[v, v, v, o] = 0.5 * b["Dvvvo"]
+    G[v, o, v, v] = 0.5 * b["Dvvvo"].transpose(2, 3, 0, 1)
+    return 0.25 * (G + G.transpose(1, 0, 3, 2) + G.transpose(2, 3, 0, 1) + G.transpose(3, 2, 1, 0))
+
+
+@pytest.mark.parametrize("no,nv,seed", [(1, 5, 41), (3, 7, 42), (4, 12, 43), (6, 4, 44)])
+def test_densities_match_oracle(pycc_amd, no, nv, seed):
+    from oracle.density_ref import OracleDensity
+    from pycc_amd.ccdensity import _blocks
+    print("densities", no, nv, seed, flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    l1, l2 = random_amplitudes(no, nv, seed + 2, scale=0.5)
+    od = OracleDensity(ref, t1, t2, l1, l2)
+    b = _blocks(cc, _dev(t1), _dev(t2), _dev(l1), _dev(l2), big=True, vvvv=True)
+    Dref, _ = od.gradient_densities()
+    assert _rel(_np(b["D"]), Dref) < 1e-11
+    for name in ("Doooo", "Dooov", "Dovov", "Doovv", "Dvvvo", "Dvvvv"):
+        assert _rel(_np(b[name]), getattr(od, name)) < 1e-11, name
+
+
+def test_density_energy_water_sto3g(pycc_amd):
+    """compute_energy from the GPU densities at the converged water STO-3G amplitudes equals the
+    reference's CCSD energy (pycc tests/test_002_ccsd_energy.py:31)."""
+    from oracle.gaussian import water_sto3g_hamiltonian
+    F, ERI, escf, nocc = water_sto3g_hamiltonian()
+    cc = pycc_amd.CCwfn(DenseHamiltonian(F, ERI), no=nocc - 1, nfzc=1, precision='DP', verbose=False)
+    e = cc.solve_cc(1e-12, 1e-12, 75)
+    lam = pycc_amd.cclambda(cc, pycc_amd.cchbar(cc))
+    lam.solve_lambda(1e-12, 1e-12)
+    dens = pycc_amd.ccdensity(cc, lam)
+    ref, tol = reference_constant("water_sto3g_frozen_core", "ccsd_correlation_energy")
+    assert abs(dens.compute_energy() - e) < 1e-10
+    assert abs(dens.compute_energy() - ref) < tol
+
+
+def test_integral_vjp_finite_differences(pycc_amd):
+    """d(c . r)/dF and d(c . r)/d<pq|rs> from the density blocks against central differences of the
+    GPU residual (Fock elements; 8-fold symmetric ERI perturbations in new contexts)."""
+    no, nv, seed = 3, 6, 45
+    print("integral vjp finite differences", flush=True)
+    H = _ham(no, nv, seed, offdiag=0.03)
+    cc = pycc_amd.CCwfn(H, no=no, precision='DP', verbose=False)
+    nmo = no + nv
+    t1, t2 = random_amplitudes(no, nv, seed + 1, scale=0.05)
+    rng = np.random.default_rng(seed)
+    c1, c2 = rng.standard_normal((no, nv)), rng.standard_normal((no, no, nv, nv))
+    g = {k: _np(x) for k, x in cc.integral_vjp(_dev(t1), _dev(t2), _dev(c1), _dev(c2), vvvv=True).items()}
+    G = _gamma_full(g, no, nv)
+
+    def cr(ctx, F):
+        r1, r2 = ctx.residuals(_dev(F), _dev(t1), _dev(t2))
+        return float(np.sum(c1 * _np(r1)) + np.sum(c2 * _np(r2)))
+
+    h = 1e-5
+    for _ in range(8):
+        p, q = rng.integers(0, nmo, 2)
+        dF = np.zeros((nmo, nmo))
+        dF[p, q] = 1.0
+        fd = (cr(cc, H.F + h * dF) - cr(cc, H.F - h * dF)) / (2 * h)
+        assert abs(fd - g["D"][p, q]) < 1e-7 * max(1.0, abs(fd)), (p, q, fd, g["D"][p, q])
+    for _ in range(8):
+        p, q, r, s = rng.integers(0, nmo, 4)
+        dE = np.zeros((nmo,) * 4)
+        for idx in [(p, q, r, s), (r, q, p, s), (p, s, r, q), (r, s, p, q),
+                    (q, p, s, r), (s, p, q, r), (q, r, s, p), (s, r, q, p)]:
+            dE[idx] = 1.0
+        vals = []
+        for sign in (1.0, -1.0):
+            ctx = pycc_amd.CCwfn(DenseHamiltonian(H.F, H.ERI + sign * h * dE), no=no, precision='DP',
+                                 verbose=False)
+            vals.append(cr(ctx, H.F))
+            del ctx
+        fd = (vals[0] - vals[1]) / (2 * h)
+        an = float(np.sum(G * dE))
+        assert abs(fd - an) < 1e-7 * max(1.0, abs(fd)), ((p, q, r, s), fd, an)
