@@ -296,11 +296,102 @@ def main():
     print(json.dumps(res), flush=True)
 
 
+def _gamma_element(g, no, idx):
+    """Element of the four-fold symmetrised Dirac-order Gamma (ccdensity.py:315-334) from density
+    blocks held on the device: dL/d<pq|rs> for one orbital index quadruple."""
+    table = {"oooo": (0.5, "Doooo", False), "vvvv": (0.5, "Dvvvv", False), "ovov": (1.0, "Dovov", False),
+             "oovv": (0.25, "Doovv", False), "vvoo": (0.25, "Doovv", True), "ooov": (0.5, "Dooov", False),
+             "ovoo": (0.5, "Dooov", True), "vvvo": (0.5, "Dvvvo", False), "vovv": (0.5, "Dvvvo", True)}
+
+    def g0(a, b, c, d):
+        pat = "".join("o" if x < no else "v" for x in (a, b, c, d))
+        if pat not in table:
+            return 0.0
+        coef, name, swap = table[pat]
+        k = tuple(x if x < no else x - no for x in (a, b, c, d))
+        if swap:
+            k = (k[2], k[3], k[0], k[1])
+        return coef * float(g[name][k])
+
+    a, b, c, d = idx
+    return 0.25 * (g0(a, b, c, d) + g0(b, a, d, c) + g0(c, d, a, b) + g0(d, c, b, a))
+
+
+def integral_element_checks(cc, sd, t1, t2, c1, c2, rng, n_fock=16, n_eri=32, h=1e-5):
+    """The integral half of the vJp of c.r(t) at fixed amplitudes against difference quotients.
+    Fock elements: autograd's dF vs central differences through the residual's F argument. ERI
+    elements <pq|rs> (8-fold orbits, seeded): the two-particle density vs one-sided difference
+    quotients over DF contexts with one extra auxiliary vector sqrt(h) u (c.r is linear in the
+    integrals at fixed amplitudes, and a DF context can only add positive rank-one terms), using
+    d(orbit) = q(u_P + u_Q) - q(u_P) - q(u_Q) for P = (p, r), Q = (q, s)."""
+    import types
+    import torch
+    import pycc_amd
+    from pycc_amd import autograd as ccad
+    o, v = cc.no, cc.nv
+    nmo = o + v
+    F = cc.H.F
+
+    def cr(ctx, Fx):
+        r1, r2 = ctx.residuals(Fx, t1, t2)
+        return float((c1 * r1).sum() + (c2 * r2).sum())
+
+    t0 = time.time()
+    Fg = F.clone().requires_grad_(True)
+    gF, = torch.autograd.grad(ccad.residual(cc, t1, t2, F=Fg), (Fg,), (c1, c2))
+    worst_f = 0.0
+    for _ in range(n_fock):
+        p, q = (int(x) for x in rng.integers(0, nmo, 2))
+        dF = torch.zeros_like(F)
+        dF[p, q] = 1.0
+        fd = (cr(cc, F + h * dF) - cr(cc, F - h * dF)) / (2 * h)
+        worst_f = max(worst_f, abs(fd - float(gF[p, q])) / max(1.0, abs(fd)))
+    log("[backward] %d Fock elements: max rel error %.2e" % (n_fock, worst_f))
+    g = cc.integral_vjp(t1, t2, c1, c2, vvvv=True)
+    B = torch.as_tensor(sd.B, device="cuda")
+    base = cr(cc, F)
+    cache = {}
+
+    def quotient(pairs):
+        key = tuple(sorted(set(pairs)))
+        if key not in cache:
+            u = torch.zeros((1, nmo, nmo), dtype=torch.float64, device="cuda")
+            for (a, b) in key:
+                u[0, a, b] = 1.0
+                u[0, b, a] = 1.0
+            Hx = types.SimpleNamespace(B=torch.cat([B, (h ** 0.5) * u]), fock=sd.fock, no=o)
+            ctx = pycc_amd.CCwfn(Hx, no=o, precision="DP", verbose=False)
+            cache[key] = (cr(ctx, F) - base) / h
+            del ctx, Hx
+        return cache[key]
+
+    worst_e, largest = 0.0, 0.0
+    for k in range(n_eri):
+        if k % 8 == 0:
+            log("[backward] ERI elements %d-%d of %d (%.0f s)" % (k, min(n_eri, k + 8) - 1, n_eri, time.time() - t0))
+        p, q, r, s = (int(x) for x in rng.integers(0, nmo, 4))
+        orbit = {(p, q, r, s), (r, q, p, s), (p, s, r, q), (r, s, p, q),
+                 (q, p, s, r), (s, p, q, r), (q, r, s, p), (s, r, q, p)}
+        an = sum(_gamma_element(g, o, idx) for idx in orbit)
+        P, Q = (min(p, r), max(p, r)), (min(q, s), max(q, s))
+        fd = quotient([P]) if P == Q else quotient([P, Q]) - quotient([P]) - quotient([Q])
+        worst_e = max(worst_e, abs(fd - an) / max(1.0, abs(an)))
+        largest = max(largest, abs(an))
+    del g, B
+    log("[backward] %d ERI elements: max rel error %.2e (largest |dL/dERI| %.3e), %d DF contexts, %.0f s" % (
+        n_eri, worst_e, largest, len(cache), time.time() - t0))
+    return {"fock_elements": n_fock, "fock_fd_step": h, "fock_fd_max_rel_error": worst_f,
+            "eri_elements": n_eri, "eri_fd_step": h, "eri_fd": "one-sided, DF rank-one perturbations",
+            "eri_fd_max_rel_error": worst_e, "eri_max_abs_gradient": largest,
+            "seconds": time.time() - t0}
+
+
 def backward_bench(args):
     """Differentiable CCSD at o=32, v=256 (seed 2, FP64): converge, then the backward through the
     torch.autograd.Function with two cotangents (the energy Lagrangian and a seeded N(0,1) one),
-    32 seeded t2 elements against central differences (step 1e-5), timings and device memory with
-    the v^4 HBAR block stored and factored, and the Lambda solve."""
+    32 seeded t2 elements against central differences (step 1e-5), 16 Fock and 32 ERI elements of
+    the integral half against difference quotients, timings and device memory with the v^4 HBAR
+    block stored and factored, and the Lambda solve."""
     import numpy as np
     import torch
     import pycc_amd
@@ -318,12 +409,15 @@ def backward_bench(args):
            "config": {"workload": "differentiable CCSD backward", "no": o, "nv": v, "seed": seed},
            "forward": {"energy": ecc, "iterations": cc.niter, "solve_ms": solve_ms}}
     base_GB = cc.device_bytes() / 1e9
-    for stored in (False, True):
+
+    def one_pass(stored):
         b = backward_pass(cc, store_vvvv=stored)
         b["device_GB_before"] = base_GB
         res["stored" if stored else "factored"] = b
         log("[backward] %s: HBAR %.0f ms, Lambda residual %.0f ms, device %.1f GB" % (
             "stored" if stored else "factored", b["hbar_build_ms"], b["lambda_residual_ms"], b["device_GB"]))
+
+    one_pass(False)
     # autograd with a seeded N(0,1) cotangent vs central differences on 32 seeded t2 elements
     rng = np.random.default_rng(seed)
     c1 = torch.tensor(rng.standard_normal((o, v)), device="cuda")
@@ -352,6 +446,8 @@ def backward_bench(args):
     res["random_cotangent"] = {"vjp_ms_incl_forward_and_hbar": vjp_ms, "fd_t2_elements": 32,
                                "fd_step": h, "fd_max_rel_error": worst}
     log("[backward] random cotangent: %.0f ms, FD max rel error %.2e over 32 t2 elements" % (vjp_ms, worst))
+    res["integral_elements"] = integral_element_checks(cc, sd, t1d, t2d, c1, c2, rng)
+    one_pass(True)            # after the checks above: the stored v^4 HBAR stays in the context
     # Lambda solve (factored HBAR)
     lam = pycc_amd.cclambda(cc, pycc_amd.cchbar(cc))
     lsolve_ms, lecc = _ms(lambda: lam.solve_lambda(1e-10, 1e-10, 100))

@@ -13,7 +13,12 @@ dt2 the directional derivative is sum(g2 * dt2).
 (2 Hov, L), the source terms of the Lambda equations (cclambda.py:348, :468). With both,
 ``E + sum(l1 r1) + 1/2 sum(l2 r2)`` is the CCSD Lagrangian and its gradient is the Lambda residual.
 
-F (the Fock matrix) is held constant: gradients with respect to integrals are not produced.
+When ``F`` requires grad, both Functions also return dF (and the residual's forward mode takes an F
+tangent: the residual is affine in F at fixed amplitudes): the one-particle density of
+``pycc/ccdensity.py:335-470`` at l = (c1, c2 + c2^T) minus its l = 0 part for the residual, at l = 0
+times the incoming gradient for the energy (``CCwfn.integral_vjp`` / ``CCwfn.energy_densities``). The
+two-electron integrals live in the context, not in a tensor argument: their gradient is the
+two-particle density returned by ``CCwfn.integral_vjp``.
 """
 from __future__ import annotations
 
@@ -23,7 +28,7 @@ import torch
 class CCSDResidual(torch.autograd.Function):
     @staticmethod
     def forward(ctx, t1, t2, cc, F, store_vvvv):
-        r1, r2 = cc.residuals(F, t1.detach(), t2.detach())
+        r1, r2 = cc.residuals(F.detach(), t1.detach(), t2.detach())
         ctx.cc = cc
         ctx.F = F
         ctx.store_vvvv = store_vvvv
@@ -32,11 +37,18 @@ class CCSDResidual(torch.autograd.Function):
         return r1, r2
 
     @staticmethod
-    def jvp(ctx, dt1, dt2, _cc, _F, _store):
+    def jvp(ctx, dt1, dt2, _cc, dF, _store):
         t1, t2 = ctx.saved_tensors
+        t1, t2 = t1.detach(), t2.detach()
         dt1 = torch.zeros_like(t1) if dt1 is None else dt1.contiguous()
         dt2 = torch.zeros_like(t2) if dt2 is None else dt2.contiguous()
-        return ctx.cc.jvp(t1.detach(), t2.detach(), dt1, dt2, F=ctx.F, store_vvvv=ctx.store_vvvv)
+        j1, j2 = ctx.cc.jvp(t1, t2, dt1, dt2, F=ctx.F.detach(), store_vvvv=ctx.store_vvvv)
+        if dF is not None:
+            # the residual is affine in F at fixed amplitudes: dr = r(dF, t) - r(0, t)
+            a1, a2 = ctx.cc.residuals(dF.detach().to(torch.float64).contiguous(), t1, t2)
+            b1, b2 = ctx.cc.residuals(torch.zeros_like(ctx.F, dtype=torch.float64), t1, t2)
+            j1, j2 = j1 + (a1 - b1), j2 + (a2 - b2)
+        return j1, j2
 
     @staticmethod
     def backward(ctx, c1, c2):
@@ -44,14 +56,18 @@ class CCSDResidual(torch.autograd.Function):
         cc = ctx.cc
         c1 = torch.zeros_like(t1) if c1 is None else c1.contiguous()
         c2 = torch.zeros_like(t2) if c2 is None else c2.contiguous()
-        g1, g2 = cc.vjp(t1.detach(), t2.detach(), c1, c2, F=ctx.F, store_vvvv=ctx.store_vvvv)
-        return g1, g2, None, None, None
+        F = ctx.F.detach()
+        g1, g2 = cc.vjp(t1.detach(), t2.detach(), c1, c2, F=F, store_vvvv=ctx.store_vvvv)
+        gF = None
+        if ctx.needs_input_grad[3]:
+            gF = cc.integral_vjp(t1.detach(), t2.detach(), c1, c2, fock_only=True)["D"].to(ctx.F.dtype)
+        return g1, g2, None, gF, None
 
 
 class CCSDEnergy(torch.autograd.Function):
     @staticmethod
     def forward(ctx, t1, t2, cc, F):
-        e = cc.cc_energy(F=F, t1=t1.detach(), t2=t2.detach())
+        e = cc.cc_energy(F=F.detach(), t1=t1.detach(), t2=t2.detach())
         ctx.cc = cc
         ctx.F = F
         ctx.save_for_backward(t1, t2)
@@ -60,8 +76,11 @@ class CCSDEnergy(torch.autograd.Function):
     @staticmethod
     def backward(ctx, ge):
         t1, t2 = ctx.saved_tensors
-        d1, d2 = ctx.cc.energy_gradient(t1.detach(), t2.detach(), F=ctx.F)
-        return d1 * ge, d2 * ge, None, None
+        d1, d2 = ctx.cc.energy_gradient(t1.detach(), t2.detach(), F=ctx.F.detach())
+        gF = None
+        if ctx.needs_input_grad[3]:
+            gF = ctx.cc.energy_densities(t1.detach(), t2.detach(), fock_only=True)["D"] * ge
+        return d1 * ge, d2 * ge, None, gF
 
 
 def residual(cc, t1, t2, F=None, store_vvvv=False):

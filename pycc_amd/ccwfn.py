@@ -105,7 +105,10 @@ class CCwfn(object):
             nmo_all = fock.shape[0]
             act = slice(self.nfzc, nmo_all)
             F = torch.as_tensor(np.ascontiguousarray(fock[act, act]), device=dev)
-            Bt = torch.as_tensor(np.asarray(B, dtype=np.float64), device=dev)[:, act, act].contiguous()
+            if isinstance(B, torch.Tensor):
+                Bt = B.to(dev, f64)[:, act, act].contiguous()
+            else:
+                Bt = torch.as_tensor(np.asarray(B, dtype=np.float64), device=dev)[:, act, act].contiguous()
             self._source = ("df", Bt)
         elif hasattr(H, "ERI") and hasattr(H, "F"):
             Fh = H.F.detach().cpu().numpy() if hasattr(H.F, "detach") else np.asarray(H.F)
@@ -318,12 +321,13 @@ class CCwfn(object):
                   _lib.VJP_REUSE_HBAR, self._stream())
         return j1, j2
 
-    def integral_vjp(self, t1, t2, c1, c2, vvvv=False):
+    def integral_vjp(self, t1, t2, c1, c2, vvvv=False, fock_only=False):
         """The integral half of the vector-Jacobian product of ``residuals(F, t1, t2)`` with cotangent
         (c1, c2): d(c1.r1 + c2.r2)/dF and d(...)/d<pq|rs> at fixed amplitudes, as the density blocks of
         ``ccdensity.py:335-931`` at l = (c1, c2 + c2^T) with the energy part (l = 0) removed. Returns a
         dict of device tensors: ``D`` (nmo x nmo, Fock gradient) and ``Doooo``, ``Dooov``, ``Dovov``,
-        ``Doovv``, ``Dvvvo`` (and ``Dvvvv`` with ``vvvv=True``) in the reference's density layout."""
+        ``Doovv``, ``Dvvvo`` (and ``Dvvvv`` with ``vvvv=True``) in the reference's density layout.
+        ``fock_only=True`` returns only ``D``."""
         from .ccdensity import _blocks
         self._require_dp("the integral vJp")
         t1 = self._as_dev(t1, (self.no, self.nv))
@@ -331,10 +335,30 @@ class CCwfn(object):
         c1 = self._as_dev(c1, (self.no, self.nv))
         c2 = self._as_dev(c2, (self.no, self.no, self.nv, self.nv))
         l2 = (c2 + c2.permute(1, 0, 3, 2)).contiguous()
-        with_l = _blocks(self, t1, t2, c1, l2, big=True, vvvv=vvvv)
-        z1, z2 = self._torch.zeros_like(c1), self._torch.zeros_like(c2)
-        energy_part = _blocks(self, t1, t2, z1, z2, big=True, vvvv=vvvv)
-        return {k: (x - energy_part[k]) for k, x in with_l.items() if x is not None}
+        big = not fock_only
+        with_l = _blocks(self, t1, t2, c1, l2, big=big, vvvv=vvvv and big)
+        # l = 0 part; Dvvvo and Dvvvv have an l factor in every term (ccdensity.py:609, :716-741)
+        z1, z2 = self._torch.zeros_like(t1), self._torch.zeros_like(t2)
+        energy_part = _blocks(self, t1, t2, z1, z2, big=False, vvvv=False)
+        del z1, z2
+        keys = ["D"] if fock_only else [k for k, x in with_l.items() if x is not None]
+        out = {}
+        for k in keys:
+            if energy_part.get(k) is not None:
+                with_l[k].sub_(energy_part[k])
+            out[k] = with_l[k]
+        return out
+
+    def energy_densities(self, t1, t2, vvvv=False, fock_only=False):
+        """dE/dF and dE/d<pq|rs> of ``cc_energy`` at (t1, t2): the density blocks of
+        ``ccdensity.py:335-931`` at l = 0 (same layout as ``integral_vjp``)."""
+        from .ccdensity import _blocks
+        self._require_dp("the energy densities")
+        t1 = self._as_dev(t1, (self.no, self.nv))
+        t2 = self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        z1, z2 = self._torch.zeros_like(t1), self._torch.zeros_like(t2)
+        b = _blocks(self, t1, t2, z1, z2, big=not fock_only, vvvv=vvvv and not fock_only)
+        return {"D": b["D"]} if fock_only else {k: x for k, x in b.items() if x is not None}
 
     def energy_gradient(self, t1=None, t2=None, F=None):
         """(dE/dt1, dE/dt2) of ``cc_energy`` = (2 Hov, L): the source terms of the Lambda equations

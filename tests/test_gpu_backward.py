@@ -322,6 +322,56 @@ def test_forward_mode_autograd(pycc_amd):
     assert _rel(_np(tan1), _np(j1)) < 1e-14 and _rel(_np(tan2), _np(j2)) < 1e-14
 
 
+def test_autograd_fock_gradient(pycc_amd):
+    """With F requiring grad: reverse mode through the residual and the energy returns dF (checked
+    against central differences of the GPU residual / energy), and forward mode takes an F tangent."""
+    import torch.autograd.forward_ad as fwAD
+    from pycc_amd import autograd as ccad
+    no, nv, seed = 3, 7, 52
+    print("autograd fock gradient", flush=True)
+    H, cc, ref, t1, t2 = _setup(pycc_amd, no, nv, seed)
+    rng = np.random.default_rng(seed)
+    nmo = no + nv
+    c1, c2 = rng.standard_normal((no, nv)), rng.standard_normal((no, no, nv, nv))
+    Fd = _dev(H.F).requires_grad_(True)
+    r1, r2 = ccad.residual(cc, _dev(t1), _dev(t2), F=Fd)
+    gF, = torch.autograd.grad((r1, r2), (Fd,), (_dev(c1), _dev(c2)))
+    e = ccad.energy(cc, _dev(t1), _dev(t2), F=Fd)
+    eF, = torch.autograd.grad(3.0 * e, (Fd,))
+    gF, eF = _np(gF), _np(eF)
+
+    def cr(F):
+        a1, a2 = cc.residuals(_dev(F), _dev(t1), _dev(t2))
+        return float(np.sum(c1 * _np(a1)) + np.sum(c2 * _np(a2)))
+
+    def en(F):
+        return 3.0 * cc.cc_energy(F=_dev(F), t1=_dev(t1), t2=_dev(t2))
+
+    h = 1e-5
+    for _ in range(10):
+        p, q = rng.integers(0, nmo, 2)
+        dF = np.zeros((nmo, nmo))
+        dF[p, q] = 1.0
+        fd = (cr(H.F + h * dF) - cr(H.F - h * dF)) / (2 * h)
+        assert abs(fd - gF[p, q]) < 1e-7 * max(1.0, abs(fd)), (p, q, fd, gF[p, q])
+        fe = (en(H.F + h * dF) - en(H.F - h * dF)) / (2 * h)
+        assert abs(fe - eF[p, q]) < 1e-7 * max(1.0, abs(fe)), (p, q, fe, eF[p, q])
+    X = rng.standard_normal((nmo, nmo))
+    x1 = _dev(rng.standard_normal((no, nv)))
+    x2 = _dev(_sym2(rng.standard_normal((no, no, nv, nv))))
+    with fwAD.dual_level():
+        Fdual = fwAD.make_dual(_dev(H.F), _dev(X))
+        d1, d2 = fwAD.make_dual(_dev(t1), x1), fwAD.make_dual(_dev(t2), x2)
+        q1, q2 = ccad.residual(cc, d1, d2, F=Fdual)
+        tan1, tan2 = _np(fwAD.unpack_dual(q1).tangent), _np(fwAD.unpack_dual(q2).tangent)
+    j1, j2 = cc.jvp(_dev(t1), _dev(t2), x1, x2)
+    p1, p2 = cc.residuals(_dev(H.F + h * X), _dev(t1), _dev(t2))
+    m1, m2 = cc.residuals(_dev(H.F - h * X), _dev(t1), _dev(t2))
+    fd1 = _np(j1) + (_np(p1) - _np(m1)) / (2 * h)
+    fd2 = _np(j2) + (_np(p2) - _np(m2)) / (2 * h)
+    assert _rel(tan1, fd1) < 1e-8 and _rel(tan2, fd2) < 1e-8
+
+
 # ---- densities: the integral half of the vJp (ccdensity.py:335-931) -------------------------------
 def _gamma_full(b, no, nv):
     """gradient_densities' four-fold symmetrised Gamma (ccdensity.py:315-334) from density blocks."""
@@ -368,6 +418,49 @@ def test_density_energy_water_sto3g(pycc_amd):
     ref, tol = reference_constant("water_sto3g_frozen_core", "ccsd_correlation_energy")
     assert abs(dens.compute_energy() - e) < 1e-10
     assert abs(dens.compute_energy() - ref) < tol
+
+
+def test_hellmann_feynman(pycc_amd):
+    """At converged amplitudes and Lambda, the densities are the derivatives of the converged CCSD
+    energy: central differences (step 1e-4) of E_CC over re-solved perturbed Hamiltonians (Fock
+    elements with F_pq = F_qp kept; 8-fold symmetric ERI elements) against D and Gamma."""
+    no, nv, seed = 3, 7, 61
+    print("hellmann-feynman", flush=True)
+    H = _ham(no, nv, seed, offdiag=0.01)
+    nmo = no + nv
+
+    def solve(Hx, with_density=False):
+        cc = pycc_amd.CCwfn(Hx, no=no, precision='DP', verbose=False)
+        e = cc.solve_cc(1e-13, 1e-13, 200)
+        assert e is not None
+        if not with_density:
+            return e
+        lam = pycc_amd.cclambda(cc, pycc_amd.cchbar(cc))
+        assert lam.solve_lambda(1e-13, 1e-13, 200) is not None
+        d = pycc_amd.ccdensity(cc, lam, store_vvvv=True)
+        b = {k: _np(getattr(d, k)) for k in ("Doooo", "Dooov", "Dovov", "Doovv", "Dvvvo", "Dvvvv")}
+        return e, _np(d._D), _gamma_full(b, no, nv)
+
+    e0, D, G = solve(H, with_density=True)
+    rng = np.random.default_rng(seed)
+    h = 1e-4
+    for p, q in [(0, 4), (1, 1), (5, 8), (2, 0)]:
+        dF = np.zeros((nmo, nmo))
+        dF[p, q] = dF[q, p] = 1.0
+        fd = (solve(DenseHamiltonian(H.F + h * dF, H.ERI))
+              - solve(DenseHamiltonian(H.F - h * dF, H.ERI))) / (2 * h)
+        an = float(np.sum(D * dF))
+        assert abs(fd - an) < 1e-6 * max(1.0, abs(fd)), ((p, q), fd, an)
+    for _ in range(4):
+        p, q, r, s = (int(x) for x in rng.integers(0, nmo, 4))
+        dE = np.zeros((nmo,) * 4)
+        for idx in [(p, q, r, s), (r, q, p, s), (p, s, r, q), (r, s, p, q),
+                    (q, p, s, r), (s, p, q, r), (q, r, s, p), (s, r, q, p)]:
+            dE[idx] = 1.0
+        fd = (solve(DenseHamiltonian(H.F, H.ERI + h * dE))
+              - solve(DenseHamiltonian(H.F, H.ERI - h * dE))) / (2 * h)
+        an = float(np.sum(G * dE))
+        assert abs(fd - an) < 1e-6 * max(1.0, abs(fd)), ((p, q, r, s), fd, an)
 
 
 def test_integral_vjp_finite_differences(pycc_amd):
