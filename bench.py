@@ -32,7 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-PEAK = {"DP": 78.6e12, "MIXED": 157.3e12, "MIXED_BF16X3": 157.3e12}   # ring GEMM precision peak
+# ring GEMM precision peak (FP16x3: three dense FP16 MFMA products per algorithmic product)
+PEAK = {"DP": 78.6e12, "MIXED": 157.3e12, "MIXED_BF16X3": 157.3e12, "MIXED_F16X3": 2.5e15 / 3}
 PEAK_FP64, PEAK_FP32, PEAK_BF16 = 78.6e12, 157.3e12, 2.5e15
 HBM_PEAK = 8.0e12
 
@@ -166,7 +167,7 @@ def main():
     ap.add_argument("--no", type=int, default=48)
     ap.add_argument("--nv", type=int, default=384)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--precision", default="MIXED_BF16X3", choices=["MIXED", "MIXED_BF16X3", "DP"])
+    ap.add_argument("--precision", default="MIXED_BF16X3", choices=["MIXED", "MIXED_BF16X3", "MIXED_F16X3", "DP"])
     ap.add_argument("--maxiter", type=int, default=60)
     ap.add_argument("--e-conv", type=float, default=1e-9, help="energy threshold of the FP64 leg")
     ap.add_argument("--e-conv-mixed", type=float, default=1e-7,
@@ -238,6 +239,7 @@ def main():
     res["scaling"] = "weak"
     res["vs_baseline"] = None
     res["dtype"] = {"MIXED": "f32 (ladder+rings), f64 (rest)", "MIXED_BF16X3": "bf16x3 ladder, f32 rings, f64 rest",
+                    "MIXED_F16X3": "f16x3 (ladder+rings, scaled hi+lo, f32 accumulation), f64 rest",
                     "DP": "f64"}[args.precision]
     res["data"] = "synthetic (seeded DF Hamiltonian, no external data)"
     res["config"] = {"workload": "closed-shell CCSD iteration, synthetic DF Hamiltonian",
@@ -272,14 +274,16 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get(args.precision)
-    ring_prec = "MIXED" if args.precision != "DP" else "DP"
+    ring_prec = args.precision if args.precision in ("DP", "MIXED_F16X3") else "MIXED"
     achieved = fl_r / (ms_r * 1e-3) if ms_r > 0 else 0.0
-    res["roofline"] = {"kernel": "plain_gemm_kernel (ring GEMMs, %s)" % ("f32" if ring_prec == "MIXED" else "f64"),
+    ring_kernel = {"DP": "plain_gemm_kernel (ring GEMMs, f64)", "MIXED": "plain_gemm_kernel (ring GEMMs, f32)",
+                   "MIXED_F16X3": "x3_gemm_kernel<_Float16, 256> (ring GEMMs, f16x3)"}[ring_prec]
+    res["roofline"] = {"kernel": ring_kernel,
                        "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[ring_prec] / 1e12,
                        "unit": "TFLOP/s", "frac": achieved / PEAK[ring_prec], "traffic": traffic,
                        "launches": n_r, "avg_launch_ms": ms_r / max(n_r, 1),
                        "flops_per_launch": fl_r / max(n_r, 1)}
-    lad_peak = PEAK_BF16 / 3 if args.precision == "MIXED_BF16X3" else PEAK[args.precision]
+    lad_peak = PEAK_BF16 / 3 if args.precision in ("MIXED_BF16X3", "MIXED_F16X3") else PEAK[args.precision]
     res["ladder"] = {"achieved_tflops": (fl_l / (ms_l * 1e-3) / 1e12) if ms_l > 0 else 0.0,
                      "peak_tflops": lad_peak / 1e12, "ms_per_step": ms_l / max(mixed["timed_steps"], 1),
                      "flops_per_step": f_ladder_kernel(o, v)}

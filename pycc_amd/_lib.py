@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libccx.so")
 MODE_DP = 0
 MODE_MIXED = 1
 MODE_MIXED_BF16X3 = 2
+MODE_MIXED_F16X3 = 3
 HBAR_STORED = 1
 VJP_REUSE_HBAR = 2
 

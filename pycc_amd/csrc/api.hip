@@ -80,8 +80,9 @@ int ccx_create(int no, int nv, int mode, ccx_ctx** out) {
   return guarded([&] {
     CCX_CHECK(out != nullptr, ccx::ERR_ARG, "null output handle");
     CCX_CHECK(no >= 1 && nv >= 1, ccx::ERR_ARG, "need no >= 1 and nv >= 1");
-    CCX_CHECK(mode == CCX_MODE_DP || mode == CCX_MODE_MIXED || mode == CCX_MODE_MIXED_BF16X3, ccx::ERR_ARG,
-              "mode must be CCX_MODE_DP, CCX_MODE_MIXED or CCX_MODE_MIXED_BF16X3");
+    CCX_CHECK(mode == CCX_MODE_DP || mode == CCX_MODE_MIXED || mode == CCX_MODE_MIXED_BF16X3 ||
+                  mode == CCX_MODE_MIXED_F16X3,
+              ccx::ERR_ARG, "mode must be CCX_MODE_DP, CCX_MODE_MIXED, CCX_MODE_MIXED_BF16X3 or CCX_MODE_MIXED_F16X3");
     int ndev = 0;
     CCX_HIP(hipGetDeviceCount(&ndev));
     CCX_CHECK(ndev > 0, ccx::ERR_HIP, "no HIP device");

@@ -4,11 +4,12 @@
 #include "contract.h"
 #include "gemm.h"
 #include "kernels.h"
+#include "split16.h"
 #include <vector>
 
 namespace ccx {
 
-enum Mode : int { MODE_DP = 0, MODE_MIXED = 1, MODE_MIXED_BF16X3 = 2 };
+enum Mode : int { MODE_DP = 0, MODE_MIXED = 1, MODE_MIXED_BF16X3 = 2, MODE_MIXED_F16X3 = 3 };
 
 struct Ctx {
   int64_t o = 0, v = 0, nmo = 0;
@@ -35,7 +36,14 @@ struct Ctx {
   // packed ladder matrices: rows (e<=f) / (e<f), columns (a<=b)/(a<b) then (m, b)
   void* LP = nullptr;                // double or float
   void* LM = nullptr;
+  // split ladder matrices [N][Kp]: bf16 (MODE_MIXED_BF16X3) or f16 with per-row scales lsP / lsM
+  // (MODE_MIXED_F16X3)
   bf16 *LPh = nullptr, *LPl = nullptr, *LMh = nullptr, *LMl = nullptr;
+  float *lsP = nullptr, *lsM = nullptr;
+  // FP16x3 ring operands (MODE_MIXED_F16X3): split K and J, [ov][ovp]
+  f16 *Kh = nullptr, *Kl = nullptr, *Jh = nullptr, *Jl = nullptr;
+  float* xs = nullptr;               // operand scales: [0] amplitudes, [1] J, [2] K, [3] W
+  float* xpart = nullptr;            // absmax partials (2 x x3_partials())
 
   // ---- per-residual work buffers (allocated on first use) ----
   bool have_work = false;
@@ -43,8 +51,12 @@ struct Ctx {
   double *tau = nullptr, *Wmnij = nullptr, *Hh = nullptr, *Zt = nullptr, *Wd = nullptr, *Ud = nullptr;
   void *S1 = nullptr, *A1 = nullptr, *S2 = nullptr, *Tx = nullptr, *W = nullptr;
   void *Tp = nullptr, *Tm = nullptr, *CP = nullptr, *CM = nullptr;
-  bf16 *Tph = nullptr, *Tpl = nullptr, *Tmh = nullptr, *Tml = nullptr;
+  bf16 *Tph = nullptr, *Tpl = nullptr, *Tmh = nullptr, *Tml = nullptr;   // f16 in MODE_MIXED_F16X3
   float *tau32 = nullptr, *W32 = nullptr;   // mixed modes: FP32 tau and Wmnij (hole-hole terms)
+  // FP16x3 ring operands (see ring_prep_x3): BJ [2ov][ovp], BK [ov][ovp], AP [ov][2ovp],
+  // AQ [ov][ovp], BP = split W^T [ov][2ovp]
+  f16 *BJh = nullptr, *BJl = nullptr, *BKh = nullptr, *BKl = nullptr, *APh = nullptr, *APl = nullptr;
+  f16 *AQh = nullptr, *AQl = nullptr, *BPh = nullptr, *BPl = nullptr;
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
 
@@ -104,6 +116,8 @@ struct Ctx {
   int64_t npM() const { return v * (v - 1) / 2; }
   int64_t noP() const { return o * (o + 1) / 2; }
   int64_t noM() const { return o * (o - 1) / 2; }
+  int64_t ovp() const { return (o * v + 31) / 32 * 32; }   // K padding of the x3 ring operands
+  bool split_ladder() const { return mode == MODE_MIXED_BF16X3 || mode == MODE_MIXED_F16X3; }
 
   template <typename T>
   T* alloc(size_t n) {

@@ -71,8 +71,18 @@ static void alloc_integrals(Ctx& c) {
     c.LP = c.alloc<double>(szP);
     c.LM = c.alloc<double>(szM);
   } else {
-    c.K32 = c.alloc<float>(ov * ov);
-    c.J32 = c.alloc<float>(ov * ov);
+    if (c.mode == MODE_MIXED_F16X3) {
+      const int64_t ovp = c.ovp();
+      for (f16** q : {&c.Kh, &c.Kl, &c.Jh, &c.Jl}) {
+        *q = c.alloc<f16>(ov * ovp);
+        CCX_HIP(hipMemset(*q, 0, ov * ovp * sizeof(f16)));   // K padding
+      }
+      c.xs = c.alloc<float>(64);
+      c.xpart = c.alloc<float>(2 * x3_partials());
+    } else {
+      c.K32 = c.alloc<float>(ov * ov);
+      c.J32 = c.alloc<float>(ov * ov);
+    }
     c.Io32 = c.alloc<float>(ov * ov);
     if (c.mode == MODE_MIXED) {
       c.LP = c.alloc<float>(szP);
@@ -85,6 +95,10 @@ static void alloc_integrals(Ctx& c) {
       c.LMl = c.alloc<bf16>(bM);
       for (bf16* q : {c.LPh, c.LPl}) CCX_HIP(hipMemset(q, 0, bP * sizeof(bf16)));   // K padding
       for (bf16* q : {c.LMh, c.LMl}) CCX_HIP(hipMemset(q, 0, bM * sizeof(bf16)));
+      if (c.mode == MODE_MIXED_F16X3) {
+        c.lsP = c.alloc<float>(c.npP() + ov);
+        c.lsM = c.alloc<float>(c.npM() + ov);
+      }
     }
   }
   c.red = c.alloc<double>(1024 * 64);
@@ -96,6 +110,8 @@ static void pack_ladder(Ctx& c, int64_t e0, int64_t e1, VVVVSrc src, hipStream_t
     ladder_pack<double>(c.o, c.v, e0, e1, src, c.Covvv, (double*)c.LP, (double*)c.LM, s);
   else if (c.mode == MODE_MIXED)
     ladder_pack<float>(c.o, c.v, e0, e1, src, c.Covvv, (float*)c.LP, (float*)c.LM, s);
+  else if (c.mode == MODE_MIXED_F16X3)
+    ladder_pack_x3(c.v, e0, e1, src, (f16*)c.LPh, (f16*)c.LPl, (f16*)c.LMh, (f16*)c.LMl, c.lsP, c.lsM, s);
   else
     ladder_pack_bf16x3(c.o, c.v, e0, e1, src, c.LPh, c.LPl, c.LMh, c.LMl, s);
 }
@@ -115,12 +131,23 @@ static void finish_integrals(Ctx& c, const double* fock, hipStream_t s) {
   axpby<double>(-1.0, tv(c.Cooov, "nime", {o, o, o, v}), 1.0, tv(c.Looov, "imne", {o, o, o, v}), s);
   // J[(me),(nf)] = (mf|ne)
   axpby<double>(1.0, tv(c.Covov, "mfne", {o, v, o, v}), 0.0, tv(c.J, "menf", {o, v, o, v}), s);
-  if (c.mode != MODE_DP) {
+  if (c.mode == MODE_MIXED_F16X3) {
+    // split K and J once, each with its own power-of-two scale (xs[2], xs[1])
+    const int64_t ovp = c.ovp();
+    x3_absmax<double>(c.Covov, ov * ov, c.xpart, s);
+    x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 2, s);
+    x3_split_rows<double>(c.Covov, ov, ov, ov, c.xs + 2, c.Kh, c.Kl, ovp, s);
+    x3_absmax<double>(c.J, ov * ov, c.xpart, s);
+    x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 1, s);
+    x3_split_rows<double>(c.J, ov, ov, ov, c.xs + 1, c.Jh, c.Jl, ovp, s);
+  } else if (c.mode != MODE_DP) {
     cast<double, float>(ov * ov, c.Covov, c.K32, s);
     cast<double, float>(ov * ov, c.J, c.J32, s);
-    cast<double, float>(ov * ov, c.Ioovv, c.Io32, s);
   }
+  if (c.mode != MODE_DP) cast<double, float>(ov * ov, c.Ioovv, c.Io32, s);
   if (c.mode == MODE_MIXED_BF16X3) ladder_pack_o_bf16x3(o, v, c.Covvv, c.LPh, c.LPl, c.LMh, c.LMl, s);
+  if (c.mode == MODE_MIXED_F16X3)
+    ladder_pack_o_x3(o, v, c.Covvv, (f16*)c.LPh, (f16*)c.LPl, (f16*)c.LMh, (f16*)c.LMl, c.lsP, c.lsM, s);
   c.have_ints = true;
 }
 
@@ -189,13 +216,35 @@ void alloc_work(Ctx& c) {
   c.Ud = c.alloc<double>(ov * ov);
   const size_t es = c.mode == MODE_DP ? sizeof(double) : sizeof(float);
   auto talloc = [&](size_t n) -> void* { return es == 8 ? (void*)c.alloc<double>(n) : (void*)c.alloc<float>(n); };
-  c.S1 = talloc(2 * ov * ov);
-  c.A1 = talloc(ov * ov);
-  c.S2 = talloc(2 * ov * ov);
-  c.Tx = talloc(ov * ov);
+  if (c.mode == MODE_MIXED_F16X3) {
+    // FP32 ring outputs P (A1), Q (S1) and the :930 scratch (S2); the operands are split FP16
+    c.S1 = talloc(ov * ov);
+    c.A1 = talloc(ov * ov);
+    c.S2 = talloc(ov * ov);
+    const int64_t ovp = c.ovp();
+    auto halloc = [&](f16** q, int64_t n) {
+      *q = c.alloc<f16>(n);
+      CCX_HIP(hipMemset(*q, 0, n * sizeof(f16)));   // K padding stays zero
+    };
+    halloc(&c.BJh, 2 * ov * ovp);
+    halloc(&c.BJl, 2 * ov * ovp);
+    halloc(&c.BKh, ov * ovp);
+    halloc(&c.BKl, ov * ovp);
+    halloc(&c.APh, 2 * ov * ovp);
+    halloc(&c.APl, 2 * ov * ovp);
+    halloc(&c.AQh, ov * ovp);
+    halloc(&c.AQl, ov * ovp);
+    halloc(&c.BPh, 2 * ov * ovp);
+    halloc(&c.BPl, 2 * ov * ovp);
+  } else {
+    c.S1 = talloc(2 * ov * ov);
+    c.A1 = talloc(ov * ov);
+    c.S2 = talloc(2 * ov * ov);
+    c.Tx = talloc(ov * ov);
+  }
   c.W = c.mode == MODE_DP ? (void*)c.Wd : talloc(2 * ov * ov);
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
-  if (c.mode == MODE_MIXED_BF16X3) {
+  if (c.split_ladder()) {
     const int64_t aP = noP * bf16_kpad(npP), aM = noM * bf16_kpad(npM);
     c.Tph = c.alloc<bf16>(aP);
     c.Tpl = c.alloc<bf16>(aP);
@@ -217,6 +266,7 @@ void alloc_work(Ctx& c) {
 }
 
 void ladder_bf16x3(Ctx& c, const double* t1, const double* t2, hipStream_t s);   // ladder_bf16x3.hip
+void ladder_x3(Ctx& c, const double* t1, const double* t2, hipStream_t s);       // split16.hip
 
 // The ring and ladder GEMMs run in T (double for DP, float for the mixed modes).
 template <typename T>
@@ -282,7 +332,17 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   T* S2 = (T*)c.S2;
   T* Tx = (T*)c.Tx;
   T* W = (T*)c.W;
-  ring_prep<T>(o, v, t1, t2, S1, A1, S2, Tx, c.Ud, s);
+  const bool x3 = c.mode == MODE_MIXED_F16X3;
+  if (x3) {
+    // amplitude scale: every ring operand is bounded by 3 max|t2| + max|t1|^2
+    const int np = x3_partials();
+    x3_absmax<double>(t2, ov * ov, c.xpart, s);
+    x3_absmax<double>(t1, ov, c.xpart + np, s);
+    x3_scale(c.xpart, 3.f, c.xpart + np, 1.f, c.xs, s);
+    ring_prep_x3(o, v, c.ovp(), t1, t2, c.xs, c.BJh, c.BJl, c.BKh, c.BKl, c.APh, c.APl, c.AQh, c.AQl, c.Ud, s);
+  } else {
+    ring_prep<T>(o, v, t1, t2, S1, A1, S2, Tx, c.Ud, s);
+  }
   // W[(me)][x][(jb)]: x = 0 holds Wmbej, x = 1 holds Wmbje
   const std::vector<int64_t> WSTR{v * 2 * ov, 2 * ov, v, 1};
   const TV Wej = tvs(c.Wd, "mejb", OVOV, WSTR), Wje = tvs(c.Wd + ov, "mejb", OVOV, WSTR);
@@ -299,17 +359,40 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // :644 -tau'_jnfb <mn|ef> (tau' = t2/2 + t1 t1), :645 +1/2 t_njfb L_mnef, :683 +tau'_jnfb <mn|fe>:
   //   Wej += K.(T - Tx/2 - P1) - 1/2 J.T ;  Wje += J.tau'
   const double unit = 2.0 * (double)ov * ov * ov;   // one ov x ov x ov GEMM
+  const int64_t ovp = c.ovp();
   int kr = c.tbegin(s, Ctx::PH_RING, 2 * unit);
-  contract<T>(T(1), tv(Jt, "menf", OVOV), tv(S1, "nfxjb", OVXOV), T(1), tv(W, "mexjb", OVXOV), ws, s);
+  if (x3) {
+    X3Gemm g;
+    g.M = ov; g.N = 2 * ov; g.Kp = ovp; g.lda = ovp; g.ldb = ovp; g.ldc = 2 * ov;
+    g.Ah = c.Jh; g.Al = c.Jl; g.Bh = c.BJh; g.Bl = c.BJl; g.C = (float*)W;
+    g.sa = c.xs + 1; g.sb = c.xs; g.beta = 1.f;
+    x3_gemm<_Float16>(g, s);
+  } else {
+    contract<T>(T(1), tv(Jt, "menf", OVOV), tv(S1, "nfxjb", OVXOV), T(1), tv(W, "mexjb", OVXOV), ws, s);
+  }
   c.tend(s, kr);
   kr = c.tbegin(s, Ctx::PH_RING, unit);
-  contract<T>(T(1), tv(Kt, "menf", OVOV), tv(A1, "nfjb", OVOV), T(1), tvs(W, "mejb", OVOV, WSTR), ws, s);
+  if (x3) {
+    X3Gemm g;
+    g.M = ov; g.N = ov; g.Kp = ovp; g.lda = ovp; g.ldb = ovp; g.ldc = 2 * ov;
+    g.Ah = c.Kh; g.Al = c.Kl; g.Bh = c.BKh; g.Bl = c.BKl; g.C = (float*)W;
+    g.sa = c.xs + 2; g.sb = c.xs; g.beta = 1.f;
+    x3_gemm<_Float16>(g, s);
+    // W (FP32 [me][x][jb]) -> split BP[jb][x][me] with its own scale xs[3]
+    x3_absmax<float>((const float*)W, 2 * ov * ov, c.xpart, s);
+    x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 3, s);
+    x3_split_t((const float*)W, ov, ov, 2, 2 * ov, c.xs + 3, c.BPh, c.BPl, ovp, 2 * ovp, s);
+  } else {
+    contract<T>(T(1), tv(Kt, "menf", OVOV), tv(A1, "nfjb", OVOV), T(1), tvs(W, "mejb", OVOV, WSTR), ws, s);
+  }
   c.tend(s, kr);
 
   // ---- ladder + Z: ccwfn.py:931 (1/2 tau_ijef <ab|ef>) and :715 (Z_mbij = <mb|ef> tau_ijef) ----
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
   const double f_lad_P = 2.0 * noP * (npP + ov) * npP, f_lad_M = 2.0 * noM * (npM + ov) * npM;
-  if (c.mode == MODE_MIXED_BF16X3) {
+  if (c.mode == MODE_MIXED_F16X3) {
+    ladder_x3(c, t1, t2, s);
+  } else if (c.mode == MODE_MIXED_BF16X3) {
     ladder_bf16x3(c, t1, t2, s);
   } else {
     tau_pack<T>(o, v, t1, t2, (T*)c.Tp, (T*)c.Tm, s);
@@ -331,10 +414,26 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   T* P = A1;   // A1 is dead after the K GEMM
   T* Q = S1;   // S1 is dead after the J GEMM
   kr = c.tbegin(s, Ctx::PH_RING, 2 * unit);
-  contract<T>(T(1), tv(S2, "iamex", {o, v, o, v, 2}), tv(W, "mexjb", OVXOV), T(0), tv(P, "iajb", OVOV), ws, s);
+  if (x3) {
+    X3Gemm g;
+    g.M = ov; g.N = ov; g.Kp = 2 * ovp; g.lda = 2 * ovp; g.ldb = 2 * ovp; g.ldc = ov;
+    g.Ah = c.APh; g.Al = c.APl; g.Bh = c.BPh; g.Bl = c.BPl; g.C = (float*)P;
+    g.sa = c.xs; g.sb = c.xs + 3;
+    x3_gemm<_Float16>(g, s);
+  } else {
+    contract<T>(T(1), tv(S2, "iamex", {o, v, o, v, 2}), tv(W, "mexjb", OVXOV), T(0), tv(P, "iajb", OVOV), ws, s);
+  }
   c.tend(s, kr);
   kr = c.tbegin(s, Ctx::PH_RING, unit);
-  contract<T>(T(1), tv(Tx, "jame", OVOV), tvs(W + ov, "meib", OVOV, WSTR), T(0), tv(Q, "jaib", OVOV), ws, s);
+  if (x3) {
+    X3Gemm g;
+    g.M = ov; g.N = ov; g.Kp = ovp; g.lda = ovp; g.ldb = 2 * ovp; g.ldc = ov;
+    g.Ah = c.AQh; g.Al = c.AQl; g.Bh = c.BPh + ovp; g.Bl = c.BPl + ovp; g.C = (float*)Q;
+    g.sa = c.xs; g.sb = c.xs + 3;
+    x3_gemm<_Float16>(g, s);
+  } else {
+    contract<T>(T(1), tv(Tx, "jame", OVOV), tvs(W + ov, "meib", OVOV, WSTR), T(0), tv(Q, "jaib", OVOV), ws, s);
+  }
   c.tend(s, kr);
 
   // ---- Zt_mbij = Z_mbij + <mb|ij> + t_ie <mb|ej> + t_je <mb|ie> -----------------------------------

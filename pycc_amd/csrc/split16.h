@@ -1,0 +1,52 @@
+// Split 16-bit ("x3") GEMMs and their operand conversions (split16.hip).
+#pragma once
+#include "common.h"
+#include "kernels.h"
+
+namespace ccx {
+
+typedef _Float16 f16;
+
+// C[m][n] = alpha / (sa sb) * sum_k (Ah Bh + Ah Bl + Al Bh)[m][n][k] + beta C[m][n], with A stored
+// [M][lda] and B stored [N][ldb], both K-contiguous, K padded to Kp (a multiple of 32, zeros in
+// the padding of both operands). sa / sb are device scalars (operand scales) or both null.
+struct X3Gemm {
+  int64_t M = 0, N = 0, Kp = 0, lda = 0, ldb = 0, ldc = 0;
+  const void *Ah = nullptr, *Al = nullptr, *Bh = nullptr, *Bl = nullptr;
+  float* C = nullptr;
+  const float *sa = nullptr, *sb = nullptr;
+  const float* sbn = nullptr;        // per-row scales of B (one per n), in place of sb
+  float alpha = 1.f, beta = 0.f;
+};
+template <typename E>   // __bf16 or _Float16
+void x3_gemm(const X3Gemm& p, hipStream_t s);
+
+// max |x| over n elements as x3_partials() per-block partial maxima in `part`.
+template <typename T>
+void x3_absmax(const T* x, int64_t n, float* part, hipStream_t s);
+int x3_partials();
+// *s_out = 2^k such that (ca max(pa) + cb max(pb)^2) * 2^k < 2^15 (pb may be null).
+void x3_scale(const float* pa, float ca, const float* pb, float cb, float* s_out, hipStream_t s);
+
+// hi/lo[r * ldd + c] = split(src[r * lds + c] * sc), r < rows, c < cols.
+template <typename Tin>
+void x3_split_rows(const Tin* src, int64_t rows, int64_t cols, int64_t lds, const float* sc, f16* hi, f16* lo,
+                   int64_t ldd, hipStream_t s);
+// hi/lo[cc * ldd + x * Kb + r] = split(src[r * lds + x * Cb + cc] * sc), r < R, x < nx, cc < Cb.
+void x3_split_t(const float* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
+                int64_t Kb, int64_t ldd, hipStream_t s);
+// FP16x3 packed ladder matrices (the BF16x3 layouts of kernels.h, [N][Kp]) with one power-of-two
+// scale per row in sP / sM: the (ab) rows for e in [e0, e1) from src, and the (m, b) rows from Covvv.
+void ladder_pack_x3(int64_t v, int64_t e0, int64_t e1, VVVVSrc src, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
+                    float* sP, float* sM, hipStream_t s);
+void ladder_pack_o_x3(int64_t o, int64_t v, const double* Covvv, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
+                      float* sP, float* sM, hipStream_t s);
+// FP16x3 packed tau rows (tau_pack_bf16x3 layout) scaled by *sc.
+void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2, const float* sc,
+                 f16* Tph, f16* Tpl, f16* Tmh, f16* Tml, hipStream_t s);
+// The split ring operands BJ, BK, AP, AQ (row stride ovp, or 2 ovp for AP) and Ud from t1, t2.
+void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,
+                  f16* BJl, f16* BKh, f16* BKl, f16* APh, f16* APl, f16* AQh, f16* AQl, double* Ud,
+                  hipStream_t s);
+
+}  // namespace ccx

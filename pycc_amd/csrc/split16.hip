@@ -1,0 +1,573 @@
+// Split 16-bit ("x3") GEMMs for gfx950 and the conversions that feed them.
+//
+// An operand value x is carried as two 16-bit numbers, x*s = hi + lo + err with hi = E(x*s) and
+// lo = E(x*s - hi), s a power-of-two scale of the whole operand. A product is taken as
+// (hi_a hi_b + hi_a lo_b + lo_a hi_b) / (s_a s_b): three MFMAs accumulated in FP32, lo*lo dropped.
+//  * E = _Float16 (11-bit significands): hi + lo hold 22 bits, so the dropped and rounded parts are
+//    ~2^-22 |x|, four times FP32 rounding. s puts the operand's bound below 2^15 (FP16 max 65504);
+//    values under 2^-14 (subnormal) keep an absolute error <= 2^-25, ~2^-40 of the bound.
+//  * E = bf16 (8-bit significands, FP32 exponent range): 16 bits, ~2^-16 |x|; no scale needed.
+// Both operands are K-contiguous ("NT": C[m][n] = sum_k A[m][k] B[n][k]) with row strides lda and
+// ldb and K padded to a multiple of 32 with zeros.
+#include "ccsd.h"
+#include "split16.h"
+#include <cmath>
+
+namespace ccx {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename E> struct X3;
+template <> struct X3<__bf16> {
+  typedef __bf16 v8 __attribute__((ext_vector_type(8)));
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct X3<_Float16> {
+  typedef _Float16 v8 __attribute__((ext_vector_type(8)));
+  static __device__ __forceinline__ f4 mfma(v8 a, v8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+namespace {
+
+constexpr int BK = 32, KPAD = 8;   // LDS row = 40 elements = 80 bytes
+constexpr int GM = 8;              // M tiles per tile group (operand reuse in one XCD's L2)
+
+// Hardware workgroup id (round-robin over the 8 XCDs) -> logical tile id, so that each XCD runs a
+// contiguous range of logical tiles.
+__device__ __forceinline__ int64_t xcd_remap_x3(int64_t w, int64_t ntiles) {
+  const int64_t nx = 8;
+  int64_t q = ntiles / nx, r = ntiles % nx;
+  int64_t x = w % nx, j = w / nx;
+  return x * q + (x < r ? x : r) + j;
+}
+
+// BT x BT output tiles, 4 waves in 2 x 2, each computing (BT/2) x (BT/2) as 16 x 16 MFMA blocks.
+// Operand tiles (BT rows x 32 k, hi and lo) are double-buffered in LDS with one register stage:
+// the next k-tile's global loads are issued before the current tile's MFMAs.
+// MFMA operand layout: lane l holds A[l % 16][8 (l / 16) .. +7] and B[l % 16][8 (l / 16) .. +7];
+// C register r of lane l is C[4 (l / 16) + r][l % 16].
+template <typename E, int BT>
+__global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
+  typedef typename X3<E>::v8 v8;
+  constexpr int Q = BT / 64;     // rows per thread in one operand tile
+  constexpr int WT = BT / 2;
+  constexpr int NB = WT / 16;
+  __shared__ __attribute__((aligned(16))) E sA[2][2][BT][BK + KPAD];   // [buf][hi/lo]
+  __shared__ __attribute__((aligned(16))) E sB[2][2][BT][BK + KPAD];
+  const E* __restrict__ Ah = (const E*)p.Ah;
+  const E* __restrict__ Al = (const E*)p.Al;
+  const E* __restrict__ Bh = (const E*)p.Bh;
+  const E* __restrict__ Bl = (const E*)p.Bl;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BT - 1) / BT;
+  const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
+  // groups of GM M tiles sweep all N tiles: consecutive tiles share B rows and GM A panels
+  const int64_t group = t / (GM * tiles_n), first_m = group * GM;
+  const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
+  const int64_t within = t - group * GM * tiles_n;
+  const int64_t tm = first_m + within % gsize, tn = within / gsize;
+
+  const int lr = tid >> 2, lk = (tid & 3) * 8;
+  int64_t offA[Q], offB[Q];
+  bool okA[Q], okB[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int64_t m = tm * BT + lr + 64 * q, n = tn * BT + lr + 64 * q;
+    okA[q] = m < p.M;
+    okB[q] = n < p.N;
+    // rows beyond the matrix read row 0 (valid memory) and are zeroed before the LDS store
+    offA[q] = (okA[q] ? m : 0) * p.lda + lk;
+    offB[q] = (okB[q] ? n : 0) * p.ldb + lk;
+  }
+  const v8 zero8 = v8{};
+  v8 rA[2][Q], rB[2][Q];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      rA[0][q] = *(const v8*)(Ah + offA[q] + k0);
+      rA[1][q] = *(const v8*)(Al + offA[q] + k0);
+      rB[0][q] = *(const v8*)(Bh + offB[q] + k0);
+      rB[1][q] = *(const v8*)(Bl + offB[q] + k0);
+    }
+  };
+  auto sstore = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        *(v8*)&sA[b][h][lr + 64 * q][lk] = okA[q] ? rA[h][q] : zero8;
+        *(v8*)&sB[b][h][lr + 64 * q][lk] = okB[q] ? rB[h][q] : zero8;
+      }
+  };
+
+  f4 acc[NB][NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  const int fr = lane >> 4, fc = lane & 15;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < p.Kp; k0 += BK) {
+    const bool has_next = k0 + BK < p.Kp;
+    if (has_next) gload(k0 + BK);
+    v8 a[NB], b[NB];
+    // lo(A) x hi(B)
+#pragma unroll
+    for (int i = 0; i < NB; ++i) a[i] = *(const v8*)&sA[buf][1][wm * WT + i * 16 + fc][fr * 8];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) b[j] = *(const v8*)&sB[buf][0][wn * WT + j * 16 + fc][fr * 8];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[i][j] = X3<E>::mfma(a[i], b[j], acc[i][j]);
+    // hi(A) x hi(B)
+#pragma unroll
+    for (int i = 0; i < NB; ++i) a[i] = *(const v8*)&sA[buf][0][wm * WT + i * 16 + fc][fr * 8];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[i][j] = X3<E>::mfma(a[i], b[j], acc[i][j]);
+    // hi(A) x lo(B)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) b[j] = *(const v8*)&sB[buf][1][wn * WT + j * 16 + fc][fr * 8];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[i][j] = X3<E>::mfma(a[i], b[j], acc[i][j]);
+    if (has_next) sstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+
+  float f0 = p.alpha;
+  if (p.sa) f0 = f0 / p.sa[0];
+  if (p.sb) f0 = f0 / p.sb[0];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int64_t n = tn * BT + wn * WT + j * 16 + fc;
+    if (n >= p.N) continue;
+    const float f = p.sbn ? f0 / p.sbn[n] : f0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = tm * BT + wm * WT + i * 16 + fr * 4 + r;
+        if (m < p.M) {
+          float* d = p.C + m * p.ldc + n;
+          float val = f * acc[i][j][r];
+          if (p.beta != 0.f) val += p.beta * *d;
+          *d = val;
+        }
+      }
+  }
+}
+
+// ---- operand conversions ------------------------------------------------------------------
+constexpr int X3_RED_BLOCKS = 1024;
+
+// 2^k with bound * 2^k < 2^15 (k clamped to [-40, 60]).
+__device__ __forceinline__ float pow2_scale(float bound) {
+  int e = 0;
+  frexpf(bound, &e);
+  int k = 15 - e;
+  k = k < -40 ? -40 : (k > 60 ? 60 : k);
+  return ldexpf(1.0f, k);
+}
+
+__device__ __forceinline__ float block_max(float x) {
+  __shared__ float sh[256];
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  float r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void absmax_kernel(const T* __restrict__ x, int64_t n, float* __restrict__ part) {
+  double m = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs((double)x[e]));
+  float r = block_max((float)m);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// s = 2^(15 - e) with bound = ca max|a| + cb max|b|^2 < 2^e: bound * s < 2^15.
+__global__ __launch_bounds__(256) void x3_scale_kernel(const float* __restrict__ pa, int na, float ca,
+                                                       const float* __restrict__ pb, int nb, float cb,
+                                                       float* __restrict__ s_out) {
+  float ma = 0.f, mb = 0.f;
+  for (int k = threadIdx.x; k < na; k += blockDim.x) ma = fmaxf(ma, pa[k]);
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) mb = fmaxf(mb, pb[k]);
+  ma = block_max(ma);
+  mb = block_max(mb);
+  const float sc = pow2_scale(ca * ma + cb * mb * mb);
+  if (threadIdx.x == 0) s_out[0] = sc;
+}
+
+template <typename E>
+__device__ __forceinline__ void split_store(float x, E* hi, E* lo, int64_t at) {
+  const E h = (E)x;
+  hi[at] = h;
+  lo[at] = (E)(x - (float)h);
+}
+
+template <typename Tin, typename E>
+__global__ __launch_bounds__(256) void split_rows_kernel(const Tin* __restrict__ src, int64_t rows, int64_t cols,
+                                                         int64_t lds, const float* __restrict__ sc, E* hi, E* lo,
+                                                         int64_t ldd) {
+  const double s = sc[0];
+  const int64_t total = rows * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / cols, c = e - r * cols;
+    split_store<E>((float)((double)src[r * lds + c] * s), hi, lo, r * ldd + c);
+  }
+}
+
+// dst[cc * ldd + x * Kb + r] = split(src[r * lds + x * Cb + cc]) for r < R, x < nx, cc < Cb,
+// through 32 x 33 LDS tiles (coalesced reads along cc, writes along r).
+template <typename E>
+__global__ __launch_bounds__(256) void split_t_kernel(const float* __restrict__ src, int64_t R, int64_t Cb, int nx,
+                                                      int64_t lds, const float* __restrict__ sc, E* hi, E* lo,
+                                                      int64_t Kb, int64_t ldd) {
+  __shared__ float tile[32][33];
+  const float s = sc[0];
+  const int64_t tr = (R + 31) / 32, tc = (Cb + 31) / 32;
+  const int64_t per_x = tr * tc;
+  for (int64_t w = blockIdx.x; w < per_x * nx; w += gridDim.x) {
+    const int64_t x = w / per_x, rem = w - x * per_x;
+    const int64_t r0 = (rem / tc) * 32, c0 = (rem % tc) * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int y = ty; y < 32; y += 8) {
+      const int64_t r = r0 + y, cc = c0 + tx;
+      tile[y][tx] = (r < R && cc < Cb) ? src[r * lds + x * Cb + cc] : 0.f;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+      const int64_t cc = c0 + y, r = r0 + tx;
+      if (cc < Cb && r < R) split_store<E>(tile[tx][y] * s, hi, lo, cc * ldd + x * Kb + r);
+    }
+    __syncthreads();
+  }
+}
+
+// Split ring operands straight from t1, t2 (layouts of the FP32 ring_prep, regrouped so that
+// every ring GEMM is K-contiguous on both sides; T = t2_ijab, X = t2_ijba, P = t_ja t_ib at
+// (ia, jb), ov = o v, Kp = ovp):
+//   BJ[x][(jb)][(ia)]  x = 0: -T/2, x = 1: X/2 + P     (B of the J GEMM, N = (x, jb), K = ia)
+//   BK[(jb)][(ia)]     T - X/2 - P                      (B of the K GEMM)
+//   AP[(ia)][x][(jb)]  x = 0: 2T - X, x = 1: T          (A of the ring GEMM, K = (x, me))
+//   AQ[(ia)][(jb)]     X                                (A of the crossed ring GEMM)
+//   Ud[(ia)][(jb)]     2T - X in FP64
+template <typename E>
+__global__ __launch_bounds__(256) void ring_prep_x3_kernel(int64_t o, int64_t v, int64_t ovp,
+                                                           const double* __restrict__ t1,
+                                                           const double* __restrict__ t2,
+                                                           const float* __restrict__ sc, E* BJh, E* BJl,
+                                                           E* BKh, E* BKl, E* APh, E* APl, E* AQh, E* AQl,
+                                                           double* __restrict__ Ud) {
+  __shared__ double tn[32][33];
+  __shared__ double tx[32][33];
+  const double s = sc[0];
+  const int64_t ij = blockIdx.y;
+  const int64_t i = ij / o, j = ij % o;
+  const int64_t tiles_b = (v + 31) / 32;
+  const int64_t a0 = (blockIdx.x / tiles_b) * 32, b0 = (blockIdx.x % tiles_b) * 32;
+  const double* blk = t2 + ij * v * v;
+  const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+  for (int y = y0; y < 32; y += 8) {
+    int64_t a = a0 + y, b = b0 + x;
+    tn[y][x] = (a < v && b < v) ? blk[a * v + b] : 0.0;   // T(a0 + y, b0 + x)
+    int64_t bb = b0 + y, aa = a0 + x;
+    tx[y][x] = (bb < v && aa < v) ? blk[bb * v + aa] : 0.0;   // X(a0 + x, b0 + y)
+  }
+  __syncthreads();
+  const int64_t ov = o * v;
+  // b fastest: AP, AQ, Ud
+  for (int y = y0; y < 32; y += 8) {
+    const int64_t a = a0 + y, b = b0 + x;
+    if (a >= v || b >= v) continue;
+    const double T = tn[y][x], X = tx[x][y];
+    const int64_t ia = i * v + a, jb = j * v + b;
+    split_store<E>((float)((2.0 * T - X) * s), APh, APl, ia * 2 * ovp + jb);
+    split_store<E>((float)(T * s), APh, APl, ia * 2 * ovp + ovp + jb);
+    split_store<E>((float)(X * s), AQh, AQl, ia * ovp + jb);
+    Ud[ia * ov + jb] = 2.0 * T - X;
+  }
+  // a fastest: BJ, BK
+  for (int y = y0; y < 32; y += 8) {
+    const int64_t a = a0 + x, b = b0 + y;
+    if (a >= v || b >= v) continue;
+    const double T = tn[x][y], X = tx[y][x];
+    const double P = t1[j * v + a] * t1[i * v + b];
+    const int64_t ia = i * v + a, jb = j * v + b;
+    split_store<E>((float)(-0.5 * T * s), BJh, BJl, jb * ovp + ia);
+    split_store<E>((float)((0.5 * X + P) * s), BJh, BJl, (ov + jb) * ovp + ia);
+    split_store<E>((float)((T - 0.5 * X - P) * s), BKh, BKl, jb * ovp + ia);
+  }
+}
+
+// ---- packed ladder operands (layouts of the BF16x3 ones in kernels.hip) ----------------------
+__device__ __forceinline__ void decode_pair(int64_t c, int64_t& a, int64_t& b) {
+  int64_t f = (int64_t)((sqrt(8.0 * (double)c + 1.0) - 1.0) * 0.5);
+  while (f * (f + 1) / 2 > c) --f;
+  while ((f + 1) * (f + 2) / 2 <= c) ++f;
+  b = f;
+  a = c - f * (f + 1) / 2;
+}
+
+// One workgroup per row (e <= f) of the (ab) block: row maxima first, then the scaled split values
+// 1/2(<ab|ef> + <ab|fe>) (row pairP(e, f) of LP) and, for e < f, 1/2(<ab|ef> - <ab|fe>) (row
+// pairM(e, f) of LM, columns a < b).
+template <typename E>
+__global__ __launch_bounds__(256) void ladder_pack_x3_kernel(int64_t v, int64_t e0, VVVVSrc src, E* LPh, E* LPl,
+                                                             E* LMh, E* LMl, float* sP, float* sM, int64_t KpP,
+                                                             int64_t KpM) {
+  const int64_t npP = v * (v + 1) / 2;
+  const int64_t e = e0 + blockIdx.y / v, f = blockIdx.y % v;
+  if (f < e) return;
+  const int64_t rowP = pairP(e, f);
+  const bool hasM = e < f;
+  const int64_t rowM = hasM ? pairM(e, f) : 0;
+  auto vals = [&](int64_t c, double& x, double& y, bool& mcol, int64_t& cm) {
+    int64_t a, b;
+    decode_pair(c, a, b);
+    x = src.base[e * src.se + a * src.sa + f * src.sf + b * src.sb];   // <ab|ef> = (ea|fb)
+    y = src.base[e * src.se + b * src.sa + f * src.sf + a * src.sb];   // <ab|fe> = (eb|fa)
+    mcol = a < b;
+    cm = mcol ? pairM(a, b) : 0;
+  };
+  double mp = 0.0, mm = 0.0;
+  for (int64_t c = threadIdx.x; c < npP; c += blockDim.x) {
+    double x, y;
+    bool mcol;
+    int64_t cm;
+    vals(c, x, y, mcol, cm);
+    mp = fmax(mp, fabs(0.5 * (x + y)));
+    if (mcol) mm = fmax(mm, fabs(0.5 * (x - y)));
+  }
+  const float scP = pow2_scale(block_max((float)mp));
+  const float scM = pow2_scale(block_max((float)mm));
+  if (threadIdx.x == 0) {
+    sP[rowP] = scP;
+    if (hasM) sM[rowM] = scM;
+  }
+  for (int64_t c = threadIdx.x; c < npP; c += blockDim.x) {
+    double x, y;
+    bool mcol;
+    int64_t cm;
+    vals(c, x, y, mcol, cm);
+    split_store<E>((float)(0.5 * (x + y) * scP), LPh, LPl, rowP * KpP + c);
+    if (hasM && mcol) split_store<E>((float)(0.5 * (x - y) * scM), LMh, LMl, rowM * KpM + cm);
+  }
+}
+
+// One workgroup per (m, b): row npP + mb of LP holds 1/2(<mb|ef> + <mb|fe>) over (e <= f), row
+// npM + mb of LM holds 1/2(<mb|ef> - <mb|fe>) over (e < f).
+template <typename E>
+__global__ __launch_bounds__(256) void ladder_pack_o_x3_kernel(int64_t o, int64_t v, const double* __restrict__ Covvv,
+                                                               E* LPh, E* LPl, E* LMh, E* LMl, float* sP, float* sM) {
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
+  const int64_t KpP = bf16_kpad(npP), KpM = bf16_kpad(npM);
+  const int64_t mb = blockIdx.y, m = mb / v, b = mb % v;
+  auto vals = [&](int64_t k, double& x, double& y, int64_t& ef_m, bool& lt) {
+    int64_t e, f;
+    decode_pair(k, e, f);
+    x = Covvv[((m * v + e) * v + b) * v + f];   // <mb|ef> = (me|bf)
+    y = Covvv[((m * v + f) * v + b) * v + e];   // <mb|fe> = (mf|be)
+    lt = e < f;
+    ef_m = lt ? pairM(e, f) : 0;
+  };
+  double mp = 0.0, mm = 0.0;
+  for (int64_t k = threadIdx.x; k < npP; k += blockDim.x) {
+    double x, y;
+    int64_t q;
+    bool lt;
+    vals(k, x, y, q, lt);
+    mp = fmax(mp, fabs(0.5 * (x + y)));
+    if (lt) mm = fmax(mm, fabs(0.5 * (x - y)));
+  }
+  const float scP = pow2_scale(block_max((float)mp));
+  const float scM = pow2_scale(block_max((float)mm));
+  if (threadIdx.x == 0) {
+    sP[npP + mb] = scP;
+    sM[npM + mb] = scM;
+  }
+  for (int64_t k = threadIdx.x; k < npP; k += blockDim.x) {
+    double x, y;
+    int64_t q;
+    bool lt;
+    vals(k, x, y, q, lt);
+    split_store<E>((float)(0.5 * (x + y) * scP), LPh, LPl, (npP + mb) * KpP + k);
+    if (lt) split_store<E>((float)(0.5 * (x - y) * scM), LMh, LMl, (npM + mb) * KpM + q);
+  }
+}
+
+// Row (i <= j) of the packed tau+ (and, for i < j, tau-) amplitudes, scaled by *sc.
+template <typename E>
+__global__ __launch_bounds__(256) void tau_pack_x3_kernel(int64_t o, int64_t v, int64_t KpP, int64_t KpM,
+                                                          const double* __restrict__ t1,
+                                                          const double* __restrict__ t2,
+                                                          const float* __restrict__ sc, E* Tph, E* Tpl, E* Tmh,
+                                                          E* Tml) {
+  const int64_t npP = v * (v + 1) / 2;
+  const double s = sc[0];
+  int64_t i, j;
+  decode_pair(blockIdx.y, i, j);
+  const int64_t rowP = blockIdx.y;
+  const bool hasM = i < j;
+  const int64_t rowM = hasM ? pairM(i, j) : 0;
+  const double* blk = t2 + (i * o + j) * v * v;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npP; c += (int64_t)gridDim.x * blockDim.x) {
+    int64_t e, f;
+    decode_pair(c, e, f);
+    const double tef = blk[e * v + f] + t1[i * v + e] * t1[j * v + f];
+    const double tfe = blk[f * v + e] + t1[i * v + f] * t1[j * v + e];
+    split_store<E>((float)((e < f ? tef + tfe : tef) * s), Tph, Tpl, rowP * KpP + c);
+    if (hasM && e < f) split_store<E>((float)((tef - tfe) * s), Tmh, Tml, rowM * KpM + pairM(e, f));
+  }
+}
+
+inline int grid_x3(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 65536)); }
+
+}  // namespace
+
+template <typename E>
+void x3_gemm(const X3Gemm& p, hipStream_t s) {
+  if (p.M == 0 || p.N == 0) return;
+  CCX_CHECK(p.Kp % BK == 0 && p.lda % 8 == 0 && p.ldb % 8 == 0 && p.lda >= p.Kp && p.ldb >= p.Kp, ERR_INTERNAL,
+            "x3 GEMM operand padding");
+  if (p.Kp == 0) {   // empty contraction: C = beta C
+    scale<float>(p.beta, tvs(p.C, "mn", {p.M, p.N}, {p.ldc, 1}), s);
+    return;
+  }
+  if (p.M >= 256) {
+    const int64_t tiles = ceil_div(p.M, 256) * ceil_div(p.N, 256);
+    CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
+    x3_gemm_kernel<E, 256><<<dim3((unsigned)tiles), 256, 0, s>>>(p);
+  } else {
+    const int64_t tiles = ceil_div(p.M, 128) * ceil_div(p.N, 128);
+    CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
+    x3_gemm_kernel<E, 128><<<dim3((unsigned)tiles), 256, 0, s>>>(p);
+  }
+  CCX_HIP(hipGetLastError());
+}
+template void x3_gemm<__bf16>(const X3Gemm&, hipStream_t);
+template void x3_gemm<_Float16>(const X3Gemm&, hipStream_t);
+
+template <typename T>
+void x3_absmax(const T* x, int64_t n, float* part, hipStream_t s) {
+  absmax_kernel<T><<<X3_RED_BLOCKS, 256, 0, s>>>(x, n, part);
+  CCX_HIP(hipGetLastError());
+}
+template void x3_absmax<double>(const double*, int64_t, float*, hipStream_t);
+template void x3_absmax<float>(const float*, int64_t, float*, hipStream_t);
+
+void x3_scale(const float* pa, float ca, const float* pb, float cb, float* s_out, hipStream_t s) {
+  x3_scale_kernel<<<1, 256, 0, s>>>(pa, X3_RED_BLOCKS, ca, pb ? pb : pa, pb ? X3_RED_BLOCKS : 0, cb, s_out);
+  CCX_HIP(hipGetLastError());
+}
+int x3_partials() { return X3_RED_BLOCKS; }
+
+template <typename Tin>
+void x3_split_rows(const Tin* src, int64_t rows, int64_t cols, int64_t lds, const float* sc, f16* hi, f16* lo,
+                   int64_t ldd, hipStream_t s) {
+  split_rows_kernel<Tin, _Float16><<<grid_x3(rows * cols), 256, 0, s>>>(src, rows, cols, lds, sc, (_Float16*)hi,
+                                                                        (_Float16*)lo, ldd);
+  CCX_HIP(hipGetLastError());
+}
+template void x3_split_rows<double>(const double*, int64_t, int64_t, int64_t, const float*, f16*, f16*, int64_t,
+                                    hipStream_t);
+template void x3_split_rows<float>(const float*, int64_t, int64_t, int64_t, const float*, f16*, f16*, int64_t,
+                                   hipStream_t);
+
+void x3_split_t(const float* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
+                int64_t Kb, int64_t ldd, hipStream_t s) {
+  const int64_t work = ceil_div(R, 32) * ceil_div(Cb, 32) * nx;
+  split_t_kernel<_Float16><<<(unsigned)std::min<int64_t>(work, 1 << 20), 256, 0, s>>>(
+      src, R, Cb, nx, lds, sc, (_Float16*)hi, (_Float16*)lo, Kb, ldd);
+  CCX_HIP(hipGetLastError());
+}
+
+void ladder_pack_x3(int64_t v, int64_t e0, int64_t e1, VVVVSrc src, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
+                    float* sP, float* sM, hipStream_t s) {
+  if (e1 <= e0) return;
+  const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
+  dim3 grid(1, (unsigned)((e1 - e0) * v));
+  ladder_pack_x3_kernel<_Float16><<<grid, 256, 0, s>>>(v, e0, src, (_Float16*)LPh, (_Float16*)LPl, (_Float16*)LMh,
+                                                       (_Float16*)LMl, sP, sM, bf16_kpad(npP), bf16_kpad(npM));
+  CCX_HIP(hipGetLastError());
+}
+
+void ladder_pack_o_x3(int64_t o, int64_t v, const double* Covvv, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
+                      float* sP, float* sM, hipStream_t s) {
+  dim3 grid(1, (unsigned)(o * v));
+  ladder_pack_o_x3_kernel<_Float16><<<grid, 256, 0, s>>>(o, v, Covvv, (_Float16*)LPh, (_Float16*)LPl,
+                                                         (_Float16*)LMh, (_Float16*)LMl, sP, sM);
+  CCX_HIP(hipGetLastError());
+}
+
+void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2, const float* sc,
+                 f16* Tph, f16* Tpl, f16* Tmh, f16* Tml, hipStream_t s) {
+  const int64_t npP = v * (v + 1) / 2;
+  dim3 grid((unsigned)std::min<int64_t>(ceil_div(npP, 256), 1024), (unsigned)(o * (o + 1) / 2));
+  tau_pack_x3_kernel<_Float16><<<grid, 256, 0, s>>>(o, v, KpP, KpM, t1, t2, sc, (_Float16*)Tph, (_Float16*)Tpl,
+                                                    (_Float16*)Tmh, (_Float16*)Tml);
+  CCX_HIP(hipGetLastError());
+}
+
+void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,
+                  f16* BJl, f16* BKh, f16* BKl, f16* APh, f16* APl, f16* AQh, f16* AQl, double* Ud,
+                  hipStream_t s) {
+  const int64_t tiles = ceil_div(v, 32);
+  dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * o));
+  ring_prep_x3_kernel<_Float16><<<grid, 256, 0, s>>>(o, v, ovp, t1, t2, sc, (_Float16*)BJh, (_Float16*)BJl,
+                                                     (_Float16*)BKh, (_Float16*)BKl, (_Float16*)APh,
+                                                     (_Float16*)APl, (_Float16*)AQh, (_Float16*)AQl, Ud);
+  CCX_HIP(hipGetLastError());
+}
+
+// The packed ladder (ccwfn.py:931 and Z, :715) as FP16x3 GEMMs: scaled tau rows times the
+// per-row-scaled ladder matrices, FP32 outputs CP / CM.
+void ladder_x3(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, ov = c.ov();
+  const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
+  const int64_t KpP = bf16_kpad(npP), KpM = bf16_kpad(npM);
+  const int np = x3_partials();
+  // packed tau+- are bounded by 2 max|t2| + 2 max|t1|^2
+  x3_absmax<double>(t2, ov * ov, c.xpart, s);
+  x3_absmax<double>(t1, ov, c.xpart + np, s);
+  x3_scale(c.xpart, 2.f, c.xpart + np, 2.f, c.xs + 4, s);
+  tau_pack_x3(o, v, KpP, KpM, t1, t2, c.xs + 4, (f16*)c.Tph, (f16*)c.Tpl, (f16*)c.Tmh, (f16*)c.Tml, s);
+  X3Gemm g;
+  g.M = noP; g.N = npP + ov; g.Kp = KpP; g.lda = KpP; g.ldb = KpP; g.ldc = npP + ov;
+  g.Ah = c.Tph; g.Al = c.Tpl; g.Bh = c.LPh; g.Bl = c.LPl; g.C = (float*)c.CP;
+  g.sa = c.xs + 4; g.sbn = c.lsP;
+  int k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noP * (npP + ov) * npP);
+  x3_gemm<_Float16>(g, s);
+  c.tend(s, k);
+  if (noM > 0) {
+    X3Gemm h = g;
+    h.M = noM; h.N = npM + ov; h.Kp = KpM; h.lda = KpM; h.ldb = KpM; h.ldc = npM + ov;
+    h.Ah = c.Tmh; h.Al = c.Tml; h.Bh = c.LMh; h.Bl = c.LMl; h.C = (float*)c.CM; h.sbn = c.lsM;
+    k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noM * (npM + ov) * npM);
+    x3_gemm<_Float16>(h, s);
+    c.tend(s, k);
+  }
+}
+
+}  // namespace ccx

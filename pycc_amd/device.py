@@ -7,7 +7,9 @@ resolves the compute device. It differs from the reference in two deliberate way
   (pycc's own numpy path is the CPU implementation) and a missing GPU raises ``PyCCError``;
 * ``precision`` selects the MI355X precision mode of the contraction kernels: ``'DP'`` (FP64
   everywhere), ``'MIXED'`` (FP32 MFMA ladder + ring GEMMs, everything else FP64; ``'SP'`` is
-  accepted as its alias) or ``'MIXED_BF16X3'`` (as MIXED, ladder as BF16x3 split products).
+  accepted as its alias), ``'MIXED_BF16X3'`` (as MIXED, ladder as BF16x3 split products) or
+  ``'MIXED_F16X3'`` (BF16x3 ladder, ring GEMMs as FP16x3 split products with power-of-two
+  operand scales).
 
 ``ContractionBackend.__call__(subscripts, A, B)`` mirrors ``pycc/device.py:64-86`` for two torch
 operands on the GPU and runs on the library's MFMA contraction kernel (``ccx_contract``).
@@ -20,7 +22,7 @@ from . import _lib
 from .exceptions import InvalidKeywordError, PyCCError
 
 PRECISION_MODES = {"DP": _lib.MODE_DP, "SP": _lib.MODE_MIXED, "MIXED": _lib.MODE_MIXED,
-                   "MIXED_BF16X3": _lib.MODE_MIXED_BF16X3}
+                   "MIXED_BF16X3": _lib.MODE_MIXED_BF16X3, "MIXED_F16X3": _lib.MODE_MIXED_F16X3}
 
 
 def _torch():

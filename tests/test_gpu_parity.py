@@ -141,7 +141,8 @@ def test_solve_returns_none_when_not_converged(pycc_amd):
 
 # Low-precision contractions leave a noise floor in the Jacobi step norm, so the mixed solves use
 # r_conv above that floor; the energy bar is the north star's 1 microhartree.
-@pytest.mark.parametrize("precision,tol_r,r_conv", [("MIXED", 1e-5, 1e-7), ("MIXED_BF16X3", 3e-4, 1e-6)])
+@pytest.mark.parametrize("precision,tol_r,r_conv", [("MIXED", 1e-5, 1e-7), ("MIXED_BF16X3", 3e-4, 1e-6),
+                                                     ("MIXED_F16X3", 3e-5, 1e-7)])
 def test_mixed_precision_residual_and_energy(pycc_amd, precision, tol_r, r_conv):
     no, nv = 6, 30
     sd, H = _ham(no, nv, 11, t2max=0.03)
@@ -164,7 +165,7 @@ def test_mixed_precision_residual_and_energy(pycc_amd, precision, tol_r, r_conv)
     assert abs(e - e_ref) < 1e-6
 
 
-@pytest.mark.parametrize("precision,tol_r", [("MIXED", 1e-5), ("MIXED_BF16X3", 3e-4)])
+@pytest.mark.parametrize("precision,tol_r", [("MIXED", 1e-5), ("MIXED_BF16X3", 3e-4), ("MIXED_F16X3", 3e-5)])
 def test_mixed_residual_large_pair_count(pycc_amd, precision, tol_r):
     """o = 24 gives 300 occupied pairs: the ladder takes the 256 x 256 BF16x3 tiles (with partial edge
     tiles in M and N) and the hole-hole terms the FP32 split-K path."""
