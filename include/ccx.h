@@ -180,6 +180,14 @@ int ccx_contract_f32(const char* subscripts, float alpha, const float* A, const 
                      const int64_t* b_strides, float beta, float* C, const int64_t* c_shape,
                      const int64_t* c_strides, void* stream);
 
+/* C = alpha op(A) op(B) + beta C for FP32 row-major device matrices (op = transpose when transa /
+ * transb is nonzero; C is M x N, op(A) M x K, op(B) K x N) through FP16x3 split products: each
+ * operand is held as hi + lo FP16 with a power-of-two scale (22 significant bits), three FP16 MFMA
+ * products per product, FP32 accumulation. No counterpart in the reference: it is the split GEMM
+ * of CCX_MODE_MIXED_F16X3 exposed on its own. Uses a per-thread device workspace. */
+int ccx_gemm_f16x3(int transa, int transb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
+                   int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* ccx_last_error(void);
 

@@ -44,6 +44,11 @@ void ladder_pack_o_x3(int64_t o, int64_t v, const double* Covvv, f16* LPh, f16* 
 // FP16x3 packed tau rows (tau_pack_bf16x3 layout) scaled by *sc.
 void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2, const float* sc,
                  f16* Tph, f16* Tpl, f16* Tmh, f16* Tml, hipStream_t s);
+// C = alpha op(A) op(B) + beta C (FP32 row-major, op = transpose when ta / tb) through FP16x3 with
+// workspace copies of the split operands.
+struct Workspace;
+void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
+                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, Workspace& ws, hipStream_t s);
 // The split ring operands BJ, BK, AP, AQ (row stride ovp, or 2 ovp for AP) and Ud from t1, t2.
 void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,
                   f16* BJl, f16* BKh, f16* BKl, f16* APh, f16* APl, f16* AQh, f16* AQl, double* Ud,

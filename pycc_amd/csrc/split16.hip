@@ -172,6 +172,133 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
   }
 }
 
+// 256 x 256 output tiles for large GEMMs, 8 waves (2 per SIMD) in 2 x 4, each computing 128 x 64.
+// Operand k-tiles (256 rows x 32 k, hi and lo) go from global memory straight into LDS
+// (global_load_lds_dwordx4, issued from inline assembly so that the only waits are the explicit
+// ones below) and are double-buffered: the next tile's loads are in flight while the current one
+// feeds the MFMAs. LDS rows are 32 elements (64 bytes) with the 16-byte chunk c of row r stored at
+// chunk c ^ ((r >> 1) & 3), which keeps both the DMA writes and the fragment reads free of bank
+// conflicts. An LDS-DMA instruction writes 16 bytes per lane at M0 + 16 lane, so lane l of an
+// instruction covering rows R0 .. R0 + 15 loads row R0 + l / 4, chunk (l % 4) ^ ((row >> 1) & 3).
+// Rows past M / N load a valid row (their results are not stored).
+__device__ __forceinline__ int x3_swz(int r, int c) { return r * 32 + ((c ^ ((r >> 1) & 3)) * 8); }
+
+template <typename E>
+__device__ __forceinline__ void x3_dma16(const E* g, const E* lds) {
+  const uint32_t base = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const E*)lds;
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+               :: "s"(__builtin_amdgcn_readfirstlane(base)), "v"(g) : "memory");
+}
+
+template <typename E>
+__global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
+  typedef typename X3<E>::v8 v8;
+  constexpr int BT = 256, WTM = 128, WTN = 64, NBM = WTM / 16, NBN = WTN / 16, IPW = 2;
+  __shared__ __attribute__((aligned(1024))) E sA[2][2][BT * 32];   // [stage][hi/lo]
+  __shared__ __attribute__((aligned(1024))) E sB[2][2][BT * 32];
+  const E* __restrict__ Ah = (const E*)p.Ah;
+  const E* __restrict__ Al = (const E*)p.Al;
+  const E* __restrict__ Bh = (const E*)p.Bh;
+  const E* __restrict__ Bl = (const E*)p.Bl;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BT - 1) / BT;
+  const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
+  const int64_t group = t / (GM * tiles_n), first_m = group * GM;
+  const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
+  const int64_t within = t - group * GM * tiles_n;
+  const int64_t tm = first_m + within % gsize, tn = within / gsize;
+
+  int64_t gA[IPW], gB[IPW];
+  int ldso[IPW];
+#pragma unroll
+  for (int i = 0; i < IPW; ++i) {
+    const int R0 = (wave * IPW + i) * 16;
+    const int row = R0 + (lane >> 2), c = (lane & 3) ^ ((row >> 1) & 3);
+    int64_t m = tm * BT + row, n = tn * BT + row;
+    m = m < p.M ? m : p.M - 1;
+    n = n < p.N ? n : p.N - 1;
+    gA[i] = m * p.lda + c * 8;
+    gB[i] = n * p.ldb + c * 8;
+    ldso[i] = R0 * 32;
+  }
+  auto issue = [&](int st, int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < IPW; ++i) {
+      x3_dma16<E>(Ah + gA[i] + k0, &sA[st][0][ldso[i]]);
+      x3_dma16<E>(Al + gA[i] + k0, &sA[st][1][ldso[i]]);
+      x3_dma16<E>(Bh + gB[i] + k0, &sB[st][0][ldso[i]]);
+      x3_dma16<E>(Bl + gB[i] + k0, &sB[st][1][ldso[i]]);
+    }
+  };
+  const int fr = lane >> 4, fc = lane & 15;
+  int ra[NBM], rb[NBN];
+#pragma unroll
+  for (int i = 0; i < NBM; ++i) ra[i] = x3_swz(wm * WTM + i * 16 + fc, fr);
+#pragma unroll
+  for (int j = 0; j < NBN; ++j) rb[j] = x3_swz(wn * WTN + j * 16 + fc, fr);
+  f4 acc[NBM][NBN];
+#pragma unroll
+  for (int i = 0; i < NBM; ++i)
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+  issue(0, 0);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < p.Kp; k0 += BK) {
+    if (k0 + BK < p.Kp) issue(buf ^ 1, k0 + BK);
+    v8 a[NBM], b[NBN];
+    // lo(A) x hi(B), hi(A) x hi(B), hi(A) x lo(B)
+#pragma unroll
+    for (int i = 0; i < NBM; ++i) a[i] = *(const v8*)&sA[buf][1][ra[i]];
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) b[j] = *(const v8*)&sB[buf][0][rb[j]];
+#pragma unroll
+    for (int i = 0; i < NBM; ++i)
+#pragma unroll
+      for (int j = 0; j < NBN; ++j) acc[i][j] = X3<E>::mfma(a[i], b[j], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < NBM; ++i) a[i] = *(const v8*)&sA[buf][0][ra[i]];
+#pragma unroll
+    for (int i = 0; i < NBM; ++i)
+#pragma unroll
+      for (int j = 0; j < NBN; ++j) acc[i][j] = X3<E>::mfma(a[i], b[j], acc[i][j]);
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) b[j] = *(const v8*)&sB[buf][1][rb[j]];
+#pragma unroll
+    for (int i = 0; i < NBM; ++i)
+#pragma unroll
+      for (int j = 0; j < NBN; ++j) acc[i][j] = X3<E>::mfma(a[i], b[j], acc[i][j]);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this wave's loads of the next k-tile
+    __syncthreads();                                    // everyone's loads landed, reads of buf done
+    buf ^= 1;
+  }
+
+  float f0 = p.alpha;
+  if (p.sa) f0 = f0 / p.sa[0];
+  if (p.sb) f0 = f0 / p.sb[0];
+#pragma unroll
+  for (int j = 0; j < NBN; ++j) {
+    const int64_t n = tn * BT + wn * WTN + j * 16 + fc;
+    if (n >= p.N) continue;
+    const float f = p.sbn ? f0 / p.sbn[n] : f0;
+#pragma unroll
+    for (int i = 0; i < NBM; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = tm * BT + wm * WTM + i * 16 + fr * 4 + r;
+        if (m < p.M) {
+          float* d = p.C + m * p.ldc + n;
+          float val = f * acc[i][j][r];
+          if (p.beta != 0.f) val += p.beta * *d;
+          *d = val;
+        }
+      }
+  }
+}
+
 // ---- operand conversions ------------------------------------------------------------------
 constexpr int X3_RED_BLOCKS = 1024;
 
@@ -197,11 +324,21 @@ __device__ __forceinline__ float block_max(float x) {
   return r;
 }
 
+// max |x[r * ld + c]| over r < rows, c < cols, one partial per workgroup
 template <typename T>
-__global__ __launch_bounds__(256) void absmax_kernel(const T* __restrict__ x, int64_t n, float* __restrict__ part) {
+__global__ __launch_bounds__(256) void absmax_kernel(const T* __restrict__ x, int64_t rows, int64_t cols, int64_t ld,
+                                                     float* __restrict__ part) {
   double m = 0.0;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-    m = fmax(m, fabs((double)x[e]));
+  const int64_t n = rows * cols;
+  if (ld == cols) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+      m = fmax(m, fabs((double)x[e]));
+  } else {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t r = e / cols;
+      m = fmax(m, fabs((double)x[r * ld + (e - r * cols)]));
+    }
+  }
   float r = block_max((float)m);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
@@ -458,7 +595,7 @@ void x3_gemm(const X3Gemm& p, hipStream_t s) {
   if (p.M >= 256) {
     const int64_t tiles = ceil_div(p.M, 256) * ceil_div(p.N, 256);
     CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
-    x3_gemm_kernel<E, 256><<<dim3((unsigned)tiles), 256, 0, s>>>(p);
+    x3_gemm_dma_kernel<E><<<dim3((unsigned)tiles), 512, 0, s>>>(p);
   } else {
     const int64_t tiles = ceil_div(p.M, 128) * ceil_div(p.N, 128);
     CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
@@ -471,7 +608,7 @@ template void x3_gemm<_Float16>(const X3Gemm&, hipStream_t);
 
 template <typename T>
 void x3_absmax(const T* x, int64_t n, float* part, hipStream_t s) {
-  absmax_kernel<T><<<X3_RED_BLOCKS, 256, 0, s>>>(x, n, part);
+  absmax_kernel<T><<<X3_RED_BLOCKS, 256, 0, s>>>(x, 1, n, n, part);
   CCX_HIP(hipGetLastError());
 }
 template void x3_absmax<double>(const double*, int64_t, float*, hipStream_t);
@@ -539,6 +676,42 @@ void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const dou
                                                      (_Float16*)BKh, (_Float16*)BKl, (_Float16*)APh,
                                                      (_Float16*)APl, (_Float16*)AQh, (_Float16*)AQl, Ud);
   CCX_HIP(hipGetLastError());
+}
+
+// C = alpha op(A) op(B) + beta C for FP32 row-major matrices through FP16x3: op(A) (M x K) and
+// op(B)^T (N x K) are split into K-contiguous workspace copies with their own scales.
+void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
+                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, Workspace& ws, hipStream_t s) {
+  CCX_CHECK(M >= 0 && N >= 0 && K >= 0, ERR_ARG, "negative GEMM extent");
+  if (M == 0 || N == 0) return;
+  const int64_t Kp = bf16_kpad(K);
+  const int np = x3_partials();
+  const size_t half = (size_t)(M + N) * Kp * 2 * sizeof(f16);
+  char* w = (char*)ws.get(half + (2 * (size_t)np + 64) * sizeof(float));
+  f16* Ah = (f16*)w;
+  f16* Al = Ah + M * Kp;
+  f16* Bh = Al + M * Kp;
+  f16* Bl = Bh + N * Kp;
+  float* part = (float*)(w + half);
+  float* sc = part + 2 * np;
+  CCX_HIP(hipMemsetAsync(w, 0, half, s));   // K padding
+  // op(A)[m][k] = ta ? A[k][m] : A[m][k]; op(B)^T[n][k] = tb ? B[n][k] : B[k][n]
+  absmax_kernel<float><<<np, 256, 0, s>>>(A, ta ? K : M, ta ? M : K, lda, part);
+  absmax_kernel<float><<<np, 256, 0, s>>>(B, tb ? N : K, tb ? K : N, ldb, part + np);
+  CCX_HIP(hipGetLastError());
+  x3_scale(part, 1.f, nullptr, 0.f, sc, s);
+  x3_scale(part + np, 1.f, nullptr, 0.f, sc + 1, s);
+  if (K > 0) {
+    if (ta) x3_split_t(A, K, M, 1, lda, sc, Ah, Al, 0, Kp, s);
+    else x3_split_rows<float>(A, M, K, lda, sc, Ah, Al, Kp, s);
+    if (tb) x3_split_rows<float>(B, N, K, ldb, sc + 1, Bh, Bl, Kp, s);
+    else x3_split_t(B, K, N, 1, ldb, sc + 1, Bh, Bl, 0, Kp, s);
+  }
+  X3Gemm g;
+  g.M = M; g.N = N; g.Kp = Kp; g.lda = Kp; g.ldb = Kp; g.ldc = ldc;
+  g.Ah = Ah; g.Al = Al; g.Bh = Bh; g.Bl = Bl; g.C = C;
+  g.sa = sc; g.sb = sc + 1; g.alpha = alpha; g.beta = beta;
+  x3_gemm<_Float16>(g, s);
 }
 
 // The packed ladder (ccwfn.py:931 and Z, :715) as FP16x3 GEMMs: scaled tau rows times the

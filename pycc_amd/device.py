@@ -75,6 +75,38 @@ class ContractionBackend(object):
                   i64(out.stride()), ctypes.c_void_p(current_stream_handle()))
         return out
 
+    def matmul_f16x3(self, A, B, out=None, alpha=1.0, beta=0.0):
+        """alpha A @ B (+ beta out) for 2-D float32 tensors through the FP16x3 split GEMM
+        (ccx_gemm_f16x3): ~22-bit operands, FP32 accumulation. Transposed views are passed as
+        transposes, other strides are made contiguous."""
+        torch = _torch()
+        if A.dim() != 2 or B.dim() != 2 or A.dtype != torch.float32 or B.dtype != torch.float32:
+            raise PyCCError("matmul_f16x3 takes two 2-D float32 tensors")
+        if A.shape[1] != B.shape[0]:
+            raise PyCCError("inner extents differ: %s @ %s" % (tuple(A.shape), tuple(B.shape)))
+        A = A.to(self.device1)
+        B = B.to(self.device1)
+
+        def op(X):
+            if X.stride(1) == 1 and X.stride(0) >= X.shape[1]:
+                return X, 0, X.stride(0)
+            if X.stride(0) == 1 and X.stride(1) >= X.shape[0]:
+                return X, 1, X.stride(1)
+            X = X.contiguous()
+            return X, 0, X.shape[1]
+
+        A, ta, lda = op(A)
+        B, tb, ldb = op(B)
+        M, K, N = A.shape[0], A.shape[1], B.shape[1]
+        if out is None:
+            out = torch.zeros((M, N), dtype=torch.float32, device=self.device1)
+            beta = 0.0
+        elif out.shape != (M, N) or out.stride(1) != 1 or out.dtype != torch.float32:
+            raise PyCCError("out must be a row-major float32 (M, N) tensor")
+        _lib.call("ccx_gemm_f16x3", ta, tb, M, N, K, alpha, dptr(A), lda, dptr(B), ldb, beta, dptr(out),
+                  out.stride(0), ctypes.c_void_p(current_stream_handle()))
+        return out
+
 
 class DeviceManager(object):
     """Owns the device/precision policy and the contraction backend (pycc/device.py:89-173)."""
