@@ -288,11 +288,11 @@ int ccx_contract_f32(const char* subscripts, float alpha, const float* A, const 
 
 int ccx_gemm_f16x3(int transa, int transb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
                    int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream) {
-  static thread_local ccx::Workspace ws;
+  static thread_local ccx::Workspace ws, kws;
   return guarded([&] {
     CCX_CHECK(((A && B) || K == 0) && (C || M == 0 || N == 0), ccx::ERR_ARG, "null matrix pointer");
     CCX_CHECK(lda >= (transa ? M : K) && ldb >= (transb ? K : N) && ldc >= N, ccx::ERR_ARG, "leading dimension too small");
-    ccx::gemm_f16x3(transa != 0, transb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, st(stream));
+    ccx::gemm_f16x3(transa != 0, transb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ws, kws, st(stream));
   });
 }
 

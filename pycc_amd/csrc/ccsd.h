@@ -42,7 +42,9 @@ struct Ctx {
   float *lsP = nullptr, *lsM = nullptr;
   // FP16x3 ring operands (MODE_MIXED_F16X3): split K and J, [ov][ovp]
   f16 *Kh = nullptr, *Kl = nullptr, *Jh = nullptr, *Jl = nullptr;
-  float* xs = nullptr;               // operand scales: [0] amplitudes, [1] J, [2] K, [3] W
+  f16 *Ioh = nullptr, *Iol = nullptr; // split <mn|ef> [(mn)][(ef) padded] (hole-hole term :603)
+  float* xs = nullptr;               // operand scales: [0] ring amplitudes, [1] J, [2] K, [3] W,
+                                     // [4] packed tau, [5] tau, [6] <mn|ef>, [7] Wmnij
   float* xpart = nullptr;            // absmax partials (2 x x3_partials())
 
   // ---- per-residual work buffers (allocated on first use) ----
@@ -57,6 +59,8 @@ struct Ctx {
   // AQ [ov][ovp], BP = split W^T [ov][2ovp]
   f16 *BJh = nullptr, *BJl = nullptr, *BKh = nullptr, *BKl = nullptr, *APh = nullptr, *APl = nullptr;
   f16 *AQh = nullptr, *AQl = nullptr, *BPh = nullptr, *BPl = nullptr;
+  // FP16x3 hole-hole operands: tau [(ij)][(ef)], tau^T [(ab)][(mn)], Wmnij^T [(ij)][(mn)]
+  f16 *TAh = nullptr, *TAl = nullptr, *TTh = nullptr, *TTl = nullptr, *WTh = nullptr, *WTl = nullptr;
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
 
@@ -117,6 +121,8 @@ struct Ctx {
   int64_t noP() const { return o * (o + 1) / 2; }
   int64_t noM() const { return o * (o - 1) / 2; }
   int64_t ovp() const { return (o * v + 31) / 32 * 32; }   // K padding of the x3 ring operands
+  int64_t o2p() const { return (o * o + 31) / 32 * 32; }
+  int64_t v2p() const { return (v * v + 31) / 32 * 32; }
   bool split_ladder() const { return mode == MODE_MIXED_BF16X3 || mode == MODE_MIXED_F16X3; }
 
   template <typename T>

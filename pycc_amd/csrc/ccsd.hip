@@ -77,13 +77,17 @@ static void alloc_integrals(Ctx& c) {
         *q = c.alloc<f16>(ov * ovp);
         CCX_HIP(hipMemset(*q, 0, ov * ovp * sizeof(f16)));   // K padding
       }
+      for (f16** q : {&c.Ioh, &c.Iol}) {
+        *q = c.alloc<f16>(o * o * c.v2p());
+        CCX_HIP(hipMemset(*q, 0, o * o * c.v2p() * sizeof(f16)));   // K padding
+      }
       c.xs = c.alloc<float>(64);
       c.xpart = c.alloc<float>(2 * x3_partials());
     } else {
       c.K32 = c.alloc<float>(ov * ov);
       c.J32 = c.alloc<float>(ov * ov);
+      c.Io32 = c.alloc<float>(ov * ov);
     }
-    c.Io32 = c.alloc<float>(ov * ov);
     if (c.mode == MODE_MIXED) {
       c.LP = c.alloc<float>(szP);
       c.LM = c.alloc<float>(szM);
@@ -140,11 +144,15 @@ static void finish_integrals(Ctx& c, const double* fock, hipStream_t s) {
     x3_absmax<double>(c.J, ov * ov, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 1, s);
     x3_split_rows<double>(c.J, ov, ov, ov, c.xs + 1, c.Jh, c.Jl, ovp, s);
+    // <mn|ef> as [(mn)][(ef)] for the hole-hole term :603
+    x3_absmax<double>(c.Ioovv, ov * ov, c.xpart, s);
+    x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 6, s);
+    x3_split_rows<double>(c.Ioovv, o * o, v * v, v * v, c.xs + 6, c.Ioh, c.Iol, c.v2p(), s);
   } else if (c.mode != MODE_DP) {
     cast<double, float>(ov * ov, c.Covov, c.K32, s);
     cast<double, float>(ov * ov, c.J, c.J32, s);
+    cast<double, float>(ov * ov, c.Ioovv, c.Io32, s);
   }
-  if (c.mode != MODE_DP) cast<double, float>(ov * ov, c.Ioovv, c.Io32, s);
   if (c.mode == MODE_MIXED_BF16X3) ladder_pack_o_bf16x3(o, v, c.Covvv, c.LPh, c.LPl, c.LMh, c.LMl, s);
   if (c.mode == MODE_MIXED_F16X3)
     ladder_pack_o_x3(o, v, c.Covvv, (f16*)c.LPh, (f16*)c.LPl, (f16*)c.LMh, (f16*)c.LMl, c.lsP, c.lsM, s);
@@ -236,6 +244,12 @@ void alloc_work(Ctx& c) {
     halloc(&c.AQl, ov * ovp);
     halloc(&c.BPh, 2 * ov * ovp);
     halloc(&c.BPl, 2 * ov * ovp);
+    halloc(&c.TAh, o * o * c.v2p());
+    halloc(&c.TAl, o * o * c.v2p());
+    halloc(&c.TTh, v * v * c.o2p());
+    halloc(&c.TTl, v * v * c.o2p());
+    halloc(&c.WTh, o * o * c.o2p());
+    halloc(&c.WTl, o * o * c.o2p());
   } else {
     c.S1 = talloc(2 * ov * ov);
     c.A1 = talloc(ov * ov);
@@ -257,7 +271,7 @@ void alloc_work(Ctx& c) {
     c.Tm = talloc(noM * npM);
   }
   if (c.mode != MODE_DP) {
-    c.tau32 = c.alloc<float>(ov * ov);
+    if (c.mode != MODE_MIXED_F16X3) c.tau32 = c.alloc<float>(ov * ov);
     c.W32 = c.alloc<float>(o * o * o * o);
   }
   c.CP = talloc(noP * (npP + ov));
@@ -317,6 +331,19 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   cd(1.0, T1("ie"), tv(c.Cooov, "njme", OOOV), 1.0, Wmnij);        // <mn|ej> = (nj|me)
   if (std::is_same<T, double>::value) {
     cd(1.0, tv(c.tau, "ijef", OOVV), tv(c.Ioovv, "mnef", OOVV), 1.0, Wmnij);  // <mn|ef>
+  } else if (c.mode == MODE_MIXED_F16X3) {
+    // the two o^4 v^2 hole-hole contractions (:603 and :930) as FP16x3 GEMMs; tau is split once in
+    // both orientations ([(ij)][(ef)] here, [(ab)][(mn)] for :930) with one scale
+    x3_absmax<double>(c.tau, ov * ov, c.xpart, s);
+    x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 5, s);
+    x3_split_rows<double>(c.tau, o * o, v * v, v * v, c.xs + 5, c.TAh, c.TAl, c.v2p(), s);
+    x3_split_t<double>(c.tau, o * o, v * v, 1, v * v, c.xs + 5, c.TTh, c.TTl, 0, c.o2p(), s);
+    X3Gemm g;   // W32[(mn)][(ij)] = sum_(ef) <mn|ef> tau_ijef
+    g.M = o * o; g.N = o * o; g.Kp = c.v2p(); g.lda = c.v2p(); g.ldb = c.v2p(); g.ldc = o * o;
+    g.Ah = c.Ioh; g.Al = c.Iol; g.Bh = c.TAh; g.Bl = c.TAl; g.C = c.W32;
+    g.sa = c.xs + 6; g.sb = c.xs + 5;
+    x3_gemm<_Float16>(g, s, &ws);
+    accumulate(o * o * o * o, 1.0, c.W32, c.Wmnij, s);
   } else {
     // mixed modes: the two o^4 v^2 hole-hole ladder contractions (:603 and :930) run in FP32
     cast<double, float>(ov * ov, c.tau, c.tau32, s);
@@ -462,6 +489,18 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // :930  1/2 tau_mnab Wmnij
   if (std::is_same<T, double>::value) {
     cd(0.5, tv(c.tau, "mnab", OOVV), Wmnij, 1.0, Hh);
+  } else if (c.mode == MODE_MIXED_F16X3) {
+    float* X32 = (float*)c.S2;
+    cast<double, float>(o * o * o * o, c.Wmnij, c.W32, s);
+    x3_absmax<float>(c.W32, o * o * o * o, c.xpart, s);
+    x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 7, s);
+    x3_split_t<float>(c.W32, o * o, o * o, 1, o * o, c.xs + 7, c.WTh, c.WTl, 0, c.o2p(), s);   // [(ij)][(mn)]
+    X3Gemm g;   // X32[(ij)][(ab)] = 1/2 sum_(mn) Wmnij tau_mnab
+    g.M = o * o; g.N = v * v; g.Kp = c.o2p(); g.lda = c.o2p(); g.ldb = c.o2p(); g.ldc = v * v;
+    g.Ah = c.WTh; g.Al = c.WTl; g.Bh = c.TTh; g.Bl = c.TTl; g.C = X32;
+    g.sa = c.xs + 7; g.sb = c.xs + 5; g.alpha = 0.5f;
+    x3_gemm<_Float16>(g, s, &ws);
+    accumulate(ov * ov, 1.0, X32, c.Hh, s);
   } else {
     float* X32 = (float*)c.S2;   // S2 is dead after the P ring GEMM
     cast<double, float>(o * o * o * o, c.Wmnij, c.W32, s);

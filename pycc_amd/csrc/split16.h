@@ -17,9 +17,14 @@ struct X3Gemm {
   const float *sa = nullptr, *sb = nullptr;
   const float* sbn = nullptr;        // per-row scales of B (one per n), in place of sb
   float alpha = 1.f, beta = 0.f;
+  // split-K (set by x3_gemm): grid.y = splits K ranges of kchunk, raw partials in work[s][M][N]
+  int64_t kchunk = 0;
+  float* work = nullptr;
 };
+struct Workspace;
+// ws (optional) enables split-K when the tile grid cannot fill the chip.
 template <typename E>   // __bf16 or _Float16
-void x3_gemm(const X3Gemm& p, hipStream_t s);
+void x3_gemm(const X3Gemm& p, hipStream_t s, Workspace* ws = nullptr);
 
 // max |x| over n elements as x3_partials() per-block partial maxima in `part`.
 template <typename T>
@@ -33,7 +38,8 @@ template <typename Tin>
 void x3_split_rows(const Tin* src, int64_t rows, int64_t cols, int64_t lds, const float* sc, f16* hi, f16* lo,
                    int64_t ldd, hipStream_t s);
 // hi/lo[cc * ldd + x * Kb + r] = split(src[r * lds + x * Cb + cc] * sc), r < R, x < nx, cc < Cb.
-void x3_split_t(const float* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
+template <typename Tin>
+void x3_split_t(const Tin* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
                 int64_t Kb, int64_t ldd, hipStream_t s);
 // FP16x3 packed ladder matrices (the BF16x3 layouts of kernels.h, [N][Kp]) with one power-of-two
 // scale per row in sP / sM: the (ab) rows for e in [e0, e1) from src, and the (m, b) rows from Covvv.
@@ -45,10 +51,10 @@ void ladder_pack_o_x3(int64_t o, int64_t v, const double* Covvv, f16* LPh, f16* 
 void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2, const float* sc,
                  f16* Tph, f16* Tpl, f16* Tmh, f16* Tml, hipStream_t s);
 // C = alpha op(A) op(B) + beta C (FP32 row-major, op = transpose when ta / tb) through FP16x3 with
-// workspace copies of the split operands.
-struct Workspace;
+// copies of the split operands in ws and split-K partials in kws.
 void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
-                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, Workspace& ws, hipStream_t s);
+                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, Workspace& ws, Workspace& kws,
+                hipStream_t s);
 // The split ring operands BJ, BK, AP, AQ (row stride ovp, or 2 ovp for AP) and Ud from t1, t2.
 void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,
                   f16* BJl, f16* BKh, f16* BKl, f16* APh, f16* APl, f16* AQh, f16* AQl, double* Ud,

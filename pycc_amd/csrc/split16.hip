@@ -11,6 +11,7 @@
 // ldb and K padded to a multiple of 32 with zeros.
 #include "ccsd.h"
 #include "split16.h"
+#include <algorithm>
 #include <cmath>
 
 namespace ccx {
@@ -113,12 +114,14 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
     for (int j = 0; j < NB; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
   const int fr = lane >> 4, fc = lane & 15;
-  gload(0);
+  const int64_t kbeg = p.work ? (int64_t)blockIdx.y * p.kchunk : 0;
+  const int64_t kend = p.work ? (kbeg + p.kchunk < p.Kp ? kbeg + p.kchunk : p.Kp) : p.Kp;
+  gload(kbeg);
   sstore(0);
   __syncthreads();
   int buf = 0;
-  for (int64_t k0 = 0; k0 < p.Kp; k0 += BK) {
-    const bool has_next = k0 + BK < p.Kp;
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool has_next = k0 + BK < kend;
     if (has_next) gload(k0 + BK);
     v8 a[NB], b[NB];
     // lo(A) x hi(B)
@@ -149,6 +152,22 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
     buf ^= 1;
   }
 
+  if (p.work) {   // split-K: raw partial sums
+    float* Wb = p.work + (int64_t)blockIdx.y * p.M * p.N;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int64_t n = tn * BT + wn * WT + j * 16 + fc;
+      if (n >= p.N) continue;
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t m = tm * BT + wm * WT + i * 16 + fr * 4 + r;
+          if (m < p.M) Wb[m * p.N + n] = acc[i][j][r];
+        }
+    }
+    return;
+  }
   float f0 = p.alpha;
   if (p.sa) f0 = f0 / p.sa[0];
   if (p.sb) f0 = f0 / p.sb[0];
@@ -243,12 +262,14 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
 #pragma unroll
     for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  issue(0, 0);
+  const int64_t kbeg = p.work ? (int64_t)blockIdx.y * p.kchunk : 0;
+  const int64_t kend = p.work ? (kbeg + p.kchunk < p.Kp ? kbeg + p.kchunk : p.Kp) : p.Kp;
+  issue(0, kbeg);
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   int buf = 0;
-  for (int64_t k0 = 0; k0 < p.Kp; k0 += BK) {
-    if (k0 + BK < p.Kp) issue(buf ^ 1, k0 + BK);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    if (k0 + BK < kend) issue(buf ^ 1, k0 + BK);
     v8 a[NBM], b[NBN];
     // lo(A) x hi(B), hi(A) x hi(B), hi(A) x lo(B)
 #pragma unroll
@@ -276,6 +297,22 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
     buf ^= 1;
   }
 
+  if (p.work) {   // split-K: raw partial sums
+    float* Wb = p.work + (int64_t)blockIdx.y * p.M * p.N;
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) {
+      const int64_t n = tn * BT + wn * WTN + j * 16 + fc;
+      if (n >= p.N) continue;
+#pragma unroll
+      for (int i = 0; i < NBM; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t m = tm * BT + wm * WTM + i * 16 + fr * 4 + r;
+          if (m < p.M) Wb[m * p.N + n] = acc[i][j][r];
+        }
+    }
+    return;
+  }
   float f0 = p.alpha;
   if (p.sa) f0 = f0 / p.sa[0];
   if (p.sb) f0 = f0 / p.sb[0];
@@ -296,6 +333,24 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
           *d = val;
         }
       }
+  }
+}
+
+// C = alpha / (sa sb sbn[n]) sum_s work[s] + beta C, summed in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void x3_reduce_kernel(X3Gemm p, int splits) {
+  float f0 = p.alpha;
+  if (p.sa) f0 = f0 / p.sa[0];
+  if (p.sb) f0 = f0 / p.sb[0];
+  const int64_t total = p.M * p.N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    float acc = 0.f;
+    for (int k = 0; k < splits; ++k) acc += p.work[k * total + e];
+    const int64_t m = e / p.N, n = e - m * p.N;
+    const float f = p.sbn ? f0 / p.sbn[n] : f0;
+    float* d = p.C + m * p.ldc + n;
+    float val = f * acc;
+    if (p.beta != 0.f) val += p.beta * *d;
+    *d = val;
   }
 }
 
@@ -377,12 +432,12 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const Tin* __restrict__
 
 // dst[cc * ldd + x * Kb + r] = split(src[r * lds + x * Cb + cc]) for r < R, x < nx, cc < Cb,
 // through 32 x 33 LDS tiles (coalesced reads along cc, writes along r).
-template <typename E>
-__global__ __launch_bounds__(256) void split_t_kernel(const float* __restrict__ src, int64_t R, int64_t Cb, int nx,
+template <typename Tin, typename E>
+__global__ __launch_bounds__(256) void split_t_kernel(const Tin* __restrict__ src, int64_t R, int64_t Cb, int nx,
                                                       int64_t lds, const float* __restrict__ sc, E* hi, E* lo,
                                                       int64_t Kb, int64_t ldd) {
   __shared__ float tile[32][33];
-  const float s = sc[0];
+  const double s = sc[0];
   const int64_t tr = (R + 31) / 32, tc = (Cb + 31) / 32;
   const int64_t per_x = tr * tc;
   for (int64_t w = blockIdx.x; w < per_x * nx; w += gridDim.x) {
@@ -391,12 +446,12 @@ __global__ __launch_bounds__(256) void split_t_kernel(const float* __restrict__ 
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int y = ty; y < 32; y += 8) {
       const int64_t r = r0 + y, cc = c0 + tx;
-      tile[y][tx] = (r < R && cc < Cb) ? src[r * lds + x * Cb + cc] : 0.f;
+      tile[y][tx] = (r < R && cc < Cb) ? (float)((double)src[r * lds + x * Cb + cc] * s) : 0.f;
     }
     __syncthreads();
     for (int y = ty; y < 32; y += 8) {
       const int64_t cc = c0 + y, r = r0 + tx;
-      if (cc < Cb && r < R) split_store<E>(tile[tx][y] * s, hi, lo, cc * ldd + x * Kb + r);
+      if (cc < Cb && r < R) split_store<E>(tile[tx][y], hi, lo, cc * ldd + x * Kb + r);
     }
     __syncthreads();
   }
@@ -584,7 +639,8 @@ inline int grid_x3(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_
 }  // namespace
 
 template <typename E>
-void x3_gemm(const X3Gemm& p, hipStream_t s) {
+void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
+  X3Gemm p = p0;
   if (p.M == 0 || p.N == 0) return;
   CCX_CHECK(p.Kp % BK == 0 && p.lda % 8 == 0 && p.ldb % 8 == 0 && p.lda >= p.Kp && p.ldb >= p.Kp, ERR_INTERNAL,
             "x3 GEMM operand padding");
@@ -592,19 +648,34 @@ void x3_gemm(const X3Gemm& p, hipStream_t s) {
     scale<float>(p.beta, tvs(p.C, "mn", {p.M, p.N}, {p.ldc, 1}), s);
     return;
   }
-  if (p.M >= 256) {
-    const int64_t tiles = ceil_div(p.M, 256) * ceil_div(p.N, 256);
-    CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
-    x3_gemm_dma_kernel<E><<<dim3((unsigned)tiles), 512, 0, s>>>(p);
-  } else {
-    const int64_t tiles = ceil_div(p.M, 128) * ceil_div(p.N, 128);
-    CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
-    x3_gemm_kernel<E, 128><<<dim3((unsigned)tiles), 256, 0, s>>>(p);
+  const bool big = p.M >= 256;
+  const int64_t bt = big ? 256 : 128;
+  const int64_t tiles = ceil_div(p.M, bt) * ceil_div(p.N, bt);
+  CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
+  // split K into workspace partials when the tile grid cannot fill 256 CUs (keep >= 32 k-tiles each)
+  int64_t splits = 1;
+  p.work = nullptr;
+  if (ws && tiles < 512) {
+    splits = std::min<int64_t>({64, ceil_div(1024, tiles), p.Kp / (32 * BK)});
+    if (splits > 1) {
+      p.kchunk = ceil_div(ceil_div(p.Kp, splits), BK) * BK;
+      splits = ceil_div(p.Kp, p.kchunk);
+      p.work = (float*)ws->get((size_t)splits * p.M * p.N * sizeof(float));
+    } else {
+      splits = 1;
+    }
   }
+  dim3 grid((unsigned)tiles, (unsigned)splits);
+  if (big) x3_gemm_dma_kernel<E><<<grid, 512, 0, s>>>(p);
+  else x3_gemm_kernel<E, 128><<<grid, 256, 0, s>>>(p);
   CCX_HIP(hipGetLastError());
+  if (p.work) {
+    x3_reduce_kernel<<<(unsigned)std::min<int64_t>(ceil_div(p.M * p.N, 256), 4096), 256, 0, s>>>(p, (int)splits);
+    CCX_HIP(hipGetLastError());
+  }
 }
-template void x3_gemm<__bf16>(const X3Gemm&, hipStream_t);
-template void x3_gemm<_Float16>(const X3Gemm&, hipStream_t);
+template void x3_gemm<__bf16>(const X3Gemm&, hipStream_t, Workspace*);
+template void x3_gemm<_Float16>(const X3Gemm&, hipStream_t, Workspace*);
 
 template <typename T>
 void x3_absmax(const T* x, int64_t n, float* part, hipStream_t s) {
@@ -632,13 +703,18 @@ template void x3_split_rows<double>(const double*, int64_t, int64_t, int64_t, co
 template void x3_split_rows<float>(const float*, int64_t, int64_t, int64_t, const float*, f16*, f16*, int64_t,
                                    hipStream_t);
 
-void x3_split_t(const float* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
+template <typename Tin>
+void x3_split_t(const Tin* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
                 int64_t Kb, int64_t ldd, hipStream_t s) {
   const int64_t work = ceil_div(R, 32) * ceil_div(Cb, 32) * nx;
-  split_t_kernel<_Float16><<<(unsigned)std::min<int64_t>(work, 1 << 20), 256, 0, s>>>(
+  split_t_kernel<Tin, _Float16><<<(unsigned)std::min<int64_t>(work, 1 << 20), 256, 0, s>>>(
       src, R, Cb, nx, lds, sc, (_Float16*)hi, (_Float16*)lo, Kb, ldd);
   CCX_HIP(hipGetLastError());
 }
+template void x3_split_t<float>(const float*, int64_t, int64_t, int, int64_t, const float*, f16*, f16*, int64_t,
+                                int64_t, hipStream_t);
+template void x3_split_t<double>(const double*, int64_t, int64_t, int, int64_t, const float*, f16*, f16*, int64_t,
+                                 int64_t, hipStream_t);
 
 void ladder_pack_x3(int64_t v, int64_t e0, int64_t e1, VVVVSrc src, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
                     float* sP, float* sM, hipStream_t s) {
@@ -681,7 +757,8 @@ void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const dou
 // C = alpha op(A) op(B) + beta C for FP32 row-major matrices through FP16x3: op(A) (M x K) and
 // op(B)^T (N x K) are split into K-contiguous workspace copies with their own scales.
 void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
-                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, Workspace& ws, hipStream_t s) {
+                const float* B, int64_t ldb, float beta, float* C, int64_t ldc, Workspace& ws, Workspace& kws,
+                hipStream_t s) {
   CCX_CHECK(M >= 0 && N >= 0 && K >= 0, ERR_ARG, "negative GEMM extent");
   if (M == 0 || N == 0) return;
   const int64_t Kp = bf16_kpad(K);
@@ -711,7 +788,7 @@ void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, 
   g.M = M; g.N = N; g.Kp = Kp; g.lda = Kp; g.ldb = Kp; g.ldc = ldc;
   g.Ah = Ah; g.Al = Al; g.Bh = Bh; g.Bl = Bl; g.C = C;
   g.sa = sc; g.sb = sc + 1; g.alpha = alpha; g.beta = beta;
-  x3_gemm<_Float16>(g, s);
+  x3_gemm<_Float16>(g, s, &kws);
 }
 
 // The packed ladder (ccwfn.py:931 and Z, :715) as FP16x3 GEMMs: scaled tau rows times the
