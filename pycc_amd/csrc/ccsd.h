@@ -43,8 +43,11 @@ struct Ctx {
   // FP16x3 ring operands (MODE_MIXED_F16X3): split K and J, [ov][ovp]
   f16 *Kh = nullptr, *Kl = nullptr, *Jh = nullptr, *Jl = nullptr;
   f16 *Ioh = nullptr, *Iol = nullptr; // split <mn|ef> [(mn)][(ef) padded] (hole-hole term :603)
+  // split Covvv: CvA[(m,e,a)][f] = (me|af) and CvT[(m,b,e)][f] = (mf|be), f padded to v32
+  f16 *CvAh = nullptr, *CvAl = nullptr, *CvTh = nullptr, *CvTl = nullptr;
   float* xs = nullptr;               // operand scales: [0] ring amplitudes, [1] J, [2] K, [3] W,
-                                     // [4] packed tau, [5] tau, [6] <mn|ef>, [7] Wmnij
+                                     // [4] packed tau, [5] tau, [6] <mn|ef>, [7] Wmnij,
+                                     // [8] Covvv, [9] t1, [10] Ud
   float* xpart = nullptr;            // absmax partials (2 x x3_partials())
 
   // ---- per-residual work buffers (allocated on first use) ----
@@ -61,6 +64,10 @@ struct Ctx {
   f16 *AQh = nullptr, *AQl = nullptr, *BPh = nullptr, *BPl = nullptr;
   // FP16x3 hole-hole operands: tau [(ij)][(ef)], tau^T [(ab)][(mn)], Wmnij^T [(ij)][(mn)]
   f16 *TAh = nullptr, *TAl = nullptr, *TTh = nullptr, *TTl = nullptr, *WTh = nullptr, *WTl = nullptr;
+  // FP16x3 <ov|vv> terms: split t1 [o][v32], Ud as [i][(m,e)][v32]; PT[j][(m,x,y)] =
+  // sum_z t_jz Covvv[m][x][y][z], P2T[j][(m,b,e)] = sum_f t_jf (mf|be), Qr1[i][a] (:759)
+  f16 *T1h = nullptr, *T1l = nullptr, *UTh = nullptr, *UTl = nullptr;
+  float *PT = nullptr, *P2T = nullptr, *Qr1 = nullptr;
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
 
@@ -123,6 +130,7 @@ struct Ctx {
   int64_t ovp() const { return (o * v + 31) / 32 * 32; }   // K padding of the x3 ring operands
   int64_t o2p() const { return (o * o + 31) / 32 * 32; }
   int64_t v2p() const { return (v * v + 31) / 32 * 32; }
+  int64_t v32() const { return (v + 31) / 32 * 32; }
   bool split_ladder() const { return mode == MODE_MIXED_BF16X3 || mode == MODE_MIXED_F16X3; }
 
   template <typename T>

@@ -148,6 +148,7 @@ static void finish_integrals(Ctx& c, const double* fock, hipStream_t s) {
     x3_absmax<double>(c.Ioovv, ov * ov, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 6, s);
     x3_split_rows<double>(c.Ioovv, o * o, v * v, v * v, c.xs + 6, c.Ioh, c.Iol, c.v2p(), s);
+    covvv_x3_setup(c, s);
   } else if (c.mode != MODE_DP) {
     cast<double, float>(ov * ov, c.Covov, c.K32, s);
     cast<double, float>(ov * ov, c.J, c.J32, s);
@@ -302,6 +303,9 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
     contract<double>(a, A, B, b, C, ws, s);
   };
 
+  const bool x3 = c.mode == MODE_MIXED_F16X3;
+  if (x3) covvv_x3_products(c, t1, s);   // the <ov|vv> products of t1 (Fae, W, Hh terms below)
+
   // ---- Fme: ccwfn.py:563-564 -------------------------------------------------------------
   const TV Fme = tv(c.Fme, "me", OV);
   axpby<double>(1.0, Fov, 0.0, Fme, s);
@@ -312,8 +316,12 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   const TV Fae = tv(c.Fae, "ae", VV);
   axpby<double>(1.0, Fvv, 0.0, Fae, s);
   cd(-0.5, Fov, T1("ma"), 1.0, Fae);
-  cd(2.0, T1("mf"), tv(c.Covvv, "mfae", OVVV), 1.0, Fae);     // L_mafe = 2(mf|ae) - (me|af)
-  cd(-1.0, T1("mf"), tv(c.Covvv, "meaf", OVVV), 1.0, Fae);
+  if (x3) {
+    covvv_x3_fae(c, c.Fae, s);
+  } else {
+    cd(2.0, T1("mf"), tv(c.Covvv, "mfae", OVVV), 1.0, Fae);   // L_mafe = 2(mf|ae) - (me|af)
+    cd(-1.0, T1("mf"), tv(c.Covvv, "meaf", OVVV), 1.0, Fae);
+  }
   cd(-1.0, tv(c.tau, "mnaf", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fae);
 
   // ---- Fmi: ccwfn.py:530-533 --------------------------------------------------------------
@@ -359,7 +367,6 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   T* S2 = (T*)c.S2;
   T* Tx = (T*)c.Tx;
   T* W = (T*)c.W;
-  const bool x3 = c.mode == MODE_MIXED_F16X3;
   if (x3) {
     // amplitude scale: every ring operand is bounded by 3 max|t2| + max|t1|^2
     const int np = x3_partials();
@@ -374,11 +381,12 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   const std::vector<int64_t> WSTR{v * 2 * ov, 2 * ov, v, 1};
   const TV Wej = tvs(c.Wd, "mejb", OVOV, WSTR), Wje = tvs(c.Wd + ov, "mejb", OVOV, WSTR);
   axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, Wej, s);      // <mb|ej> = (me|jb)        :641
-  cd(1.0, T1("jf"), tv(c.Covvv, "mebf", OVVV), 1.0, Wej);          // t_jf <mb|ef>, (me|bf)     :642
+  if (!x3) cd(1.0, T1("jf"), tv(c.Covvv, "mebf", OVVV), 1.0, Wej); // t_jf <mb|ef>, (me|bf)     :642
   cd(-1.0, T1("nb"), tv(c.Cooov, "njme", OOOV), 1.0, Wej);         // t_nb <mn|ej>, (nj|me)     :643
   axpby<double>(-1.0, tv(c.Coovv, "mjbe", OOVV), 0.0, Wje, s);     // -<mb|je> = -(mj|be)       :680
-  cd(-1.0, T1("jf"), tv(c.Covvv, "mfbe", OVVV), 1.0, Wje);         // t_jf <mb|fe>, (mf|be)     :681
+  if (!x3) cd(-1.0, T1("jf"), tv(c.Covvv, "mfbe", OVVV), 1.0, Wje);   // t_jf <mb|fe>, (mf|be)  :681
   cd(1.0, T1("nb"), tv(c.Cooov, "mjne", OOOV), 1.0, Wje);          // t_nb <mn|je>, (mj|ne)     :682
+  if (x3) covvv_x3_w(c, c.Wd, s);                                  // :642 and :681
   if (!std::is_same<T, double>::value) cast<double, T>(2 * ov * ov, c.Wd, W, s);
   const T* Kt = std::is_same<T, double>::value ? (const T*)c.Covov : (const T*)c.K32;
   const T* Jt = std::is_same<T, double>::value ? (const T*)c.J : (const T*)c.J32;
@@ -510,7 +518,8 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // :932, :937, :938, :940 through Zt
   cd(-1.0, T1("ma"), Zt, 1.0, Hh);
   // :939  t_ie <ab|ej> = t_ie (jb|ae)
-  cd(1.0, T1("ie"), tv(c.Covvv, "jbae", OVVV), 1.0, Hh);
+  if (x3) covvv_x3_hh(c, c.Hh, s);
+  else cd(1.0, T1("ie"), tv(c.Covvv, "jbae", OVVV), 1.0, Hh);
 
   // ---- r2 = Hh + Hh^T + ladder + rings (ccwfn.py:790) --------------------------------------------
   if (c.mode == MODE_DP)
@@ -529,7 +538,8 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   cd(1.0, tv(c.Ud, "iame", OVOV), Fme, 1.0, R1);                     // (2t2 - t2^T) Fme     :757
   cd(2.0, T1("nf"), tv(c.Covov, "nfia", OVOV), 1.0, R1);             // L_nafi = 2(nf|ia)    :758
   cd(-1.0, T1("nf"), tv(c.Coovv, "niaf", OOVV), 1.0, R1);            //        - (ni|af)
-  cd(1.0, tv(c.Ud, "meif", OVOV), tv(c.Covvv, "meaf", OVVV), 1.0, R1); // <ma|ef> = (me|af)    :759
+  if (x3) covvv_x3_r1(c, c.Ud, r1, s);                              // <ma|ef> = (me|af)    :759
+  else cd(1.0, tv(c.Ud, "meif", OVOV), tv(c.Covvv, "meaf", OVVV), 1.0, R1);
   cd(-1.0, T2("mnae"), tv(c.Looov, "imne", OOOV), 1.0, R1);         // L_nmei = 2(mi|ne) - (ni|me) :760
 }
 

@@ -17,6 +17,9 @@ struct X3Gemm {
   const float *sa = nullptr, *sb = nullptr;
   const float* sbn = nullptr;        // per-row scales of B (one per n), in place of sb
   float alpha = 1.f, beta = 0.f;
+  // blocked K (kb > 0): k = q kb + r lives at offset q ska + r in a row of A (q skb + r in B); kb
+  // is a multiple of 32, so a 32-wide k-tile never straddles two blocks
+  int64_t kb = 0, ska = 0, skb = 0;
   // split-K (set by x3_gemm): grid.y = splits K ranges of kchunk, raw partials in work[s][M][N]
   int64_t kchunk = 0;
   float* work = nullptr;
@@ -55,6 +58,15 @@ void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t
 void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda,
                 const float* B, int64_t ldb, float beta, float* C, int64_t ldc, Workspace& ws, Workspace& kws,
                 hipStream_t s);
+// The <ov|vv> terms of MIXED_F16X3 (csrc/split16.hip): split copies of Covvv, the three split
+// GEMMs and their scatter into Fae, W, Hh and r1.
+struct Ctx;
+void covvv_x3_setup(Ctx& c, hipStream_t s);
+void covvv_x3_products(Ctx& c, const double* t1, hipStream_t s);   // PT, P2T from t1
+void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s);              // + 2 t_mf (mf|ae) - t_mf (me|af)
+void covvv_x3_w(Ctx& c, double* Wd, hipStream_t s);                 // :642, :681 into Wd
+void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s);                // :939
+void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s);   // :759
 // The split ring operands BJ, BK, AP, AQ (row stride ovp, or 2 ovp for AP) and Ud from t1, t2.
 void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,
                   f16* BJl, f16* BKh, f16* BKl, f16* APh, f16* APl, f16* AQh, f16* AQl, double* Ud,

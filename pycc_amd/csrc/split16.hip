@@ -89,12 +89,18 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
   const v8 zero8 = v8{};
   v8 rA[2][Q], rB[2][Q];
   auto gload = [&](int64_t k0) {
+    int64_t ka = k0, kb = k0;
+    if (p.kb) {
+      const int64_t q = k0 / p.kb, r = k0 - q * p.kb;
+      ka = q * p.ska + r;
+      kb = q * p.skb + r;
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      rA[0][q] = *(const v8*)(Ah + offA[q] + k0);
-      rA[1][q] = *(const v8*)(Al + offA[q] + k0);
-      rB[0][q] = *(const v8*)(Bh + offB[q] + k0);
-      rB[1][q] = *(const v8*)(Bl + offB[q] + k0);
+      rA[0][q] = *(const v8*)(Ah + offA[q] + ka);
+      rA[1][q] = *(const v8*)(Al + offA[q] + ka);
+      rB[0][q] = *(const v8*)(Bh + offB[q] + kb);
+      rB[1][q] = *(const v8*)(Bl + offB[q] + kb);
     }
   };
   auto sstore = [&](int b) {
@@ -242,12 +248,18 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
     ldso[i] = R0 * 32;
   }
   auto issue = [&](int st, int64_t k0) {
+    int64_t ka = k0, kb = k0;
+    if (p.kb) {
+      const int64_t q = k0 / p.kb, r = k0 - q * p.kb;
+      ka = q * p.ska + r;
+      kb = q * p.skb + r;
+    }
 #pragma unroll
     for (int i = 0; i < IPW; ++i) {
-      x3_dma16<E>(Ah + gA[i] + k0, &sA[st][0][ldso[i]]);
-      x3_dma16<E>(Al + gA[i] + k0, &sA[st][1][ldso[i]]);
-      x3_dma16<E>(Bh + gB[i] + k0, &sB[st][0][ldso[i]]);
-      x3_dma16<E>(Bl + gB[i] + k0, &sB[st][1][ldso[i]]);
+      x3_dma16<E>(Ah + gA[i] + ka, &sA[st][0][ldso[i]]);
+      x3_dma16<E>(Al + gA[i] + ka, &sA[st][1][ldso[i]]);
+      x3_dma16<E>(Bh + gB[i] + kb, &sB[st][0][ldso[i]]);
+      x3_dma16<E>(Bl + gB[i] + kb, &sB[st][1][ldso[i]]);
     }
   };
   const int fr = lane >> 4, fc = lane & 15;
@@ -642,8 +654,9 @@ template <typename E>
 void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
   X3Gemm p = p0;
   if (p.M == 0 || p.N == 0) return;
-  CCX_CHECK(p.Kp % BK == 0 && p.lda % 8 == 0 && p.ldb % 8 == 0 && p.lda >= p.Kp && p.ldb >= p.Kp, ERR_INTERNAL,
-            "x3 GEMM operand padding");
+  CCX_CHECK(p.Kp % BK == 0 && p.lda % 8 == 0 && p.ldb % 8 == 0, ERR_INTERNAL, "x3 GEMM operand padding");
+  CCX_CHECK(p.kb ? (p.kb % BK == 0 && p.ska % 8 == 0 && p.skb % 8 == 0) : (p.lda >= p.Kp && p.ldb >= p.Kp),
+            ERR_INTERNAL, "x3 GEMM K layout");
   if (p.Kp == 0) {   // empty contraction: C = beta C
     scale<float>(p.beta, tvs(p.C, "mn", {p.M, p.N}, {p.ldc, 1}), s);
     return;
@@ -751,6 +764,173 @@ void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const dou
   ring_prep_x3_kernel<_Float16><<<grid, 256, 0, s>>>(o, v, ovp, t1, t2, sc, (_Float16*)BJh, (_Float16*)BJl,
                                                      (_Float16*)BKh, (_Float16*)BKl, (_Float16*)APh,
                                                      (_Float16*)APl, (_Float16*)AQh, (_Float16*)AQl, Ud);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---- the <ov|vv> terms of MIXED_F16X3 ----------------------------------------------------------
+// Six contractions stream Covvv = (me|af) [m][e][a][f] (ccwfn.py:496 twice, :642, :681, :939, :759).
+// With t1 as the small operand they are three split GEMMs over two split copies of Covvv:
+//   PT[j][(m,x,y)]  = sum_z t_jz Covvv[m][x][y][z]        (copy CvA = Covvv rows, K = z)
+//   P2T[j][(m,b,e)] = sum_f t_jf (mf|be)                   (copy CvT, rows (m,b,e), K = f)
+//   Qr1[i][a]       = sum_(m,e,f) Ud[m][e][i][f] (me|af)   (CvA with blocked K = (m,e), f)
+// and their uses are permuted adds:
+//   Fae[a][e] += 2 sum_m P2T[m][(m,a,e)] - sum_m PT[m][(m,e,a)]               (:496)
+//   Wmbej[m][e][j][b] += PT[j][(m,e,b)],  Wmbje[m][e][j][b] -= P2T[j][(m,b,e)]  (:642, :681)
+//   Hh[i][j][a][b] += PT[i][(j,b,a)]                                            (:939)
+//   r1 += Qr1                                                                   (:759)
+namespace {
+
+// dst[(i, q)][f] (row stride ldd, q < nq) = split(src[q][i][f] * sc): Ud [(m,e)][(i,f)] -> [i][(m,e)][f]
+__global__ __launch_bounds__(256) void split_udt_kernel(const double* __restrict__ Ud, int64_t o, int64_t v, int64_t nq,
+                                                        const float* __restrict__ sc, _Float16* hi, _Float16* lo,
+                                                        int64_t v32) {
+  const double s = sc[0];
+  const int64_t total = o * nq * v;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = e % v, t = e / v, q = t % nq, i = t / nq;   // destination order: f fastest
+    split_store<_Float16>((float)(Ud[(q * o + i) * v + f] * s), hi, lo, (i * nq + q) * v32 + f);
+  }
+}
+
+// Wd[(m,e)][0][j][b] += PT[j][(m,e)][b]: one row of v per (j, (m,e))
+__global__ __launch_bounds__(256) void wej_add_kernel(const float* __restrict__ PT, int64_t o, int64_t v,
+                                                      double* __restrict__ Wd) {
+  const int64_t ov = o * v, total = o * ov * v;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e % v, t = e / v, me = t % ov, j = t / ov;
+    Wd[me * 2 * ov + j * v + b] += (double)PT[e];
+  }
+}
+
+// For each plane q = q1 n2 + q2 (q1 < n1, q2 < n2): out[q1 s1 + q2 s2 + off + c ldo + r] += alpha in[q v^2 +
+// r v + c] for r, c < v, through a 32 x 33 LDS transpose (coalesced on both sides).
+__global__ __launch_bounds__(256) void plane_t_add_kernel(const float* __restrict__ in, int64_t v, int64_t n1,
+                                                          int64_t n2, int64_t s1, int64_t s2, int64_t off,
+                                                          int64_t ldo, double alpha, double* __restrict__ out) {
+  __shared__ float tile[32][33];
+  const int64_t tv_ = (v + 31) / 32, per = tv_ * tv_;
+  for (int64_t w = blockIdx.x; w < n1 * n2 * per; w += gridDim.x) {
+    const int64_t q = w / per, rem = w - q * per;
+    const int64_t r0 = (rem / tv_) * 32, c0 = (rem % tv_) * 32;
+    const int64_t q1 = q / n2, q2 = q - q1 * n2;
+    const float* src = in + q * v * v;
+    double* dst = out + q1 * s1 + q2 * s2 + off;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int y = ty; y < 32; y += 8) {
+      const int64_t r = r0 + y, c = c0 + tx;
+      tile[y][tx] = (r < v && c < v) ? src[r * v + c] : 0.f;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+      const int64_t c = c0 + y, r = r0 + tx;
+      if (c < v && r < v) dst[c * ldo + r] += alpha * (double)tile[tx][y];
+    }
+    __syncthreads();
+  }
+}
+
+// Fae[a][e] += 2 sum_m P2T[m][(m,a,e)] - sum_m PT[m][(m,e,a)]
+__global__ __launch_bounds__(256) void fae_x3_kernel(const float* __restrict__ PT, const float* __restrict__ P2T,
+                                                     int64_t o, int64_t v, double* __restrict__ Fae) {
+  const int64_t vv = v * v, ov2 = o * vv;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < vv; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = e / v, ee = e - a * v;
+    double acc = 0.0;
+    for (int64_t m = 0; m < o; ++m) {
+      const int64_t base = m * ov2 + m * vv;
+      acc += 2.0 * (double)P2T[base + a * v + ee] - (double)PT[base + ee * v + a];
+    }
+    Fae[e] += acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void add_f32_kernel(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    y[e] += (double)x[e];
+}
+
+inline unsigned grid_e(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 65536)); }
+
+}  // namespace
+
+void covvv_x3_setup(Ctx& c, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, ov = c.ov(), v32 = c.v32();
+  auto halloc = [&](f16** q, int64_t n) {
+    *q = c.alloc<f16>(n);
+    CCX_HIP(hipMemsetAsync(*q, 0, n * sizeof(f16), s));   // K padding stays zero
+  };
+  halloc(&c.CvAh, ov * v * v32);
+  halloc(&c.CvAl, ov * v * v32);
+  halloc(&c.CvTh, ov * v * v32);
+  halloc(&c.CvTl, ov * v * v32);
+  halloc(&c.T1h, o * v32);
+  halloc(&c.T1l, o * v32);
+  halloc(&c.UTh, o * ov * v32);
+  halloc(&c.UTl, o * ov * v32);
+  c.PT = c.alloc<float>(o * ov * v);
+  c.P2T = c.alloc<float>(o * ov * v);
+  c.Qr1 = c.alloc<float>(ov);
+  x3_absmax<double>(c.Covvv, ov * v * v, c.xpart, s);
+  x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 8, s);
+  x3_split_rows<double>(c.Covvv, ov * v, v, v, c.xs + 8, c.CvAh, c.CvAl, v32, s);
+  for (int64_t m = 0; m < o; ++m)   // CvT[m][(b,e)][f] = Covvv[m][f][b][e]
+    x3_split_t<double>(c.Covvv + m * v * v * v, v, v * v, 1, v * v, c.xs + 8, c.CvTh + m * v * v * v32,
+                       c.CvTl + m * v * v * v32, 0, v32, s);
+}
+
+void covvv_x3_products(Ctx& c, const double* t1, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, ov = c.ov(), v32 = c.v32();
+  x3_absmax<double>(t1, ov, c.xpart, s);
+  x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 9, s);
+  x3_split_rows<double>(t1, o, v, v, c.xs + 9, c.T1h, c.T1l, v32, s);
+  X3Gemm g;
+  g.M = o; g.N = ov * v; g.Kp = v32; g.lda = v32; g.ldb = v32; g.ldc = ov * v;
+  g.Ah = c.T1h; g.Al = c.T1l; g.Bh = c.CvAh; g.Bl = c.CvAl; g.C = c.PT;
+  g.sa = c.xs + 9; g.sb = c.xs + 8;
+  x3_gemm<_Float16>(g, s);
+  g.Bh = c.CvTh; g.Bl = c.CvTl; g.C = c.P2T;
+  x3_gemm<_Float16>(g, s);
+}
+
+void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s) {
+  fae_x3_kernel<<<grid_e(c.v * c.v), 256, 0, s>>>(c.PT, c.P2T, c.o, c.v, Fae);
+  CCX_HIP(hipGetLastError());
+}
+
+void covvv_x3_w(Ctx& c, double* Wd, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, ov = c.ov();
+  wej_add_kernel<<<grid_e(o * ov * v), 256, 0, s>>>(c.PT, o, v, Wd);
+  CCX_HIP(hipGetLastError());
+  // Wmbje[m][e][j][b] -= P2T[j][m][b][e]: plane (j, m), out Wd[(m v + e) 2 ov + ov + j v + b]
+  const int64_t planes = o * o * ceil_div(v, 32) * ceil_div(v, 32);
+  plane_t_add_kernel<<<(unsigned)std::min<int64_t>(planes, 1 << 20), 256, 0, s>>>(
+      c.P2T, v, o, o, v, 2 * v * ov, ov, 2 * ov, -1.0, Wd);
+  CCX_HIP(hipGetLastError());
+}
+
+void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s) {
+  const int64_t o = c.o, v = c.v;
+  // Hh[i][j][a][b] += PT[i][j][b][a]: plane (i, j), out Hh[(i o + j) v^2 + a v + b]
+  const int64_t planes = o * o * ceil_div(v, 32) * ceil_div(v, 32);
+  plane_t_add_kernel<<<(unsigned)std::min<int64_t>(planes, 1 << 20), 256, 0, s>>>(
+      c.PT, v, o, o, o * v * v, v * v, 0, v, 1.0, Hh);
+  CCX_HIP(hipGetLastError());
+}
+
+void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, ov = c.ov(), v32 = c.v32();
+  x3_absmax<double>(Ud, ov * ov, c.xpart, s);
+  x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 10, s);
+  split_udt_kernel<<<grid_e(o * ov * v), 256, 0, s>>>(Ud, o, v, ov, c.xs + 10, (_Float16*)c.UTh, (_Float16*)c.UTl,
+                                                      v32);
+  CCX_HIP(hipGetLastError());
+  X3Gemm g;   // Qr1[i][a] = sum_(me, f) UT[i][(me)][f] CvA[(me)][a][f]
+  g.M = o; g.N = v; g.Kp = ov * v32; g.lda = ov * v32; g.ldb = v32; g.ldc = v;
+  g.kb = v32; g.ska = v32; g.skb = v * v32;
+  g.Ah = c.UTh; g.Al = c.UTl; g.Bh = c.CvAh; g.Bl = c.CvAl; g.C = c.Qr1;
+  g.sa = c.xs + 10; g.sb = c.xs + 8;
+  x3_gemm<_Float16>(g, s, &c.ws);
+  add_f32_kernel<<<grid_e(ov), 256, 0, s>>>(ov, c.Qr1, r1);
   CCX_HIP(hipGetLastError());
 }
 
