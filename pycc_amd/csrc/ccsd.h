@@ -47,7 +47,7 @@ struct Ctx {
   f16 *CvAh = nullptr, *CvAl = nullptr, *CvTh = nullptr, *CvTl = nullptr;
   float* xs = nullptr;               // operand scales: [0] ring amplitudes, [1] J, [2] K, [3] W,
                                      // [4] packed tau, [5] tau, [6] <mn|ef>, [7] Wmnij,
-                                     // [8] Covvv, [9] t1, [10] Ud
+                                     // [8] Covvv, [9] t1, [10] Ud, [11] tau~, [12] L
   float* xpart = nullptr;            // absmax partials (2 x x3_partials())
 
   // ---- per-residual work buffers (allocated on first use) ----
@@ -68,6 +68,11 @@ struct Ctx {
   // sum_z t_jz Covvv[m][x][y][z], P2T[j][(m,b,e)] = sum_f t_jf (mf|be), Qr1[i][a] (:759)
   f16 *T1h = nullptr, *T1l = nullptr, *UTh = nullptr, *UTl = nullptr;
   float *PT = nullptr, *P2T = nullptr, *Qr1 = nullptr;
+  // W0 = the integral part of [Wmbej | Wmbje] ((me|jb), -(mj|be)) in the W layout; split tau~ and L
+  // as [(i,j,a)][f padded] for :497 / :533; FP32 results of those two GEMMs
+  double* W0 = nullptr;
+  f16 *TFh = nullptr, *TFl = nullptr, *LFh = nullptr, *LFl = nullptr;
+  float *Fae32 = nullptr, *Fmi32 = nullptr;
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
 

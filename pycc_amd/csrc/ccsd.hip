@@ -304,7 +304,10 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   };
 
   const bool x3 = c.mode == MODE_MIXED_F16X3;
-  if (x3) covvv_x3_products(c, t1, s);   // the <ov|vv> products of t1 (Fae, W, Hh terms below)
+  if (x3) {
+    x3_amp_scales(c, t1, t2, s);          // every amplitude-derived FP16x3 operand scale
+    covvv_x3_products(c, t1, s);          // the <ov|vv> products of t1 (Fae, W, Hh terms below)
+  }
 
   // ---- Fme: ccwfn.py:563-564 -------------------------------------------------------------
   const TV Fme = tv(c.Fme, "me", OV);
@@ -322,14 +325,15 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
     cd(2.0, T1("mf"), tv(c.Covvv, "mfae", OVVV), 1.0, Fae);   // L_mafe = 2(mf|ae) - (me|af)
     cd(-1.0, T1("mf"), tv(c.Covvv, "meaf", OVVV), 1.0, Fae);
   }
-  cd(-1.0, tv(c.tau, "mnaf", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fae);
+  if (!x3) cd(-1.0, tv(c.tau, "mnaf", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fae);   // FP16x3: below
 
   // ---- Fmi: ccwfn.py:530-533 --------------------------------------------------------------
   const TV Fmi = tv(c.Fmi, "mi", OO);
   axpby<double>(1.0, Foo, 0.0, Fmi, s);
   cd(0.5, T1("ie"), Fov, 1.0, Fmi);
   cd(1.0, T1("ne"), tv(c.Looov, "imne", OOOV), 1.0, Fmi);     // L_mnie = 2(mi|ne) - (ni|me)
-  cd(1.0, tv(c.tau, "inef", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fmi);
+  if (x3) tauL_x3(c, c.Fae, c.Fmi, s);   // :497 and this term, tau~ L as FP16x3 GEMMs
+  else cd(1.0, tv(c.tau, "inef", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fmi);
 
   // ---- Wmnij: ccwfn.py:596-603 (tau = t2 + t1 t1) -------------------------------------------
   build_tau(o, v, 1.0, 1.0, t1, t2, c.tau, s);
@@ -342,8 +346,6 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   } else if (c.mode == MODE_MIXED_F16X3) {
     // the two o^4 v^2 hole-hole contractions (:603 and :930) as FP16x3 GEMMs; tau is split once in
     // both orientations ([(ij)][(ef)] here, [(ab)][(mn)] for :930) with one scale
-    x3_absmax<double>(c.tau, ov * ov, c.xpart, s);
-    x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 5, s);
     x3_split_rows<double>(c.tau, o * o, v * v, v * v, c.xs + 5, c.TAh, c.TAl, c.v2p(), s);
     x3_split_t<double>(c.tau, o * o, v * v, 1, v * v, c.xs + 5, c.TTh, c.TTl, 0, c.o2p(), s);
     X3Gemm g;   // W32[(mn)][(ij)] = sum_(ef) <mn|ef> tau_ijef
@@ -368,11 +370,7 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   T* Tx = (T*)c.Tx;
   T* W = (T*)c.W;
   if (x3) {
-    // amplitude scale: every ring operand is bounded by 3 max|t2| + max|t1|^2
-    const int np = x3_partials();
-    x3_absmax<double>(t2, ov * ov, c.xpart, s);
-    x3_absmax<double>(t1, ov, c.xpart + np, s);
-    x3_scale(c.xpart, 3.f, c.xpart + np, 1.f, c.xs, s);
+    // every ring operand is bounded by 3 max|t2| + max|t1|^2 (scale xs[0])
     ring_prep_x3(o, v, c.ovp(), t1, t2, c.xs, c.BJh, c.BJl, c.BKh, c.BKl, c.APh, c.APl, c.AQh, c.AQl, c.Ud, s);
   } else {
     ring_prep<T>(o, v, t1, t2, S1, A1, S2, Tx, c.Ud, s);
@@ -380,13 +378,16 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // W[(me)][x][(jb)]: x = 0 holds Wmbej, x = 1 holds Wmbje
   const std::vector<int64_t> WSTR{v * 2 * ov, 2 * ov, v, 1};
   const TV Wej = tvs(c.Wd, "mejb", OVOV, WSTR), Wje = tvs(c.Wd + ov, "mejb", OVOV, WSTR);
-  axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, Wej, s);      // <mb|ej> = (me|jb)        :641
-  if (!x3) cd(1.0, T1("jf"), tv(c.Covvv, "mebf", OVVV), 1.0, Wej); // t_jf <mb|ef>, (me|bf)     :642
+  if (x3) {
+    covvv_x3_w(c, c.Wd, s);                                        // :641 + :642, :680 + :681
+  } else {
+    axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, Wej, s);    // <mb|ej> = (me|jb)        :641
+    cd(1.0, T1("jf"), tv(c.Covvv, "mebf", OVVV), 1.0, Wej);        // t_jf <mb|ef>, (me|bf)     :642
+    axpby<double>(-1.0, tv(c.Coovv, "mjbe", OOVV), 0.0, Wje, s);   // -<mb|je> = -(mj|be)       :680
+    cd(-1.0, T1("jf"), tv(c.Covvv, "mfbe", OVVV), 1.0, Wje);       // t_jf <mb|fe>, (mf|be)     :681
+  }
   cd(-1.0, T1("nb"), tv(c.Cooov, "njme", OOOV), 1.0, Wej);         // t_nb <mn|ej>, (nj|me)     :643
-  axpby<double>(-1.0, tv(c.Coovv, "mjbe", OOVV), 0.0, Wje, s);     // -<mb|je> = -(mj|be)       :680
-  if (!x3) cd(-1.0, T1("jf"), tv(c.Covvv, "mfbe", OVVV), 1.0, Wje);   // t_jf <mb|fe>, (mf|be)  :681
   cd(1.0, T1("nb"), tv(c.Cooov, "mjne", OOOV), 1.0, Wje);          // t_nb <mn|je>, (mj|ne)     :682
-  if (x3) covvv_x3_w(c, c.Wd, s);                                  // :642 and :681
   if (!std::is_same<T, double>::value) cast<double, T>(2 * ov * ov, c.Wd, W, s);
   const T* Kt = std::is_same<T, double>::value ? (const T*)c.Covov : (const T*)c.K32;
   const T* Jt = std::is_same<T, double>::value ? (const T*)c.J : (const T*)c.J32;

@@ -61,6 +61,10 @@ void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, 
 // The <ov|vv> terms of MIXED_F16X3 (csrc/split16.hip): split copies of Covvv, the three split
 // GEMMs and their scatter into Fae, W, Hh and r1.
 struct Ctx;
+// max|t2| and max|t1| once per residual, and from them every amplitude-derived operand scale
+void x3_amp_scales(Ctx& c, const double* t1, const double* t2, hipStream_t s);
+// Fae -= tau~_mnaf L_mnef (:497) and Fmi += tau~_inef L_mnef (:533) as FP16x3 GEMMs (tau~ in c.tau)
+void tauL_x3(Ctx& c, double* Fae, double* Fmi, hipStream_t s);
 void covvv_x3_setup(Ctx& c, hipStream_t s);
 void covvv_x3_products(Ctx& c, const double* t1, hipStream_t s);   // PT, P2T from t1
 void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s);              // + 2 t_mf (mf|ae) - t_mf (me|af)

@@ -669,7 +669,7 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
   int64_t splits = 1;
   p.work = nullptr;
   if (ws && tiles < 512) {
-    splits = std::min<int64_t>({64, ceil_div(1024, tiles), p.Kp / (32 * BK)});
+    splits = std::min<int64_t>({256, ceil_div(1024, tiles), p.Kp / (32 * BK)});
     if (splits > 1) {
       p.kchunk = ceil_div(ceil_div(p.Kp, splits), BK) * BK;
       splits = ceil_div(p.Kp, p.kchunk);
@@ -792,21 +792,24 @@ __global__ __launch_bounds__(256) void split_udt_kernel(const double* __restrict
   }
 }
 
-// Wd[(m,e)][0][j][b] += PT[j][(m,e)][b]: one row of v per (j, (m,e))
+// Wd[(m,e)][0][j][b] = W0[same] + PT[j][(m,e)][b]: one row of v per (j, (m,e))
 __global__ __launch_bounds__(256) void wej_add_kernel(const float* __restrict__ PT, int64_t o, int64_t v,
-                                                      double* __restrict__ Wd) {
+                                                      const double* __restrict__ W0, double* __restrict__ Wd) {
   const int64_t ov = o * v, total = o * ov * v;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = e % v, t = e / v, me = t % ov, j = t / ov;
-    Wd[me * 2 * ov + j * v + b] += (double)PT[e];
+    const int64_t at = me * 2 * ov + j * v + b;
+    Wd[at] = W0[at] + (double)PT[e];
   }
 }
 
-// For each plane q = q1 n2 + q2 (q1 < n1, q2 < n2): out[q1 s1 + q2 s2 + off + c ldo + r] += alpha in[q v^2 +
-// r v + c] for r, c < v, through a 32 x 33 LDS transpose (coalesced on both sides).
+// For each plane q = q1 n2 + q2 (q1 < n1, q2 < n2): out[x] = (base ? base[x] : out[x]) + alpha in[q v^2 + r v + c]
+// with x = q1 s1 + q2 s2 + off + c ldo + r, for r, c < v, through a 32 x 33 LDS transpose (coalesced on
+// both sides).
 __global__ __launch_bounds__(256) void plane_t_add_kernel(const float* __restrict__ in, int64_t v, int64_t n1,
                                                           int64_t n2, int64_t s1, int64_t s2, int64_t off,
-                                                          int64_t ldo, double alpha, double* __restrict__ out) {
+                                                          int64_t ldo, double alpha, const double* base,
+                                                          double* out) {
   __shared__ float tile[32][33];
   const int64_t tv_ = (v + 31) / 32, per = tv_ * tv_;
   for (int64_t w = blockIdx.x; w < n1 * n2 * per; w += gridDim.x) {
@@ -814,7 +817,9 @@ __global__ __launch_bounds__(256) void plane_t_add_kernel(const float* __restric
     const int64_t r0 = (rem / tv_) * 32, c0 = (rem % tv_) * 32;
     const int64_t q1 = q / n2, q2 = q - q1 * n2;
     const float* src = in + q * v * v;
-    double* dst = out + q1 * s1 + q2 * s2 + off;
+    const int64_t o0 = q1 * s1 + q2 * s2 + off;
+    double* dst = out + o0;
+    const double* bse = base ? base + o0 : dst;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     for (int y = ty; y < 32; y += 8) {
       const int64_t r = r0 + y, c = c0 + tx;
@@ -823,7 +828,7 @@ __global__ __launch_bounds__(256) void plane_t_add_kernel(const float* __restric
     __syncthreads();
     for (int y = ty; y < 32; y += 8) {
       const int64_t c = c0 + y, r = r0 + tx;
-      if (c < v && r < v) dst[c * ldo + r] += alpha * (double)tile[tx][y];
+      if (c < v && r < v) dst[c * ldo + r] = bse[c * ldo + r] + alpha * (double)tile[tx][y];
     }
     __syncthreads();
   }
@@ -870,6 +875,20 @@ void covvv_x3_setup(Ctx& c, hipStream_t s) {
   c.PT = c.alloc<float>(o * ov * v);
   c.P2T = c.alloc<float>(o * ov * v);
   c.Qr1 = c.alloc<float>(ov);
+  halloc(&c.TFh, o * ov * v32);
+  halloc(&c.TFl, o * ov * v32);
+  halloc(&c.LFh, o * ov * v32);
+  halloc(&c.LFl, o * ov * v32);
+  c.Fae32 = c.alloc<float>(v * v);
+  c.Fmi32 = c.alloc<float>(o * o);
+  // W0[(m,e)][x][(j,b)]: x = 0 (me|jb) (:641), x = 1 -(mj|be) (:680)
+  c.W0 = c.alloc<double>(2 * ov * ov);
+  const std::vector<int64_t> OVOV{o, v, o, v}, WSTR{v * 2 * ov, 2 * ov, v, 1};
+  axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, tvs(c.W0, "mejb", OVOV, WSTR), s);
+  axpby<double>(-1.0, tv(c.Coovv, "mjbe", {o, o, v, v}), 0.0, tvs(c.W0 + ov, "mejb", OVOV, WSTR), s);
+  x3_absmax<double>(c.Loovv, ov * ov, c.xpart, s);
+  x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 12, s);
+  x3_split_rows<double>(c.Loovv, o * ov, v, v, c.xs + 12, c.LFh, c.LFl, v32, s);
   x3_absmax<double>(c.Covvv, ov * v * v, c.xpart, s);
   x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 8, s);
   x3_split_rows<double>(c.Covvv, ov * v, v, v, c.xs + 8, c.CvAh, c.CvAl, v32, s);
@@ -880,9 +899,7 @@ void covvv_x3_setup(Ctx& c, hipStream_t s) {
 
 void covvv_x3_products(Ctx& c, const double* t1, hipStream_t s) {
   const int64_t o = c.o, v = c.v, ov = c.ov(), v32 = c.v32();
-  x3_absmax<double>(t1, ov, c.xpart, s);
-  x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 9, s);
-  x3_split_rows<double>(t1, o, v, v, c.xs + 9, c.T1h, c.T1l, v32, s);
+  x3_split_rows<double>(t1, o, v, v, c.xs + 9, c.T1h, c.T1l, v32, s);   // scale from x3_amp_scales
   X3Gemm g;
   g.M = o; g.N = ov * v; g.Kp = v32; g.lda = v32; g.ldb = v32; g.ldc = ov * v;
   g.Ah = c.T1h; g.Al = c.T1l; g.Bh = c.CvAh; g.Bl = c.CvAl; g.C = c.PT;
@@ -897,14 +914,15 @@ void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s) {
   CCX_HIP(hipGetLastError());
 }
 
+// Wd = W0 + the :642 / :681 products (writes every element of Wd)
 void covvv_x3_w(Ctx& c, double* Wd, hipStream_t s) {
   const int64_t o = c.o, v = c.v, ov = c.ov();
-  wej_add_kernel<<<grid_e(o * ov * v), 256, 0, s>>>(c.PT, o, v, Wd);
+  wej_add_kernel<<<grid_e(o * ov * v), 256, 0, s>>>(c.PT, o, v, c.W0, Wd);
   CCX_HIP(hipGetLastError());
-  // Wmbje[m][e][j][b] -= P2T[j][m][b][e]: plane (j, m), out Wd[(m v + e) 2 ov + ov + j v + b]
+  // Wmbje[m][e][j][b] = W0 - P2T[j][m][b][e]: plane (j, m), out Wd[(m v + e) 2 ov + ov + j v + b]
   const int64_t planes = o * o * ceil_div(v, 32) * ceil_div(v, 32);
   plane_t_add_kernel<<<(unsigned)std::min<int64_t>(planes, 1 << 20), 256, 0, s>>>(
-      c.P2T, v, o, o, v, 2 * v * ov, ov, 2 * ov, -1.0, Wd);
+      c.P2T, v, o, o, v, 2 * v * ov, ov, 2 * ov, -1.0, c.W0, Wd);
   CCX_HIP(hipGetLastError());
 }
 
@@ -913,14 +931,12 @@ void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s) {
   // Hh[i][j][a][b] += PT[i][j][b][a]: plane (i, j), out Hh[(i o + j) v^2 + a v + b]
   const int64_t planes = o * o * ceil_div(v, 32) * ceil_div(v, 32);
   plane_t_add_kernel<<<(unsigned)std::min<int64_t>(planes, 1 << 20), 256, 0, s>>>(
-      c.PT, v, o, o, o * v * v, v * v, 0, v, 1.0, Hh);
+      c.PT, v, o, o, o * v * v, v * v, 0, v, 1.0, nullptr, Hh);
   CCX_HIP(hipGetLastError());
 }
 
 void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s) {
   const int64_t o = c.o, v = c.v, ov = c.ov(), v32 = c.v32();
-  x3_absmax<double>(Ud, ov * ov, c.xpart, s);
-  x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 10, s);
   split_udt_kernel<<<grid_e(o * ov * v), 256, 0, s>>>(Ud, o, v, ov, c.xs + 10, (_Float16*)c.UTh, (_Float16*)c.UTl,
                                                       v32);
   CCX_HIP(hipGetLastError());
@@ -931,6 +947,41 @@ void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s) {
   g.sa = c.xs + 10; g.sb = c.xs + 8;
   x3_gemm<_Float16>(g, s, &c.ws);
   add_f32_kernel<<<grid_e(ov), 256, 0, s>>>(ov, c.Qr1, r1);
+  CCX_HIP(hipGetLastError());
+}
+
+void x3_amp_scales(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
+  const int np = x3_partials();
+  const int64_t ov = c.ov();
+  x3_absmax<double>(t2, ov * ov, c.xpart, s);
+  x3_absmax<double>(t1, ov, c.xpart + np, s);
+  const float* p2 = c.xpart;
+  const float* p1 = c.xpart + np;
+  x3_scale(p2, 3.f, p1, 1.f, c.xs + 0, s);     // ring operands: |2T - X|, |T - X/2 - P|, ...
+  x3_scale(p2, 2.f, p1, 2.f, c.xs + 4, s);     // packed tau+-
+  x3_scale(p2, 1.f, p1, 1.f, c.xs + 5, s);     // tau = t2 + t1 t1
+  x3_scale(p1, 1.f, nullptr, 0.f, c.xs + 9, s);   // t1
+  x3_scale(p2, 3.f, nullptr, 0.f, c.xs + 10, s);  // Ud = 2 t2 - t2^T
+  x3_scale(p2, 1.f, p1, 0.5f, c.xs + 11, s);   // tau~ = t2 + 1/2 t1 t1
+}
+
+void tauL_x3(Ctx& c, double* Fae, double* Fmi, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, ov = c.ov(), v32 = c.v32();
+  x3_split_rows<double>(c.tau, o * ov, v, v, c.xs + 11, c.TFh, c.TFl, v32, s);   // tau~ [(m,n,a)][f]
+  X3Gemm g;   // Fae32[a][e] = -sum_(mn, f) tau~[(mn)][a][f] L[(mn)][e][f]
+  g.M = v; g.N = v; g.Kp = o * o * v32; g.lda = v32; g.ldb = v32; g.ldc = v;
+  g.kb = v32; g.ska = v * v32; g.skb = v * v32;
+  g.Ah = c.TFh; g.Al = c.TFl; g.Bh = c.LFh; g.Bl = c.LFl; g.C = c.Fae32;
+  g.sa = c.xs + 11; g.sb = c.xs + 12; g.alpha = -1.f;
+  x3_gemm<_Float16>(g, s, &c.ws);
+  add_f32_kernel<<<grid_e(v * v), 256, 0, s>>>(v * v, c.Fae32, Fae);
+  CCX_HIP(hipGetLastError());
+  X3Gemm h;   // Fmi32[m][i] = sum_(n,e,f) L[m][(n,e)][f] tau~[i][(n,e)][f]
+  h.M = o; h.N = o; h.Kp = ov * v32; h.lda = ov * v32; h.ldb = ov * v32; h.ldc = o;
+  h.Ah = c.LFh; h.Al = c.LFl; h.Bh = c.TFh; h.Bl = c.TFl; h.C = c.Fmi32;
+  h.sa = c.xs + 12; h.sb = c.xs + 11;
+  x3_gemm<_Float16>(h, s, &c.ws);
+  add_f32_kernel<<<grid_e(o * o), 256, 0, s>>>(o * o, c.Fmi32, Fmi);
   CCX_HIP(hipGetLastError());
 }
 
@@ -977,11 +1028,7 @@ void ladder_x3(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
   const int64_t o = c.o, v = c.v, ov = c.ov();
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
   const int64_t KpP = bf16_kpad(npP), KpM = bf16_kpad(npM);
-  const int np = x3_partials();
-  // packed tau+- are bounded by 2 max|t2| + 2 max|t1|^2
-  x3_absmax<double>(t2, ov * ov, c.xpart, s);
-  x3_absmax<double>(t1, ov, c.xpart + np, s);
-  x3_scale(c.xpart, 2.f, c.xpart + np, 2.f, c.xs + 4, s);
+  // packed tau+- are bounded by 2 max|t2| + 2 max|t1|^2 (scale xs[4] from x3_amp_scales)
   tau_pack_x3(o, v, KpP, KpM, t1, t2, c.xs + 4, (f16*)c.Tph, (f16*)c.Tpl, (f16*)c.Tmh, (f16*)c.Tml, s);
   X3Gemm g;
   g.M = noP; g.N = npP + ov; g.Kp = KpP; g.lda = KpP; g.ldb = KpP; g.ldc = npP + ov;
