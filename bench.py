@@ -7,7 +7,7 @@ one iteration of the CCSD solve loop (residual, Jacobi update, energy, DIIS), ex
 ``solve_cc`` runs it.
 
 Legs (each in its own context, run one after the other):
-  mixed  -- precision ``--precision`` (default MIXED_BF16X3): W warmup + K timed iterations,
+  mixed  -- precision ``--precision`` (default MIXED_F16X3): W warmup + K timed iterations,
             then iterated (untimed) to convergence;
   fp64   -- precision DP, the same; its converged energy is the FP64 reference at this size.
 ``value`` is the mixed leg's iterations/s; the FP64 leg, the speed-up and the energy / amplitude
@@ -167,7 +167,7 @@ def main():
     ap.add_argument("--no", type=int, default=48)
     ap.add_argument("--nv", type=int, default=384)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--precision", default="MIXED_BF16X3", choices=["MIXED", "MIXED_BF16X3", "MIXED_F16X3", "DP"])
+    ap.add_argument("--precision", default="MIXED_F16X3", choices=["MIXED", "MIXED_BF16X3", "MIXED_F16X3", "DP"])
     ap.add_argument("--maxiter", type=int, default=60)
     ap.add_argument("--e-conv", type=float, default=1e-9, help="energy threshold of the FP64 leg")
     ap.add_argument("--e-conv-mixed", type=float, default=1e-7,
@@ -277,7 +277,7 @@ def main():
     ring_prec = args.precision if args.precision in ("DP", "MIXED_F16X3") else "MIXED"
     achieved = fl_r / (ms_r * 1e-3) if ms_r > 0 else 0.0
     ring_kernel = {"DP": "plain_gemm_kernel (ring GEMMs, f64)", "MIXED": "plain_gemm_kernel (ring GEMMs, f32)",
-                   "MIXED_F16X3": "x3_gemm_kernel<_Float16, 256> (ring GEMMs, f16x3)"}[ring_prec]
+                   "MIXED_F16X3": "x3_gemm_dma_kernel<_Float16> (ring GEMMs, f16x3)"}[ring_prec]
     res["roofline"] = {"kernel": ring_kernel,
                        "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[ring_prec] / 1e12,
                        "unit": "TFLOP/s", "frac": achieved / PEAK[ring_prec], "traffic": traffic,

@@ -65,14 +65,13 @@ struct Ctx {
   // FP16x3 hole-hole operands: tau [(ij)][(ef)], tau^T [(ab)][(mn)], Wmnij^T [(ij)][(mn)]
   f16 *TAh = nullptr, *TAl = nullptr, *TTh = nullptr, *TTl = nullptr, *WTh = nullptr, *WTl = nullptr;
   // FP16x3 <ov|vv> terms: split t1 [o][v32], Ud as [i][(m,e)][v32]; PT[j][(m,x,y)] =
-  // sum_z t_jz Covvv[m][x][y][z], P2T[j][(m,b,e)] = sum_f t_jf (mf|be), Qr1[i][a] (:759)
+  // sum_z t_jz Covvv[m][x][y][z], P2T[j][(m,b,e)] = sum_f t_jf (mf|be)
   f16 *T1h = nullptr, *T1l = nullptr, *UTh = nullptr, *UTl = nullptr;
-  float *PT = nullptr, *P2T = nullptr, *Qr1 = nullptr;
+  float *PT = nullptr, *P2T = nullptr;
   // W0 = the integral part of [Wmbej | Wmbje] ((me|jb), -(mj|be)) in the W layout; split tau~ and L
-  // as [(i,j,a)][f padded] for :497 / :533; FP32 results of those two GEMMs
+  // as [(i,j,a)][f padded] for :497 / :533
   double* W0 = nullptr;
   f16 *TFh = nullptr, *TFl = nullptr, *LFh = nullptr, *LFl = nullptr;
-  float *Fae32 = nullptr, *Fmi32 = nullptr;
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
 

@@ -332,8 +332,8 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   axpby<double>(1.0, Foo, 0.0, Fmi, s);
   cd(0.5, T1("ie"), Fov, 1.0, Fmi);
   cd(1.0, T1("ne"), tv(c.Looov, "imne", OOOV), 1.0, Fmi);     // L_mnie = 2(mi|ne) - (ni|me)
-  if (x3) tauL_x3(c, c.Fae, c.Fmi, s);   // :497 and this term, tau~ L as FP16x3 GEMMs
-  else cd(1.0, tv(c.tau, "inef", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fmi);
+  cd(1.0, tv(c.tau, "inef", OOVV), tv(c.Loovv, "mnef", OOVV), 1.0, Fmi);
+  if (x3) fae_tauL_x3(c, c.Fae, s);   // :497 as an FP16x3 GEMM (tau~ still in c.tau)
 
   // ---- Wmnij: ccwfn.py:596-603 (tau = t2 + t1 t1) -------------------------------------------
   build_tau(o, v, 1.0, 1.0, t1, t2, c.tau, s);
@@ -348,12 +348,11 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
     // both orientations ([(ij)][(ef)] here, [(ab)][(mn)] for :930) with one scale
     x3_split_rows<double>(c.tau, o * o, v * v, v * v, c.xs + 5, c.TAh, c.TAl, c.v2p(), s);
     x3_split_t<double>(c.tau, o * o, v * v, 1, v * v, c.xs + 5, c.TTh, c.TTl, 0, c.o2p(), s);
-    X3Gemm g;   // W32[(mn)][(ij)] = sum_(ef) <mn|ef> tau_ijef
+    X3Gemm g;   // Wmnij[(mn)][(ij)] += sum_(ef) <mn|ef> tau_ijef
     g.M = o * o; g.N = o * o; g.Kp = c.v2p(); g.lda = c.v2p(); g.ldb = c.v2p(); g.ldc = o * o;
-    g.Ah = c.Ioh; g.Al = c.Iol; g.Bh = c.TAh; g.Bl = c.TAl; g.C = c.W32;
+    g.Ah = c.Ioh; g.Al = c.Iol; g.Bh = c.TAh; g.Bl = c.TAl; g.C64 = c.Wmnij; g.beta = 1.f;
     g.sa = c.xs + 6; g.sb = c.xs + 5;
-    x3_gemm<_Float16>(g, s, &ws);
-    accumulate(o * o * o * o, 1.0, c.W32, c.Wmnij, s);
+    x3_gemm<_Float16>(g, s, &ws);   // into Wmnij (FP64), through split K
   } else {
     // mixed modes: the two o^4 v^2 hole-hole ladder contractions (:603 and :930) run in FP32
     cast<double, float>(ov * ov, c.tau, c.tau32, s);

@@ -20,6 +20,9 @@ struct X3Gemm {
   // blocked K (kb > 0): k = q kb + r lives at offset q ska + r in a row of A (q skb + r in B); kb
   // is a multiple of 32, so a 32-wide k-tile never straddles two blocks
   int64_t kb = 0, ska = 0, skb = 0;
+  // FP64 destination (optional, needs ws): C64[m ldc + n] = alpha / (sa sb) sum + beta C64, the
+  // split-K partials (FP32, each over a short K range) summed in FP64
+  double* C64 = nullptr;
   // split-K (set by x3_gemm): grid.y = splits K ranges of kchunk, raw partials in work[s][M][N]
   int64_t kchunk = 0;
   float* work = nullptr;
@@ -63,8 +66,9 @@ void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, 
 struct Ctx;
 // max|t2| and max|t1| once per residual, and from them every amplitude-derived operand scale
 void x3_amp_scales(Ctx& c, const double* t1, const double* t2, hipStream_t s);
-// Fae -= tau~_mnaf L_mnef (:497) and Fmi += tau~_inef L_mnef (:533) as FP16x3 GEMMs (tau~ in c.tau)
-void tauL_x3(Ctx& c, double* Fae, double* Fmi, hipStream_t s);
+// Fae -= tau~_mnaf L_mnef (:497) as an FP16x3 GEMM (tau~ in c.tau). Its twin in Fmi (:533) stays
+// FP64: its ~1 Eh diagonal sums over o v^2 terms made the converged energy drift by ~1 uEh in FP16x3.
+void fae_tauL_x3(Ctx& c, double* Fae, hipStream_t s);
 void covvv_x3_setup(Ctx& c, hipStream_t s);
 void covvv_x3_products(Ctx& c, const double* t1, hipStream_t s);   // PT, P2T from t1
 void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s);              // + 2 t_mf (mf|ae) - t_mf (me|af)

@@ -348,21 +348,28 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   }
 }
 
-// C = alpha / (sa sb sbn[n]) sum_s work[s] + beta C, summed in a fixed order (deterministic).
+// C = alpha / (sa sb sbn[n]) sum_s work[s] + beta C, summed in FP64 in a fixed order
+// (deterministic); written to C64 (FP64) when given, else to C.
 __global__ __launch_bounds__(256) void x3_reduce_kernel(X3Gemm p, int splits) {
-  float f0 = p.alpha;
+  double f0 = p.alpha;
   if (p.sa) f0 = f0 / p.sa[0];
   if (p.sb) f0 = f0 / p.sb[0];
   const int64_t total = p.M * p.N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    float acc = 0.f;
-    for (int k = 0; k < splits; ++k) acc += p.work[k * total + e];
+    double acc = 0.0;
+    for (int k = 0; k < splits; ++k) acc += (double)p.work[k * total + e];
     const int64_t m = e / p.N, n = e - m * p.N;
-    const float f = p.sbn ? f0 / p.sbn[n] : f0;
-    float* d = p.C + m * p.ldc + n;
-    float val = f * acc;
-    if (p.beta != 0.f) val += p.beta * *d;
-    *d = val;
+    const double f = p.sbn ? f0 / p.sbn[n] : f0;
+    double val = f * acc;
+    if (p.C64) {
+      double* d = p.C64 + m * p.ldc + n;
+      if (p.beta != 0.f) val += (double)p.beta * *d;
+      *d = val;
+    } else {
+      float* d = p.C + m * p.ldc + n;
+      if (p.beta != 0.f) val += (double)p.beta * (double)*d;
+      *d = (float)val;
+    }
   }
 }
 
@@ -668,14 +675,14 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
   // split K into workspace partials when the tile grid cannot fill 256 CUs (keep >= 32 k-tiles each)
   int64_t splits = 1;
   p.work = nullptr;
-  if (ws && tiles < 512) {
-    splits = std::min<int64_t>({256, ceil_div(1024, tiles), p.Kp / (32 * BK)});
-    if (splits > 1) {
+  CCX_CHECK(!p.C64 || ws, ERR_INTERNAL, "an FP64 destination needs a workspace");
+  if (ws && (tiles < 512 || p.C64)) {
+    splits = tiles < 512 ? std::min<int64_t>({1024, ceil_div(1024, tiles), p.Kp / (32 * BK)}) : 1;
+    splits = std::max<int64_t>(splits, 1);
+    if (splits > 1 || p.C64) {
       p.kchunk = ceil_div(ceil_div(p.Kp, splits), BK) * BK;
       splits = ceil_div(p.Kp, p.kchunk);
       p.work = (float*)ws->get((size_t)splits * p.M * p.N * sizeof(float));
-    } else {
-      splits = 1;
     }
   }
   dim3 grid((unsigned)tiles, (unsigned)splits);
@@ -772,12 +779,11 @@ void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const dou
 // With t1 as the small operand they are three split GEMMs over two split copies of Covvv:
 //   PT[j][(m,x,y)]  = sum_z t_jz Covvv[m][x][y][z]        (copy CvA = Covvv rows, K = z)
 //   P2T[j][(m,b,e)] = sum_f t_jf (mf|be)                   (copy CvT, rows (m,b,e), K = f)
-//   Qr1[i][a]       = sum_(m,e,f) Ud[m][e][i][f] (me|af)   (CvA with blocked K = (m,e), f)
+//   r1[i][a]       += sum_(m,e,f) Ud[m][e][i][f] (me|af)   (CvA with blocked K = (m,e), f)
 // and their uses are permuted adds:
 //   Fae[a][e] += 2 sum_m P2T[m][(m,a,e)] - sum_m PT[m][(m,e,a)]               (:496)
 //   Wmbej[m][e][j][b] += PT[j][(m,e,b)],  Wmbje[m][e][j][b] -= P2T[j][(m,b,e)]  (:642, :681)
 //   Hh[i][j][a][b] += PT[i][(j,b,a)]                                            (:939)
-//   r1 += Qr1                                                                   (:759)
 namespace {
 
 // dst[(i, q)][f] (row stride ldd, q < nq) = split(src[q][i][f] * sc): Ud [(m,e)][(i,f)] -> [i][(m,e)][f]
@@ -849,11 +855,6 @@ __global__ __launch_bounds__(256) void fae_x3_kernel(const float* __restrict__ P
   }
 }
 
-__global__ __launch_bounds__(256) void add_f32_kernel(int64_t n, const float* __restrict__ x, double* __restrict__ y) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-    y[e] += (double)x[e];
-}
-
 inline unsigned grid_e(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 65536)); }
 
 }  // namespace
@@ -874,13 +875,10 @@ void covvv_x3_setup(Ctx& c, hipStream_t s) {
   halloc(&c.UTl, o * ov * v32);
   c.PT = c.alloc<float>(o * ov * v);
   c.P2T = c.alloc<float>(o * ov * v);
-  c.Qr1 = c.alloc<float>(ov);
   halloc(&c.TFh, o * ov * v32);
   halloc(&c.TFl, o * ov * v32);
   halloc(&c.LFh, o * ov * v32);
   halloc(&c.LFl, o * ov * v32);
-  c.Fae32 = c.alloc<float>(v * v);
-  c.Fmi32 = c.alloc<float>(o * o);
   // W0[(m,e)][x][(j,b)]: x = 0 (me|jb) (:641), x = 1 -(mj|be) (:680)
   c.W0 = c.alloc<double>(2 * ov * ov);
   const std::vector<int64_t> OVOV{o, v, o, v}, WSTR{v * 2 * ov, 2 * ov, v, 1};
@@ -940,14 +938,12 @@ void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s) {
   split_udt_kernel<<<grid_e(o * ov * v), 256, 0, s>>>(Ud, o, v, ov, c.xs + 10, (_Float16*)c.UTh, (_Float16*)c.UTl,
                                                       v32);
   CCX_HIP(hipGetLastError());
-  X3Gemm g;   // Qr1[i][a] = sum_(me, f) UT[i][(me)][f] CvA[(me)][a][f]
+  X3Gemm g;   // r1[i][a] += sum_(me, f) UT[i][(me)][f] CvA[(me)][a][f], partials summed in FP64
   g.M = o; g.N = v; g.Kp = ov * v32; g.lda = ov * v32; g.ldb = v32; g.ldc = v;
   g.kb = v32; g.ska = v32; g.skb = v * v32;
-  g.Ah = c.UTh; g.Al = c.UTl; g.Bh = c.CvAh; g.Bl = c.CvAl; g.C = c.Qr1;
-  g.sa = c.xs + 10; g.sb = c.xs + 8;
+  g.Ah = c.UTh; g.Al = c.UTl; g.Bh = c.CvAh; g.Bl = c.CvAl; g.C64 = r1;
+  g.sa = c.xs + 10; g.sb = c.xs + 8; g.beta = 1.f;
   x3_gemm<_Float16>(g, s, &c.ws);
-  add_f32_kernel<<<grid_e(ov), 256, 0, s>>>(ov, c.Qr1, r1);
-  CCX_HIP(hipGetLastError());
 }
 
 void x3_amp_scales(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
@@ -965,24 +961,17 @@ void x3_amp_scales(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
   x3_scale(p2, 1.f, p1, 0.5f, c.xs + 11, s);   // tau~ = t2 + 1/2 t1 t1
 }
 
-void tauL_x3(Ctx& c, double* Fae, double* Fmi, hipStream_t s) {
+void fae_tauL_x3(Ctx& c, double* Fae, hipStream_t s) {
   const int64_t o = c.o, v = c.v, ov = c.ov(), v32 = c.v32();
   x3_split_rows<double>(c.tau, o * ov, v, v, c.xs + 11, c.TFh, c.TFl, v32, s);   // tau~ [(m,n,a)][f]
-  X3Gemm g;   // Fae32[a][e] = -sum_(mn, f) tau~[(mn)][a][f] L[(mn)][e][f]
+  // the diagonal sums are of one sign (pair-energy-like): split K finely, sum the FP32 partials in
+  // FP64 straight into Fae
+  X3Gemm g;   // Fae[a][e] -= sum_(mn, f) tau~[(mn)][a][f] L[(mn)][e][f]
   g.M = v; g.N = v; g.Kp = o * o * v32; g.lda = v32; g.ldb = v32; g.ldc = v;
   g.kb = v32; g.ska = v * v32; g.skb = v * v32;
-  g.Ah = c.TFh; g.Al = c.TFl; g.Bh = c.LFh; g.Bl = c.LFl; g.C = c.Fae32;
-  g.sa = c.xs + 11; g.sb = c.xs + 12; g.alpha = -1.f;
+  g.Ah = c.TFh; g.Al = c.TFl; g.Bh = c.LFh; g.Bl = c.LFl; g.C64 = Fae;
+  g.sa = c.xs + 11; g.sb = c.xs + 12; g.alpha = -1.f; g.beta = 1.f;
   x3_gemm<_Float16>(g, s, &c.ws);
-  add_f32_kernel<<<grid_e(v * v), 256, 0, s>>>(v * v, c.Fae32, Fae);
-  CCX_HIP(hipGetLastError());
-  X3Gemm h;   // Fmi32[m][i] = sum_(n,e,f) L[m][(n,e)][f] tau~[i][(n,e)][f]
-  h.M = o; h.N = o; h.Kp = ov * v32; h.lda = ov * v32; h.ldb = ov * v32; h.ldc = o;
-  h.Ah = c.LFh; h.Al = c.LFl; h.Bh = c.TFh; h.Bl = c.TFl; h.C = c.Fmi32;
-  h.sa = c.xs + 12; h.sb = c.xs + 11;
-  x3_gemm<_Float16>(h, s, &c.ws);
-  add_f32_kernel<<<grid_e(o * o), 256, 0, s>>>(o * o, c.Fmi32, Fmi);
-  CCX_HIP(hipGetLastError());
 }
 
 // C = alpha op(A) op(B) + beta C for FP32 row-major matrices through FP16x3: op(A) (M x K) and
