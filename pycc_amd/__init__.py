@@ -8,6 +8,7 @@ Public API (names as exported by ``pycc/__init__.py``):
   backward is the Lambda-equation vector-Jacobian product; ``ccdensity`` -- the one- and
   two-particle densities (the integral half of the vJp);
 * ``DeviceManager``, ``ContractionBackend`` -- device/precision policy and the GPU einsum;
+* ``integrals`` -- FCIDUMP / ``.npz`` integral files and ``MOHamiltonian`` (the psi4-free input);
 * ``PyCCError``, ``PyCCWarning``, ``InvalidKeywordError``, ``DeviceError``.
 
 The arithmetic runs in ``libccx.so`` (HIP, gfx950), declared in ``include/ccx.h``.
@@ -19,6 +20,7 @@ from .cchbar import cchbar
 from .cclambda import cclambda
 from .ccdensity import ccdensity
 from . import synth
+from . import integrals
 
 __all__ = ["CCwfn", "ccwfn", "cchbar", "cclambda", "ccdensity", "DeviceManager", "ContractionBackend", "PyCCError", "PyCCWarning",
-           "InvalidKeywordError", "DeviceError", "synth"]
+           "InvalidKeywordError", "DeviceError", "synth", "integrals"]

@@ -90,6 +90,8 @@ class CCwfn(object):
 
         if no is None:
             no = getattr(H, "no", None)
+        if no is None and getattr(H, "nocc", None) is not None:
+            no = int(H.nocc) - int(nfzc)      # doubly occupied minus frozen core (wavefunction.py:305)
         if no is None:
             raise PyCCError("the number of active occupied orbitals `no` is required")
         self.nfzc = int(nfzc)
