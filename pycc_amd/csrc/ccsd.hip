@@ -106,7 +106,7 @@ static void alloc_integrals(Ctx& c) {
     }
   }
   c.red = c.alloc<double>(1024 * 64);
-  c.scal = c.alloc<double>(64);
+  c.scal = c.alloc<double>(128);   // [0, 64): per-slot overlaps, [64, 72): diis_push overlaps
 }
 
 static void pack_ladder(Ctx& c, int64_t e0, int64_t e1, VVVVSrc src, hipStream_t s) {
@@ -609,18 +609,21 @@ void diis_add(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
   while (used[slot]) ++slot;
   double* vv = c.dv + (int64_t)slot * n;
   double* ee = c.de + (int64_t)slot * n;
-  CCX_HIP(hipMemcpyAsync(vv, t1, n1 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  CCX_HIP(hipMemcpyAsync(vv + n1, t2, c.n2() * sizeof(double), hipMemcpyDeviceToDevice, s));
-  sub(n, vv, c.doldt, ee, s);                                   // e = t - t_old  (utils.py:289-291)
-  CCX_HIP(hipMemcpyAsync(c.doldt, vv, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   c.order.push_back(slot);
-  // new row of the overlap matrix: <e_k, e_new> for every stored error
-  for (int k : c.order) {
-    dots(n, 1, c.de + (int64_t)k * n, n, ee, c.red, c.scal + k, s);
-  }
+  // one pass: vv = t, ee = t - t_old (utils.py:287-291), t_old = t, and the first (up to 8) entries
+  // of the new overlap row <e_k, e_new> (the rest, for histories past 8, in further passes)
+  DiisSlots d{};
+  d.n = std::min<int>((int)c.order.size(), 8);
+  for (int k = 0; k < d.n; ++k) d.slot[k] = c.order[k];
+  diis_push(n1, c.n2(), t1, t2, c.doldt, vv, ee, c.de, d, c.red, c.scal + 64, s);
+  std::vector<double> first(d.n);
+  for (int k = (int)d.n; k < (int)c.order.size(); ++k)
+    dots(n, 1, c.de + (int64_t)c.order[k] * n, n, ee, c.red, c.scal + c.order[k], s);
+  CCX_HIP(hipMemcpyAsync(first.data(), c.scal + 64, d.n * sizeof(double), hipMemcpyDeviceToHost, s));
   std::vector<double> hrow(c.cap);
   CCX_HIP(hipMemcpyAsync(hrow.data(), c.scal, c.cap * sizeof(double), hipMemcpyDeviceToHost, s));
   CCX_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < d.n; ++k) hrow[c.order[k]] = first[k];
   for (int k : c.order) {
     c.bdots[k][slot] = hrow[k];
     c.bdots[slot][k] = hrow[k];
@@ -673,10 +676,9 @@ void diis_extrapolate(Ctx& c, double* t1, double* t2, double* coef_out, hipStrea
   if (coef_out)
     for (int i = 0; i < m; ++i) coef_out[i] = x[i];
   CCX_HIP(hipMemcpyAsync(c.dcoef, coef.data(), c.cap * sizeof(double), hipMemcpyHostToDevice, s));
-  lincomb(n1, c.cap, c.dcoef, c.dv, n, t1, s);                  // utils.py:351-355
-  lincomb(c.n2(), c.cap, c.dcoef, c.dv + n1, n, t2, s);
-  CCX_HIP(hipMemcpyAsync(c.doldt, t1, n1 * sizeof(double), hipMemcpyDeviceToDevice, s));
-  CCX_HIP(hipMemcpyAsync(c.doldt + n1, t2, c.n2() * sizeof(double), hipMemcpyDeviceToDevice, s));
+  // t = sum_k c_k t_k (utils.py:351-355), written to t_old as well (utils.py:358-359)
+  lincomb2(n1, c.cap, c.dcoef, c.dv, n, t1, c.doldt, s);
+  lincomb2(c.n2(), c.cap, c.dcoef, c.dv + n1, n, t2, c.doldt + n1, s);
   CCX_HIP(hipStreamSynchronize(s));   // `coef` (host) must outlive the copy
 }
 

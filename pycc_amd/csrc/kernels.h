@@ -80,6 +80,18 @@ void lincomb(int64_t n, int nvec, const double* c, const double* x0, int64_t str
 // out = x - y
 void sub(int64_t n, const double* x, const double* y, double* out, hipStream_t s);
 
+// DIIS push (one pass): v = [t1; t2], e = v - oldt, oldt = v, and out[k] = <e, E_k> for the nd stored
+// error vectors E_k = E + slots[k] * n (a slot equal to the new one uses e itself); nd <= 8.
+struct DiisSlots {
+  int n;
+  int slot[8];
+};
+void diis_push(int64_t n1, int64_t n2, const double* t1, const double* t2, double* oldt, double* v, double* e,
+               const double* E, DiisSlots d, double* partials, double* out, hipStream_t s);
+// lincomb into two outputs: out = out2 = sum_k c_k x_k (the extrapolated amplitudes and the new oldt)
+void lincomb2(int64_t n, int nvec, const double* c, const double* x0, int64_t stride, double* out, double* out2,
+              hipStream_t s);
+
 template <typename Tin, typename Tout>
 void cast(int64_t n, const Tin* in, Tout* out, hipStream_t s);
 

@@ -541,6 +541,63 @@ __global__ void sub_kernel(int64_t n, const double* __restrict__ x, const double
     out[e] = x[e] - y[e];
 }
 
+constexpr int DIIS_BLOCKS = 1024;
+
+__global__ __launch_bounds__(256) void diis_push_kernel(int64_t n1, int64_t n2, const double* __restrict__ t1,
+                                                        const double* __restrict__ t2, double* oldt,
+                                                        double* __restrict__ v, double* __restrict__ e,
+                                                        const double* __restrict__ E, int64_t stride, DiisSlots d,
+                                                        int self_slot, double* __restrict__ partials) {
+  double acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+  const int64_t n = n1 + n2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double t = i < n1 ? t1[i] : t2[i - n1];
+    const double err = t - oldt[i];
+    v[i] = t;
+    e[i] = err;
+    oldt[i] = t;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < d.n) acc[k] += err * (d.slot[k] == self_slot ? err : E[d.slot[k] * stride + i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k >= d.n) break;
+    const double r = block_sum(acc[k]);
+    if (threadIdx.x == 0) partials[k * gridDim.x + blockIdx.x] = r;
+  }
+}
+
+void diis_push(int64_t n1, int64_t n2, const double* t1, const double* t2, double* oldt, double* v, double* e,
+               const double* E, DiisSlots d, double* partials, double* out, hipStream_t s) {
+  CCX_CHECK(d.n >= 1 && d.n <= 8, ERR_INTERNAL, "diis_push takes 1..8 overlaps");
+  const int64_t stride = n1 + n2;
+  const int self_slot = (int)((e - E) / stride);
+  diis_push_kernel<<<DIIS_BLOCKS, 256, 0, s>>>(n1, n2, t1, t2, oldt, v, e, E, stride, d, self_slot, partials);
+  CCX_HIP(hipGetLastError());
+  dots_finish_kernel<<<d.n, 256, 0, s>>>(partials, DIIS_BLOCKS, out);
+  CCX_HIP(hipGetLastError());
+}
+
+__global__ void lincomb2_kernel(int64_t n, int nvec, const double* __restrict__ c, const double* __restrict__ x0,
+                                int64_t stride, double* __restrict__ out, double* __restrict__ out2) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int k = 0; k < nvec; ++k)
+      if (c[k] != 0.0) s += c[k] * x0[k * stride + e];   // unused slots carry c = 0
+    out[e] = s;
+    out2[e] = s;
+  }
+}
+
+void lincomb2(int64_t n, int nvec, const double* c, const double* x0, int64_t stride, double* out, double* out2,
+              hipStream_t s) {
+  lincomb2_kernel<<<grid_for(n), 256, 0, s>>>(n, nvec, c, x0, stride, out, out2);
+  CCX_HIP(hipGetLastError());
+}
+
 void sub(int64_t n, const double* x, const double* y, double* out, hipStream_t s) {
   sub_kernel<<<grid_for(n), 256, 0, s>>>(n, x, y, out);
   CCX_HIP(hipGetLastError());
