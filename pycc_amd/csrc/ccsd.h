@@ -72,6 +72,10 @@ struct Ctx {
   // as [(i,j,a)][f padded] for :497 / :533
   double* W0 = nullptr;
   f16 *TFh = nullptr, *TFl = nullptr, *LFh = nullptr, *LFl = nullptr;
+  // side stream for the packed ladder in the split-GEMM modes: it depends only on t1, t2, so it runs
+  // beside the rest of the residual (fork after the amplitude scales, join before zt_add)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* red = nullptr;             // reduction partials
   double* scal = nullptr;            // small device scalars
 

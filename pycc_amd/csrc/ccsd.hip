@@ -28,6 +28,11 @@ void Ctx::destroy() {
   for (void* p : owned) (void)hipFree(p);
   owned.clear();
   ws.release();
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
+  ev_fork = ev_join = nullptr;
+  side = nullptr;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -260,6 +265,10 @@ void alloc_work(Ctx& c) {
   c.W = c.mode == MODE_DP ? (void*)c.Wd : talloc(2 * ov * ov);
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
   if (c.split_ladder()) {
+    // default priority; ordered against the caller's stream by events only
+    CCX_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    CCX_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    CCX_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     const int64_t aP = noP * bf16_kpad(npP), aM = noM * bf16_kpad(npM);
     c.Tph = c.alloc<bf16>(aP);
     c.Tpl = c.alloc<bf16>(aP);
@@ -304,10 +313,18 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   };
 
   const bool x3 = c.mode == MODE_MIXED_F16X3;
-  if (x3) {
-    x3_amp_scales(c, t1, t2, s);          // every amplitude-derived FP16x3 operand scale
-    covvv_x3_products(c, t1, s);          // the <ov|vv> products of t1 (Fae, W, Hh terms below)
+  if (x3) x3_amp_scales(c, t1, t2, s);   // every amplitude-derived FP16x3 operand scale
+  // ---- ladder + Z: ccwfn.py:931 (1/2 tau_ijef <ab|ef>) and :715 (Z_mbij = <mb|ef> tau_ijef) ----
+  // In the split-GEMM modes it needs only t1, t2 (and the scales above): it runs on the side stream
+  // beside everything up to zt_add. Its kernels use no shared workspace.
+  if (c.split_ladder()) {
+    CCX_HIP(hipEventRecord(c.ev_fork, s));
+    CCX_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+    if (x3) ladder_x3(c, t1, t2, c.side);
+    else ladder_bf16x3(c, t1, t2, c.side);
+    CCX_HIP(hipEventRecord(c.ev_join, c.side));
   }
+  if (x3) covvv_x3_products(c, t1, s);   // the <ov|vv> products of t1 (Fae, W, Hh terms below)
 
   // ---- Fme: ccwfn.py:563-564 -------------------------------------------------------------
   const TV Fme = tv(c.Fme, "me", OV);
@@ -422,14 +439,10 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   }
   c.tend(s, kr);
 
-  // ---- ladder + Z: ccwfn.py:931 (1/2 tau_ijef <ab|ef>) and :715 (Z_mbij = <mb|ef> tau_ijef) ----
+  // ---- ladder + Z in the FP64 / FP32 modes (through the contraction planner and its workspace) ----
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
   const double f_lad_P = 2.0 * noP * (npP + ov) * npP, f_lad_M = 2.0 * noM * (npM + ov) * npM;
-  if (c.mode == MODE_MIXED_F16X3) {
-    ladder_x3(c, t1, t2, s);
-  } else if (c.mode == MODE_MIXED_BF16X3) {
-    ladder_bf16x3(c, t1, t2, s);
-  } else {
+  if (!c.split_ladder()) {
     tau_pack<T>(o, v, t1, t2, (T*)c.Tp, (T*)c.Tm, s);
     int k = c.tbegin(s, Ctx::PH_LADDER, f_lad_P);
     contract<T>(T(1), tv(c.Tp, "xk", {noP, npP}), tv(c.LP, "ky", {npP, npP + ov}), T(0),
@@ -478,6 +491,7 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   axpby<double>(1.0, tv(c.Cooov, "mijb", OOOV), 0.0, Zt, s);        // <mb|ij> = (mi|jb)
   cd(1.0, T1("ie"), tv(c.Covov, "mejb", OVOV), 1.0, Zt);            // <mb|ej> = (me|jb)
   cd(1.0, T1("je"), tv(c.Coovv, "mibe", OOVV), 1.0, Zt);            // <mb|ie> = (mi|be)
+  if (c.split_ladder()) CCX_HIP(hipStreamWaitEvent(s, c.ev_join, 0));   // CP, CM from the side stream
   if (c.mode == MODE_DP) zt_add<double>(o, v, (const double*)c.CP, (const double*)c.CM, c.Zt, s);
   else zt_add<float>(o, v, (const float*)c.CP, (const float*)c.CM, c.Zt, s);
 
