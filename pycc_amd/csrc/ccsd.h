@@ -40,7 +40,7 @@ struct Ctx {
   // (MODE_MIXED_F16X3)
   bf16 *LPh = nullptr, *LPl = nullptr, *LMh = nullptr, *LMl = nullptr;
   float *lsP = nullptr, *lsM = nullptr;
-  // FP16x3 ring operands (MODE_MIXED_F16X3): split K and J, [ov][ovp]
+  // FP16x3 ring operands (MODE_MIXED_F16X3): split K and J, [ov][ovp] interleaved (lo = hi + 32)
   f16 *Kh = nullptr, *Kl = nullptr, *Jh = nullptr, *Jl = nullptr;
   f16 *Ioh = nullptr, *Iol = nullptr; // split <mn|ef> [(mn)][(ef) padded] (hole-hole term :603)
   // split Covvv: CvA[(m,e,a)][f] = (me|af) and CvT[(m,b,e)][f] = (mf|be), f padded to v32
@@ -59,7 +59,7 @@ struct Ctx {
   bf16 *Tph = nullptr, *Tpl = nullptr, *Tmh = nullptr, *Tml = nullptr;   // f16 in MODE_MIXED_F16X3
   float *tau32 = nullptr, *W32 = nullptr;   // mixed modes: FP32 tau and Wmnij (hole-hole terms)
   // FP16x3 ring operands (see ring_prep_x3): BJ [2ov][ovp], BK [ov][ovp], AP [ov][2ovp],
-  // AQ [ov][ovp], BP = split W^T [ov][2ovp]
+  // AQ [ov][ovp], BP = split W^T [ov][2ovp], all interleaved (lo = hi + 32, rows twice as long)
   f16 *BJh = nullptr, *BJl = nullptr, *BKh = nullptr, *BKl = nullptr, *APh = nullptr, *APl = nullptr;
   f16 *AQh = nullptr, *AQl = nullptr, *BPh = nullptr, *BPl = nullptr;
   // FP16x3 hole-hole operands: tau [(ij)][(ef)], tau^T [(ab)][(mn)], Wmnij^T [(ij)][(mn)]

@@ -78,10 +78,12 @@ static void alloc_integrals(Ctx& c) {
   } else {
     if (c.mode == MODE_MIXED_F16X3) {
       const int64_t ovp = c.ovp();
-      for (f16** q : {&c.Kh, &c.Kl, &c.Jh, &c.Jl}) {
-        *q = c.alloc<f16>(ov * ovp);
-        CCX_HIP(hipMemset(*q, 0, ov * ovp * sizeof(f16)));   // K padding
+      for (f16** q : {&c.Kh, &c.Jh}) {   // interleaved split copies: lo = hi + 32
+        *q = c.alloc<f16>(2 * ov * ovp);
+        CCX_HIP(hipMemset(*q, 0, 2 * ov * ovp * sizeof(f16)));   // K padding
       }
+      c.Kl = c.Kh + 32;
+      c.Jl = c.Jh + 32;
       for (f16** q : {&c.Ioh, &c.Iol}) {
         *q = c.alloc<f16>(o * o * c.v2p());
         CCX_HIP(hipMemset(*q, 0, o * o * c.v2p() * sizeof(f16)));   // K padding
@@ -98,12 +100,22 @@ static void alloc_integrals(Ctx& c) {
       c.LM = c.alloc<float>(szM);
     } else {
       const int64_t bP = (c.npP() + ov) * bf16_kpad(c.npP()), bM = (c.npM() + ov) * bf16_kpad(c.npM());
-      c.LPh = c.alloc<bf16>(bP);
-      c.LPl = c.alloc<bf16>(bP);
-      c.LMh = c.alloc<bf16>(bM);
-      c.LMl = c.alloc<bf16>(bM);
-      for (bf16* q : {c.LPh, c.LPl}) CCX_HIP(hipMemset(q, 0, bP * sizeof(bf16)));   // K padding
-      for (bf16* q : {c.LMh, c.LMl}) CCX_HIP(hipMemset(q, 0, bM * sizeof(bf16)));
+      if (c.mode == MODE_MIXED_F16X3) {
+        // interleaved split rows (hi and lo of each 32-wide k-block side by side, lo = hi + 32)
+        c.LPh = c.alloc<bf16>(2 * bP);
+        c.LMh = c.alloc<bf16>(2 * bM);
+        CCX_HIP(hipMemset(c.LPh, 0, 2 * bP * sizeof(bf16)));   // K padding
+        if (bM) CCX_HIP(hipMemset(c.LMh, 0, 2 * bM * sizeof(bf16)));
+        c.LPl = c.LPh + 32;
+        c.LMl = bM ? c.LMh + 32 : nullptr;
+      } else {
+        c.LPh = c.alloc<bf16>(bP);
+        c.LPl = c.alloc<bf16>(bP);
+        c.LMh = c.alloc<bf16>(bM);
+        c.LMl = c.alloc<bf16>(bM);
+        for (bf16* q : {c.LPh, c.LPl}) CCX_HIP(hipMemset(q, 0, bP * sizeof(bf16)));   // K padding
+        for (bf16* q : {c.LMh, c.LMl}) CCX_HIP(hipMemset(q, 0, bM * sizeof(bf16)));
+      }
       if (c.mode == MODE_MIXED_F16X3) {
         c.lsP = c.alloc<float>(c.npP() + ov);
         c.lsM = c.alloc<float>(c.npM() + ov);
@@ -145,10 +157,10 @@ static void finish_integrals(Ctx& c, const double* fock, hipStream_t s) {
     const int64_t ovp = c.ovp();
     x3_absmax<double>(c.Covov, ov * ov, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 2, s);
-    x3_split_rows<double>(c.Covov, ov, ov, ov, c.xs + 2, c.Kh, c.Kl, ovp, s);
+    x3_split_rows<double>(c.Covov, ov, ov, ov, c.xs + 2, c.Kh, c.Kl, ovp, s, true);
     x3_absmax<double>(c.J, ov * ov, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 1, s);
-    x3_split_rows<double>(c.J, ov, ov, ov, c.xs + 1, c.Jh, c.Jl, ovp, s);
+    x3_split_rows<double>(c.J, ov, ov, ov, c.xs + 1, c.Jh, c.Jl, ovp, s, true);
     // <mn|ef> as [(mn)][(ef)] for the hole-hole term :603
     x3_absmax<double>(c.Ioovv, ov * ov, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 6, s);
@@ -240,16 +252,16 @@ void alloc_work(Ctx& c) {
       *q = c.alloc<f16>(n);
       CCX_HIP(hipMemset(*q, 0, n * sizeof(f16)));   // K padding stays zero
     };
-    halloc(&c.BJh, 2 * ov * ovp);
-    halloc(&c.BJl, 2 * ov * ovp);
-    halloc(&c.BKh, ov * ovp);
-    halloc(&c.BKl, ov * ovp);
-    halloc(&c.APh, 2 * ov * ovp);
-    halloc(&c.APl, 2 * ov * ovp);
-    halloc(&c.AQh, ov * ovp);
-    halloc(&c.AQl, ov * ovp);
-    halloc(&c.BPh, 2 * ov * ovp);
-    halloc(&c.BPl, 2 * ov * ovp);
+    // the ring operands are interleaved (hi and lo of each 32-wide k-block side by side, lo = hi + 32)
+    auto ialloc = [&](f16** h, f16** l, int64_t n) {
+      halloc(h, 2 * n);
+      *l = *h + 32;
+    };
+    ialloc(&c.BJh, &c.BJl, 2 * ov * ovp);
+    ialloc(&c.BKh, &c.BKl, ov * ovp);
+    ialloc(&c.APh, &c.APl, 2 * ov * ovp);
+    ialloc(&c.AQh, &c.AQl, ov * ovp);
+    ialloc(&c.BPh, &c.BPl, 2 * ov * ovp);
     halloc(&c.TAh, o * o * c.v2p());
     halloc(&c.TAl, o * o * c.v2p());
     halloc(&c.TTh, v * v * c.o2p());
@@ -270,12 +282,21 @@ void alloc_work(Ctx& c) {
     CCX_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
     CCX_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     const int64_t aP = noP * bf16_kpad(npP), aM = noM * bf16_kpad(npM);
-    c.Tph = c.alloc<bf16>(aP);
-    c.Tpl = c.alloc<bf16>(aP);
-    c.Tmh = c.alloc<bf16>(aM);
-    c.Tml = c.alloc<bf16>(aM);
-    for (bf16* q : {c.Tph, c.Tpl}) CCX_HIP(hipMemset(q, 0, aP * sizeof(bf16)));   // K padding
-    for (bf16* q : {c.Tmh, c.Tml}) CCX_HIP(hipMemset(q, 0, aM * sizeof(bf16)));
+    if (c.mode == MODE_MIXED_F16X3) {   // interleaved, lo = hi + 32
+      c.Tph = c.alloc<bf16>(2 * aP);
+      c.Tmh = c.alloc<bf16>(2 * aM);
+      CCX_HIP(hipMemset(c.Tph, 0, 2 * aP * sizeof(bf16)));   // K padding
+      if (aM) CCX_HIP(hipMemset(c.Tmh, 0, 2 * aM * sizeof(bf16)));
+      c.Tpl = c.Tph + 32;
+      c.Tml = aM ? c.Tmh + 32 : nullptr;
+    } else {
+      c.Tph = c.alloc<bf16>(aP);
+      c.Tpl = c.alloc<bf16>(aP);
+      c.Tmh = c.alloc<bf16>(aM);
+      c.Tml = c.alloc<bf16>(aM);
+      for (bf16* q : {c.Tph, c.Tpl}) CCX_HIP(hipMemset(q, 0, aP * sizeof(bf16)));   // K padding
+      for (bf16* q : {c.Tmh, c.Tml}) CCX_HIP(hipMemset(q, 0, aM * sizeof(bf16)));
+    }
   } else {
     c.Tp = talloc(noP * npP);
     c.Tm = talloc(noM * npM);
@@ -414,8 +435,9 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   const int64_t ovp = c.ovp();
   int kr = c.tbegin(s, Ctx::PH_RING, 2 * unit);
   if (x3) {
-    X3Gemm g;
-    g.M = ov; g.N = 2 * ov; g.Kp = ovp; g.lda = ovp; g.ldb = ovp; g.ldc = 2 * ov;
+    X3Gemm g;   // interleaved operands: rows of 2 ovp, kb = 32, ska = skb = 64
+    g.M = ov; g.N = 2 * ov; g.Kp = ovp; g.lda = 2 * ovp; g.ldb = 2 * ovp; g.ldc = 2 * ov;
+    g.kb = 32; g.ska = 64; g.skb = 64;
     g.Ah = c.Jh; g.Al = c.Jl; g.Bh = c.BJh; g.Bl = c.BJl; g.C = (float*)W;
     g.sa = c.xs + 1; g.sb = c.xs; g.beta = 1.f;
     x3_gemm<_Float16>(g, s);
@@ -426,14 +448,15 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   kr = c.tbegin(s, Ctx::PH_RING, unit);
   if (x3) {
     X3Gemm g;
-    g.M = ov; g.N = ov; g.Kp = ovp; g.lda = ovp; g.ldb = ovp; g.ldc = 2 * ov;
+    g.M = ov; g.N = ov; g.Kp = ovp; g.lda = 2 * ovp; g.ldb = 2 * ovp; g.ldc = 2 * ov;
+    g.kb = 32; g.ska = 64; g.skb = 64;
     g.Ah = c.Kh; g.Al = c.Kl; g.Bh = c.BKh; g.Bl = c.BKl; g.C = (float*)W;
     g.sa = c.xs + 2; g.sb = c.xs; g.beta = 1.f;
     x3_gemm<_Float16>(g, s);
     // W (FP32 [me][x][jb]) -> split BP[jb][x][me] with its own scale xs[3]
     x3_absmax<float>((const float*)W, 2 * ov * ov, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 3, s);
-    x3_split_t((const float*)W, ov, ov, 2, 2 * ov, c.xs + 3, c.BPh, c.BPl, ovp, 2 * ovp, s);
+    x3_split_t((const float*)W, ov, ov, 2, 2 * ov, c.xs + 3, c.BPh, c.BPl, ovp, 2 * ovp, s, true);
   } else {
     contract<T>(T(1), tv(Kt, "menf", OVOV), tv(A1, "nfjb", OVOV), T(1), tvs(W, "mejb", OVOV, WSTR), ws, s);
   }
@@ -464,7 +487,8 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   kr = c.tbegin(s, Ctx::PH_RING, 2 * unit);
   if (x3) {
     X3Gemm g;
-    g.M = ov; g.N = ov; g.Kp = 2 * ovp; g.lda = 2 * ovp; g.ldb = 2 * ovp; g.ldc = ov;
+    g.M = ov; g.N = ov; g.Kp = 2 * ovp; g.lda = 4 * ovp; g.ldb = 4 * ovp; g.ldc = ov;
+    g.kb = 32; g.ska = 64; g.skb = 64;
     g.Ah = c.APh; g.Al = c.APl; g.Bh = c.BPh; g.Bl = c.BPl; g.C = (float*)P;
     g.sa = c.xs; g.sb = c.xs + 3;
     x3_gemm<_Float16>(g, s);
@@ -474,9 +498,10 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   c.tend(s, kr);
   kr = c.tbegin(s, Ctx::PH_RING, unit);
   if (x3) {
-    X3Gemm g;
-    g.M = ov; g.N = ov; g.Kp = ovp; g.lda = ovp; g.ldb = 2 * ovp; g.ldc = ov;
-    g.Ah = c.AQh; g.Al = c.AQl; g.Bh = c.BPh + ovp; g.Bl = c.BPl + ovp; g.C = (float*)Q;
+    X3Gemm g;   // B = the x = 1 half of BP rows: k offset ovp, at 2 ovp in the interleaved row
+    g.M = ov; g.N = ov; g.Kp = ovp; g.lda = 2 * ovp; g.ldb = 4 * ovp; g.ldc = ov;
+    g.kb = 32; g.ska = 64; g.skb = 64;
+    g.Ah = c.AQh; g.Al = c.AQl; g.Bh = c.BPh + 2 * ovp; g.Bl = c.BPl + 2 * ovp; g.C = (float*)Q;
     g.sa = c.xs; g.sb = c.xs + 3;
     x3_gemm<_Float16>(g, s);
   } else {

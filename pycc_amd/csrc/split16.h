@@ -18,7 +18,8 @@ struct X3Gemm {
   const float* sbn = nullptr;        // per-row scales of B (one per n), in place of sb
   float alpha = 1.f, beta = 0.f;
   // blocked K (kb > 0): k = q kb + r lives at offset q ska + r in a row of A (q skb + r in B); kb
-  // is a multiple of 32, so a 32-wide k-tile never straddles two blocks
+  // is a multiple of 32, so a 32-wide k-tile never straddles two blocks. Interleaved operands
+  // (hi and lo of each 32-wide k-block side by side, lo = hi + 32) are kb = 32, ska / skb = 64.
   int64_t kb = 0, ska = 0, skb = 0;
   // FP64 destination (optional, needs ws): C64[m ldc + n] = alpha / (sa sb) sum + beta C64, the
   // split-K partials (FP32, each over a short K range) summed in FP64
@@ -39,21 +40,23 @@ int x3_partials();
 // *s_out = 2^k such that (ca max(pa) + cb max(pb)^2) * 2^k < 2^15 (pb may be null).
 void x3_scale(const float* pa, float ca, const float* pb, float cb, float* s_out, hipStream_t s);
 
-// hi/lo[r * ldd + c] = split(src[r * lds + c] * sc), r < rows, c < cols.
+// hi/lo[r * ldd + c] = split(src[r * lds + c] * sc), r < rows, c < cols. il: interleaved layout
+// (lo = hi + 32, ldd a multiple of 32; k-block q of a row at 64 q, the row at 2 ldd r).
 template <typename Tin>
 void x3_split_rows(const Tin* src, int64_t rows, int64_t cols, int64_t lds, const float* sc, f16* hi, f16* lo,
-                   int64_t ldd, hipStream_t s);
+                   int64_t ldd, hipStream_t s, bool il = false);
 // hi/lo[cc * ldd + x * Kb + r] = split(src[r * lds + x * Cb + cc] * sc), r < R, x < nx, cc < Cb.
 template <typename Tin>
 void x3_split_t(const Tin* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
-                int64_t Kb, int64_t ldd, hipStream_t s);
-// FP16x3 packed ladder matrices (the BF16x3 layouts of kernels.h, [N][Kp]) with one power-of-two
-// scale per row in sP / sM: the (ab) rows for e in [e0, e1) from src, and the (m, b) rows from Covvv.
+                int64_t Kb, int64_t ldd, hipStream_t s, bool il = false);
+// FP16x3 packed ladder matrices (the BF16x3 layouts of kernels.h, [N][Kp], interleaved: lo = hi + 32)
+// with one power-of-two scale per row in sP / sM: the (ab) rows for e in [e0, e1) from src, and the
+// (m, b) rows from Covvv.
 void ladder_pack_x3(int64_t v, int64_t e0, int64_t e1, VVVVSrc src, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
                     float* sP, float* sM, hipStream_t s);
 void ladder_pack_o_x3(int64_t o, int64_t v, const double* Covvv, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
                       float* sP, float* sM, hipStream_t s);
-// FP16x3 packed tau rows (tau_pack_bf16x3 layout) scaled by *sc.
+// FP16x3 packed tau rows (tau_pack_bf16x3 layout, interleaved: lo = hi + 32) scaled by *sc.
 void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2, const float* sc,
                  f16* Tph, f16* Tpl, f16* Tmh, f16* Tml, hipStream_t s);
 // C = alpha op(A) op(B) + beta C (FP32 row-major, op = transpose when ta / tb) through FP16x3 with
@@ -75,7 +78,8 @@ void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s);              // + 2 t_mf 
 void covvv_x3_w(Ctx& c, double* Wd, hipStream_t s);                 // :642, :681 into Wd
 void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s);                // :939
 void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s);   // :759
-// The split ring operands BJ, BK, AP, AQ (row stride ovp, or 2 ovp for AP) and Ud from t1, t2.
+// The split ring operands BJ, BK, AP, AQ (K = ovp, or 2 ovp for AP; interleaved, lo = hi + 32) and Ud
+// from t1, t2.
 void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,
                   f16* BJl, f16* BKh, f16* BKl, f16* APh, f16* APl, f16* AQh, f16* AQl, double* Ud,
                   hipStream_t s);

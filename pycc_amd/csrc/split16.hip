@@ -46,6 +46,29 @@ __device__ __forceinline__ int64_t xcd_remap_x3(int64_t w, int64_t ntiles) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// Workgroup id -> output tile of the 256 x 256 kernel. Groups of 8 M-tiles are swept in chunks of
+// 32 N-tiles. In a full chunk (256 consecutive workgroups, dispatched round-robin to the 8 XCDs)
+// workgroup i takes M-tile (i / 8) % 8 and N-tile 4 (i % 8) + i / 64: each XCD works on an 8 x 4
+// block of tiles (A and B panels reused in its L2) and all XCDs read the same 8 A panels at the
+// same time (shared through the memory-side cache). Partial chunks run M-fastest.
+__device__ __forceinline__ void chunk_order(int64_t t, int64_t tiles_m, int64_t tiles_n, int64_t& tm, int64_t& tn) {
+  const int64_t g = t / (GM * tiles_n), first_m = g * GM;
+  const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
+  const int64_t r = t - g * GM * tiles_n;
+  const int64_t c = r / (gsize * 32), i = r - c * gsize * 32;
+  const int64_t nc = tiles_n - 32 * c < 32 ? tiles_n - 32 * c : 32;
+  int64_t ml, nl;
+  if (gsize == GM && nc == 32) {
+    ml = (i >> 3) & 7;
+    nl = 4 * (i & 7) + (i >> 6);
+  } else {
+    ml = i % gsize;
+    nl = i / gsize;
+  }
+  tm = first_m + ml;
+  tn = 32 * c + nl;
+}
+
 // BT x BT output tiles, 4 waves in 2 x 2, each computing (BT/2) x (BT/2) as 16 x 16 MFMA blocks.
 // Operand tiles (BT rows x 32 k, hi and lo) are double-buffered in LDS with one register stage:
 // the next k-tile's global loads are issued before the current tile's MFMAs.
@@ -228,11 +251,8 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 2, wn = wave & 3;
   const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BT - 1) / BT;
-  const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
-  const int64_t group = t / (GM * tiles_n), first_m = group * GM;
-  const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
-  const int64_t within = t - group * GM * tiles_n;
-  const int64_t tm = first_m + within % gsize, tn = within / gsize;
+  int64_t tm, tn;
+  chunk_order(blockIdx.x, tiles_m, tiles_n, tm, tn);
 
   int64_t gA[IPW], gB[IPW];
   int ldso[IPW];
@@ -249,7 +269,10 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   }
   auto issue = [&](int st, int64_t k0) {
     int64_t ka = k0, kb = k0;
-    if (p.kb) {
+    if (p.kb == BK) {   // interleaved operands and other 32-blocked layouts
+      ka = (k0 >> 5) * p.ska;
+      kb = (k0 >> 5) * p.skb;
+    } else if (p.kb) {
       const int64_t q = k0 / p.kb, r = k0 - q * p.kb;
       ka = q * p.ska + r;
       kb = q * p.skb + r;
@@ -437,15 +460,22 @@ __device__ __forceinline__ void split_store(float x, E* hi, E* lo, int64_t at) {
   lo[at] = (E)(x - (float)h);
 }
 
+// Interleaved ("IL") split operands: a row of K (padded to a multiple of 32) is stored as 2 K
+// elements, k-block q of 32 as hi at 64 q and lo at 64 q + 32, so that one 128-byte line carries
+// both halves of a row's k-tile. With hi = base and lo = base + 32, element `at` = row * ld + k
+// (ld a multiple of 32) of the plain layout lives at il_at(at) in both.
+__device__ __forceinline__ int64_t il_at(int64_t at) { return ((at >> 5) << 6) | (at & 31); }
+
 template <typename Tin, typename E>
 __global__ __launch_bounds__(256) void split_rows_kernel(const Tin* __restrict__ src, int64_t rows, int64_t cols,
                                                          int64_t lds, const float* __restrict__ sc, E* hi, E* lo,
-                                                         int64_t ldd) {
+                                                         int64_t ldd, bool il) {
   const double s = sc[0];
   const int64_t total = rows * cols;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = e / cols, c = e - r * cols;
-    split_store<E>((float)((double)src[r * lds + c] * s), hi, lo, r * ldd + c);
+    const int64_t at = r * ldd + c;
+    split_store<E>((float)((double)src[r * lds + c] * s), hi, lo, il ? il_at(at) : at);
   }
 }
 
@@ -454,7 +484,7 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const Tin* __restrict__
 template <typename Tin, typename E>
 __global__ __launch_bounds__(256) void split_t_kernel(const Tin* __restrict__ src, int64_t R, int64_t Cb, int nx,
                                                       int64_t lds, const float* __restrict__ sc, E* hi, E* lo,
-                                                      int64_t Kb, int64_t ldd) {
+                                                      int64_t Kb, int64_t ldd, bool il) {
   __shared__ float tile[32][33];
   const double s = sc[0];
   const int64_t tr = (R + 31) / 32, tc = (Cb + 31) / 32;
@@ -470,7 +500,8 @@ __global__ __launch_bounds__(256) void split_t_kernel(const Tin* __restrict__ sr
     __syncthreads();
     for (int y = ty; y < 32; y += 8) {
       const int64_t cc = c0 + y, r = r0 + tx;
-      if (cc < Cb && r < R) split_store<E>(tile[tx][y], hi, lo, cc * ldd + x * Kb + r);
+      const int64_t at = cc * ldd + x * Kb + r;
+      if (cc < Cb && r < R) split_store<E>(tile[tx][y], hi, lo, il ? il_at(at) : at);
     }
     __syncthreads();
   }
@@ -484,6 +515,7 @@ __global__ __launch_bounds__(256) void split_t_kernel(const Tin* __restrict__ sr
 //   AP[(ia)][x][(jb)]  x = 0: 2T - X, x = 1: T          (A of the ring GEMM, K = (x, me))
 //   AQ[(ia)][(jb)]     X                                (A of the crossed ring GEMM)
 //   Ud[(ia)][(jb)]     2T - X in FP64
+// The split operands are written interleaved (il_at; the lo pointers are hi + 32).
 template <typename E>
 __global__ __launch_bounds__(256) void ring_prep_x3_kernel(int64_t o, int64_t v, int64_t ovp,
                                                            const double* __restrict__ t1,
@@ -514,9 +546,9 @@ __global__ __launch_bounds__(256) void ring_prep_x3_kernel(int64_t o, int64_t v,
     if (a >= v || b >= v) continue;
     const double T = tn[y][x], X = tx[x][y];
     const int64_t ia = i * v + a, jb = j * v + b;
-    split_store<E>((float)((2.0 * T - X) * s), APh, APl, ia * 2 * ovp + jb);
-    split_store<E>((float)(T * s), APh, APl, ia * 2 * ovp + ovp + jb);
-    split_store<E>((float)(X * s), AQh, AQl, ia * ovp + jb);
+    split_store<E>((float)((2.0 * T - X) * s), APh, APl, il_at(ia * 2 * ovp + jb));
+    split_store<E>((float)(T * s), APh, APl, il_at(ia * 2 * ovp + ovp + jb));
+    split_store<E>((float)(X * s), AQh, AQl, il_at(ia * ovp + jb));
     Ud[ia * ov + jb] = 2.0 * T - X;
   }
   // a fastest: BJ, BK
@@ -526,9 +558,9 @@ __global__ __launch_bounds__(256) void ring_prep_x3_kernel(int64_t o, int64_t v,
     const double T = tn[x][y], X = tx[y][x];
     const double P = t1[j * v + a] * t1[i * v + b];
     const int64_t ia = i * v + a, jb = j * v + b;
-    split_store<E>((float)(-0.5 * T * s), BJh, BJl, jb * ovp + ia);
-    split_store<E>((float)((0.5 * X + P) * s), BJh, BJl, (ov + jb) * ovp + ia);
-    split_store<E>((float)((T - 0.5 * X - P) * s), BKh, BKl, jb * ovp + ia);
+    split_store<E>((float)(-0.5 * T * s), BJh, BJl, il_at(jb * ovp + ia));
+    split_store<E>((float)((0.5 * X + P) * s), BJh, BJl, il_at((ov + jb) * ovp + ia));
+    split_store<E>((float)((T - 0.5 * X - P) * s), BKh, BKl, il_at(jb * ovp + ia));
   }
 }
 
@@ -582,8 +614,8 @@ __global__ __launch_bounds__(256) void ladder_pack_x3_kernel(int64_t v, int64_t 
     bool mcol;
     int64_t cm;
     vals(c, x, y, mcol, cm);
-    split_store<E>((float)(0.5 * (x + y) * scP), LPh, LPl, rowP * KpP + c);
-    if (hasM && mcol) split_store<E>((float)(0.5 * (x - y) * scM), LMh, LMl, rowM * KpM + cm);
+    split_store<E>((float)(0.5 * (x + y) * scP), LPh, LPl, il_at(rowP * KpP + c));
+    if (hasM && mcol) split_store<E>((float)(0.5 * (x - y) * scM), LMh, LMl, il_at(rowM * KpM + cm));
   }
 }
 
@@ -623,8 +655,8 @@ __global__ __launch_bounds__(256) void ladder_pack_o_x3_kernel(int64_t o, int64_
     int64_t q;
     bool lt;
     vals(k, x, y, q, lt);
-    split_store<E>((float)(0.5 * (x + y) * scP), LPh, LPl, (npP + mb) * KpP + k);
-    if (lt) split_store<E>((float)(0.5 * (x - y) * scM), LMh, LMl, (npM + mb) * KpM + q);
+    split_store<E>((float)(0.5 * (x + y) * scP), LPh, LPl, il_at((npP + mb) * KpP + k));
+    if (lt) split_store<E>((float)(0.5 * (x - y) * scM), LMh, LMl, il_at((npM + mb) * KpM + q));
   }
 }
 
@@ -648,8 +680,8 @@ __global__ __launch_bounds__(256) void tau_pack_x3_kernel(int64_t o, int64_t v, 
     decode_pair(c, e, f);
     const double tef = blk[e * v + f] + t1[i * v + e] * t1[j * v + f];
     const double tfe = blk[f * v + e] + t1[i * v + f] * t1[j * v + e];
-    split_store<E>((float)((e < f ? tef + tfe : tef) * s), Tph, Tpl, rowP * KpP + c);
-    if (hasM && e < f) split_store<E>((float)((tef - tfe) * s), Tmh, Tml, rowM * KpM + pairM(e, f));
+    split_store<E>((float)((e < f ? tef + tfe : tef) * s), Tph, Tpl, il_at(rowP * KpP + c));
+    if (hasM && e < f) split_store<E>((float)((tef - tfe) * s), Tmh, Tml, il_at(rowM * KpM + pairM(e, f)));
   }
 }
 
@@ -713,28 +745,30 @@ int x3_partials() { return X3_RED_BLOCKS; }
 
 template <typename Tin>
 void x3_split_rows(const Tin* src, int64_t rows, int64_t cols, int64_t lds, const float* sc, f16* hi, f16* lo,
-                   int64_t ldd, hipStream_t s) {
+                   int64_t ldd, hipStream_t s, bool il) {
+  CCX_CHECK(!il || (ldd % 32 == 0 && lo == hi + 32), ERR_INTERNAL, "interleaved split layout");
   split_rows_kernel<Tin, _Float16><<<grid_x3(rows * cols), 256, 0, s>>>(src, rows, cols, lds, sc, (_Float16*)hi,
-                                                                        (_Float16*)lo, ldd);
+                                                                        (_Float16*)lo, ldd, il);
   CCX_HIP(hipGetLastError());
 }
 template void x3_split_rows<double>(const double*, int64_t, int64_t, int64_t, const float*, f16*, f16*, int64_t,
-                                    hipStream_t);
+                                    hipStream_t, bool);
 template void x3_split_rows<float>(const float*, int64_t, int64_t, int64_t, const float*, f16*, f16*, int64_t,
-                                   hipStream_t);
+                                   hipStream_t, bool);
 
 template <typename Tin>
 void x3_split_t(const Tin* src, int64_t R, int64_t Cb, int nx, int64_t lds, const float* sc, f16* hi, f16* lo,
-                int64_t Kb, int64_t ldd, hipStream_t s) {
+                int64_t Kb, int64_t ldd, hipStream_t s, bool il) {
+  CCX_CHECK(!il || (ldd % 32 == 0 && Kb % 32 == 0 && lo == hi + 32), ERR_INTERNAL, "interleaved split layout");
   const int64_t work = ceil_div(R, 32) * ceil_div(Cb, 32) * nx;
   split_t_kernel<Tin, _Float16><<<(unsigned)std::min<int64_t>(work, 1 << 20), 256, 0, s>>>(
-      src, R, Cb, nx, lds, sc, (_Float16*)hi, (_Float16*)lo, Kb, ldd);
+      src, R, Cb, nx, lds, sc, (_Float16*)hi, (_Float16*)lo, Kb, ldd, il);
   CCX_HIP(hipGetLastError());
 }
 template void x3_split_t<float>(const float*, int64_t, int64_t, int, int64_t, const float*, f16*, f16*, int64_t,
-                                int64_t, hipStream_t);
+                                int64_t, hipStream_t, bool);
 template void x3_split_t<double>(const double*, int64_t, int64_t, int, int64_t, const float*, f16*, f16*, int64_t,
-                                 int64_t, hipStream_t);
+                                 int64_t, hipStream_t, bool);
 
 void ladder_pack_x3(int64_t v, int64_t e0, int64_t e1, VVVVSrc src, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
                     float* sP, float* sM, hipStream_t s) {
@@ -766,6 +800,8 @@ void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t
 void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,
                   f16* BJl, f16* BKh, f16* BKl, f16* APh, f16* APl, f16* AQh, f16* AQl, double* Ud,
                   hipStream_t s) {
+  CCX_CHECK(ovp % 32 == 0 && BJl == BJh + 32 && BKl == BKh + 32 && APl == APh + 32 && AQl == AQh + 32,
+            ERR_INTERNAL, "ring operands must be interleaved");
   const int64_t tiles = ceil_div(v, 32);
   dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * o));
   ring_prep_x3_kernel<_Float16><<<grid, 256, 0, s>>>(o, v, ovp, t1, t2, sc, (_Float16*)BJh, (_Float16*)BJl,
@@ -1020,7 +1056,9 @@ void ladder_x3(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
   // packed tau+- are bounded by 2 max|t2| + 2 max|t1|^2 (scale xs[4] from x3_amp_scales)
   tau_pack_x3(o, v, KpP, KpM, t1, t2, c.xs + 4, (f16*)c.Tph, (f16*)c.Tpl, (f16*)c.Tmh, (f16*)c.Tml, s);
   X3Gemm g;
-  g.M = noP; g.N = npP + ov; g.Kp = KpP; g.lda = KpP; g.ldb = KpP; g.ldc = npP + ov;
+  // all operands interleaved: rows of 2 Kp, kb = 32, ska = skb = 64
+  g.M = noP; g.N = npP + ov; g.Kp = KpP; g.lda = 2 * KpP; g.ldb = 2 * KpP; g.ldc = npP + ov;
+  g.kb = 32; g.ska = 64; g.skb = 64;
   g.Ah = c.Tph; g.Al = c.Tpl; g.Bh = c.LPh; g.Bl = c.LPl; g.C = (float*)c.CP;
   g.sa = c.xs + 4; g.sbn = c.lsP;
   int k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noP * (npP + ov) * npP);
@@ -1028,7 +1066,7 @@ void ladder_x3(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
   c.tend(s, k);
   if (noM > 0) {
     X3Gemm h = g;
-    h.M = noM; h.N = npM + ov; h.Kp = KpM; h.lda = KpM; h.ldb = KpM; h.ldc = npM + ov;
+    h.M = noM; h.N = npM + ov; h.Kp = KpM; h.lda = 2 * KpM; h.ldb = 2 * KpM; h.ldc = npM + ov;
     h.Ah = c.Tmh; h.Al = c.Tml; h.Bh = c.LMh; h.Bl = c.LMl; h.C = (float*)c.CM; h.sbn = c.lsM;
     k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noM * (npM + ov) * npM);
     x3_gemm<_Float16>(h, s);
