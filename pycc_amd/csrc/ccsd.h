@@ -70,7 +70,7 @@ struct Ctx {
   float *PT = nullptr, *P2T = nullptr;
   // W0 = the integral part of [Wmbej | Wmbje] ((me|jb), -(mj|be)) in the W layout; split tau~ and L
   // as [(i,j,a)][f padded] for :497 / :533
-  double* W0 = nullptr;
+  float* W0 = nullptr;
   f16 *TFh = nullptr, *TFl = nullptr, *LFh = nullptr, *LFl = nullptr;
   // side stream for the packed ladder in the split-GEMM modes: it depends only on t1, t2, so it runs
   // beside the rest of the residual (fork after the amplitude scales, join before zt_add)

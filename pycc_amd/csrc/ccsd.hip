@@ -238,7 +238,7 @@ void alloc_work(Ctx& c) {
   c.Wmnij = c.alloc<double>(o * o * o * o);
   c.Hh = c.alloc<double>(ov * ov);
   c.Zt = c.alloc<double>(ov * o * o);
-  c.Wd = c.alloc<double>(2 * ov * ov);
+  if (c.mode != MODE_MIXED_F16X3) c.Wd = c.alloc<double>(2 * ov * ov);   // FP16x3 builds W directly
   c.Ud = c.alloc<double>(ov * ov);
   const size_t es = c.mode == MODE_DP ? sizeof(double) : sizeof(float);
   auto talloc = [&](size_t n) -> void* { return es == 8 ? (void*)c.alloc<double>(n) : (void*)c.alloc<float>(n); };
@@ -414,18 +414,18 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   }
   // W[(me)][x][(jb)]: x = 0 holds Wmbej, x = 1 holds Wmbje
   const std::vector<int64_t> WSTR{v * 2 * ov, 2 * ov, v, 1};
-  const TV Wej = tvs(c.Wd, "mejb", OVOV, WSTR), Wje = tvs(c.Wd + ov, "mejb", OVOV, WSTR);
   if (x3) {
-    covvv_x3_w(c, c.Wd, s);                                        // :641 + :642, :680 + :681
+    w_build_x3(c, t1, (float*)W, s);   // :641-643, :680-682 in one pass, straight to FP32
   } else {
+    const TV Wej = tvs(c.Wd, "mejb", OVOV, WSTR), Wje = tvs(c.Wd + ov, "mejb", OVOV, WSTR);
     axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, Wej, s);    // <mb|ej> = (me|jb)        :641
     cd(1.0, T1("jf"), tv(c.Covvv, "mebf", OVVV), 1.0, Wej);        // t_jf <mb|ef>, (me|bf)     :642
     axpby<double>(-1.0, tv(c.Coovv, "mjbe", OOVV), 0.0, Wje, s);   // -<mb|je> = -(mj|be)       :680
     cd(-1.0, T1("jf"), tv(c.Covvv, "mfbe", OVVV), 1.0, Wje);       // t_jf <mb|fe>, (mf|be)     :681
+    cd(-1.0, T1("nb"), tv(c.Cooov, "njme", OOOV), 1.0, Wej);       // t_nb <mn|ej>, (nj|me)     :643
+    cd(1.0, T1("nb"), tv(c.Cooov, "mjne", OOOV), 1.0, Wje);        // t_nb <mn|je>, (mj|ne)     :682
+    if (!std::is_same<T, double>::value) cast<double, T>(2 * ov * ov, c.Wd, W, s);
   }
-  cd(-1.0, T1("nb"), tv(c.Cooov, "njme", OOOV), 1.0, Wej);         // t_nb <mn|ej>, (nj|me)     :643
-  cd(1.0, T1("nb"), tv(c.Cooov, "mjne", OOOV), 1.0, Wje);          // t_nb <mn|je>, (mj|ne)     :682
-  if (!std::is_same<T, double>::value) cast<double, T>(2 * ov * ov, c.Wd, W, s);
   const T* Kt = std::is_same<T, double>::value ? (const T*)c.Covov : (const T*)c.K32;
   const T* Jt = std::is_same<T, double>::value ? (const T*)c.J : (const T*)c.J32;
   const std::vector<int64_t> OVXOV{o, v, 2, o, v};

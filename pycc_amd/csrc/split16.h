@@ -75,7 +75,8 @@ void fae_tauL_x3(Ctx& c, double* Fae, hipStream_t s);
 void covvv_x3_setup(Ctx& c, hipStream_t s);
 void covvv_x3_products(Ctx& c, const double* t1, hipStream_t s);   // PT, P2T from t1
 void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s);              // + 2 t_mf (mf|ae) - t_mf (me|af)
-void covvv_x3_w(Ctx& c, double* Wd, hipStream_t s);                 // :642, :681 into Wd
+// the FP32 ring W[(m,e)][x][(j,b)] (:641-643, :680-682) from W0, PT, P2T and t1 (Cooov terms)
+void w_build_x3(Ctx& c, const double* t1, float* W, hipStream_t s);
 void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s);                // :939
 void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s);   // :759
 // The split ring operands BJ, BK, AP, AQ (K = ovp, or 2 ovp for AP; interleaved, lo = hi + 32) and Ud

@@ -834,14 +834,68 @@ __global__ __launch_bounds__(256) void split_udt_kernel(const double* __restrict
   }
 }
 
-// Wd[(m,e)][0][j][b] = W0[same] + PT[j][(m,e)][b]: one row of v per (j, (m,e))
-__global__ __launch_bounds__(256) void wej_add_kernel(const float* __restrict__ PT, int64_t o, int64_t v,
-                                                      const double* __restrict__ W0, double* __restrict__ Wd) {
-  const int64_t ov = o * v, total = o * ov * v;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = e % v, t = e / v, me = t % ov, j = t / ov;
-    const int64_t at = me * 2 * ov + j * v + b;
-    Wd[at] = W0[at] + (double)PT[e];
+// The FP32 ring W in one pass (ccwfn.py:641-643 Wmbej, :680-682 Wmbje), W[(m,e)][x][(j,b)]:
+//   x = 0: W0 + PT[j][(m,e,b)]  - sum_n t_nb (nj|me)
+//   x = 1: W0 - P2T[j][(m,b,e)] + sum_n t_nb (mj|ne)
+// summed in FP64 and rounded once. One workgroup per (m, j, block of WE e's); thread = b. The two
+// Cooov slices of a block of n go to LDS (read as broadcasts); the P2T tile [b][e] is transposed
+// through LDS so that every global access is along b or e.
+constexpr int WE = 16, WN = 64, WB = 256;
+__global__ __launch_bounds__(WB) void w_build_x3_kernel(int64_t o, int64_t v, const double* __restrict__ t1,
+                                                        const double* __restrict__ Cooov,
+                                                        const float* __restrict__ W0, const float* __restrict__ PT,
+                                                        const float* __restrict__ P2T, float* __restrict__ W) {
+  __shared__ double c1[WN][WE], c2[WN][WE];
+  __shared__ float p2[WB][WE + 1];
+  const int64_t ov = o * v, neb = (v + WE - 1) / WE;
+  const int64_t eb = blockIdx.x % neb, mj = blockIdx.x / neb;
+  const int64_t m = mj / o, j = mj - m * o, e0 = eb * WE;
+  const int ne = (int)(v - e0 < WE ? v - e0 : WE);
+  const int tid = threadIdx.x;
+  for (int64_t b0 = 0; b0 < v; b0 += WB) {
+    const int64_t b = b0 + tid;
+    const bool bok = b < v;
+    double a0[WE], a1[WE];
+#pragma unroll
+    for (int e = 0; e < WE; ++e) a0[e] = a1[e] = 0.0;
+    for (int64_t n0 = 0; n0 < o; n0 += WN) {
+      const int nn = (int)(o - n0 < WN ? o - n0 : WN);
+      __syncthreads();
+      for (int q = tid; q < WN * WE; q += WB) {
+        const int n = q / WE, e = q - n * WE;
+        const bool ok = n < nn && e < ne;
+        c1[n][e] = ok ? Cooov[(((n0 + n) * o + j) * o + m) * v + e0 + e] : 0.0;   // (nj|me)
+        c2[n][e] = ok ? Cooov[((m * o + j) * o + n0 + n) * v + e0 + e] : 0.0;     // (mj|ne)
+      }
+      __syncthreads();
+      if (bok) {
+        for (int n = 0; n < nn; ++n) {
+          const double tb = t1[(n0 + n) * v + b];
+#pragma unroll
+          for (int e = 0; e < WE; ++e) {
+            a0[e] = fma(c1[n][e], tb, a0[e]);
+            a1[e] = fma(c2[n][e], tb, a1[e]);
+          }
+        }
+      }
+    }
+    // P2T[j][m][b][e0 .. e0 + WE) for the WB b's of this chunk, through LDS
+    const float* p2src = P2T + (j * o + m) * v * v;
+    for (int q = tid; q < WB * WE; q += WB) {
+      const int bb = q / WE, e = q - bb * WE;
+      p2[bb][e] = (b0 + bb < v && e < ne) ? p2src[(b0 + bb) * v + e0 + e] : 0.f;
+    }
+    __syncthreads();
+    if (bok) {
+#pragma unroll
+      for (int e = 0; e < WE; ++e) {
+        if (e >= ne) break;
+        const int64_t me = m * v + e0 + e;
+        const int64_t at0 = me * 2 * ov + j * v + b, at1 = at0 + ov;
+        W[at0] = (float)((double)W0[at0] + (double)PT[(j * ov + me) * v + b] - a0[e]);
+        W[at1] = (float)((double)W0[at1] - (double)p2[tid][e] + a1[e]);
+      }
+    }
   }
 }
 
@@ -915,11 +969,18 @@ void covvv_x3_setup(Ctx& c, hipStream_t s) {
   halloc(&c.TFl, o * ov * v32);
   halloc(&c.LFh, o * ov * v32);
   halloc(&c.LFl, o * ov * v32);
-  // W0[(m,e)][x][(j,b)]: x = 0 (me|jb) (:641), x = 1 -(mj|be) (:680)
-  c.W0 = c.alloc<double>(2 * ov * ov);
-  const std::vector<int64_t> OVOV{o, v, o, v}, WSTR{v * 2 * ov, 2 * ov, v, 1};
-  axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, tvs(c.W0, "mejb", OVOV, WSTR), s);
-  axpby<double>(-1.0, tv(c.Coovv, "mjbe", {o, o, v, v}), 0.0, tvs(c.W0 + ov, "mejb", OVOV, WSTR), s);
+  // W0[(m,e)][x][(j,b)] (FP32): x = 0 (me|jb) (:641), x = 1 -(mj|be) (:680), through an FP64 temporary
+  c.W0 = c.alloc<float>(2 * ov * ov);
+  {
+    double* W0d = nullptr;
+    CCX_HIP(hipMalloc(&W0d, 2 * ov * ov * sizeof(double)));
+    const std::vector<int64_t> OVOV{o, v, o, v}, WSTR{v * 2 * ov, 2 * ov, v, 1};
+    axpby<double>(1.0, tv(c.Covov, "mejb", OVOV), 0.0, tvs(W0d, "mejb", OVOV, WSTR), s);
+    axpby<double>(-1.0, tv(c.Coovv, "mjbe", {o, o, v, v}), 0.0, tvs(W0d + ov, "mejb", OVOV, WSTR), s);
+    cast<double, float>(2 * ov * ov, W0d, c.W0, s);
+    CCX_HIP(hipStreamSynchronize(s));
+    CCX_HIP(hipFree(W0d));
+  }
   x3_absmax<double>(c.Loovv, ov * ov, c.xpart, s);
   x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 12, s);
   x3_split_rows<double>(c.Loovv, o * ov, v, v, c.xs + 12, c.LFh, c.LFl, v32, s);
@@ -948,15 +1009,12 @@ void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s) {
   CCX_HIP(hipGetLastError());
 }
 
-// Wd = W0 + the :642 / :681 products (writes every element of Wd)
-void covvv_x3_w(Ctx& c, double* Wd, hipStream_t s) {
-  const int64_t o = c.o, v = c.v, ov = c.ov();
-  wej_add_kernel<<<grid_e(o * ov * v), 256, 0, s>>>(c.PT, o, v, c.W0, Wd);
-  CCX_HIP(hipGetLastError());
-  // Wmbje[m][e][j][b] = W0 - P2T[j][m][b][e]: plane (j, m), out Wd[(m v + e) 2 ov + ov + j v + b]
-  const int64_t planes = o * o * ceil_div(v, 32) * ceil_div(v, 32);
-  plane_t_add_kernel<<<(unsigned)std::min<int64_t>(planes, 1 << 20), 256, 0, s>>>(
-      c.P2T, v, o, o, v, 2 * v * ov, ov, 2 * ov, -1.0, c.W0, Wd);
+// W = the FP32 ring W (every element written)
+void w_build_x3(Ctx& c, const double* t1, float* W, hipStream_t s) {
+  const int64_t o = c.o, v = c.v;
+  const int64_t blocks = o * o * ceil_div(v, WE);
+  CCX_CHECK(blocks < (1ll << 31), ERR_ARG, "W build grid too large");
+  w_build_x3_kernel<<<(unsigned)blocks, WB, 0, s>>>(o, v, t1, c.Cooov, c.W0, c.PT, c.P2T, W);
   CCX_HIP(hipGetLastError());
 }
 
