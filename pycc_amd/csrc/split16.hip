@@ -46,29 +46,6 @@ __device__ __forceinline__ int64_t xcd_remap_x3(int64_t w, int64_t ntiles) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// Workgroup id -> output tile of the 256 x 256 kernel. Groups of 8 M-tiles are swept in chunks of
-// 32 N-tiles. In a full chunk (256 consecutive workgroups, dispatched round-robin to the 8 XCDs)
-// workgroup i takes M-tile (i / 8) % 8 and N-tile 4 (i % 8) + i / 64: each XCD works on an 8 x 4
-// block of tiles (A and B panels reused in its L2) and all XCDs read the same 8 A panels at the
-// same time (shared through the memory-side cache). Partial chunks run M-fastest.
-__device__ __forceinline__ void chunk_order(int64_t t, int64_t tiles_m, int64_t tiles_n, int64_t& tm, int64_t& tn) {
-  const int64_t g = t / (GM * tiles_n), first_m = g * GM;
-  const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
-  const int64_t r = t - g * GM * tiles_n;
-  const int64_t c = r / (gsize * 32), i = r - c * gsize * 32;
-  const int64_t nc = tiles_n - 32 * c < 32 ? tiles_n - 32 * c : 32;
-  int64_t ml, nl;
-  if (gsize == GM && nc == 32) {
-    ml = (i >> 3) & 7;
-    nl = 4 * (i & 7) + (i >> 6);
-  } else {
-    ml = i % gsize;
-    nl = i / gsize;
-  }
-  tm = first_m + ml;
-  tn = 32 * c + nl;
-}
-
 // BT x BT output tiles, 4 waves in 2 x 2, each computing (BT/2) x (BT/2) as 16 x 16 MFMA blocks.
 // Operand tiles (BT rows x 32 k, hi and lo) are double-buffered in LDS with one register stage:
 // the next k-tile's global loads are issued before the current tile's MFMAs.
@@ -251,8 +228,11 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 2, wn = wave & 3;
   const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BT - 1) / BT;
-  int64_t tm, tn;
-  chunk_order(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
+  const int64_t group = t / (GM * tiles_n), first_m = group * GM;
+  const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
+  const int64_t within = t - group * GM * tiles_n;
+  const int64_t tm = first_m + within % gsize, tn = within / gsize;
 
   int64_t gA[IPW], gB[IPW];
   int ldso[IPW];
