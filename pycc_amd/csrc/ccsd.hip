@@ -243,10 +243,9 @@ void alloc_work(Ctx& c) {
   const size_t es = c.mode == MODE_DP ? sizeof(double) : sizeof(float);
   auto talloc = [&](size_t n) -> void* { return es == 8 ? (void*)c.alloc<double>(n) : (void*)c.alloc<float>(n); };
   if (c.mode == MODE_MIXED_F16X3) {
-    // FP32 ring outputs P (A1), Q (S1) and the :930 scratch (S2); the operands are split FP16
+    // FP32 ring outputs P (A1) and Q (S1); the operands are split FP16
     c.S1 = talloc(ov * ov);
     c.A1 = talloc(ov * ov);
-    c.S2 = talloc(ov * ov);
     const int64_t ovp = c.ovp();
     auto halloc = [&](f16** q, int64_t n) {
       *q = c.alloc<f16>(n);
@@ -350,7 +349,8 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // ---- Fme: ccwfn.py:563-564 -------------------------------------------------------------
   const TV Fme = tv(c.Fme, "me", OV);
   axpby<double>(1.0, Fov, 0.0, Fme, s);
-  cd(1.0, T1("nf"), tv(c.Loovv, "mnef", OOVV), 1.0, Fme);
+  // + t_nf L_mnef: Fme[m][e] += sum_(n,f) t[n][f] Loovv[m][n][e][f]
+  wsum(o, v, o, v, 1.0, t1, v, c.Loovv, o * v * v, v, v * v, 1.0, c.Fme, v, s);
 
   // ---- Fae: ccwfn.py:494-497 (tau~ = t2 + 1/2 t1 t1) ----------------------------------------
   build_tau(o, v, 1.0, 0.5, t1, t2, c.tau, s);
@@ -537,17 +537,15 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   if (std::is_same<T, double>::value) {
     cd(0.5, tv(c.tau, "mnab", OOVV), Wmnij, 1.0, Hh);
   } else if (c.mode == MODE_MIXED_F16X3) {
-    float* X32 = (float*)c.S2;
     cast<double, float>(o * o * o * o, c.Wmnij, c.W32, s);
     x3_absmax<float>(c.W32, o * o * o * o, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 7, s);
     x3_split_t<float>(c.W32, o * o, o * o, 1, o * o, c.xs + 7, c.WTh, c.WTl, 0, c.o2p(), s);   // [(ij)][(mn)]
-    X3Gemm g;   // X32[(ij)][(ab)] = 1/2 sum_(mn) Wmnij tau_mnab
+    X3Gemm g;   // Hh[(ij)][(ab)] += 1/2 sum_(mn) Wmnij tau_mnab, added into the FP64 Hh
     g.M = o * o; g.N = v * v; g.Kp = c.o2p(); g.lda = c.o2p(); g.ldb = c.o2p(); g.ldc = v * v;
-    g.Ah = c.WTh; g.Al = c.WTl; g.Bh = c.TTh; g.Bl = c.TTl; g.C = X32;
+    g.Ah = c.WTh; g.Al = c.WTl; g.Bh = c.TTh; g.Bl = c.TTl; g.C64 = c.Hh; g.beta = 1.f;
     g.sa = c.xs + 7; g.sb = c.xs + 5; g.alpha = 0.5f;
     x3_gemm<_Float16>(g, s, &ws);
-    accumulate(ov * ov, 1.0, X32, c.Hh, s);
   } else {
     float* X32 = (float*)c.S2;   // S2 is dead after the P ring GEMM
     cast<double, float>(o * o * o * o, c.Wmnij, c.W32, s);
@@ -574,9 +572,11 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   cd(1.0, T1("ie"), Fae, 1.0, R1);                                   //                      :755
   cd(-1.0, Fmi, T1("ma"), 1.0, R1);                                  //                      :756
   // U = 2 t2 - t2^T in ring layout: Ud[i][a][m][e] = 2 t2_imae - t2_imea
-  cd(1.0, tv(c.Ud, "iame", OVOV), Fme, 1.0, R1);                     // (2t2 - t2^T) Fme     :757
-  cd(2.0, T1("nf"), tv(c.Covov, "nfia", OVOV), 1.0, R1);             // L_nafi = 2(nf|ia)    :758
-  cd(-1.0, T1("nf"), tv(c.Coovv, "niaf", OOVV), 1.0, R1);            //        - (ni|af)
+  // (2t2 - t2^T) Fme :757: r1[i][a] += sum_(m,e) Ud[i][a][m][e] Fme[m][e]
+  wsum(o, v, o, v, 1.0, c.Fme, v, c.Ud, v * ov, ov, v, 1.0, r1, v, s);
+  // L_nafi = 2(nf|ia) - (ni|af) :758: 2 sum_(nf) t_nf Covov[(nf)][(ia)] - sum_(n,f) t_nf Coovv[n][i][a][f]
+  colsum(ov, ov, 2.0, t1, c.Covov, ov, 1.0, r1, (double*)ws.get((size_t)colsum_chunks(ov) * ov * sizeof(double)), s);
+  wsum(o, v, o, v, -1.0, t1, v, c.Coovv, v * v, v, o * v * v, 1.0, r1, v, s);
   if (x3) covvv_x3_r1(c, c.Ud, r1, s);                              // <ma|ef> = (me|af)    :759
   else cd(1.0, tv(c.Ud, "meif", OVOV), tv(c.Covvv, "meaf", OVVV), 1.0, R1);
   cd(-1.0, T2("mnae"), tv(c.Looov, "imne", OOOV), 1.0, R1);         // L_nmei = 2(mi|ne) - (ni|me) :760

@@ -98,4 +98,13 @@ void cast(int64_t n, const Tin* in, Tout* out, hipStream_t s);
 // out += alpha * in (FP32 partial results folded into FP64 sums)
 void accumulate(int64_t n, double alpha, const float* in, double* out, hipStream_t s);
 
+// out[p ldo + q] = beta out + alpha sum_{n < N, f < F} w[n ldw + f] X[p sp + q sq + n sn + f] (FP64,
+// one wave per output; for contractions with a small output and a contiguous innermost index).
+void wsum(int64_t P, int64_t Q, int64_t N, int64_t F, double alpha, const double* w, int64_t ldw, const double* X,
+          int64_t sp, int64_t sq, int64_t sn, double beta, double* out, int64_t ldo, hipStream_t s);
+// out[c] = beta out[c] + alpha sum_r w[r] X[r ldx + c] (FP64; partial needs colsum_chunks(R) * C doubles).
+int colsum_chunks(int64_t R);
+void colsum(int64_t R, int64_t C, double alpha, const double* w, const double* X, int64_t ldx, double beta,
+            double* out, double* partial, hipStream_t s);
+
 }  // namespace ccx

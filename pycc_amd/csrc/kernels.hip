@@ -3,6 +3,7 @@
 // These are HBM-bound: each reads its inputs once and writes its outputs once. Transposed reads
 // of t2 blocks go through 32x33 LDS tiles so that global loads and stores stay coalesced.
 #include "kernels.h"
+#include <algorithm>
 #include <cmath>
 
 namespace ccx {
@@ -626,6 +627,92 @@ __global__ void accumulate_kernel(int64_t n, double alpha, const float* __restri
 void accumulate(int64_t n, double alpha, const float* in, double* out, hipStream_t s) {
   if (!n) return;
   accumulate_kernel<<<grid_for(n), 256, 0, s>>>(n, alpha, in, out);
+  CCX_HIP(hipGetLastError());
+}
+
+// ---- bandwidth-bound weighted sums (the GEMV-shaped FP64 contractions of the residual) ----
+// out[p ldo + q] = beta out + alpha sum_{n < N, f < F} w[n ldw + f] X[p sp + q sq + n sn + f]:
+// one wave per output, lanes along the contiguous f, a fixed-order butterfly at the end.
+__global__ __launch_bounds__(256) void wsum_kernel(int64_t P, int64_t Q, int64_t N, int64_t F, double alpha,
+                                                   const double* __restrict__ w, int64_t ldw,
+                                                   const double* __restrict__ X, int64_t sp, int64_t sq, int64_t sn,
+                                                   double beta, double* __restrict__ out, int64_t ldo) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; t < P * Q; t += nw) {
+    const int64_t p = t / Q, q = t - p * Q;
+    const double* x = X + p * sp + q * sq;
+    double a0 = 0.0, a1 = 0.0;
+    int64_t n = 0;
+    for (; n + 1 < N; n += 2) {
+      const double *x0 = x + n * sn, *x1 = x0 + sn, *w0 = w + n * ldw, *w1 = w0 + ldw;
+      for (int64_t f = lane; f < F; f += 64) {
+        a0 = fma(w0[f], x0[f], a0);
+        a1 = fma(w1[f], x1[f], a1);
+      }
+    }
+    if (n < N) {
+      const double *x0 = x + n * sn, *w0 = w + n * ldw;
+      for (int64_t f = lane; f < F; f += 64) a0 = fma(w0[f], x0[f], a0);
+    }
+    double acc = a0 + a1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) {
+      double* d = out + p * ldo + q;
+      *d = (beta == 0.0 ? 0.0 : beta * *d) + alpha * acc;
+    }
+  }
+}
+
+// partial[rc][c] = sum_{r in row chunk rc} w[r] X[r ldx + c]
+__global__ __launch_bounds__(256) void colsum_partial_kernel(int64_t R, int64_t C, int64_t rchunk,
+                                                             const double* __restrict__ w,
+                                                             const double* __restrict__ X, int64_t ldx,
+                                                             double* __restrict__ partial) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, rc = blockIdx.y;
+  if (c >= C) return;
+  const int64_t r0 = rc * rchunk, r1 = r0 + rchunk < R ? r0 + rchunk : R;
+  double a0 = 0.0, a1 = 0.0;
+  int64_t r = r0;
+  for (; r + 1 < r1; r += 2) {
+    a0 = fma(w[r], X[r * ldx + c], a0);
+    a1 = fma(w[r + 1], X[(r + 1) * ldx + c], a1);
+  }
+  if (r < r1) a0 = fma(w[r], X[r * ldx + c], a0);
+  partial[rc * C + c] = a0 + a1;
+}
+
+// out[c] = beta out[c] + alpha sum_rc partial[rc][c], rc in order
+__global__ __launch_bounds__(256) void colsum_finish_kernel(int64_t C, int nrc, const double* __restrict__ partial,
+                                                            double alpha, double beta, double* __restrict__ out) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  double acc = 0.0;
+  for (int rc = 0; rc < nrc; ++rc) acc += partial[rc * C + c];
+  out[c] = (beta == 0.0 ? 0.0 : beta * out[c]) + alpha * acc;
+}
+
+void wsum(int64_t P, int64_t Q, int64_t N, int64_t F, double alpha, const double* w, int64_t ldw, const double* X,
+          int64_t sp, int64_t sq, int64_t sn, double beta, double* out, int64_t ldo, hipStream_t s) {
+  if (P * Q == 0) return;
+  const int64_t blocks = std::min<int64_t>(ceil_div(P * Q, 4), 65536);   // 4 waves per block
+  wsum_kernel<<<(unsigned)blocks, 256, 0, s>>>(P, Q, N, F, alpha, w, ldw, X, sp, sq, sn, beta, out, ldo);
+  CCX_HIP(hipGetLastError());
+}
+
+int colsum_chunks(int64_t R) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(R, 128))); }
+
+void colsum(int64_t R, int64_t C, double alpha, const double* w, const double* X, int64_t ldx, double beta,
+            double* out, double* partial, hipStream_t s) {
+  if (C == 0) return;
+  const int nrc = colsum_chunks(R);
+  const int64_t rchunk = ceil_div(std::max<int64_t>(R, 1), nrc);
+  CCX_CHECK(ceil_div(C, 256) < (1ll << 31), ERR_ARG, "column sum too wide");
+  colsum_partial_kernel<<<dim3((unsigned)ceil_div(C, 256), (unsigned)nrc), 256, 0, s>>>(R, C, rchunk, w, X, ldx,
+                                                                                       partial);
+  CCX_HIP(hipGetLastError());
+  colsum_finish_kernel<<<(unsigned)ceil_div(C, 256), 256, 0, s>>>(C, nrc, partial, alpha, beta, out);
   CCX_HIP(hipGetLastError());
 }
 

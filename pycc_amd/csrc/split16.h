@@ -21,8 +21,9 @@ struct X3Gemm {
   // is a multiple of 32, so a 32-wide k-tile never straddles two blocks. Interleaved operands
   // (hi and lo of each 32-wide k-block side by side, lo = hi + 32) are kb = 32, ska / skb = 64.
   int64_t kb = 0, ska = 0, skb = 0;
-  // FP64 destination (optional, needs ws): C64[m ldc + n] = alpha / (sa sb) sum + beta C64, the
-  // split-K partials (FP32, each over a short K range) summed in FP64
+  // FP64 destination (optional): C64[m ldc + n] = alpha / (sa sb) sum + beta C64. On grids under 512
+  // tiles (needs ws) the split-K partials (FP32, each over a short K range) are summed in FP64; on
+  // larger grids the kernel adds its FP32 sum into C64 directly.
   double* C64 = nullptr;
   // split-K (set by x3_gemm): grid.y = splits K ranges of kchunk, raw partials in work[s][M][N]
   int64_t kchunk = 0;

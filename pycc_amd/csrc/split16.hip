@@ -188,10 +188,17 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
       for (int r = 0; r < 4; ++r) {
         const int64_t m = tm * BT + wm * WT + i * 16 + fr * 4 + r;
         if (m < p.M) {
-          float* d = p.C + m * p.ldc + n;
-          float val = f * acc[i][j][r];
-          if (p.beta != 0.f) val += p.beta * *d;
-          *d = val;
+          if (p.C64) {   // FP64 destination without split K
+            double* d = p.C64 + m * p.ldc + n;
+            double val = (double)(f * acc[i][j][r]);
+            if (p.beta != 0.f) val += (double)p.beta * *d;
+            *d = val;
+          } else {
+            float* d = p.C + m * p.ldc + n;
+            float val = f * acc[i][j][r];
+            if (p.beta != 0.f) val += p.beta * *d;
+            *d = val;
+          }
         }
       }
   }
@@ -342,10 +349,17 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
       for (int r = 0; r < 4; ++r) {
         const int64_t m = tm * BT + wm * WTM + i * 16 + fr * 4 + r;
         if (m < p.M) {
-          float* d = p.C + m * p.ldc + n;
-          float val = f * acc[i][j][r];
-          if (p.beta != 0.f) val += p.beta * *d;
-          *d = val;
+          if (p.C64) {   // FP64 destination without split K
+            double* d = p.C64 + m * p.ldc + n;
+            double val = (double)(f * acc[i][j][r]);
+            if (p.beta != 0.f) val += (double)p.beta * *d;
+            *d = val;
+          } else {
+            float* d = p.C + m * p.ldc + n;
+            float val = f * acc[i][j][r];
+            if (p.beta != 0.f) val += p.beta * *d;
+            *d = val;
+          }
         }
       }
   }
@@ -687,9 +701,11 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
   // split K into workspace partials when the tile grid cannot fill 256 CUs (keep >= 32 k-tiles each)
   int64_t splits = 1;
   p.work = nullptr;
-  CCX_CHECK(!p.C64 || ws, ERR_INTERNAL, "an FP64 destination needs a workspace");
-  if (ws && (tiles < 512 || p.C64)) {
-    splits = tiles < 512 ? std::min<int64_t>({1024, ceil_div(1024, tiles), p.Kp / (32 * BK)}) : 1;
+  // An FP64 destination is written by the kernel directly when the tile grid fills the chip, else
+  // through split-K partials summed in FP64 (which need a workspace).
+  CCX_CHECK(!p.C64 || ws || tiles >= 512, ERR_INTERNAL, "an FP64 destination on a small grid needs a workspace");
+  if (ws && tiles < 512) {
+    splits = std::min<int64_t>({1024, ceil_div(1024, tiles), p.Kp / (32 * BK)});
     splits = std::max<int64_t>(splits, 1);
     if (splits > 1 || p.C64) {
       p.kchunk = ceil_div(ceil_div(p.Kp, splits), BK) * BK;

@@ -183,6 +183,29 @@ def test_mixed_residual_large_pair_count(pycc_amd, precision, tol_r):
     assert rel2 < tol_r and rel1 < tol_r
 
 
+def test_f16x3_residual_matches_dp_on_a_full_grid(pycc_amd):
+    """o = 24, v = 256: the :930 hole-hole split GEMM has 3 x 256 = 768 tiles, so it adds into the FP64
+    Hh straight from the kernel (no split-K partials); the ring, ladder and W paths run at a size where
+    every split GEMM takes the 256 x 256 LDS-DMA kernel. The FP64 residual of the same library (pinned
+    to the oracle at smaller sizes) is the reference; the bar is the FP16x3 residual tolerance."""
+    no, nv = 24, 256
+    print("f16x3 vs dp residual", no, nv, flush=True)
+    sd = synthetic_df(no, nv, seed=21)
+    t1, t2 = random_amplitudes(no, nv, 21, scale=0.02)
+    out = {}
+    for precision in ("DP", "MIXED_F16X3"):
+        cc = pycc_amd.CCwfn(sd, no=no, precision=precision, verbose=False)
+        r1, r2 = cc.residuals(cc.H.F, t1, t2)
+        out[precision] = (r1.cpu().numpy(), r2.cpu().numpy())
+        del cc, r1, r2
+        torch.cuda.empty_cache()
+    (a1, a2), (b1, b2) = out["DP"], out["MIXED_F16X3"]
+    rel1 = np.abs(b1 - a1).max() / np.abs(a1).max()
+    rel2 = np.abs(b2 - a2).max() / np.abs(a2).max()
+    print("residual rel err r1 %.2e r2 %.2e" % (rel1, rel2))
+    assert np.isfinite(b2).all() and rel1 < 3e-5 and rel2 < 3e-5
+
+
 def test_contract_f32_large_tiles(pycc_amd):
     """8200 x 8200 output: the FP32 plain GEMM takes its 256 x 256 tiles (1089 tiles, partial edge
     tiles in M and N), with a K that is not a multiple of the 32-wide k tile."""
