@@ -102,12 +102,13 @@ static void alloc_integrals(Ctx& c) {
       const int64_t bP = (c.npP() + ov) * bf16_kpad(c.npP()), bM = (c.npM() + ov) * bf16_kpad(c.npM());
       if (c.mode == MODE_MIXED_F16X3) {
         // interleaved split rows (hi and lo of each 32-wide k-block side by side, lo = hi + 32)
-        c.LPh = c.alloc<bf16>(2 * bP);
-        c.LMh = c.alloc<bf16>(2 * bM);
+        // (at least 64 elements, so that lo = hi + 32 stays inside an empty one)
+        c.LPh = c.alloc<bf16>(std::max<int64_t>(2 * bP, 64));
+        c.LMh = c.alloc<bf16>(std::max<int64_t>(2 * bM, 64));
         CCX_HIP(hipMemset(c.LPh, 0, 2 * bP * sizeof(bf16)));   // K padding
         if (bM) CCX_HIP(hipMemset(c.LMh, 0, 2 * bM * sizeof(bf16)));
         c.LPl = c.LPh + 32;
-        c.LMl = bM ? c.LMh + 32 : nullptr;
+        c.LMl = c.LMh + 32;
       } else {
         c.LPh = c.alloc<bf16>(bP);
         c.LPl = c.alloc<bf16>(bP);
@@ -282,12 +283,12 @@ void alloc_work(Ctx& c) {
     CCX_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     const int64_t aP = noP * bf16_kpad(npP), aM = noM * bf16_kpad(npM);
     if (c.mode == MODE_MIXED_F16X3) {   // interleaved, lo = hi + 32
-      c.Tph = c.alloc<bf16>(2 * aP);
-      c.Tmh = c.alloc<bf16>(2 * aM);
+      c.Tph = c.alloc<bf16>(std::max<int64_t>(2 * aP, 64));
+      c.Tmh = c.alloc<bf16>(std::max<int64_t>(2 * aM, 64));
       CCX_HIP(hipMemset(c.Tph, 0, 2 * aP * sizeof(bf16)));   // K padding
       if (aM) CCX_HIP(hipMemset(c.Tmh, 0, 2 * aM * sizeof(bf16)));
       c.Tpl = c.Tph + 32;
-      c.Tml = aM ? c.Tmh + 32 : nullptr;
+      c.Tml = c.Tmh + 32;
     } else {
       c.Tph = c.alloc<bf16>(aP);
       c.Tpl = c.alloc<bf16>(aP);

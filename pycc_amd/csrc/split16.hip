@@ -769,6 +769,7 @@ template void x3_split_t<double>(const double*, int64_t, int64_t, int, int64_t, 
 void ladder_pack_x3(int64_t v, int64_t e0, int64_t e1, VVVVSrc src, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
                     float* sP, float* sM, hipStream_t s) {
   if (e1 <= e0) return;
+  CCX_CHECK(LPl == LPh + 32 && LMl == LMh + 32, ERR_INTERNAL, "ladder operands must be interleaved");
   const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
   dim3 grid(1, (unsigned)((e1 - e0) * v));
   ladder_pack_x3_kernel<_Float16><<<grid, 256, 0, s>>>(v, e0, src, (_Float16*)LPh, (_Float16*)LPl, (_Float16*)LMh,
@@ -778,6 +779,7 @@ void ladder_pack_x3(int64_t v, int64_t e0, int64_t e1, VVVVSrc src, f16* LPh, f1
 
 void ladder_pack_o_x3(int64_t o, int64_t v, const double* Covvv, f16* LPh, f16* LPl, f16* LMh, f16* LMl,
                       float* sP, float* sM, hipStream_t s) {
+  CCX_CHECK(LPl == LPh + 32 && LMl == LMh + 32, ERR_INTERNAL, "ladder operands must be interleaved");
   dim3 grid(1, (unsigned)(o * v));
   ladder_pack_o_x3_kernel<_Float16><<<grid, 256, 0, s>>>(o, v, Covvv, (_Float16*)LPh, (_Float16*)LPl,
                                                          (_Float16*)LMh, (_Float16*)LMl, sP, sM);
@@ -786,6 +788,8 @@ void ladder_pack_o_x3(int64_t o, int64_t v, const double* Covvv, f16* LPh, f16* 
 
 void tau_pack_x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2, const float* sc,
                  f16* Tph, f16* Tpl, f16* Tmh, f16* Tml, hipStream_t s) {
+  CCX_CHECK(KpP % 32 == 0 && KpM % 32 == 0 && Tpl == Tph + 32 && Tml == Tmh + 32, ERR_INTERNAL,
+            "packed tau must be interleaved");
   const int64_t npP = v * (v + 1) / 2;
   dim3 grid((unsigned)std::min<int64_t>(ceil_div(npP, 256), 1024), (unsigned)(o * (o + 1) / 2));
   tau_pack_x3_kernel<_Float16><<<grid, 256, 0, s>>>(o, v, KpP, KpM, t1, t2, sc, (_Float16*)Tph, (_Float16*)Tpl,

@@ -96,6 +96,23 @@ def test_residual_dp_matches_oracle(pycc_amd, no, nv, seed, offdiag):
     assert abs(cc.cc_energy(t1=t1, t2=t2) - ref.energy(t1, t2)) < 1e-12
 
 
+@pytest.mark.parametrize("no,nv,seed", [(1, 5, 1), (3, 1, 2), (2, 3, 3)])
+def test_f16x3_residual_edge_shapes(pycc_amd, no, nv, seed):
+    """MIXED_F16X3 with empty packed parts: o = 1 has no (i < j) pairs (no tau- rows), v = 1 no (e < f)
+    pairs (no LM columns but the Z ones). Bar: the FP16x3 residual tolerance, relative to max|r|."""
+    print("f16x3 edge", no, nv, flush=True)
+    sd, H = _ham(no, nv, seed, t2max=0.1, offdiag=0.05)
+    ref = OracleCCSD(H, no)
+    t1, t2 = random_amplitudes(no, nv, seed, scale=0.05)
+    r1_ref, r2_ref = ref.residuals(H.F, t1, t2)
+    cc = pycc_amd.CCwfn(H, no=no, precision='MIXED_F16X3', verbose=False)
+    r1, r2 = cc.residuals(H.F, t1, t2)
+    rel1 = np.abs(r1.cpu().numpy() - r1_ref).max() / np.abs(r1_ref).max()
+    rel2 = np.abs(r2.cpu().numpy() - r2_ref).max() / np.abs(r2_ref).max()
+    print("rel err r1 %.2e r2 %.2e" % (rel1, rel2))
+    assert rel1 < 3e-5 and rel2 < 3e-5
+
+
 def test_df_integrals_match_dense(pycc_amd):
     no, nv = 4, 21
     sd, H = _ham(no, nv, 7, t2max=0.1)
