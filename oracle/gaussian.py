@@ -1,13 +1,20 @@
 """A small psi4-free Gaussian-integral + RHF front end (oracle -- test infrastructure).
 
-Purpose: reproduce the MO Hamiltonian psi4 hands pycc for H2O/STO-3G (``pycc/tests/conftest.py:28-36``
-SCF options; geometry ``pycc/data/molecules.py:42-46``), so the oracle can be checked against the
-reference's own known answers (``tests/test_002_ccsd_energy.py:30``, ``tests/test_003_ccsd_lambda.py:50``).
+Purpose: reproduce the MO Hamiltonian psi4 hands pycc for H2O in STO-3G and cc-pVDZ
+(``pycc/tests/conftest.py:28-36`` SCF options; geometry ``pycc/data/molecules.py:42-46``), so the
+oracle can be checked against the reference's own known answers (``tests/test_002_ccsd_energy.py:31,38``,
+``tests/test_003_ccsd_lambda.py:49,61``, ``tests/test_030_sp.py:30,39``).
 
 Integrals use the McMurchie-Davidson scheme (Hermite expansion coefficients E, Hermite Coulomb
 integrals R with the Boys function), written from the published recurrences
 (McMurchie & Davidson, J. Comput. Phys. 26, 218 (1978); Helgaker, Jorgensen, Olsen, "Molecular
-Electronic-Structure Theory", ch. 9). Cartesian s/p shells only, which is all STO-3G needs.
+Electronic-Structure Theory", ch. 9). ``ao_integrals`` / ``water_sto3g_hamiltonian`` take Cartesian
+s/p shells in pure Python (all STO-3G needs). ``ao_integrals_shells`` / ``water_hamiltonian`` also take
+d shells (cc-pVDZ): the one-electron integrals come from the same Python primitives, the ERIs from
+``md_eri.c`` (compiled by ``build_md_eri`` into ``oracle/_build``), and each d shell is reduced to its
+five real solid harmonics (psi4 uses spherical functions for the correlation-consistent sets: nmo = 24 for
+water). Energies do not depend on how a shell's functions are normalised or ordered, only on the
+space they span, so the plain combinations 2z^2 - x^2 - y^2, xz, yz, x^2 - y^2, xy are used.
 """
 from __future__ import annotations
 
@@ -29,6 +36,24 @@ STO3G = {
           (1, [5.0331513, 1.1695961, 0.3803890], [0.15591627, 0.60768372, 0.39195739])],
 }
 ZNUC = {"H": 1.0, "O": 8.0}
+
+# cc-pVDZ (Dunning, J. Chem. Phys. 90, 1007 (1989)), as distributed in the EMSL/psi4 library
+# (segmented form: the two contracted O s functions carry the 0.3023 primitive too).
+_O_S = [11720.0, 1759.0, 400.8, 113.7, 37.03, 13.27, 5.025, 1.013, 0.3023]
+CCPVDZ = {
+    "H": [(0, [13.01, 1.962, 0.4446], [0.019685, 0.137977, 0.478148]),
+          (0, [0.1220], [1.0]),
+          (1, [0.7270], [1.0])],
+    "O": [(0, _O_S, [0.000710, 0.005470, 0.027837, 0.104800, 0.283062, 0.448719, 0.270952, 0.015458,
+                     -0.002585]),
+          (0, _O_S, [-0.000160, -0.001263, -0.006267, -0.025716, -0.070924, -0.165411, -0.116955,
+                     0.557368, 0.572759]),
+          (0, [0.3023], [1.0]),
+          (1, [17.70, 3.854, 1.046], [0.043018, 0.228913, 0.508728]),
+          (1, [0.2753], [1.0]),
+          (2, [1.185], [1.0])],
+}
+BASIS_SETS = {"sto-3g": STO3G, "cc-pvdz": CCPVDZ}
 
 
 def water_geometry(r_oh_ang=1.1, angle_deg=104.0):
@@ -211,6 +236,140 @@ def ao_integrals(atoms, basis=STO3G):
     for (s1, x1), (s2, x2) in itertools.combinations(atoms, 2):
         enuc += ZNUC[s1] * ZNUC[s2] / np.linalg.norm(x1 - x2)
     return S, T, V, eri, enuc
+
+
+# ---- shells with d functions (ERIs in C) -----------------------------------------------------------
+def _cart_comps(l):
+    """Cartesian components of a shell, i descending then j descending (the order md_eri.c uses)."""
+    return [(i, j, l - i - j) for i in range(l, -1, -1) for j in range(l - i, -1, -1)]
+
+
+class _ShellFunction:
+    """One Cartesian component with the shell's coefficients (radial normalisation of the x^l
+    component, contraction normalised for it); used for the one-electron integrals."""
+
+    def __init__(self, A, lmn, exps, c):
+        self.A, self.lmn, self.exps, self.c = A, lmn, exps, c
+
+
+def _shell_coefs(l, exps, coefs):
+    exps = np.asarray(exps, float)
+    c = np.asarray(coefs, float) * ((2.0 * exps / math.pi) ** 0.75 * (4.0 * exps) ** (l / 2.0)
+                                    / math.sqrt(_df(2 * l - 1)))
+    lmn = (l, 0, 0)
+    s = sum(ca * cb * _overlap_prim(a, lmn, np.zeros(3), b, lmn, np.zeros(3))
+            for ca, a in zip(c, exps) for cb, b in zip(c, exps))
+    return exps, c / math.sqrt(s)
+
+
+_LIB = None
+
+
+def _md_lib():
+    """ctypes handle of md_eri.c (compiled into oracle/_build by build_md_eri when missing)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    import ctypes
+    import os
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "libmderi.so")
+    if not os.path.exists(out):
+        build_md_eri()
+    lib = ctypes.CDLL(out)
+    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    lib.md_eri.argtypes = [ctypes.c_int, ip, dp, ip, ip, dp, dp, ip, ctypes.c_int, dp]
+    lib.md_eri.restype = ctypes.c_int
+    _LIB = lib
+    return lib
+
+
+def build_md_eri():
+    """Compile md_eri.c with gcc into oracle/_build/libmderi.so."""
+    import os
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    out = os.path.join(here, "_build", "libmderi.so")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    subprocess.check_call(["gcc", "-O2", "-shared", "-fPIC", "-o", out + ".tmp", os.path.join(here, "md_eri.c"), "-lm"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def _sph_transform(shell_l):
+    """Cartesian -> (spherical for d) basis transformation matrix, columns = final functions."""
+    d = np.zeros((6, 5))
+    xx, xy, xz, yy, yz, zz = range(6)
+    d[[xx, yy, zz], 0] = [-1.0, -1.0, 2.0]      # 2z^2 - x^2 - y^2
+    d[xz, 1] = 1.0
+    d[yz, 2] = 1.0
+    d[[xx, yy], 3] = [1.0, -1.0]                # x^2 - y^2
+    d[xy, 4] = 1.0
+    blocks = [np.eye(3 if l == 1 else 1) if l < 2 else d for l in shell_l]
+    ncart = sum(b.shape[0] for b in blocks)
+    nsph = sum(b.shape[1] for b in blocks)
+    T = np.zeros((ncart, nsph))
+    r = c = 0
+    for b in blocks:
+        T[r:r + b.shape[0], c:c + b.shape[1]] = b
+        r += b.shape[0]
+        c += b.shape[1]
+    return T
+
+
+def ao_integrals_shells(atoms, basis=CCPVDZ, spherical=True):
+    """(S, T, V, ERI_chemist, E_nuc) for shells up to d; d shells reduced to 5 solid harmonics."""
+    import ctypes
+    shells = []
+    for sym, xyz in atoms:
+        for l, exps, coefs in basis[sym]:
+            e, c = _shell_coefs(l, exps, coefs)
+            shells.append((l, np.asarray(xyz, float), e, c))
+    funcs = [_ShellFunction(A, lmn, e, c) for l, A, e, c in shells for lmn in _cart_comps(l)]
+    n = len(funcs)
+    S = np.zeros((n, n)); T = np.zeros((n, n)); V = np.zeros((n, n))
+    for i in range(n):
+        for j in range(i + 1):
+            f, g = funcs[i], funcs[j]
+            S[i, j] = S[j, i] = _contract2(f, g, _overlap_prim)
+            T[i, j] = T[j, i] = _contract2(f, g, _kinetic_prim)
+            V[i, j] = V[j, i] = sum(
+                -ZNUC[sym] * _contract2(f, g, lambda a, l1, A, b, l2, B: _nuclear_prim(a, l1, A, b, l2, B, xyz))
+                for sym, xyz in atoms)
+    ls = np.array([sh[0] for sh in shells], dtype=np.int32)
+    xyz = np.ascontiguousarray(np.concatenate([sh[1] for sh in shells]), dtype=np.float64)
+    nprim = np.array([len(sh[2]) for sh in shells], dtype=np.int32)
+    first = np.concatenate([[0], np.cumsum(nprim)[:-1]]).astype(np.int32)
+    exps = np.ascontiguousarray(np.concatenate([sh[2] for sh in shells]), dtype=np.float64)
+    coef = np.ascontiguousarray(np.concatenate([sh[3] for sh in shells]), dtype=np.float64)
+    ncomp = np.array([(l + 1) * (l + 2) // 2 for l in ls], dtype=np.int32)
+    bf0 = np.concatenate([[0], np.cumsum(ncomp)[:-1]]).astype(np.int32)
+    eri = np.zeros((n, n, n, n))
+    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    rc = _md_lib().md_eri(len(shells), ls.ctypes.data_as(ip), xyz.ctypes.data_as(dp), nprim.ctypes.data_as(ip),
+                          first.ctypes.data_as(ip), exps.ctypes.data_as(dp), coef.ctypes.data_as(dp),
+                          bf0.ctypes.data_as(ip), n, eri.ctypes.data_as(dp))
+    if rc != 0:
+        raise ValueError("md_eri: angular momentum above 2")
+    if spherical:
+        X = _sph_transform(ls)
+        S, T, V = (X.T @ M @ X for M in (S, T, V))
+        eri = np.einsum('pqrs,pi,qj,rk,sl->ijkl', eri, X, X, X, X, optimize=True)
+    enuc = 0.0
+    for (s1, x1), (s2, x2) in itertools.combinations(atoms, 2):
+        enuc += ZNUC[s1] * ZNUC[s2] / np.linalg.norm(x1 - x2)
+    return S, T, V, eri, enuc
+
+
+def water_hamiltonian(basis="cc-pvdz"):
+    """(F_mo, ERI_mo Dirac, E_scf, nocc, h_mo, eri_mo chemist, E_nuc) for pycc's "H2O" molecule."""
+    atoms = water_geometry()
+    S, T, V, eri, enuc = ao_integrals_shells(atoms, BASIS_SETS[basis.lower()])
+    nocc = 5
+    E, C, eps, F_ao = rhf(S, T + V, eri, nocc)
+    F_mo = C.T @ F_ao @ C
+    h_mo = C.T @ (T + V) @ C
+    eri_mo = np.einsum('pqrs,pi,qj,rk,sl->ijkl', eri, C, C, C, C, optimize=True)
+    return F_mo, np.ascontiguousarray(eri_mo.transpose(0, 2, 1, 3)), E + enuc, nocc, h_mo, eri_mo, enuc
 
 
 def rhf(S, Hcore, eri, nocc, conv=1e-13, maxiter=200):

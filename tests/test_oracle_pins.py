@@ -3,6 +3,10 @@
 * H2O/STO-3G frozen-core CCSD energy and Lambda pseudoenergy: pycc's ``test_002_ccsd_energy.py:31``
   and ``test_003_ccsd_lambda.py:49`` (psi4/CFOUR constants, 1e-11), reproduced through the
   psi4-free integral front end in ``oracle.gaussian``.
+* H2O/cc-pVDZ (nmo = 24, d shells as solid harmonics), frozen core (``test_002_ccsd_energy.py:38``,
+  ``test_003_ccsd_lambda.py:61``) and all electrons (``test_030_sp.py:30,39``): CCSD energy and
+  Lambda pseudoenergy at 1e-11. The C two-electron integrals behind it equal the pure-Python ones
+  (STO-3G, and random quartets over s, p and d shells).
 * every r1/r2 element at arbitrary amplitudes and a non-diagonal Fock: the exact similarity
   transform ``e^{-T} H e^{T}`` in the determinant space (``oracle.brute``).
 * o = 1: CCSD equals two-electron FCI.
@@ -52,6 +56,51 @@ def test_water_sto3g_lambda_matches_reference_constant(water):
     lecc = lam.solve_lambda(1e-12, 1e-12)
     ref, tol = reference_constant("water_sto3g_frozen_core", "lambda_pseudoenergy")
     assert abs(lecc - ref) < tol  # pycc tests/test_003_ccsd_lambda.py:49
+
+
+@pytest.fixture(scope="module")
+def water_dz():
+    from oracle.gaussian import water_hamiltonian
+    F, ERI, escf, nocc = water_hamiltonian("cc-pvdz")[:4]
+    return DenseHamiltonian(F, ERI), nocc
+
+
+@pytest.mark.parametrize("nfzc,key", [(1, "water_ccpvdz_frozen_core"), (0, "water_ccpvdz_all_electron")])
+def test_water_ccpvdz_matches_reference_constants(water_dz, nfzc, key):
+    H, nocc = water_dz
+    assert H.F.shape == (24, 24)
+    cc = OracleCCSD(H, no=nocc - nfzc, nfzc=nfzc)
+    e = cc.solve_cc(1e-12, 1e-12, 100)
+    ref, tol = reference_constant(key, "ccsd_correlation_energy")
+    assert abs(e - ref) < tol, (e, ref)      # test_002_ccsd_energy.py:38 / test_030_sp.py:30
+    lecc = OracleLambda(cc, OracleHBAR(cc)).solve_lambda(1e-12, 1e-12)
+    ref, tol = reference_constant(key, "lambda_pseudoenergy")
+    assert abs(lecc - ref) < tol, (lecc, ref)  # test_003_ccsd_lambda.py:61 / test_030_sp.py:39
+
+
+def test_c_eri_equals_python_eri():
+    """md_eri.c against the pure-Python McMurchie-Davidson primitives: the whole STO-3G ERI tensor, and
+    random quartets of a basis with s, p and two-primitive d shells on all three centres."""
+    from oracle import gaussian as g
+    atoms = g.water_geometry()
+    eri_py = g.ao_integrals(atoms)[3]
+    eri_c = g.ao_integrals_shells(atoms, g.STO3G, spherical=False)[3]
+    assert np.abs(eri_c - eri_py).max() < 1e-14
+    basis = {"O": [(2, [1.185, 0.4], [0.7, 0.4]), (1, [1.046], [1.0]), (0, [5.0], [1.0])],
+             "H": [(0, [0.4446], [1.0]), (2, [0.727], [1.0])]}
+    eri = g.ao_integrals_shells(atoms, basis, spherical=False)[3]
+    funcs = []
+    for sym, xyz in atoms:
+        for l, exps, coefs in basis[sym]:
+            e, c = g._shell_coefs(l, exps, coefs)
+            funcs += [g._ShellFunction(np.asarray(xyz, float), lmn, e, c) for lmn in g._cart_comps(l)]
+    rng = np.random.default_rng(0)
+    for i, j, k, l in rng.integers(0, len(funcs), (150, 4)):
+        f, h, u, w = funcs[i], funcs[j], funcs[k], funcs[l]
+        val = sum(ca * cb * cc * cd * g._eri_prim(a, f.lmn, f.A, b, h.lmn, h.A, c, u.lmn, u.A, d, w.lmn, w.A)
+                  for ca, a in zip(f.c, f.exps) for cb, b in zip(h.c, h.exps)
+                  for cc, c in zip(u.c, u.exps) for cd, d in zip(w.c, w.exps))
+        assert abs(val - eri[i, j, k, l]) < 1e-14, (i, j, k, l)
 
 
 @pytest.mark.parametrize("no,nv,seed", [(2, 2, 1), (2, 3, 2), (1, 4, 3)])
