@@ -140,6 +140,30 @@ def test_solve_cc_dp_matches_oracle(pycc_amd, no, nv, seed):
     assert np.abs(cc.t2.cpu().numpy() - ref.t2).max() < 1e-8
 
 
+def test_solve_cc_dp_matches_oracle_o16_v128(pycc_amd):
+    """The FP64 benchmark configuration o=16, v=128 (nmo = 144, seeded synthetic DF Hamiltonian with
+    max |MP2 t2| = 0.03): the GPU solve (from the DF factors) against the oracle on the same dense
+    integrals, to 1e-10 Eh in the energy and 1e-8 max-abs in T1 and T2, in the same number of
+    iterations. The oracle needs ~30-40 s of host time here; it prints each iteration."""
+    no, nv = 16, 128
+    print("o=16 v=128 FP64 solve vs oracle", flush=True)
+    sd = synthetic_df(no, nv, seed=0)
+    cc = pycc_amd.CCwfn(sd, no=no, precision='DP', verbose=False)
+    e = cc.solve_cc(1e-10, 1e-10, 100, 8, 1)
+    print("GPU: E %.14f in %d iterations" % (e, cc.niter), flush=True)
+    H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
+    ref = OracleCCSD(H, no)
+    e_ref = ref.solve_cc(1e-10, 1e-10, 100, 8, 1, verbose=True)
+    assert e is not None and e_ref is not None
+    d1 = np.abs(cc.t1.cpu().numpy() - ref.t1).max()
+    d2 = np.abs(cc.t2.cpu().numpy() - ref.t2).max()
+    print("E %.14f oracle %.14f, iterations %d / %d, max|dT1| %.2e max|dT2| %.2e" % (
+        e, e_ref, cc.niter, ref.niter, d1, d2))
+    assert abs(e - e_ref) < 1e-10
+    assert cc.niter == ref.niter
+    assert d1 < 1e-8 and d2 < 1e-8
+
+
 def test_water_sto3g_reference_constant_on_gpu(pycc_amd):
     from oracle.gaussian import water_sto3g_hamiltonian
     F, ERI, escf, nocc = water_sto3g_hamiltonian()
