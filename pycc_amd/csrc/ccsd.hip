@@ -401,7 +401,8 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   }
 
   // ---- ring operands and the W intermediates: ccwfn.py:641-645 (Wmbej), :680-683 (Wmbje) ----
-  // Wd[0] = Wmbej and Wd[1] = Wmbje, both stored [m][e][j][b].
+  // W[(m,e)][x][(j,b)]: x = 0 Wmbej, x = 1 Wmbje (built in FP64 Wd and cast, or in MIXED_F16X3
+  // written in FP32 by one kernel).
   T* S1 = (T*)c.S1;
   T* A1 = (T*)c.A1;
   T* S2 = (T*)c.S2;
