@@ -47,7 +47,8 @@ struct Ctx {
   f16 *CvAh = nullptr, *CvAl = nullptr, *CvTh = nullptr, *CvTl = nullptr;
   float* xs = nullptr;               // operand scales: [0] ring amplitudes, [1] J, [2] K, [3] W,
                                      // [4] packed tau, [5] tau, [6] <mn|ef>, [7] Wmnij,
-                                     // [8] Covvv, [9] t1, [10] Ud, [11] tau~, [12] L
+                                     // [8] Covvv, [9] t1, [10] Ud, [11] tau~, [12] L,
+                                     // [13] t2, [14] off-diagonal G (:925)
   float* xpart = nullptr;            // absmax partials (2 x x3_partials())
 
   // ---- per-residual work buffers (allocated on first use) ----
@@ -67,6 +68,10 @@ struct Ctx {
   // FP16x3 <ov|vv> terms: split t1 [o][v32], Ud as [i][(m,e)][v32]; PT[j][(m,x,y)] =
   // sum_z t_jz Covvv[m][x][y][z], P2T[j][(m,b,e)] = sum_f t_jf (mf|be)
   f16 *T1h = nullptr, *T1l = nullptr, *UTh = nullptr, *UTl = nullptr;
+  // FP16x3 :925 operands (interleaved, lo = hi + 32): t2 as [(i,j,a)][e], off-diagonal G as [b][e];
+  // G's off-diagonal part and diagonal in FP64
+  f16 *T2h = nullptr, *T2l = nullptr, *GPh = nullptr, *GPl = nullptr;
+  double *Gp = nullptr, *Gd = nullptr;
   float *PT = nullptr, *P2T = nullptr;
   // W0 = the integral part of [Wmbej | Wmbje] ((me|jb), -(mj|be)) in the W layout; split tau~ and L
   // as [(i,j,a)][f padded] for :497 / :533

@@ -262,6 +262,12 @@ void alloc_work(Ctx& c) {
     ialloc(&c.APh, &c.APl, 2 * ov * ovp);
     ialloc(&c.AQh, &c.AQl, ov * ovp);
     ialloc(&c.BPh, &c.BPl, 2 * ov * ovp);
+    if (hh_g_x3_fits(o, v)) {   // :925 as an FP16x3 GEMM
+      ialloc(&c.T2h, &c.T2l, o * o * v * c.v32());
+      ialloc(&c.GPh, &c.GPl, v * c.v32());
+      c.Gp = c.alloc<double>(v * v);
+      c.Gd = c.alloc<double>(v);
+    }
     halloc(&c.TAh, o * o * c.v2p());
     halloc(&c.TAl, o * o * c.v2p());
     halloc(&c.TTh, v * v * c.o2p());
@@ -524,12 +530,16 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
 
   // ---- half residual Hh (before P(ij,ab)): ccwfn.py:922-940 ------------------------------------
   const TV Hh = tv(c.Hh, "ijab", OOVV);
-  axpby<double>(0.5, tv(c.Ioovv, "ijab", OOVV), 0.0, Hh, s);        // 1/2 <ab|ij> = 1/2 <ij|ab> :922
   // :923-925  t2_ijae (Fae_be - 1/2 t_mb Fme_me)
   const TV G = tv(c.G, "be", VV);
   axpby<double>(1.0, tv(c.Fae, "be", VV), 0.0, G, s);
   cd(-0.5, T1("mb"), Fme, 1.0, G);
-  cd(1.0, T2("ijae"), G, 1.0, Hh);
+  if (x3 && c.T2h) {
+    hh_g_x3(c, t2, c.Hh, s);   // :922 and :925 in one FP16x3 GEMM (G's diagonal in FP64)
+  } else {
+    axpby<double>(0.5, tv(c.Ioovv, "ijab", OOVV), 0.0, Hh, s);      // 1/2 <ab|ij> = 1/2 <ij|ab> :922
+    cd(1.0, T2("ijae"), G, 1.0, Hh);
+  }
   // :926-928  -t2_imab (Fmi_mj + 1/2 t_je Fme_me)
   const TV Hm = tv(c.Hm, "mj", OO);
   axpby<double>(1.0, tv(c.Fmi, "mj", OO), 0.0, Hm, s);

@@ -25,6 +25,11 @@ struct X3Gemm {
   // tiles (needs ws) the split-K partials (FP32, each over a short K range) are summed in FP64; on
   // larger grids the kernel adds its FP32 sum into C64 directly.
   double* C64 = nullptr;
+  // with C64 written straight from the kernel (full grids only): beta scales C64in instead of C64
+  // when given, and D64[m ldc + n] dvec[n] is added in FP64 (a diagonal term kept out of the split)
+  const double* C64in = nullptr;
+  const double* D64 = nullptr;
+  const double* dvec = nullptr;
   // split-K (set by x3_gemm): grid.y = splits K ranges of kchunk, raw partials in work[s][M][N]
   int64_t kchunk = 0;
   float* work = nullptr;
@@ -80,6 +85,11 @@ void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s);              // + 2 t_mf 
 void w_build_x3(Ctx& c, const double* t1, float* W, hipStream_t s);
 void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s);                // :939
 void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s);   // :759
+// Hh = 1/2 <ij|ab> + t2_ijae G_be (:922, :925): the off-diagonal G as one FP16x3 GEMM, the diagonal
+// t2_ijab G_bb (orbital-energy sized) and 1/2 <ij|ab> added in FP64 by its epilogue. Needs a grid
+// the direct FP64 epilogue takes (hh_g_x3_fits).
+bool hh_g_x3_fits(int64_t o, int64_t v);
+void hh_g_x3(Ctx& c, const double* t2, double* Hh, hipStream_t s);
 // The split ring operands BJ, BK, AP, AQ (K = ovp, or 2 ovp for AP; interleaved, lo = hi + 32) and Ud
 // from t1, t2.
 void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,

@@ -189,10 +189,11 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
         const int64_t m = tm * BT + wm * WT + i * 16 + fr * 4 + r;
         if (m < p.M) {
           if (p.C64) {   // FP64 destination without split K
-            double* d = p.C64 + m * p.ldc + n;
+            const int64_t at = m * p.ldc + n;
             double val = (double)(f * acc[i][j][r]);
-            if (p.beta != 0.f) val += (double)p.beta * *d;
-            *d = val;
+            if (p.beta != 0.f) val += (double)p.beta * (p.C64in ? p.C64in[at] : p.C64[at]);
+            if (p.D64) val += p.D64[at] * p.dvec[n];
+            p.C64[at] = val;
           } else {
             float* d = p.C + m * p.ldc + n;
             float val = f * acc[i][j][r];
@@ -350,10 +351,11 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
         const int64_t m = tm * BT + wm * WTM + i * 16 + fr * 4 + r;
         if (m < p.M) {
           if (p.C64) {   // FP64 destination without split K
-            double* d = p.C64 + m * p.ldc + n;
+            const int64_t at = m * p.ldc + n;
             double val = (double)(f * acc[i][j][r]);
-            if (p.beta != 0.f) val += (double)p.beta * *d;
-            *d = val;
+            if (p.beta != 0.f) val += (double)p.beta * (p.C64in ? p.C64in[at] : p.C64[at]);
+            if (p.D64) val += p.D64[at] * p.dvec[n];
+            p.C64[at] = val;
           } else {
             float* d = p.C + m * p.ldc + n;
             float val = f * acc[i][j][r];
@@ -704,6 +706,8 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
   // An FP64 destination is written by the kernel directly when the tile grid fills the chip, else
   // through split-K partials summed in FP64 (which need a workspace).
   CCX_CHECK(!p.C64 || ws || tiles >= 512, ERR_INTERNAL, "an FP64 destination on a small grid needs a workspace");
+  CCX_CHECK(!(p.C64in || p.D64) || (p.C64 && tiles >= 512 && (!p.D64 || p.dvec)), ERR_INTERNAL,
+            "FP64 epilogue inputs need the direct FP64 destination");
   if (ws && tiles < 512) {
     splits = std::min<int64_t>({1024, ceil_div(1024, tiles), p.Kp / (32 * BK)});
     splits = std::max<int64_t>(splits, 1);
@@ -1053,6 +1057,40 @@ void x3_amp_scales(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
   x3_scale(p1, 1.f, nullptr, 0.f, c.xs + 9, s);   // t1
   x3_scale(p2, 3.f, nullptr, 0.f, c.xs + 10, s);  // Ud = 2 t2 - t2^T
   x3_scale(p2, 1.f, p1, 0.5f, c.xs + 11, s);   // tau~ = t2 + 1/2 t1 t1
+  x3_scale(p2, 1.f, nullptr, 0.f, c.xs + 13, s);  // t2
+}
+
+namespace {
+// Gp = G with a zero diagonal, Gd = diag(G) (G is v x v)
+__global__ __launch_bounds__(256) void g_offdiag_kernel(const double* __restrict__ G, int64_t v,
+                                                        double* __restrict__ Gp, double* __restrict__ Gd) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < v * v; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / v, f = e - b * v;
+    Gp[e] = b == f ? 0.0 : G[e];
+    if (b == f) Gd[b] = G[e];
+  }
+}
+}  // namespace
+
+bool hh_g_x3_fits(int64_t o, int64_t v) {
+  return o * o * v >= 256 && ceil_div(o * o * v, 256) * ceil_div(v, 256) >= 512;
+}
+
+void hh_g_x3(Ctx& c, const double* t2, double* Hh, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, v32 = c.v32();
+  CCX_CHECK(hh_g_x3_fits(o, v) && c.T2h && c.GPh, ERR_INTERNAL, "hh_g_x3 on a grid it does not fit");
+  g_offdiag_kernel<<<grid_e(v * v), 256, 0, s>>>(c.G, v, c.Gp, c.Gd);
+  CCX_HIP(hipGetLastError());
+  x3_absmax<double>(c.Gp, v * v, c.xpart, s);
+  x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 14, s);
+  x3_split_rows<double>(c.Gp, v, v, v, c.xs + 14, c.GPh, c.GPl, v32, s, true);
+  x3_split_rows<double>(t2, o * o * v, v, v, c.xs + 13, c.T2h, c.T2l, v32, s, true);
+  X3Gemm g;   // Hh[(ija)][b] = 1/2 <ij|ab> + sum_(e != b) t2[(ija)][e] G[b][e] + t2[(ija)][b] G[b][b]
+  g.M = o * o * v; g.N = v; g.Kp = v32; g.lda = 2 * v32; g.ldb = 2 * v32; g.ldc = v;
+  g.kb = 32; g.ska = 64; g.skb = 64;
+  g.Ah = c.T2h; g.Al = c.T2l; g.Bh = c.GPh; g.Bl = c.GPl; g.sa = c.xs + 13; g.sb = c.xs + 14;
+  g.C64 = Hh; g.C64in = c.Ioovv; g.beta = 0.5f; g.D64 = t2; g.dvec = c.Gd;
+  x3_gemm<_Float16>(g, s);
 }
 
 void fae_tauL_x3(Ctx& c, double* Fae, hipStream_t s) {
