@@ -223,37 +223,48 @@ __device__ __forceinline__ void x3_dma16(const E* g, const E* lds) {
                :: "s"(__builtin_amdgcn_readfirstlane(base)), "v"(g) : "memory");
 }
 
-template <typename E>
+// BN = 256 or 320 output columns per tile (320: 128 x 80 per wave, 144 KiB of LDS; B's 20 row
+// blocks of 16 go to wave q % 8, so waves 0-3 load three and 4-7 two).
+template <typename E, int BN>
 __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   typedef typename X3<E>::v8 v8;
-  constexpr int BT = 256, WTM = 128, WTN = 64, NBM = WTM / 16, NBN = WTN / 16, IPW = 2;
+  constexpr int BT = 256, WTM = 128, WTN = BN / 4, NBM = WTM / 16, NBN = WTN / 16, IPW = 2;
+  constexpr int NIB = BN / 16, IPB = (NIB + 7) / 8;   // B row blocks, per wave (rounded up)
   __shared__ __attribute__((aligned(1024))) E sA[2][2][BT * 32];   // [stage][hi/lo]
-  __shared__ __attribute__((aligned(1024))) E sB[2][2][BT * 32];
+  __shared__ __attribute__((aligned(1024))) E sB[2][2][BN * 32];
   const E* __restrict__ Ah = (const E*)p.Ah;
   const E* __restrict__ Al = (const E*)p.Al;
   const E* __restrict__ Bh = (const E*)p.Bh;
   const E* __restrict__ Bl = (const E*)p.Bl;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 2, wn = wave & 3;
-  const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BT - 1) / BT;
+  const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BN - 1) / BN;
   const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
   const int64_t group = t / (GM * tiles_n), first_m = group * GM;
   const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
   const int64_t within = t - group * GM * tiles_n;
   const int64_t tm = first_m + within % gsize, tn = within / gsize;
 
-  int64_t gA[IPW], gB[IPW];
-  int ldso[IPW];
+  int64_t gA[IPW], gB[IPB];
+  int ldso[IPW], ldsb[IPB];
 #pragma unroll
   for (int i = 0; i < IPW; ++i) {
     const int R0 = (wave * IPW + i) * 16;
     const int row = R0 + (lane >> 2), c = (lane & 3) ^ ((row >> 1) & 3);
-    int64_t m = tm * BT + row, n = tn * BT + row;
+    int64_t m = tm * BT + row;
     m = m < p.M ? m : p.M - 1;
-    n = n < p.N ? n : p.N - 1;
     gA[i] = m * p.lda + c * 8;
-    gB[i] = n * p.ldb + c * 8;
     ldso[i] = R0 * 32;
+  }
+#pragma unroll
+  for (int i = 0; i < IPB; ++i) {
+    const int q = wave + 8 * i;
+    const int R0 = (q < NIB ? q : 0) * 16;   // blocks past NIB are not loaded
+    const int row = R0 + (lane >> 2), c = (lane & 3) ^ ((row >> 1) & 3);
+    int64_t n = tn * BN + row;
+    n = n < p.N ? n : p.N - 1;
+    gB[i] = n * p.ldb + c * 8;
+    ldsb[i] = R0 * 32;
   }
   auto issue = [&](int st, int64_t k0) {
     int64_t ka = k0, kb = k0;
@@ -269,8 +280,13 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
     for (int i = 0; i < IPW; ++i) {
       x3_dma16<E>(Ah + gA[i] + ka, &sA[st][0][ldso[i]]);
       x3_dma16<E>(Al + gA[i] + ka, &sA[st][1][ldso[i]]);
-      x3_dma16<E>(Bh + gB[i] + kb, &sB[st][0][ldso[i]]);
-      x3_dma16<E>(Bl + gB[i] + kb, &sB[st][1][ldso[i]]);
+    }
+#pragma unroll
+    for (int i = 0; i < IPB; ++i) {
+      if (wave + 8 * i < NIB) {   // wave-uniform
+        x3_dma16<E>(Bh + gB[i] + kb, &sB[st][0][ldsb[i]]);
+        x3_dma16<E>(Bl + gB[i] + kb, &sB[st][1][ldsb[i]]);
+      }
     }
   };
   const int fr = lane >> 4, fc = lane & 15;
@@ -324,7 +340,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
     float* Wb = p.work + (int64_t)blockIdx.y * p.M * p.N;
 #pragma unroll
     for (int j = 0; j < NBN; ++j) {
-      const int64_t n = tn * BT + wn * WTN + j * 16 + fc;
+      const int64_t n = tn * BN + wn * WTN + j * 16 + fc;
       if (n >= p.N) continue;
 #pragma unroll
       for (int i = 0; i < NBM; ++i)
@@ -341,7 +357,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   if (p.sb) f0 = f0 / p.sb[0];
 #pragma unroll
   for (int j = 0; j < NBN; ++j) {
-    const int64_t n = tn * BT + wn * WTN + j * 16 + fc;
+    const int64_t n = tn * BN + wn * WTN + j * 16 + fc;
     if (n >= p.N) continue;
     const float f = p.sbn ? f0 / p.sbn[n] : f0;
 #pragma unroll
@@ -697,8 +713,9 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
     return;
   }
   const bool big = p.M >= 256;
-  const int64_t bt = big ? 256 : 128;
-  const int64_t tiles = ceil_div(p.M, bt) * ceil_div(p.N, bt);
+  const bool wide = big && p.N >= 8192;   // 320-column tiles: fewer operand panel fetches per output
+  const int64_t bt = big ? 256 : 128, bn = wide ? 320 : bt;
+  const int64_t tiles = ceil_div(p.M, bt) * ceil_div(p.N, bn);
   CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
   // split K into workspace partials when the tile grid cannot fill 256 CUs (keep >= 32 k-tiles each)
   int64_t splits = 1;
@@ -718,7 +735,8 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
     }
   }
   dim3 grid((unsigned)tiles, (unsigned)splits);
-  if (big) x3_gemm_dma_kernel<E><<<grid, 512, 0, s>>>(p);
+  if (wide) x3_gemm_dma_kernel<E, 320><<<grid, 512, 0, s>>>(p);
+  else if (big) x3_gemm_dma_kernel<E, 256><<<grid, 512, 0, s>>>(p);
   else x3_gemm_kernel<E, 128><<<grid, 256, 0, s>>>(p);
   CCX_HIP(hipGetLastError());
   if (p.work) {

@@ -4,7 +4,8 @@ FP64 products of the same FP32 inputs.
 Each operand is carried as hi + lo FP16 with a power-of-two scale (22 significant bits) and
 accumulated in FP32, so the bar is set relative to |A| |B|: max |C - C_ref| <= 1e-6 max (|A| |B|),
 about four FP32 roundings of the largest term. Shapes cover both kernels (M < 256: 128 x 128
-tiles with register staging, down to M = 1; M >= 256: 256 x 256 tiles fed by LDS DMA),
+tiles with register staging, down to M = 1; M >= 256: 256 x 256 tiles fed by LDS DMA, 256 x 320
+from N = 8192, with ragged last tiles),
 split K (few tiles, long K),
 ragged M / N edges, K not a multiple of 32, transposed operands, alpha / beta and operands many
 decades apart in magnitude.
@@ -43,7 +44,8 @@ def _check(mgr, A, B, alpha=1.0, beta=0.0, C0=None, ta=False, tb=False):
 
 @pytest.mark.parametrize("M,N,K", [(100, 70, 33), (300, 517, 1000), (257, 1030, 2048), (512, 256, 64),
                                    (300, 260, 50000), (100, 90, 40000), (48, 3000, 384), (64, 70, 33),
-                                   (1, 5, 40), (48, 384, 100000)])
+                                   (1, 5, 40), (48, 384, 100000), (300, 9000, 200), (512, 8200, 64),
+                                   (260, 16500, 1000)])
 def test_split_gemm_shapes(mgr, M, N, K):
     print("f16x3 gemm", M, N, K, flush=True)
     rng = np.random.default_rng(M + N + K)
