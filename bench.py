@@ -277,7 +277,7 @@ def main():
     ring_prec = args.precision if args.precision in ("DP", "MIXED_F16X3") else "MIXED"
     achieved = fl_r / (ms_r * 1e-3) if ms_r > 0 else 0.0
     ring_kernel = {"DP": "plain_gemm_kernel (ring GEMMs, f64)", "MIXED": "plain_gemm_kernel (ring GEMMs, f32)",
-                   "MIXED_F16X3": "x3_gemm_dma_kernel<_Float16> (ring GEMMs, f16x3)"}[ring_prec]
+                   "MIXED_F16X3": "x3_gemm_dma_kernel<_Float16, 320> (ring GEMMs, f16x3)"}[ring_prec]
     res["roofline"] = {"kernel": ring_kernel,
                        "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[ring_prec] / 1e12,
                        "unit": "TFLOP/s", "frac": achieved / PEAK[ring_prec], "traffic": traffic,
