@@ -11,6 +11,7 @@
  *   pycc/ccwfn.py:1122-1162  CCwfn.cc_energy(o, v, F, L, t1, t2)                 -> ccx_energy
  *   pycc/ccwfn.py:281-284    Jacobi update t += r/D and the 2-norm "rms"         -> ccx_update
  *   pycc/utils.py:257-361    helper_diis (__init__, add_error_vector, extrapolate) -> ccx_diis_*
+ *   pycc/cctriples.py:177-240  t_tjl(ccwfn), the (T) correction                   -> ccx_triples
  *
  * Conventions
  *   - All amplitude/residual/integral pointers are DEVICE pointers (FP64, C-contiguous):
@@ -154,6 +155,14 @@ int ccx_densities(ccx_ctx* ctx, const double* t1, const double* t2, const double
 int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const double* l2, const double* fock,
                        const double* D, const double* Doooo, const double* Dooov, const double* Dovov,
                        const double* Doovv, const double* Dvvvo, double* e_dev, void* stream);
+
+/* *e_dev = the (T) correction of closed-shell CCSD(T) at (fock, t1, t2), FP64 in every precision
+ * mode: pycc/cctriples.py:177-240 (t_tjl; the same energy as t_vikings :243 and t_vikings_inverted
+ * :311), with the connected and disconnected T3 numerators of cctriples.py:27-72 and :108-146 and
+ * the orbital-energy denominators from the diagonal of `fock`. Synchronises `stream`; uses v^3
+ * doubles of device scratch for the duration of the call. */
+int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
+                void* stream);
 
 /* Measurement hooks (no counterpart in the reference, which has only wall-clock prints,
  * pycc/utils.py:229-238): HIP events recorded on the residual's stream around the ladder GEMMs,

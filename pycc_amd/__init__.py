@@ -7,6 +7,7 @@ Public API (names as exported by ``pycc/__init__.py``):
   ``pseudoenergy``); ``autograd.residual`` -- the residual as a ``torch.autograd.Function`` whose
   backward is the Lambda-equation vector-Jacobian product; ``ccdensity`` -- the one- and
   two-particle densities (the integral half of the vJp);
+* ``cctriples`` -- the (T) correction (``t_tjl``, ``t_vikings``, ``t_vikings_inverted``);
 * ``DeviceManager``, ``ContractionBackend`` -- device/precision policy and the GPU einsum;
 * ``integrals`` -- FCIDUMP / ``.npz`` integral files and ``MOHamiltonian`` (the psi4-free input);
 * ``PyCCError``, ``PyCCWarning``, ``InvalidKeywordError``, ``DeviceError``.
@@ -19,8 +20,9 @@ from .ccwfn import CCwfn, ccwfn
 from .cchbar import cchbar
 from .cclambda import cclambda
 from .ccdensity import ccdensity
+from . import cctriples
 from . import synth
 from . import integrals
 
 __all__ = ["CCwfn", "ccwfn", "cchbar", "cclambda", "ccdensity", "DeviceManager", "ContractionBackend", "PyCCError", "PyCCWarning",
-           "InvalidKeywordError", "DeviceError", "synth", "integrals"]
+           "InvalidKeywordError", "DeviceError", "cctriples", "synth", "integrals"]

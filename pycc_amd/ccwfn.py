@@ -5,6 +5,8 @@ start_diis)`` reproduces ``pycc/ccwfn.py:216-319`` for the spin-adapted CCSD mod
 Jacobi update ``t += r/D``, the 2-norm "rms", convergence when both ``|dE| < e_conv`` and
 ``rms < r_conv``, DIIS after the convergence test, and ``None`` when ``maxiter`` is exhausted.
 ``residuals(F, t1, t2)`` and ``cc_energy(o, v, F, L, t1, t2)`` keep the reference signatures.
+With ``model='CCSD(T)'`` the converged energy includes the (T) correction (``ccwfn.py:298-310``,
+``triples_energy`` / ``pycc_amd.cctriples.t_tjl``).
 
 The Hamiltonian ``H`` is either
 
@@ -68,7 +70,7 @@ class _Ctx:
 class CCwfn(object):
     """Spin-adapted closed-shell CCSD on one AMD GPU (see the module docstring)."""
 
-    VALID_MODELS = ['CCSD']
+    VALID_MODELS = ['CCSD', 'CCSD(T)']
 
     def __init__(self, H, no=None, nfzc=0, model='CCSD', device='GPU', precision='DP',
                  verbose=True, **kwargs):
@@ -211,10 +213,20 @@ class CCwfn(object):
         name = "T-amplitudes (%s)" % self.model
         for step in self.iterate(e_conv, r_conv, maxiter, max_diis, start_diis):
             if step["converged"]:
+                ecc = step["ecc"]
                 if self.verbose:
                     print("%s converged in %.3f seconds." % (name, time.time() - t0))
-                    print("E(%s) = %20.15f" % (self.model, step["ecc"]))
-                return step["ecc"]
+                if self.model == 'CCSD(T)':                                 # ccwfn.py:298-310
+                    et = self.triples_energy()
+                    self.et = et
+                    if self.verbose:
+                        print("E(CCSD) = %20.15f" % ecc)
+                        print("E(T)    = %20.15f" % et)
+                    ecc = ecc + et
+                    self.ecc = ecc
+                elif self.verbose:
+                    print("E(%s) = %20.15f" % (self.model, ecc))
+                return ecc
         return None
 
     def iterate(self, e_conv=1e-7, r_conv=1e-7, maxiter=100, max_diis=8, start_diis=1):
@@ -379,6 +391,16 @@ class CCwfn(object):
         """1/2 <ij|ab> l2_ijab (cclambda.py:570)."""
         l2 = self._as_dev(l2, (self.no, self.no, self.nv, self.nv))
         _lib.call("ccx_pseudoenergy", self._ctx.h, dptr(l2), dptr(self._e_dev[:1]), self._stream())
+        return float(self._e_dev[0].item())
+
+    def triples_energy(self, t1=None, t2=None, F=None):
+        """The (T) correction E(T) at (F, t1, t2), FP64 on the GPU in every precision mode
+        (``pycc/cctriples.py:177-240``, ``t_tjl``)."""
+        F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
+        t1 = self.t1 if t1 is None else self._as_dev(t1, (self.no, self.nv))
+        t2 = self.t2 if t2 is None else self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
+        _lib.call("ccx_triples", self._ctx.h, dptr(F), dptr(t1), dptr(t2), dptr(self._e_dev[:1]),
+                  self._stream())
         return float(self._e_dev[0].item())
 
     # ---- measurement hooks -----------------------------------------------------------------

@@ -253,6 +253,14 @@ int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const d
   });
 }
 
+int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
+                void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::triples_energy(ctx->c, fock, t1, t2, e_dev, st(stream));
+  });
+}
+
 int ccx_timing_start(ccx_ctx* ctx) {
   return guarded([&] {
     CCX_CHECK(ctx, ccx::ERR_ARG, "null context");

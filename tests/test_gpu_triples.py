@@ -1,0 +1,90 @@
+"""GPU (T) correction (``ccx_triples``, ``pycc/cctriples.py:177-240``) against the oracle
+(``oracle/triples_ref.py``, pinned by the reference's water (T) energies) and against those energies
+themselves.
+
+Bars: the reference's water (T) energies (``pycc/tests/test_005_ccsd_t_energy.py:33,44``) to 1e-11
+Eh through a GPU CCSD(T) solve; the oracle's E(T) at arbitrary amplitudes and a non-diagonal Fock to
+1e-12 relative, for dense and density-fitted contexts and odd and even sizes; E(T) from a mixed
+precision context equals the FP64 context's (the (T) path is FP64 in every mode)."""
+import numpy as np
+import pytest
+
+from conftest import random_amplitudes, reference_constant
+from oracle.triples_ref import t_tjl
+from pycc_amd.synth import synthetic_df, DenseHamiltonian, dirac_eri
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _ham(no, nv, seed, offdiag=0.05):
+    sd = synthetic_df(no, nv, seed=seed, t2max=0.1)
+    H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
+    X = np.random.default_rng(seed + 100).standard_normal(H.F.shape) * offdiag
+    H.F = H.F + X + X.T
+    H.eps = np.diagonal(H.F).copy()
+    return sd, H
+
+
+@pytest.mark.parametrize("basis,key", [("sto-3g", "water_sto3g_frozen_core"),
+                                       ("cc-pvdz", "water_ccpvdz_frozen_core")])
+def test_water_ccsd_t_matches_reference_constants(tmp_path, basis, key):
+    _need_gpu()
+    import pycc_amd
+    from pycc_amd.integrals import MOHamiltonian, write_fcidump
+    if basis == "sto-3g":
+        from oracle.gaussian import ao_integrals, rhf, water_geometry
+        S, T, V, eri, enuc = ao_integrals(water_geometry())
+        _, C, _, _ = rhf(S, T + V, eri, 5)
+        h_mo = C.T @ (T + V) @ C
+        eri_mo = np.einsum('pqrs,pi,qj,rk,sl->ijkl', eri, C, C, C, C, optimize=True)
+        nocc = 5
+    else:
+        from oracle.gaussian import water_hamiltonian
+        _, _, _, nocc, h_mo, eri_mo, enuc = water_hamiltonian("cc-pvdz")
+    path = tmp_path / "FCIDUMP"
+    write_fcidump(path, h_mo, eri_mo, nelec=2 * nocc, ecore=enuc)
+    print("water", basis, "CCSD(T) from FCIDUMP", flush=True)
+    cc = pycc_amd.CCwfn(MOHamiltonian.from_fcidump(path), nfzc=1, model='CCSD(T)', precision='DP',
+                        verbose=False)
+    e = cc.solve_cc(1e-12, 1e-12, 100)
+    eccsd, tol = reference_constant(key, "ccsd_correlation_energy")
+    et_ref, tol_t = reference_constant(key, "triples_correction")
+    assert abs(cc.et - et_ref) < tol_t, (cc.et, et_ref)
+    assert abs(e - (eccsd + et_ref)) < tol + tol_t, (e, eccsd + et_ref)
+    assert pycc_amd.cctriples.t_tjl(cc) == cc.et
+
+
+@pytest.mark.parametrize("no,nv,seed", [(1, 5, 1), (2, 3, 2), (3, 17, 3), (5, 24, 4), (6, 33, 5)])
+def test_triples_matches_oracle(no, nv, seed):
+    _need_gpu()
+    import pycc_amd
+    print("triples dense no=%d nv=%d seed=%d" % (no, nv, seed), flush=True)
+    _, H = _ham(no, nv, seed)
+    t1, t2 = random_amplitudes(no, nv, seed, scale=0.1)
+    cc = pycc_amd.CCwfn(H, no=no, precision='DP', verbose=False)
+    et = cc.triples_energy(t1, t2)
+    ref = t_tjl(H.F, H.ERI, t1, t2, no)
+    assert abs(et - ref) < 1e-12 * max(abs(ref), 1e-3), (et, ref)
+
+
+def test_triples_density_fitted_and_mixed_contexts():
+    _need_gpu()
+    import pycc_amd
+    no, nv, seed = 4, 21, 7
+    print("triples DF / mixed no=%d nv=%d" % (no, nv), flush=True)
+    sd = synthetic_df(no, nv, seed=seed, t2max=0.1)
+    H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
+    t1, t2 = random_amplitudes(no, nv, seed, scale=0.1)
+    ref = t_tjl(H.F, H.ERI, t1, t2, no)
+    df = pycc_amd.CCwfn(sd, no=no, precision='DP', verbose=False).triples_energy(t1, t2)
+    assert abs(df - ref) < 1e-12 * abs(ref), (df, ref)
+    dp = pycc_amd.CCwfn(H, no=no, precision='DP', verbose=False).triples_energy(t1, t2)
+    for precision in ("MIXED", "MIXED_F16X3"):
+        mx = pycc_amd.CCwfn(H, no=no, precision=precision, verbose=False).triples_energy(t1, t2)
+        assert abs(mx - dp) < 1e-14 * abs(dp), (precision, mx, dp)

@@ -1,0 +1,46 @@
+"""The (T) oracle (``oracle/triples_ref.py``, a restatement of ``pycc/cctriples.py:177-240``) pinned
+against the reference's own (T) energies for water, frozen core
+(``pycc/tests/test_005_ccsd_t_energy.py:33`` STO-3G, ``:44`` cc-pVDZ, 1e-11), and against the
+full-loop textbook form over all i, j, k (``t_full``) at arbitrary amplitudes and a non-diagonal Fock.
+A dropped permutation, a wrong weight or a swapped index in either form fails one of these."""
+import numpy as np
+import pytest
+
+from conftest import random_amplitudes, reference_constant
+from oracle.ccsd_ref import OracleCCSD
+from oracle.triples_ref import t_full, t_tjl
+from pycc_amd.synth import synthetic_df, DenseHamiltonian, dirac_eri
+
+
+@pytest.mark.parametrize("basis,key", [("sto-3g", "water_sto3g_frozen_core"),
+                                       ("cc-pvdz", "water_ccpvdz_frozen_core")])
+def test_water_triples_match_reference_constants(basis, key):
+    if basis == "sto-3g":
+        from oracle.gaussian import water_sto3g_hamiltonian
+        F, ERI, _, nocc = water_sto3g_hamiltonian()
+    else:
+        from oracle.gaussian import water_hamiltonian
+        F, ERI, _, nocc = water_hamiltonian("cc-pvdz")[:4]
+    H = DenseHamiltonian(F, ERI)
+    cc = OracleCCSD(H, no=nocc - 1, nfzc=1)
+    cc.solve_cc(1e-12, 1e-12, 100)
+    et = t_tjl(H.F, H.ERI, cc.t1, cc.t2, cc.no, nfzc=1)
+    ref, tol = reference_constant(key, "triples_correction")
+    assert abs(et - ref) < tol, (et, ref)
+    assert abs(t_full(H.F, H.ERI, cc.t1, cc.t2, cc.no, nfzc=1) - et) < 1e-14
+
+
+@pytest.mark.parametrize("no,nv,seed", [(1, 4, 1), (3, 5, 2), (4, 7, 3)])
+def test_triangular_sum_equals_full_loop(no, nv, seed):
+    sd = synthetic_df(no, nv, seed=seed, t2max=0.1)
+    H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
+    X = np.random.default_rng(seed + 100).standard_normal(H.F.shape) * 0.05
+    F = H.F + X + X.T                      # f_ov != 0 exercises the t2 f_kc terms of t3d
+    t1, t2 = random_amplitudes(no, nv, seed, scale=0.1)
+    a = t_tjl(F, H.ERI, t1, t2, no)
+    b = t_full(F, H.ERI, t1, t2, no)
+    assert abs(a - b) < 1e-13 * max(1.0, abs(b)), (a, b)
+    if no == 1:
+        assert abs(a) < 1e-15              # one electron pair: no triple excitations
+    else:
+        assert abs(a) > 1e-6               # not a vacuous zero
