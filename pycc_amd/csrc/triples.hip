@@ -1,10 +1,12 @@
 // (T) correction of closed-shell CCSD(T) in FP64 (pycc/cctriples.py:177-240, t_tjl).
 //
 // For every occupied triple i >= j >= k the connected numerator W[a][b][c] (cctriples.py:27-72,
-// t3c_ijk without the denominator) is accumulated into one v^3 buffer by twelve contractions: six
-// <ab|ei> t2 terms (GEMMs of v^2 x v x v) and six <mc|jk> t2 terms (v^2 x v x o), written straight
-// into W's [a][b][c] order by the strided contraction engine. A fused kernel then adds the
-// disconnected part (t3d_ijk, cctriples.py:108-146: <ij|ab> t1_kc + t2_ijab f_kc and their
+// t3c_ijk without the denominator) comes from three GEMMs and one tiled permute-and-add: for each p
+// in (i, j, k) the four terms that share p (two <ab|ep> t2 and two <mc|qr> t2 products) form one
+// GEMM of v^2 x 2v x (v + o), whose A operand [Covvv[p] | t2[p]^T] is built once per call and whose
+// B operand (t2 and -(mq|rc) columns of the two (q, r) orders side by side) is packed per triple;
+// the three v^2 x 2v results are then added into W's [a][b][c] order through 16^3 LDS tiles.
+// A tiled kernel then adds the disconnected part (t3d_ijk, cctriples.py:108-146: <ij|ab> t1_kc + t2_ijab f_kc and their
 // permutations) to form V and reduces
 //
 //   E_ijk = sum_abc V_abc (4 W_abc + W_bca + W_cab - 2 W_acb - 2 W_bac - 2 W_cba) / D_ijk^abc,
@@ -21,8 +23,6 @@
 namespace ccx {
 
 namespace {
-
-constexpr int T3_BLOCKS = 512;
 
 __device__ __forceinline__ double t3_block_sum(double x) {
   __shared__ double sh[256];
@@ -41,7 +41,104 @@ __global__ void t3_zero_kernel(double* e) {
   if (threadIdx.x == 0) e[0] = 0.0;
 }
 
-// partials[block] = sum over this block's abc of V (4W + W_bca + W_cab - 2(W_acb + W_bac + W_cba)) / D
+// ---- 16^3 tiles of v^3 arrays: output (a, b, c) in [org_a, org_a + 16) x ... ----------------------
+// A thread owns (b_l, c_l) = (tid / 16, tid % 16) and the 16 elements a_l = 0..15 of its column.
+constexpr int TT = 16, TP = TT + 1;   // tile edge; LDS row pitch (padded against bank conflicts)
+
+// acc[a_l] += coef * S[g0][g1][off + g2], where source dimension k is indexed by the output
+// dimension Dk (0 = a, 1 = b, 2 = c): the 16^3 box is staged through LDS with 128-byte rows read
+// along the source's contiguous dimension, then read back in the output's order.
+template <int D0, int D1, int D2>
+__device__ __forceinline__ void t3_gather(const double* __restrict__ S, int64_t s0, int64_t s1, int64_t off,
+                                          int64_t v, const int64_t* org, double (*sh)[TT][TP], double coef,
+                                          double* acc, int bl, int cl) {
+  __syncthreads();
+  for (int le = threadIdx.x; le < TT * TT * TT; le += 256) {
+    const int l2 = le % TT, l1 = (le / TT) % TT, l0 = le / (TT * TT);
+    const int64_t g0 = org[D0] + l0, g1 = org[D1] + l1, g2 = org[D2] + l2;
+    sh[l0][l1][l2] = (g0 < v && g1 < v && g2 < v) ? S[g0 * s0 + g1 * s1 + off + g2] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int al = 0; al < TT; ++al) {
+    const int L[3] = {al, bl, cl};
+    acc[al] += coef * sh[L[D0]][L[D1]][L[D2]];
+  }
+}
+
+__device__ __forceinline__ void t3_tile_origin(int64_t v, int64_t* org) {
+  const int64_t nt = (v + TT - 1) / TT;
+  const int64_t t = blockIdx.x;
+  org[0] = t / (nt * nt) * TT;
+  org[1] = t / nt % nt * TT;
+  org[2] = t % nt * TT;
+}
+
+// A[p][(x, y)][kk] = Covvv[p][x][y][kk] for kk < v, t2[p][kk - v][x][y] for v <= kk < v + o, 0 above:
+// the two t3c_ijk products that share p as one GEMM operand with K = v + o (padded to Kp).
+__global__ void t3_operand_a_kernel(int64_t o, int64_t v, int64_t Kp, const double* __restrict__ Covvv,
+                                    const double* __restrict__ t2, double* __restrict__ A) {
+  const int64_t total = o * v * v * Kp;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t kk = e % Kp, row = e / Kp;   // row = (p, x, y)
+    const int64_t xy = row % (v * v), p = row / (v * v);
+    double val = 0.0;
+    if (kk < v) val = Covvv[row * v + kk];
+    else if (kk < v + o) val = t2[(p * o + (kk - v)) * v * v + xy];
+    A[e] = val;
+  }
+}
+
+// B[s][z'][kk] = t2[r][q][z][kk] for kk < v and -(m q|r z) = -<m z|q r> with m = kk - v below v + o,
+// for the two (q, r) orders of slot s (z' < v: first order, z' >= v: second): slot 0 is p = i with
+// (j, k), (k, j); slot 1 is p = j with (i, k), (k, i); slot 2 is p = k with (i, j), (j, i).
+__global__ void t3_pack_kernel(int64_t o, int64_t v, int64_t Kp, int i, int j, int k,
+                               const double* __restrict__ t2, const double* __restrict__ Cooov,
+                               double* __restrict__ B) {
+  const int64_t total = 3 * 2 * v * Kp;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t kk = e % Kp, zz = e / Kp % (2 * v), sl = e / (2 * v * Kp);
+    const int h = zz >= v, z = (int)(zz - h * v);
+    const int q0 = sl == 0 ? j : i, r0 = sl == 2 ? j : k;   // the slot's first (q, r)
+    const int q = h ? r0 : q0, r = h ? q0 : r0;
+    double val = 0.0;
+    if (kk < v) val = t2[(((int64_t)r * o + q) * v + z) * v + kk];
+    else if (kk < v + o) val = -Cooov[(((kk - v) * o + q) * o + r) * v + z];
+    B[e] = val;
+  }
+}
+
+// W[a][b][c] = X_ijk[a,b,c] + X_ikj[a,c,b] + X_jik[b,a,c] + X_jki[b,c,a] + X_kij[c,a,b] + X_kji[c,b,a]
+// from the slot products X_s[x][y][z'] (v x v x 2v; z' >= v holds the slot's second order).
+__global__ __launch_bounds__(256) void t3_assemble_kernel(int64_t v, const double* __restrict__ X,
+                                                          double* __restrict__ W) {
+  __shared__ double sh[TT][TT][TP];
+  int64_t org[3];
+  t3_tile_origin(v, org);
+  const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
+  const int64_t s0 = 2 * v * v, s1 = 2 * v, xs = v * v * 2 * v;
+  const double *X0 = X, *X1 = X + xs, *X2 = X + 2 * xs;
+  double acc[TT];
+#pragma unroll
+  for (int al = 0; al < TT; ++al) acc[al] = 0.0;
+  t3_gather<0, 1, 2>(X0, s0, s1, 0, v, org, sh, 1.0, acc, bl, cl);   // X_ijk[a][b][c]
+  t3_gather<0, 2, 1>(X0, s0, s1, v, v, org, sh, 1.0, acc, bl, cl);   // X_ikj[a][c][b]
+  t3_gather<1, 0, 2>(X1, s0, s1, 0, v, org, sh, 1.0, acc, bl, cl);   // X_jik[b][a][c]
+  t3_gather<1, 2, 0>(X1, s0, s1, v, v, org, sh, 1.0, acc, bl, cl);   // X_jki[b][c][a]
+  t3_gather<2, 0, 1>(X2, s0, s1, 0, v, org, sh, 1.0, acc, bl, cl);   // X_kij[c][a][b]
+  t3_gather<2, 1, 0>(X2, s0, s1, v, v, org, sh, 1.0, acc, bl, cl);   // X_kji[c][b][a]
+  const int64_t b = org[1] + bl, c = org[2] + cl;
+  if (b < v && c < v)
+    for (int al = 0; al < TT; ++al) {
+      const int64_t a = org[0] + al;
+      if (a < v) W[(a * v + b) * v + c] = acc[al];
+    }
+}
+
+// partials[block] = sum over the block's tile of V_abc (4 W_abc + W_bca + W_cab - 2 (W_acb + W_bac +
+// W_cba)) / D_ijk^abc with V = W + the disconnected numerator of t3d_ijk
 __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, int i, int j, int k,
                                                         const double* __restrict__ W,
                                                         const double* __restrict__ F, int64_t nmo,
@@ -49,43 +146,53 @@ __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, in
                                                         const double* __restrict__ t2,
                                                         const double* __restrict__ Ioovv,
                                                         double* __restrict__ partials) {
-  const int64_t vv = v * v, total = vv * v;
-  const double dijk = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k];
-  const double* Iij = Ioovv + ((int64_t)i * o + j) * vv;
-  const double* Iik = Ioovv + ((int64_t)i * o + k) * vv;
-  const double* Ijk = Ioovv + ((int64_t)j * o + k) * vv;
-  const double* Tij = t2 + ((int64_t)i * o + j) * vv;
-  const double* Tik = t2 + ((int64_t)i * o + k) * vv;
-  const double* Tjk = t2 + ((int64_t)j * o + k) * vv;
-  const double* Fi = F + (int64_t)i * nmo + o;   // f_ia row
-  const double* Fj = F + (int64_t)j * nmo + o;
-  const double* Fk = F + (int64_t)k * nmo + o;
+  __shared__ double sh[TT][TT][TP];
+  int64_t org[3];
+  t3_tile_origin(v, org);
+  const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
+  double q[TT], w[TT];
+#pragma unroll
+  for (int al = 0; al < TT; ++al) q[al] = w[al] = 0.0;
+  const int64_t vv = v * v;
+  t3_gather<0, 1, 2>(W, vv, v, 0, v, org, sh, 1.0, w, bl, cl);    // W_abc
+  t3_gather<1, 2, 0>(W, vv, v, 0, v, org, sh, 1.0, q, bl, cl);    // W_bca = W[b][c][a]
+  t3_gather<2, 0, 1>(W, vv, v, 0, v, org, sh, 1.0, q, bl, cl);    // W_cab
+  t3_gather<0, 2, 1>(W, vv, v, 0, v, org, sh, -2.0, q, bl, cl);   // W_acb
+  t3_gather<1, 0, 2>(W, vv, v, 0, v, org, sh, -2.0, q, bl, cl);   // W_bac
+  t3_gather<2, 1, 0>(W, vv, v, 0, v, org, sh, -2.0, q, bl, cl);   // W_cba
+  const int64_t b = org[1] + bl, c = org[2] + cl;
   double s = 0.0;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = e % v, ab = e / v;
-    const int64_t b = ab % v, a = ab / v;
-    const double w_abc = W[e];
-    const double w_bca = W[(b * v + c) * v + a];
-    const double w_cab = W[(c * v + a) * v + b];
-    const double w_acb = W[(a * v + c) * v + b];
-    const double w_bac = W[(b * v + a) * v + c];
-    const double w_cba = W[(c * v + b) * v + a];
-    const double vabc = w_abc + Iij[a * v + b] * t1[k * v + c] + Iik[a * v + c] * t1[j * v + b] +
-                        Ijk[b * v + c] * t1[i * v + a] + Tij[a * v + b] * Fk[c] + Tik[a * v + c] * Fj[b] +
-                        Tjk[b * v + c] * Fi[a];
-    const int64_t oa = o + a, ob = o + b, oc = o + c;
-    const double d = dijk - F[oa * nmo + oa] - F[ob * nmo + ob] - F[oc * nmo + oc];
-    s += vabc * (4.0 * w_abc + w_bca + w_cab - 2.0 * (w_acb + w_bac + w_cba)) / d;
+  if (b < v && c < v) {
+    const double* Iij = Ioovv + ((int64_t)i * o + j) * vv;
+    const double* Iik = Ioovv + ((int64_t)i * o + k) * vv;
+    const double* Ijk = Ioovv + ((int64_t)j * o + k) * vv;
+    const double* Tij = t2 + ((int64_t)i * o + j) * vv;
+    const double* Tik = t2 + ((int64_t)i * o + k) * vv;
+    const double* Tjk = t2 + ((int64_t)j * o + k) * vv;
+    const double* Fi = F + (int64_t)i * nmo + o;   // f_ia row
+    const double* Fj = F + (int64_t)j * nmo + o;
+    const double* Fk = F + (int64_t)k * nmo + o;
+    const int64_t ob = o + b, oc = o + c;
+    const double dbc = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k] - F[ob * nmo + ob] - F[oc * nmo + oc];
+    const double ijbc = Ijk[b * v + c], tjbc = Tjk[b * v + c];
+    const double t1kc = t1[k * v + c], t1jb = t1[j * v + b], fkc = Fk[c], fjb = Fj[b];
+    for (int al = 0; al < TT; ++al) {
+      const int64_t a = org[0] + al;
+      if (a >= v) break;
+      const double vabc = w[al] + Iij[a * v + b] * t1kc + Iik[a * v + c] * t1jb + ijbc * t1[i * v + a] +
+                          Tij[a * v + b] * fkc + Tik[a * v + c] * fjb + tjbc * Fi[a];
+      const int64_t oa = o + a;
+      s += vabc * (4.0 * w[al] + q[al]) / (dbc - F[oa * nmo + oa]);
+    }
   }
   s = t3_block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 // e[0] += weight * sum(partials[0 .. n)), in a fixed order
-__global__ void t3_accumulate_kernel(const double* __restrict__ partials, int n, double weight, double* e) {
+__global__ void t3_accumulate_kernel(const double* __restrict__ partials, int64_t n, double weight, double* e) {
   double s = 0.0;
-  for (int b = threadIdx.x; b < n; b += blockDim.x) s += partials[b];
+  for (int64_t b = threadIdx.x; b < n; b += blockDim.x) s += partials[b];
   s = t3_block_sum(s);
   if (threadIdx.x == 0) e[0] += weight * s;
 }
@@ -95,52 +202,46 @@ __global__ void t3_accumulate_kernel(const double* __restrict__ partials, int n,
 void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2, double* e_dev, hipStream_t s) {
   CCX_CHECK(c.have_ints, ERR_STATE, "triples called before the integrals were set");
   CCX_CHECK(F && t1 && t2 && e_dev, ERR_ARG, "null argument");
-  const int64_t o = c.o, v = c.v, nmo = c.nmo, ov = o * v, vvv = v * v * v;
+  const int64_t o = c.o, v = c.v, nmo = c.nmo, vv = v * v, vvv = vv * v;
   CCX_CHECK(c.Covvv && c.Cooov && c.Ioovv, ERR_STATE, "the context holds no FP64 <ov|vv>/<oo|ov> blocks");
-  if (!c.red) c.red = c.alloc<double>(1024 * 64);
-  double* partials = c.red + 1024 * 48;
-  const int blocks = (int)std::min<int64_t>(T3_BLOCKS, std::max<int64_t>(1, ceil_div(vvv, 256)));
-  double* W = nullptr;
-  CCX_HIP(hipMalloc(&W, vvv * sizeof(double)));
+  const int64_t nt = ceil_div(v, TT), tiles = nt * nt * nt;
+  CCX_CHECK(tiles < (int64_t(1) << 31), ERR_ARG, "too many virtual orbitals for the (T) tiles");
+  // scratch: A (o x v^2 x Kp), X (3 slots of v x v x 2v), W (v^3), B (3 x 2v x Kp), tile partials
+  const int64_t Kp = (v + o + 7) / 8 * 8;
+  const int64_t nA = o * vv * Kp, nX = 3 * 2 * vvv, nB = 3 * 2 * v * Kp;
+  const int64_t nall = nA + nX + vvv + nB + tiles;
+  double* base = nullptr;
+  CCX_HIP(hipMalloc(&base, nall * sizeof(double)));
+  double *A = base, *X = A + nA, *W = X + nX, *B = W + vvv, *partials = B + nB;
   Workspace& ws = c.ws;
-  const std::vector<int64_t> VV{v, v}, VVV{v, v, v}, OVV{o, v, v}, OV{o, v};
-  const TV Wabc = tv(W, "abc", VVV);
-  // Wvvvo[x][y][e][p] = <xy|ep> = (xe|yp) = Covvv[p][y][x][e]: `lab` names Covvv[p]'s dimensions
-  auto Wv = [&](int p, const char* lab) { return tv(c.Covvv + (int64_t)p * vvv, lab, VVV); };
-  // Wovoo[m][c][q][r] = <mc|qr> = (mq|cr) = Cooov[m][q][r][c]
-  auto Wo = [&](int q, int r, const char* lab) {
-    return tvs(c.Cooov + (int64_t)q * ov + (int64_t)r * v, lab, OV, {o * o * v, 1});
-  };
-  auto T2p = [&](int p, int q, const char* lab) { return tv(t2 + ((int64_t)p * o + q) * v * v, lab, VV); };
-  auto T2r = [&](int p, const char* lab) { return tv(t2 + (int64_t)p * o * v * v, lab, OVV); };
-  auto cd = [&](double a, const TV& A, const TV& B, double b) { contract<double>(a, A, B, b, Wabc, ws, s); };
+  const std::vector<int64_t> VVK{v, v, Kp}, VV2{v, v, 2 * v}, ZK{2 * v, Kp};
 
+  t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A);
+  CCX_HIP(hipGetLastError());
   t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
   CCX_HIP(hipGetLastError());
   for (int i = 0; i < o; ++i)
     for (int j = 0; j <= i; ++j)
       for (int k = 0; k <= j; ++k) {
-        // t3c_ijk (cctriples.py:50-62), each term in the reference's order
-        cd(1.0, Wv(i, "abe"), T2p(k, j, "ce"), 0.0);     // bae,ce->abc  W[:,:,:,i] t2[k,j]
-        cd(1.0, Wv(i, "ace"), T2p(j, k, "be"), 1.0);     // cae,be->abc  W[:,:,:,i] t2[j,k]
-        cd(1.0, Wv(k, "cae"), T2p(j, i, "be"), 1.0);     // ace,be->abc  W[:,:,:,k] t2[j,i]
-        cd(1.0, Wv(k, "cbe"), T2p(i, j, "ae"), 1.0);     // bce,ae->abc  W[:,:,:,k] t2[i,j]
-        cd(1.0, Wv(j, "bce"), T2p(i, k, "ae"), 1.0);     // cbe,ae->abc  W[:,:,:,j] t2[i,k]
-        cd(1.0, Wv(j, "bae"), T2p(k, i, "ce"), 1.0);     // abe,ce->abc  W[:,:,:,j] t2[k,i]
-        cd(-1.0, Wo(j, k, "mc"), T2r(i, "mab"), 1.0);    // mc,mab->abc  Wovoo[:,:,j,k] t2[i]
-        cd(-1.0, Wo(k, j, "mb"), T2r(i, "mac"), 1.0);    // mb,mac->abc  Wovoo[:,:,k,j] t2[i]
-        cd(-1.0, Wo(i, j, "mb"), T2r(k, "mca"), 1.0);    // mb,mca->abc  Wovoo[:,:,i,j] t2[k]
-        cd(-1.0, Wo(j, i, "ma"), T2r(k, "mcb"), 1.0);    // ma,mcb->abc  Wovoo[:,:,j,i] t2[k]
-        cd(-1.0, Wo(k, i, "ma"), T2r(j, "mbc"), 1.0);    // ma,mbc->abc  Wovoo[:,:,k,i] t2[j]
-        cd(-1.0, Wo(i, k, "mc"), T2r(j, "mba"), 1.0);    // mc,mba->abc  Wovoo[:,:,i,k] t2[j]
-        t3_energy_kernel<<<blocks, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, partials);
+        t3_pack_kernel<<<(int)std::min<int64_t>(1024, ceil_div(nB, 256)), 256, 0, s>>>(o, v, Kp, i, j, k, t2,
+                                                                                     c.Cooov, B);
+        CCX_HIP(hipGetLastError());
+        // X_s[x][y][z'] = sum_e <yx|ep> t_rq^{ze} - sum_m <mz|qr> t_pm^{xy}, with <yx|ep> = (px|ye) =
+        // Covvv[p][x][y][e]: the twelve t3c_ijk terms of cctriples.py:50-62, four per GEMM
+        const int ps[3] = {i, j, k};
+        for (int sl = 0; sl < 3; ++sl)
+          contract<double>(1.0, tv(A + (int64_t)ps[sl] * vv * Kp, "xyk", VVK), tv(B + sl * 2 * v * Kp, "zk", ZK),
+                           0.0, tv(X + sl * 2 * vvv, "xyz", VV2), ws, s);
+        t3_assemble_kernel<<<(int)tiles, 256, 0, s>>>(v, X, W);
+        CCX_HIP(hipGetLastError());
+        t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, partials);
         CCX_HIP(hipGetLastError());
         const double weight = 2.0 - (double)((i == j) + (i == k) + (j == k));   // cctriples.py:237
-        t3_accumulate_kernel<<<1, 256, 0, s>>>(partials, blocks, weight, e_dev);
+        t3_accumulate_kernel<<<1, 256, 0, s>>>(partials, tiles, weight, e_dev);
         CCX_HIP(hipGetLastError());
       }
   CCX_HIP(hipStreamSynchronize(s));
-  CCX_HIP(hipFree(W));
+  CCX_HIP(hipFree(base));
 }
 
 }  // namespace ccx
