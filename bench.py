@@ -184,6 +184,9 @@ def main():
     ap.add_argument("--backward", action="store_true",
                     help="measure the backward instead: o=32 v=256 seed 2 FP64 (override with --no/--nv/"
                          "--seed), HBAR stored vs factored, finite-difference checks, Lambda solve")
+    ap.add_argument("--triples", action="store_true",
+                    help="measure the (T) correction instead: FP64 CCSD converged at o=32 v=256 seed 3 "
+                         "(override with --no/--nv/--seed), then E(T) timed twice")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -192,6 +195,8 @@ def main():
         sys.exit(2)
     if args.backward:
         return backward_bench(args)
+    if args.triples:
+        return triples_bench(args)
     if args.copy_probe > 0:
         import torch
         n = int(args.copy_probe * (1 << 30)) // 8
@@ -388,6 +393,59 @@ def integral_element_checks(cc, sd, t1, t2, c1, c2, rng, n_fock=16, n_eri=32, h=
             "eri_elements": n_eri, "eri_fd_step": h, "eri_fd": "one-sided, DF rank-one perturbations",
             "eri_fd_max_rel_error": worst_e, "eri_max_abs_gradient": largest,
             "seconds": time.time() - t0}
+
+
+def triples_bench(args):
+    """The (T) correction (pycc/cctriples.py:177-240) at o=32, v=256 (seed 3, FP64): converge CCSD,
+    then time E(T) twice on the converged amplitudes. Algorithmic work: the reference's twelve
+    t3c_ijk contractions per triple i >= j >= k, 12 v^3 (v + o) flops, priced at the FP64 MFMA peak.
+    The CPU baseline runs the oracle's t3c_ijk + t3d_ijk + Lee-Rendell sum on the first 4 triples and
+    scales by the triple count."""
+    import numpy as np
+    import torch
+    import pycc_amd
+    from pycc_amd.synth import synthetic_df
+    o = 32 if args.no == 48 else args.no
+    v = 256 if args.nv == 384 else args.nv
+    sd = synthetic_df(o, v, seed=args.seed)
+    cc = pycc_amd.CCwfn(sd, no=o, precision="DP", verbose=False)
+    solve_ms, ecc = _ms(lambda: cc.solve_cc(1e-10, 1e-10, 100))
+    log("[triples] FP64 CCSD o=%d v=%d: E = %.12f in %d iterations (%.1f s)" % (o, v, ecc, cc.niter, solve_ms / 1e3))
+    runs = []
+    for rep in range(2):
+        ms, et = _ms(lambda: cc.triples_energy())
+        runs.append(ms)
+        log("[triples] E(T) = %.15f in %.1f ms" % (et, ms))
+    ntri = o * (o + 1) * (o + 2) // 6
+    flops = ntri * 12.0 * v ** 3 * (v + o)
+    best = min(runs)
+    # CPU sample: the oracle on the first 4 triples (i, j, k) = (0,0,0), (1,0,0), (1,1,0), (1,1,1)
+    from oracle.triples_ref import t3c_ijk, t3d_ijk
+    t1 = cc.t1.cpu().numpy()
+    t2 = cc.t2.cpu().numpy()
+    Fh = cc.H.F.cpu().numpy()
+    B = np.asarray(sd.B)                      # (pq|rs) = sum_Q B[Q,p,q] B[Q,r,s]; <pq|rs> = (pr|qs)
+    Boo, Bvv, Bov, Bvo = B[:, :o, :o], B[:, o:, o:], B[:, :o, o:], B[:, o:, :o]
+    Wvvvo = np.einsum('Qac,Qbi->abci', Bvv, Bvo, optimize=True)     # <ab|ci> = (ac|bi)
+    Wovoo = np.einsum('Qmj,Qck->mcjk', Boo, Bvo, optimize=True)     # <mc|jk> = (mj|ck)
+    Woovv = np.einsum('Qia,Qjb->ijab', Bov, Bov, optimize=True)     # <ij|ab> = (ia|jb)
+    t0 = time.perf_counter()
+    sample = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 1)]
+    for (i, j, k) in sample:
+        W3 = t3c_ijk(i, j, k, t2, Wvvvo, Wovoo)
+        V3 = W3 + t3d_ijk(i, j, k, t1, t2, Woovv, Fh[:o, o:])
+        float((W3 * V3).sum())
+    cpu_s = (time.perf_counter() - t0) / len(sample) * ntri
+    out = {"metric": "(T) correction wall time", "value": best / 1e3, "unit": "s", "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "CCSD(T) (T) correction, synthetic DF Hamiltonian", "no": o, "nv": v,
+                      "seed": args.seed, "triples": ntri},
+           "E_T": et, "E_CCSD": ecc, "runs_ms": runs,
+           "roofline": {"bound": "mfma", "achieved": flops / (best / 1e3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                        "unit": "TFLOP/s", "frac": flops / (best / 1e3) / PEAK_FP64, "traffic": None},
+           "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None,
+                            "kind": "port", "sample": "oracle t3c_ijk + t3d_ijk on 4 triples, scaled to %d" % ntri}}
+    print(json.dumps(out))
 
 
 def backward_bench(args):

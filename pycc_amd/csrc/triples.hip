@@ -45,20 +45,33 @@ __global__ void t3_zero_kernel(double* e) {
 // A thread owns (b_l, c_l) = (tid / 16, tid % 16) and the 16 elements a_l = 0..15 of its column.
 constexpr int TT = 16, TP = TT + 1;   // tile edge; LDS row pitch (padded against bank conflicts)
 
-// acc[a_l] += coef * S[g0][g1][off + g2], where source dimension k is indexed by the output
-// dimension Dk (0 = a, 1 = b, 2 = c): the 16^3 box is staged through LDS with 128-byte rows read
-// along the source's contiguous dimension, then read back in the output's order.
+// Staging of one 16^3 box of a source array S[g0][g1][off + g2] whose dimension k is indexed by the
+// output dimension Dk (0 = a, 1 = b, 2 = c). A thread loads the 16 elements (l0 = 0..15, l1 = tid / 16,
+// l2 = tid % 16) -- 128-byte rows along the source's contiguous dimension -- into registers; they go
+// to LDS as sh[l0][l1][l2] and are read back in the output's order. Loads of the next box are issued
+// before the current box is read back, so they overlap the LDS work.
 template <int D0, int D1, int D2>
-__device__ __forceinline__ void t3_gather(const double* __restrict__ S, int64_t s0, int64_t s1, int64_t off,
-                                          int64_t v, const int64_t* org, double (*sh)[TT][TP], double coef,
-                                          double* acc, int bl, int cl) {
-  __syncthreads();
-  for (int le = threadIdx.x; le < TT * TT * TT; le += 256) {
-    const int l2 = le % TT, l1 = (le / TT) % TT, l0 = le / (TT * TT);
-    const int64_t g0 = org[D0] + l0, g1 = org[D1] + l1, g2 = org[D2] + l2;
-    sh[l0][l1][l2] = (g0 < v && g1 < v && g2 < v) ? S[g0 * s0 + g1 * s1 + off + g2] : 0.0;
-  }
-  __syncthreads();
+__device__ __forceinline__ void t3_load(const double* __restrict__ S, int64_t s0, int64_t s1, int64_t off,
+                                        int64_t v, const int64_t* org, double* r) {
+  const int l1 = threadIdx.x / TT, l2 = threadIdx.x % TT;
+  const int64_t g1 = org[D1] + l1, g2 = org[D2] + l2;
+  const bool in = g1 < v && g2 < v;
+  const double* p = S + org[D0] * s0 + g1 * s1 + off + g2;
+  const int64_t n0 = v - org[D0];
+#pragma unroll
+  for (int l0 = 0; l0 < TT; ++l0) r[l0] = (in && l0 < n0) ? p[l0 * s0] : 0.0;
+}
+
+__device__ __forceinline__ void t3_store(double (*sh)[TT][TP], const double* r) {
+  const int l1 = threadIdx.x / TT, l2 = threadIdx.x % TT;
+#pragma unroll
+  for (int l0 = 0; l0 < TT; ++l0) sh[l0][l1][l2] = r[l0];
+}
+
+// acc[a_l] += coef * (staged box at the output position (a_l, b_l, c_l))
+template <int D0, int D1, int D2>
+__device__ __forceinline__ void t3_accumulate(double (*sh)[TT][TP], double coef, double* acc) {
+  const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
 #pragma unroll
   for (int al = 0; al < TT; ++al) {
     const int L[3] = {al, bl, cl};
@@ -120,15 +133,38 @@ __global__ __launch_bounds__(256) void t3_assemble_kernel(int64_t v, const doubl
   const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
   const int64_t s0 = 2 * v * v, s1 = 2 * v, xs = v * v * 2 * v;
   const double *X0 = X, *X1 = X + xs, *X2 = X + 2 * xs;
-  double acc[TT];
+  double acc[TT], r[TT];
 #pragma unroll
   for (int al = 0; al < TT; ++al) acc[al] = 0.0;
-  t3_gather<0, 1, 2>(X0, s0, s1, 0, v, org, sh, 1.0, acc, bl, cl);   // X_ijk[a][b][c]
-  t3_gather<0, 2, 1>(X0, s0, s1, v, v, org, sh, 1.0, acc, bl, cl);   // X_ikj[a][c][b]
-  t3_gather<1, 0, 2>(X1, s0, s1, 0, v, org, sh, 1.0, acc, bl, cl);   // X_jik[b][a][c]
-  t3_gather<1, 2, 0>(X1, s0, s1, v, v, org, sh, 1.0, acc, bl, cl);   // X_jki[b][c][a]
-  t3_gather<2, 0, 1>(X2, s0, s1, 0, v, org, sh, 1.0, acc, bl, cl);   // X_kij[c][a][b]
-  t3_gather<2, 1, 0>(X2, s0, s1, v, v, org, sh, 1.0, acc, bl, cl);   // X_kji[c][b][a]
+  t3_load<0, 1, 2>(X0, s0, s1, 0, v, org, r);                      // X_ijk[a][b][c]
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<0, 2, 1>(X0, s0, s1, v, v, org, r);                      // X_ikj[a][c][b]
+  t3_accumulate<0, 1, 2>(sh, 1.0, acc);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<1, 0, 2>(X1, s0, s1, 0, v, org, r);                      // X_jik[b][a][c]
+  t3_accumulate<0, 2, 1>(sh, 1.0, acc);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<1, 2, 0>(X1, s0, s1, v, v, org, r);                      // X_jki[b][c][a]
+  t3_accumulate<1, 0, 2>(sh, 1.0, acc);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<2, 0, 1>(X2, s0, s1, 0, v, org, r);                      // X_kij[c][a][b]
+  t3_accumulate<1, 2, 0>(sh, 1.0, acc);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<2, 1, 0>(X2, s0, s1, v, v, org, r);                      // X_kji[c][b][a]
+  t3_accumulate<2, 0, 1>(sh, 1.0, acc);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_accumulate<2, 1, 0>(sh, 1.0, acc);
   const int64_t b = org[1] + bl, c = org[2] + cl;
   if (b < v && c < v)
     for (int al = 0; al < TT; ++al) {
@@ -150,16 +186,39 @@ __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, in
   int64_t org[3];
   t3_tile_origin(v, org);
   const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
-  double q[TT], w[TT];
+  double q[TT], w[TT], r[TT];
 #pragma unroll
   for (int al = 0; al < TT; ++al) q[al] = w[al] = 0.0;
   const int64_t vv = v * v;
-  t3_gather<0, 1, 2>(W, vv, v, 0, v, org, sh, 1.0, w, bl, cl);    // W_abc
-  t3_gather<1, 2, 0>(W, vv, v, 0, v, org, sh, 1.0, q, bl, cl);    // W_bca = W[b][c][a]
-  t3_gather<2, 0, 1>(W, vv, v, 0, v, org, sh, 1.0, q, bl, cl);    // W_cab
-  t3_gather<0, 2, 1>(W, vv, v, 0, v, org, sh, -2.0, q, bl, cl);   // W_acb
-  t3_gather<1, 0, 2>(W, vv, v, 0, v, org, sh, -2.0, q, bl, cl);   // W_bac
-  t3_gather<2, 1, 0>(W, vv, v, 0, v, org, sh, -2.0, q, bl, cl);   // W_cba
+  t3_load<0, 1, 2>(W, vv, v, 0, v, org, r);     // W_abc
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<1, 2, 0>(W, vv, v, 0, v, org, r);     // W_bca = W[b][c][a]
+  t3_accumulate<0, 1, 2>(sh, 1.0, w);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<2, 0, 1>(W, vv, v, 0, v, org, r);     // W_cab
+  t3_accumulate<1, 2, 0>(sh, 1.0, q);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<0, 2, 1>(W, vv, v, 0, v, org, r);     // W_acb
+  t3_accumulate<2, 0, 1>(sh, 1.0, q);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<1, 0, 2>(W, vv, v, 0, v, org, r);     // W_bac
+  t3_accumulate<0, 2, 1>(sh, -2.0, q);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_load<2, 1, 0>(W, vv, v, 0, v, org, r);     // W_cba
+  t3_accumulate<1, 0, 2>(sh, -2.0, q);
+  __syncthreads();
+  t3_store(sh, r);
+  __syncthreads();
+  t3_accumulate<2, 1, 0>(sh, -2.0, q);
   const int64_t b = org[1] + bl, c = org[2] + cl;
   double s = 0.0;
   if (b < v && c < v) {
