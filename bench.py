@@ -186,7 +186,7 @@ def main():
                          "--seed), HBAR stored vs factored, finite-difference checks, Lambda solve")
     ap.add_argument("--triples", action="store_true",
                     help="measure the (T) correction instead: FP64 CCSD converged at o=32 v=256 seed 3 "
-                         "(override with --no/--nv/--seed), then E(T) timed twice")
+                         "(override with --no/--nv/--seed), E(T) timed in FP64 and in MIXED_F16X3")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -396,9 +396,11 @@ def integral_element_checks(cc, sd, t1, t2, c1, c2, rng, n_fock=16, n_eri=32, h=
 
 
 def triples_bench(args):
-    """The (T) correction (pycc/cctriples.py:177-240) at o=32, v=256 (seed 3, FP64): converge CCSD,
-    then time E(T) twice on the converged amplitudes. Algorithmic work: the reference's twelve
-    t3c_ijk contractions per triple i >= j >= k, 12 v^3 (v + o) flops, priced at the FP64 MFMA peak.
+    """The (T) correction (pycc/cctriples.py:177-240) at o=32, v=256 (seed 3): converge CCSD in FP64,
+    time E(T) once in FP64, then twice in MIXED_F16X3 (FP16x3 slot GEMMs) on the same amplitudes;
+    `value` is the MIXED_F16X3 time and E_T_error its difference from the FP64 E(T). Algorithmic
+    work: the reference's twelve t3c_ijk contractions per triple i >= j >= k, 12 v^3 (v + o) flops,
+    priced at the FP16x3 peak (dense FP16 MFMA / 3).
     The CPU baseline runs the oracle's t3c_ijk + t3d_ijk + Lee-Rendell sum on the first 4 triples and
     scales by the triple count."""
     import numpy as np
@@ -411,19 +413,25 @@ def triples_bench(args):
     cc = pycc_amd.CCwfn(sd, no=o, precision="DP", verbose=False)
     solve_ms, ecc = _ms(lambda: cc.solve_cc(1e-10, 1e-10, 100))
     log("[triples] FP64 CCSD o=%d v=%d: E = %.12f in %d iterations (%.1f s)" % (o, v, ecc, cc.niter, solve_ms / 1e3))
+    dp_ms, et_dp = _ms(lambda: cc.triples_energy())
+    log("[triples] FP64 E(T) = %.15f in %.1f ms" % (et_dp, dp_ms))
+    t1, t2 = cc.t1.clone(), cc.t2.clone()
+    del cc
+    torch.cuda.empty_cache()
+    mx = pycc_amd.CCwfn(sd, no=o, precision="MIXED_F16X3", verbose=False)
     runs = []
     for rep in range(2):
-        ms, et = _ms(lambda: cc.triples_energy())
+        ms, et = _ms(lambda: mx.triples_energy(t1, t2))
         runs.append(ms)
-        log("[triples] E(T) = %.15f in %.1f ms" % (et, ms))
+        log("[triples] MIXED_F16X3 E(T) = %.15f in %.1f ms (error %.2e Eh)" % (et, ms, et - et_dp))
     ntri = o * (o + 1) * (o + 2) // 6
     flops = ntri * 12.0 * v ** 3 * (v + o)
     best = min(runs)
     # CPU sample: the oracle on the first 4 triples (i, j, k) = (0,0,0), (1,0,0), (1,1,0), (1,1,1)
     from oracle.triples_ref import t3c_ijk, t3d_ijk
-    t1 = cc.t1.cpu().numpy()
-    t2 = cc.t2.cpu().numpy()
-    Fh = cc.H.F.cpu().numpy()
+    Fh = mx.H.F.cpu().numpy()
+    t1 = t1.cpu().numpy()
+    t2 = t2.cpu().numpy()
     B = np.asarray(sd.B)                      # (pq|rs) = sum_Q B[Q,p,q] B[Q,r,s]; <pq|rs> = (pr|qs)
     Boo, Bvv, Bov, Bvo = B[:, :o, :o], B[:, o:, o:], B[:, :o, o:], B[:, o:, :o]
     Wvvvo = np.einsum('Qac,Qbi->abci', Bvv, Bvo, optimize=True)     # <ab|ci> = (ac|bi)
@@ -437,12 +445,13 @@ def triples_bench(args):
         float((W3 * V3).sum())
     cpu_s = (time.perf_counter() - t0) / len(sample) * ntri
     out = {"metric": "(T) correction wall time", "value": best / 1e3, "unit": "s", "higher_is_better": False,
-           "dtype": "f64", "data": "synthetic",
+           "dtype": "f16x3", "data": "synthetic",
            "config": {"workload": "CCSD(T) (T) correction, synthetic DF Hamiltonian", "no": o, "nv": v,
                       "seed": args.seed, "triples": ntri},
-           "E_T": et, "E_CCSD": ecc, "runs_ms": runs,
-           "roofline": {"bound": "mfma", "achieved": flops / (best / 1e3) / 1e12, "peak": PEAK_FP64 / 1e12,
-                        "unit": "TFLOP/s", "frac": flops / (best / 1e3) / PEAK_FP64, "traffic": None},
+           "E_T": et, "E_T_fp64": et_dp, "E_T_error": et - et_dp, "E_CCSD": ecc, "runs_ms": runs,
+           "fp64_s": dp_ms / 1e3, "fp64_tflops": flops / (dp_ms / 1e3) / 1e12, "speedup_vs_fp64": dp_ms / best,
+           "roofline": {"bound": "mfma", "achieved": flops / (best / 1e3) / 1e12, "peak": PEAK["MIXED_F16X3"] / 1e12,
+                        "unit": "TFLOP/s", "frac": flops / (best / 1e3) / PEAK["MIXED_F16X3"], "traffic": None},
            "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None,
                             "kind": "port", "sample": "oracle t3c_ijk + t3d_ijk on 4 triples, scaled to %d" % ntri}}
     print(json.dumps(out))

@@ -156,11 +156,12 @@ int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const d
                        const double* D, const double* Doooo, const double* Dooov, const double* Dovov,
                        const double* Doovv, const double* Dvvvo, double* e_dev, void* stream);
 
-/* *e_dev = the (T) correction of closed-shell CCSD(T) at (fock, t1, t2), FP64 in every precision
- * mode: pycc/cctriples.py:177-240 (t_tjl; the same energy as t_vikings :243 and t_vikings_inverted
+/* *e_dev = the (T) correction of closed-shell CCSD(T) at (fock, t1, t2): pycc/cctriples.py:177-240 (t_tjl; the same energy as t_vikings :243 and t_vikings_inverted
  * :311), with the connected and disconnected T3 numerators of cctriples.py:27-72 and :108-146 and
- * the orbital-energy denominators from the diagonal of `fock`. Synchronises `stream`; uses v^3
- * doubles of device scratch for the duration of the call. */
+ * the orbital-energy denominators from the diagonal of `fock`. FP64, except in CCX_MODE_MIXED_F16X3,
+ * whose three GEMMs per occupied triple are FP16x3 split products with FP64 output (E(T) within
+ * ~1e-9 relative of FP64). Synchronises `stream`; holds o v^2 (v + o) + 7 v^3 doubles of device
+ * scratch (half the first term in MIXED_F16X3) for the duration of the call. */
 int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
                 void* stream);
 

@@ -2,8 +2,9 @@
 
 ``t_tjl(ccwfn)`` (``cctriples.py:177-240``), ``t_vikings(ccwfn)`` (``:243``) and
 ``t_vikings_inverted(ccwfn)`` (``:311``) are three formulations of one quantity, E(T) at the
-wavefunction's current amplitudes; all three evaluate it through ``ccx_triples`` in FP64 (one
-v^3 block of T3 numerators per occupied triple i >= j >= k, never the full o^3 v^3 array).
+wavefunction's current amplitudes; all three evaluate it through ``ccx_triples`` (one v^3 block of
+T3 numerators per occupied triple i >= j >= k, never the full o^3 v^3 array), in FP64 except in
+the MIXED_F16X3 mode, whose GEMMs are FP16x3 split products.
 """
 from __future__ import annotations
 

@@ -394,8 +394,9 @@ class CCwfn(object):
         return float(self._e_dev[0].item())
 
     def triples_energy(self, t1=None, t2=None, F=None):
-        """The (T) correction E(T) at (F, t1, t2), FP64 on the GPU in every precision mode
-        (``pycc/cctriples.py:177-240``, ``t_tjl``)."""
+        """The (T) correction E(T) at (F, t1, t2) on the GPU (``pycc/cctriples.py:177-240``,
+        ``t_tjl``): FP64, except that MIXED_F16X3 runs its GEMMs as FP16x3 split products (E(T)
+        within ~1e-9 relative of FP64)."""
         F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
         t1 = self.t1 if t1 is None else self._as_dev(t1, (self.no, self.nv))
         t2 = self.t2 if t2 is None else self._as_dev(t2, (self.no, self.no, self.nv, self.nv))

@@ -1,4 +1,5 @@
-// (T) correction of closed-shell CCSD(T) in FP64 (pycc/cctriples.py:177-240, t_tjl).
+// (T) correction of closed-shell CCSD(T) (pycc/cctriples.py:177-240, t_tjl): FP64, except that the
+// MIXED_F16X3 mode runs its three GEMMs per triple as FP16x3 split products with FP64 output.
 //
 // For every occupied triple i >= j >= k the connected numerator W[a][b][c] (cctriples.py:27-72,
 // t3c_ijk without the denominator) comes from three GEMMs and one tiled permute-and-add: for each p
@@ -19,6 +20,7 @@
 // the result is deterministic.
 #include "ccsd.h"
 #include "contract.h"
+#include "split16.h"
 
 namespace ccx {
 
@@ -265,18 +267,41 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   CCX_CHECK(c.Covvv && c.Cooov && c.Ioovv, ERR_STATE, "the context holds no FP64 <ov|vv>/<oo|ov> blocks");
   const int64_t nt = ceil_div(v, TT), tiles = nt * nt * nt;
   CCX_CHECK(tiles < (int64_t(1) << 31), ERR_ARG, "too many virtual orbitals for the (T) tiles");
-  // scratch: A (o x v^2 x Kp), X (3 slots of v x v x 2v), W (v^3), B (3 x 2v x Kp), tile partials
-  const int64_t Kp = (v + o + 7) / 8 * 8;
+  // In MIXED_F16X3 the slot GEMMs are FP16x3 split products (interleaved hi/lo operands, K padded
+  // to 32, FP32 accumulation, FP64 output); every other mode runs them in FP64.
+  const bool x3 = c.mode == MODE_MIXED_F16X3;
+  const int64_t Kp = x3 ? (v + o + 31) / 32 * 32 : (v + o + 7) / 8 * 8;
+  // scratch: X (3 slots of v x v x 2v), W (v^3), B (3 x 2v x Kp), tile partials, and A (o x v^2 x
+  // Kp) in FP64 -- or, in MIXED_F16X3, split A and B (hi/lo interleaved, rows of 2 Kp halves) plus
+  // absmax partials and the two operand scales
   const int64_t nA = o * vv * Kp, nX = 3 * 2 * vvv, nB = 3 * 2 * v * Kp;
-  const int64_t nall = nA + nX + vvv + nB + tiles;
+  const int64_t np = x3_partials();
+  const int64_t n64 = nX + vvv + nB + tiles + (x3 ? 0 : nA);
+  const int64_t nx3 = x3 ? (nA + nB) * 2 * (int64_t)sizeof(f16) / 8 + (np + 8) / 2 + 1 : 0;   // in doubles
   double* base = nullptr;
-  CCX_HIP(hipMalloc(&base, nall * sizeof(double)));
-  double *A = base, *X = A + nA, *W = X + nX, *B = W + vvv, *partials = B + nB;
+  CCX_HIP(hipMalloc(&base, (n64 + nx3) * sizeof(double)));
+  double *X = base, *W = X + nX, *B = W + vvv, *partials = B + nB, *A = partials + tiles;
+  f16* Ahl = x3 ? (f16*)(base + n64) : nullptr;        // A split, [o v^2][2 Kp]
+  f16* Bhl = x3 ? Ahl + nA * 2 : nullptr;             // B split, [3 x 2v][2 Kp]
+  float* sc = x3 ? (float*)(Bhl + nB * 2) : nullptr;  // sc[0] = scale of A, sc[1] = scale of B
+  float* part = x3 ? sc + 8 : nullptr;
   Workspace& ws = c.ws;
   const std::vector<int64_t> VVK{v, v, Kp}, VV2{v, v, 2 * v}, ZK{2 * v, Kp};
 
-  t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A);
-  CCX_HIP(hipGetLastError());
+  if (x3) {   // A in FP64 only long enough to split it
+    double* A64 = nullptr;
+    CCX_HIP(hipMalloc(&A64, nA * sizeof(double)));
+    t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A64);
+    CCX_HIP(hipGetLastError());
+    x3_absmax<double>(A64, nA, part, s);
+    x3_scale(part, 1.f, nullptr, 0.f, sc, s);
+    x3_split_rows<double>(A64, o * vv, Kp, Kp, sc, Ahl, Ahl + 32, Kp, s, true);
+    CCX_HIP(hipStreamSynchronize(s));
+    CCX_HIP(hipFree(A64));
+  } else {
+    t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A);
+    CCX_HIP(hipGetLastError());
+  }
   t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
   CCX_HIP(hipGetLastError());
   for (int i = 0; i < o; ++i)
@@ -288,9 +313,24 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
         // X_s[x][y][z'] = sum_e <yx|ep> t_rq^{ze} - sum_m <mz|qr> t_pm^{xy}, with <yx|ep> = (px|ye) =
         // Covvv[p][x][y][e]: the twelve t3c_ijk terms of cctriples.py:50-62, four per GEMM
         const int ps[3] = {i, j, k};
-        for (int sl = 0; sl < 3; ++sl)
-          contract<double>(1.0, tv(A + (int64_t)ps[sl] * vv * Kp, "xyk", VVK), tv(B + sl * 2 * v * Kp, "zk", ZK),
-                           0.0, tv(X + sl * 2 * vvv, "xyz", VV2), ws, s);
+        if (x3) {
+          x3_absmax<double>(B, nB, part, s);
+          x3_scale(part, 1.f, nullptr, 0.f, sc + 1, s);
+          x3_split_rows<double>(B, 3 * 2 * v, Kp, Kp, sc + 1, Bhl, Bhl + 32, Kp, s, true);
+          for (int sl = 0; sl < 3; ++sl) {
+            X3Gemm g;   // X_s[(x,y)][z'] = sum_kk A_p[(x,y)][kk] B_s[z'][kk]
+            g.M = vv; g.N = 2 * v; g.Kp = Kp; g.lda = 2 * Kp; g.ldb = 2 * Kp; g.ldc = 2 * v;
+            g.kb = 32; g.ska = 64; g.skb = 64;
+            g.Ah = Ahl + (int64_t)ps[sl] * vv * 2 * Kp; g.Al = (const f16*)g.Ah + 32;
+            g.Bh = Bhl + (int64_t)sl * 2 * v * 2 * Kp; g.Bl = (const f16*)g.Bh + 32;
+            g.sa = sc; g.sb = sc + 1; g.C64 = X + sl * 2 * vvv;
+            x3_gemm<_Float16>(g, s, &ws);
+          }
+        } else {
+          for (int sl = 0; sl < 3; ++sl)
+            contract<double>(1.0, tv(A + (int64_t)ps[sl] * vv * Kp, "xyk", VVK), tv(B + sl * 2 * v * Kp, "zk", ZK),
+                             0.0, tv(X + sl * 2 * vvv, "xyz", VV2), ws, s);
+        }
         t3_assemble_kernel<<<(int)tiles, 256, 0, s>>>(v, X, W);
         CCX_HIP(hipGetLastError());
         t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, partials);
