@@ -184,9 +184,9 @@ def main():
     ap.add_argument("--backward", action="store_true",
                     help="measure the backward instead: o=32 v=256 seed 2 FP64 (override with --no/--nv/"
                          "--seed), HBAR stored vs factored, finite-difference checks, Lambda solve")
-    ap.add_argument("--triples", action="store_true",
-                    help="measure the (T) correction instead: FP64 CCSD converged at o=32 v=256 seed 3 "
-                         "(override with --no/--nv/--seed), E(T) timed in FP64 and in MIXED_F16X3")
+    ap.add_argument("--triples", nargs="?", const="32,256", default=None, metavar="O,V",
+                    help="measure the (T) correction instead: FP64 CCSD converged at o=32 v=256 (or O,V) "
+                         "seed 3 (--seed), E(T) timed in FP64 and in MIXED_F16X3")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -396,7 +396,7 @@ def integral_element_checks(cc, sd, t1, t2, c1, c2, rng, n_fock=16, n_eri=32, h=
 
 
 def triples_bench(args):
-    """The (T) correction (pycc/cctriples.py:177-240) at o=32, v=256 (seed 3): converge CCSD in FP64,
+    """The (T) correction (pycc/cctriples.py:177-240) at o=32, v=256 by default (seed 3): converge CCSD in FP64,
     time E(T) once in FP64, then twice in MIXED_F16X3 (FP16x3 slot GEMMs) on the same amplitudes;
     `value` is the MIXED_F16X3 time and E_T_error its difference from the FP64 E(T). Algorithmic
     work: the reference's twelve t3c_ijk contractions per triple i >= j >= k, 12 v^3 (v + o) flops,
@@ -407,14 +407,16 @@ def triples_bench(args):
     import torch
     import pycc_amd
     from pycc_amd.synth import synthetic_df
-    o = 32 if args.no == 48 else args.no
-    v = 256 if args.nv == 384 else args.nv
+    o, v = (int(x) for x in args.triples.split(","))
     sd = synthetic_df(o, v, seed=args.seed)
-    cc = pycc_amd.CCwfn(sd, no=o, precision="DP", verbose=False)
-    solve_ms, ecc = _ms(lambda: cc.solve_cc(1e-10, 1e-10, 100))
-    log("[triples] FP64 CCSD o=%d v=%d: E = %.12f in %d iterations (%.1f s)" % (o, v, ecc, cc.niter, solve_ms / 1e3))
-    dp_ms, et_dp = _ms(lambda: cc.triples_energy())
-    log("[triples] FP64 E(T) = %.15f in %.1f ms" % (et_dp, dp_ms))
+    prec = "MIXED_F16X3" if args.no_dp else "DP"     # --no-dp: no FP64 leg (amplitudes from MIXED_F16X3)
+    cc = pycc_amd.CCwfn(sd, no=o, precision=prec, verbose=False)
+    solve_ms, ecc = _ms(lambda: cc.solve_cc(1e-10 if prec == "DP" else 1e-7, 1e-10 if prec == "DP" else 1e-6, 100))
+    log("[triples] %s CCSD o=%d v=%d: E = %.12f in %d iterations (%.1f s)" % (prec, o, v, ecc, cc.niter, solve_ms / 1e3))
+    dp_ms, et_dp = None, None
+    if prec == "DP":
+        dp_ms, et_dp = _ms(lambda: cc.triples_energy())
+        log("[triples] FP64 E(T) = %.15f in %.1f ms" % (et_dp, dp_ms))
     t1, t2 = cc.t1.clone(), cc.t2.clone()
     del cc
     torch.cuda.empty_cache()
@@ -423,11 +425,13 @@ def triples_bench(args):
     for rep in range(2):
         ms, et = _ms(lambda: mx.triples_energy(t1, t2))
         runs.append(ms)
-        log("[triples] MIXED_F16X3 E(T) = %.15f in %.1f ms (error %.2e Eh)" % (et, ms, et - et_dp))
+        log("[triples] MIXED_F16X3 E(T) = %.15f in %.1f ms (error %s Eh)" % (
+            et, ms, "%.2e" % (et - et_dp) if et_dp is not None else "n/a"))
     ntri = o * (o + 1) * (o + 2) // 6
     flops = ntri * 12.0 * v ** 3 * (v + o)
     best = min(runs)
     # CPU sample: the oracle on the first 4 triples (i, j, k) = (0,0,0), (1,0,0), (1,1,0), (1,1,1)
+    log("[triples] CPU sample: oracle on 4 triples")
     from oracle.triples_ref import t3c_ijk, t3d_ijk
     Fh = mx.H.F.cpu().numpy()
     t1 = t1.cpu().numpy()
@@ -448,8 +452,11 @@ def triples_bench(args):
            "dtype": "f16x3", "data": "synthetic",
            "config": {"workload": "CCSD(T) (T) correction, synthetic DF Hamiltonian", "no": o, "nv": v,
                       "seed": args.seed, "triples": ntri},
-           "E_T": et, "E_T_fp64": et_dp, "E_T_error": et - et_dp, "E_CCSD": ecc, "runs_ms": runs,
-           "fp64_s": dp_ms / 1e3, "fp64_tflops": flops / (dp_ms / 1e3) / 1e12, "speedup_vs_fp64": dp_ms / best,
+           "E_T": et, "E_T_fp64": et_dp, "E_T_error": (et - et_dp) if et_dp is not None else None,
+           "E_CCSD": ecc, "ccsd_precision": prec, "runs_ms": runs,
+           "fp64_s": dp_ms / 1e3 if dp_ms else None,
+           "fp64_tflops": flops / (dp_ms / 1e3) / 1e12 if dp_ms else None,
+           "speedup_vs_fp64": dp_ms / best if dp_ms else None,
            "roofline": {"bound": "mfma", "achieved": flops / (best / 1e3) / 1e12, "peak": PEAK["MIXED_F16X3"] / 1e12,
                         "unit": "TFLOP/s", "frac": flops / (best / 1e3) / PEAK["MIXED_F16X3"], "traffic": None},
            "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or None,
