@@ -19,7 +19,7 @@ def test_header_declares_the_entry_points():
     names = _declared()
     for n in ["ccx_create", "ccx_destroy", "ccx_set_integrals_dense", "ccx_set_integrals_df", "ccx_residual",
               "ccx_energy", "ccx_update", "ccx_diis_reset", "ccx_diis_add", "ccx_diis_extrapolate",
-              "ccx_contract", "ccx_last_error"]:
+              "ccx_contract", "ccx_triples", "ccx_last_error"]:
         assert n in names
 
 
@@ -46,6 +46,8 @@ def test_invalid_arguments_return_error_codes_without_a_gpu():
     assert b"no >= 1" in lib.ccx_last_error()
     assert lib.ccx_create(2, 5, 7, ctypes.byref(h)) == -1
     assert lib.ccx_residual(None, None, None, None, None, None, None) == -1
+    assert lib.ccx_triples(None, None, None, None, None, None) == -1
+    assert b"null context" in lib.ccx_last_error()
     assert lib.ccx_contract(b"ab,bc", 1.0, None, None, None, None, None, None, 0.0, None, None, None, None) == -1
 
 
@@ -74,3 +76,4 @@ def test_ccwfn_rejects_unknown_kwargs_and_models():
         CCwfn(None, no=1, bogus=1)
     with pytest.raises(InvalidKeywordError):
         CCwfn(None, no=1, model='CC3')
+    assert 'CCSD(T)' in CCwfn.VALID_MODELS
