@@ -401,8 +401,8 @@ def triples_bench(args):
     `value` is the MIXED_F16X3 time and E_T_error its difference from the FP64 E(T). Algorithmic
     work: the reference's twelve t3c_ijk contractions per triple i >= j >= k, 12 v^3 (v + o) flops,
     priced at the FP16x3 peak (dense FP16 MFMA / 3).
-    The CPU baseline runs the oracle's t3c_ijk + t3d_ijk + Lee-Rendell sum on the first 4 triples and
-    scales by the triple count."""
+    The CPU baseline runs the oracle's t3c_ijk and t3d_ijk and one v^3 reduction on the first 4
+    triples and scales by the triple count."""
     import numpy as np
     import torch
     import pycc_amd
