@@ -44,3 +44,30 @@ def test_triangular_sum_equals_full_loop(no, nv, seed):
         assert abs(a) < 1e-15              # one electron pair: no triple excitations
     else:
         assert abs(a) > 1e-6               # not a vacuous zero
+
+
+@pytest.mark.parametrize("no,nv,seed", [(2, 5, 4), (4, 6, 5)])
+def test_full_abc_rewrite_equals_triangular_sum(no, nv, seed):
+    """The device kernel sums E_ijk = sum_abc V_abc (4 W_abc + W_bca + W_cab - 2 W_acb - 2 W_bac
+    - 2 W_cba) / D over all a, b, c with the (2 - d_ij - d_ik - d_jk) triple weights, in place of
+    t_tjl's a >= b >= c sum with its X3 / Y3 / Z3 intermediates; the two agree at arbitrary inputs."""
+    from oracle.triples_ref import t3c_ijk, t3d_ijk
+    sd = synthetic_df(no, nv, seed=seed, t2max=0.1)
+    H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
+    X = np.random.default_rng(seed + 100).standard_normal(H.F.shape) * 0.05
+    F = H.F + X + X.T
+    t1, t2 = random_amplitudes(no, nv, seed, scale=0.1)
+    o, v = slice(0, no), slice(no, no + nv)
+    Fo, Fv = np.diag(F)[o], np.diag(F)[v]
+    E = 0.0
+    for i in range(no):
+        for j in range(i + 1):
+            for k in range(j + 1):
+                W = t3c_ijk(i, j, k, t2, H.ERI[v, v, v, o], H.ERI[o, v, o, o])
+                V = W + t3d_ijk(i, j, k, t1, t2, H.ERI[o, o, v, v], F[o, v])
+                D = Fo[i] + Fo[j] + Fo[k] - (Fv.reshape(-1, 1, 1) + Fv.reshape(-1, 1) + Fv)
+                Q = (4.0 * W + np.einsum('bca->abc', W) + np.einsum('cab->abc', W)
+                     - 2.0 * (np.einsum('acb->abc', W) + np.einsum('bac->abc', W) + np.einsum('cba->abc', W)))
+                E += (2.0 - ((i == j) + (i == k) + (j == k))) * float((V * Q / D).sum())
+    ref = t_tjl(F, H.ERI, t1, t2, no)
+    assert abs(E - ref) < 1e-13 * max(1.0, abs(ref)), (E, ref)
