@@ -2,11 +2,13 @@
 // MIXED_F16X3 mode runs its three GEMMs per triple as FP16x3 split products with FP64 output.
 //
 // For every occupied triple i >= j >= k the connected numerator W[a][b][c] (cctriples.py:27-72,
-// t3c_ijk without the denominator) comes from three GEMMs and one tiled permute-and-add: for each p
-// in (i, j, k) the four terms that share p (two <ab|ep> t2 and two <mc|qr> t2 products) form one
-// GEMM of v^2 x 2v x (v + o), whose A operand [Covvv[p] | t2[p]^T] is built once per call and whose
-// B operand (t2 and -(mq|rc) columns of the two (q, r) orders side by side) is packed per triple;
-// the three v^2 x 2v results are then added into W's [a][b][c] order through 16^3 LDS tiles.
+// t3c_ijk without the denominator) comes from GEMMs and one tiled permute-and-add: for each p in
+// (i, j, k) the four terms that share p (two <ab|ep> t2 and two <mc|qr> t2 products) are one GEMM
+// row block of v^2 x 2v x (v + o), whose A operand [Covvv[p] | t2[p]^T] is built once per call and
+// whose B operand (t2 and -(mq|rc) columns of the two (q, r) orders side by side) is packed per
+// triple. For fixed (i, j), up to T3_KC values of k share one GEMM per slot (wider N for p = i and
+// p = j, taller M for p = k); the three v^2 x 2v blocks of a triple are then added into W's [a][b][c]
+// order through 16^3 LDS tiles.
 // A tiled kernel then adds the disconnected part (t3d_ijk, cctriples.py:108-146: <ij|ab> t1_kc + t2_ijab f_kc and their
 // permutations) to form V and reduces
 //
@@ -105,18 +107,28 @@ __global__ void t3_operand_a_kernel(int64_t o, int64_t v, int64_t Kp, const doub
   }
 }
 
-// B[s][z'][kk] = t2[r][q][z][kk] for kk < v and -(m q|r z) = -<m z|q r> with m = kk - v below v + o,
-// for the two (q, r) orders of slot s (z' < v: first order, z' >= v: second): slot 0 is p = i with
-// (j, k), (k, j); slot 1 is p = j with (i, k), (k, i); slot 2 is p = k with (i, j), (j, i).
-__global__ void t3_pack_kernel(int64_t o, int64_t v, int64_t Kp, int i, int j, int k,
-                               const double* __restrict__ t2, const double* __restrict__ Cooov,
-                               double* __restrict__ B) {
-  const int64_t total = 3 * 2 * v * Kp;
+// Up to T3_KC (q, r) entries; entry e packs the two orders (q_e, r_e) and (r_e, q_e).
+constexpr int T3_KC = 8;
+struct T3Pairs {
+  int n;
+  int q[T3_KC], r[T3_KC];
+};
+
+// B[e][z'][kk] = t2[r][q][z][kk] for kk < v and -(m q|r z) = -<m z|q r> with m = kk - v below v + o,
+// where (q, r) is entry e's first order for z' < v and its second order for z' >= v, z = z' mod v.
+// For the triple (i, j, k) the slot of p = i takes the entry (j, k), that of p = j the entry (i, k)
+// and that of p = k the entry (i, j).
+__global__ void t3_pack_kernel(int64_t o, int64_t v, int64_t Kp, T3Pairs pr, const double* __restrict__ t2,
+                               const double* __restrict__ Cooov, double* __restrict__ B) {
+  const int64_t total = (int64_t)pr.n * 2 * v * Kp;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t kk = e % Kp, zz = e / Kp % (2 * v), sl = e / (2 * v * Kp);
+    const int64_t kk = e % Kp, zz = e / Kp % (2 * v), en = e / (2 * v * Kp);
     const int h = zz >= v, z = (int)(zz - h * v);
-    const int q0 = sl == 0 ? j : i, r0 = sl == 2 ? j : k;   // the slot's first (q, r)
+    int q0 = pr.q[0], r0 = pr.r[0];
+#pragma unroll
+    for (int x = 1; x < T3_KC; ++x)
+      if (en == x) { q0 = pr.q[x]; r0 = pr.r[x]; }
     const int q = h ? r0 : q0, r = h ? q0 : r0;
     double val = 0.0;
     if (kk < v) val = t2[(((int64_t)r * o + q) * v + z) * v + kk];
@@ -126,42 +138,45 @@ __global__ void t3_pack_kernel(int64_t o, int64_t v, int64_t Kp, int i, int j, i
 }
 
 // W[a][b][c] = X_ijk[a,b,c] + X_ikj[a,c,b] + X_jik[b,a,c] + X_jki[b,c,a] + X_kij[c,a,b] + X_kji[c,b,a]
-// from the slot products X_s[x][y][z'] (v x v x 2v; z' >= v holds the slot's second order).
-__global__ __launch_bounds__(256) void t3_assemble_kernel(int64_t v, const double* __restrict__ X,
-                                                          double* __restrict__ W) {
+// from the slot products X_s[x][y][z'] (X_s[x * s0 + y * s1 + z'], z' in [0, 2v); z' >= v holds the
+// slot's second order).
+struct T3Slots {
+  const double* X[3];
+  int64_t s0[3], s1[3];
+};
+__global__ __launch_bounds__(256) void t3_assemble_kernel(int64_t v, T3Slots xs, double* __restrict__ W) {
   __shared__ double sh[TT][TT][TP];
   int64_t org[3];
   t3_tile_origin(v, org);
   const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
-  const int64_t s0 = 2 * v * v, s1 = 2 * v, xs = v * v * 2 * v;
-  const double *X0 = X, *X1 = X + xs, *X2 = X + 2 * xs;
+  const double *X0 = xs.X[0], *X1 = xs.X[1], *X2 = xs.X[2];
   double acc[TT], r[TT];
 #pragma unroll
   for (int al = 0; al < TT; ++al) acc[al] = 0.0;
-  t3_load<0, 1, 2>(X0, s0, s1, 0, v, org, r);                      // X_ijk[a][b][c]
+  t3_load<0, 1, 2>(X0, xs.s0[0], xs.s1[0], 0, v, org, r);          // X_ijk[a][b][c]
   t3_store(sh, r);
   __syncthreads();
-  t3_load<0, 2, 1>(X0, s0, s1, v, v, org, r);                      // X_ikj[a][c][b]
+  t3_load<0, 2, 1>(X0, xs.s0[0], xs.s1[0], v, v, org, r);          // X_ikj[a][c][b]
   t3_accumulate<0, 1, 2>(sh, 1.0, acc);
   __syncthreads();
   t3_store(sh, r);
   __syncthreads();
-  t3_load<1, 0, 2>(X1, s0, s1, 0, v, org, r);                      // X_jik[b][a][c]
+  t3_load<1, 0, 2>(X1, xs.s0[1], xs.s1[1], 0, v, org, r);          // X_jik[b][a][c]
   t3_accumulate<0, 2, 1>(sh, 1.0, acc);
   __syncthreads();
   t3_store(sh, r);
   __syncthreads();
-  t3_load<1, 2, 0>(X1, s0, s1, v, v, org, r);                      // X_jki[b][c][a]
+  t3_load<1, 2, 0>(X1, xs.s0[1], xs.s1[1], v, v, org, r);          // X_jki[b][c][a]
   t3_accumulate<1, 0, 2>(sh, 1.0, acc);
   __syncthreads();
   t3_store(sh, r);
   __syncthreads();
-  t3_load<2, 0, 1>(X2, s0, s1, 0, v, org, r);                      // X_kij[c][a][b]
+  t3_load<2, 0, 1>(X2, xs.s0[2], xs.s1[2], 0, v, org, r);          // X_kij[c][a][b]
   t3_accumulate<1, 2, 0>(sh, 1.0, acc);
   __syncthreads();
   t3_store(sh, r);
   __syncthreads();
-  t3_load<2, 1, 0>(X2, s0, s1, v, v, org, r);                      // X_kji[c][b][a]
+  t3_load<2, 1, 0>(X2, xs.s0[2], xs.s1[2], v, v, org, r);          // X_kji[c][b][a]
   t3_accumulate<2, 0, 1>(sh, 1.0, acc);
   __syncthreads();
   t3_store(sh, r);
@@ -271,76 +286,121 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   // to 32, FP32 accumulation, FP64 output); every other mode runs them in FP64.
   const bool x3 = c.mode == MODE_MIXED_F16X3;
   const int64_t Kp = x3 ? (v + o + 31) / 32 * 32 : (v + o + 7) / 8 * 8;
-  // scratch: X (3 slots of v x v x 2v), W (v^3), B (3 x 2v x Kp), tile partials, and A (o x v^2 x
-  // Kp) in FP64 -- or, in MIXED_F16X3, split A and B (hi/lo interleaved, rows of 2 Kp halves) plus
-  // absmax partials and the two operand scales
-  const int64_t nA = o * vv * Kp, nX = 3 * 2 * vvv, nB = 3 * 2 * v * Kp;
+  const int64_t KC = T3_KC;
+  // For fixed (i, j) the k values run in chunks of KC: the p = i and p = j GEMMs of a chunk share
+  // their A operand and become one GEMM each with N = 2v nk, and the p = k GEMMs (one B operand,
+  // consecutive A_k rows) one GEMM with M = nk v^2.
+  // scratch: XI, XJ (v^2 x 2v KC each), XK (KC v^2 x 2v), W (v^3), B (rows of the i, j and k entries,
+  // (2 KC + 1) 2v x Kp), tile partials, and A (o v^2 x Kp) in FP64 -- or, in MIXED_F16X3, A and B
+  // split (hi/lo interleaved, rows of 2 Kp halves) plus absmax partials and the operand scales
+  const int64_t nA = o * vv * Kp, nXc = vv * 2 * v * KC, nB = (2 * KC + 1) * 2 * v * Kp;
   const int64_t np = x3_partials();
-  const int64_t n64 = nX + vvv + nB + tiles + (x3 ? 0 : nA);
-  const int64_t nx3 = x3 ? (nA + nB) * 2 * (int64_t)sizeof(f16) / 8 + (np + 8) / 2 + 1 : 0;   // in doubles
-  double* base = nullptr;
-  CCX_HIP(hipMalloc(&base, (n64 + nx3) * sizeof(double)));
-  double *X = base, *W = X + nX, *B = W + vvv, *partials = B + nB, *A = partials + tiles;
-  f16* Ahl = x3 ? (f16*)(base + n64) : nullptr;        // A split, [o v^2][2 Kp]
-  f16* Bhl = x3 ? Ahl + nA * 2 : nullptr;             // B split, [3 x 2v][2 Kp]
-  float* sc = x3 ? (float*)(Bhl + nB * 2) : nullptr;  // sc[0] = scale of A, sc[1] = scale of B
-  float* part = x3 ? sc + 8 : nullptr;
-  Workspace& ws = c.ws;
-  const std::vector<int64_t> VVK{v, v, Kp}, VV2{v, v, 2 * v}, ZK{2 * v, Kp};
-
-  if (x3) {   // A in FP64 only long enough to split it
+  f16* Ahl = nullptr;    // A split, [o v^2][2 Kp]
+  float* scA = nullptr;  // scA[0] = scale of A, then absmax partials
+  if (x3) {   // A split first; its FP64 form lives only until then
+    CCX_HIP(hipMalloc(&Ahl, nA * 2 * sizeof(f16)));
+    CCX_HIP(hipMalloc(&scA, (8 + np) * sizeof(float)));
     double* A64 = nullptr;
     CCX_HIP(hipMalloc(&A64, nA * sizeof(double)));
     t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A64);
     CCX_HIP(hipGetLastError());
-    x3_absmax<double>(A64, nA, part, s);
-    x3_scale(part, 1.f, nullptr, 0.f, sc, s);
-    x3_split_rows<double>(A64, o * vv, Kp, Kp, sc, Ahl, Ahl + 32, Kp, s, true);
+    x3_absmax<double>(A64, nA, scA + 8, s);
+    x3_scale(scA + 8, 1.f, nullptr, 0.f, scA, s);
+    x3_split_rows<double>(A64, o * vv, Kp, Kp, scA, Ahl, Ahl + 32, Kp, s, true);
     CCX_HIP(hipStreamSynchronize(s));
     CCX_HIP(hipFree(A64));
-  } else {
+  }
+  const int64_t n64 = 3 * nXc + vvv + nB + tiles + (x3 ? 0 : nA);
+  const int64_t nx3 = x3 ? nB * 2 * (int64_t)sizeof(f16) / 8 + (np + 8) / 2 + 1 : 0;   // in doubles
+  double* base = nullptr;
+  CCX_HIP(hipMalloc(&base, (n64 + nx3) * sizeof(double)));
+  double *XI = base, *XJ = XI + nXc, *XK = XJ + nXc, *W = XK + nXc, *B = W + vvv, *partials = B + nB;
+  double* A = partials + tiles;
+  f16* Bhl = x3 ? (f16*)(base + n64) : nullptr;       // B split, [(2 KC + 1) 2v][2 Kp]
+  float* sc = x3 ? (float*)(Bhl + nB * 2) : nullptr;  // sc[1] = scale of B
+  float* part = x3 ? sc + 8 : nullptr;
+  Workspace& ws = c.ws;
+
+  if (!x3) {
     t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A);
     CCX_HIP(hipGetLastError());
   }
   t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
   CCX_HIP(hipGetLastError());
+  const int64_t ldx = 2 * v * KC;   // row length of XI, XJ
   for (int i = 0; i < o; ++i)
     for (int j = 0; j <= i; ++j)
-      for (int k = 0; k <= j; ++k) {
-        t3_pack_kernel<<<(int)std::min<int64_t>(1024, ceil_div(nB, 256)), 256, 0, s>>>(o, v, Kp, i, j, k, t2,
-                                                                                     c.Cooov, B);
-        CCX_HIP(hipGetLastError());
-        // X_s[x][y][z'] = sum_e <yx|ep> t_rq^{ze} - sum_m <mz|qr> t_pm^{xy}, with <yx|ep> = (px|ye) =
-        // Covvv[p][x][y][e]: the twelve t3c_ijk terms of cctriples.py:50-62, four per GEMM
-        const int ps[3] = {i, j, k};
-        if (x3) {
-          x3_absmax<double>(B, nB, part, s);
-          x3_scale(part, 1.f, nullptr, 0.f, sc + 1, s);
-          x3_split_rows<double>(B, 3 * 2 * v, Kp, Kp, sc + 1, Bhl, Bhl + 32, Kp, s, true);
-          for (int sl = 0; sl < 3; ++sl) {
-            X3Gemm g;   // X_s[(x,y)][z'] = sum_kk A_p[(x,y)][kk] B_s[z'][kk]
-            g.M = vv; g.N = 2 * v; g.Kp = Kp; g.lda = 2 * Kp; g.ldb = 2 * Kp; g.ldc = 2 * v;
-            g.kb = 32; g.ska = 64; g.skb = 64;
-            g.Ah = Ahl + (int64_t)ps[sl] * vv * 2 * Kp; g.Al = (const f16*)g.Ah + 32;
-            g.Bh = Bhl + (int64_t)sl * 2 * v * 2 * Kp; g.Bl = (const f16*)g.Bh + 32;
-            g.sa = sc; g.sb = sc + 1; g.C64 = X + sl * 2 * vvv;
-            x3_gemm<_Float16>(g, s, &ws);
-          }
-        } else {
-          for (int sl = 0; sl < 3; ++sl)
-            contract<double>(1.0, tv(A + (int64_t)ps[sl] * vv * Kp, "xyk", VVK), tv(B + sl * 2 * v * Kp, "zk", ZK),
-                             0.0, tv(X + sl * 2 * vvv, "xyz", VV2), ws, s);
+      for (int k0 = 0; k0 <= j; k0 += (int)KC) {
+        const int nk = (int)std::min<int64_t>(KC, j + 1 - k0);
+        // B rows: [0, nk 2v) the p = i entries (j, k), [nk 2v, 2 nk 2v) the p = j entries (i, k), then
+        // the p = k entry (i, j); X_s[x][y][z'] = sum_e <yx|ep> t_rq^{ze} - sum_m <mz|qr> t_pm^{xy} with
+        // <yx|ep> = (px|ye) = Covvv[p][x][y][e]: the twelve t3c_ijk terms of cctriples.py:50-62
+        T3Pairs pi{nk, {}, {}}, pj{nk, {}, {}}, pk{1, {}, {}};
+        for (int e = 0; e < nk; ++e) {
+          pi.q[e] = j; pi.r[e] = k0 + e;
+          pj.q[e] = i; pj.r[e] = k0 + e;
         }
-        t3_assemble_kernel<<<(int)tiles, 256, 0, s>>>(v, X, W);
+        pk.q[0] = i; pk.r[0] = j;
+        double* BI = B;
+        double* BJ = BI + (int64_t)nk * 2 * v * Kp;
+        double* BK = BJ + (int64_t)nk * 2 * v * Kp;
+        const int64_t nrows = (2 * (int64_t)nk + 1) * 2 * v;
+        const int pg = (int)std::min<int64_t>(1024, ceil_div((int64_t)nk * 2 * v * Kp, 256));
+        t3_pack_kernel<<<pg, 256, 0, s>>>(o, v, Kp, pi, t2, c.Cooov, BI);
         CCX_HIP(hipGetLastError());
-        t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, partials);
+        t3_pack_kernel<<<pg, 256, 0, s>>>(o, v, Kp, pj, t2, c.Cooov, BJ);
         CCX_HIP(hipGetLastError());
-        const double weight = 2.0 - (double)((i == j) + (i == k) + (j == k));   // cctriples.py:237
-        t3_accumulate_kernel<<<1, 256, 0, s>>>(partials, tiles, weight, e_dev);
+        t3_pack_kernel<<<(int)std::min<int64_t>(1024, ceil_div(2 * v * Kp, 256)), 256, 0, s>>>(o, v, Kp, pk, t2,
+                                                                                           c.Cooov, BK);
         CCX_HIP(hipGetLastError());
+        if (x3) {
+          x3_absmax<double>(B, nrows * Kp, part, s);
+          x3_scale(part, 1.f, nullptr, 0.f, sc + 1, s);
+          x3_split_rows<double>(B, nrows, Kp, Kp, sc + 1, Bhl, Bhl + 32, Kp, s, true);
+          const f16* BhI = Bhl;
+          const f16* BhJ = Bhl + (int64_t)nk * 2 * v * 2 * Kp;
+          const f16* BhK = BhJ + (int64_t)nk * 2 * v * 2 * Kp;
+          auto run = [&](const f16* Ah, int64_t M, const f16* Bh, int64_t N, double* C, int64_t ldc) {
+            X3Gemm g;   // C[m][n] = sum_kk A[m][kk] B[n][kk]
+            g.M = M; g.N = N; g.Kp = Kp; g.lda = 2 * Kp; g.ldb = 2 * Kp; g.ldc = ldc;
+            g.kb = 32; g.ska = 64; g.skb = 64;
+            g.Ah = Ah; g.Al = Ah + 32; g.Bh = Bh; g.Bl = Bh + 32;
+            g.sa = scA; g.sb = sc + 1; g.C64 = C;
+            x3_gemm<_Float16>(g, s, &ws);
+          };
+          run(Ahl + (int64_t)i * vv * 2 * Kp, vv, BhI, 2 * v * nk, XI, ldx);
+          run(Ahl + (int64_t)j * vv * 2 * Kp, vv, BhJ, 2 * v * nk, XJ, ldx);
+          run(Ahl + (int64_t)k0 * vv * 2 * Kp, nk * vv, BhK, 2 * v, XK, 2 * v);
+        } else {
+          const std::vector<int64_t> VVK{v, v, Kp};
+          const TV XIv = tvs(XI, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
+          const TV XJv = tvs(XJ, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
+          contract<double>(1.0, tv(A + (int64_t)i * vv * Kp, "xyk", VVK), tv(BI, "zk", {2 * v * nk, Kp}), 0.0, XIv,
+                           ws, s);
+          contract<double>(1.0, tv(A + (int64_t)j * vv * Kp, "xyk", VVK), tv(BJ, "zk", {2 * v * nk, Kp}), 0.0, XJv,
+                           ws, s);
+          contract<double>(1.0, tv(A + (int64_t)k0 * vv * Kp, "mk", {nk * vv, Kp}), tv(BK, "zk", {2 * v, Kp}), 0.0,
+                           tv(XK, "mz", {nk * vv, 2 * v}), ws, s);
+        }
+        for (int e = 0; e < nk; ++e) {
+          const int k = k0 + e;
+          T3Slots xs;
+          xs.X[0] = XI + (int64_t)e * 2 * v; xs.s0[0] = v * ldx; xs.s1[0] = ldx;
+          xs.X[1] = XJ + (int64_t)e * 2 * v; xs.s0[1] = v * ldx; xs.s1[1] = ldx;
+          xs.X[2] = XK + (int64_t)e * vv * 2 * v; xs.s0[2] = 2 * vv; xs.s1[2] = 2 * v;
+          t3_assemble_kernel<<<(int)tiles, 256, 0, s>>>(v, xs, W);
+          CCX_HIP(hipGetLastError());
+          t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, partials);
+          CCX_HIP(hipGetLastError());
+          const double weight = 2.0 - (double)((i == j) + (i == k) + (j == k));   // cctriples.py:237
+          t3_accumulate_kernel<<<1, 256, 0, s>>>(partials, tiles, weight, e_dev);
+          CCX_HIP(hipGetLastError());
+        }
       }
   CCX_HIP(hipStreamSynchronize(s));
   CCX_HIP(hipFree(base));
+  if (Ahl) CCX_HIP(hipFree(Ahl));
+  if (scA) CCX_HIP(hipFree(scA));
 }
 
 }  // namespace ccx

@@ -72,7 +72,7 @@ def test_water_ccsd_t_matches_reference_constants(tmp_path, basis, key):
     assert pycc_amd.cctriples.t_tjl(cc) == cc.et
 
 
-@pytest.mark.parametrize("no,nv,seed", [(1, 5, 1), (2, 3, 2), (3, 17, 3), (5, 24, 4), (6, 33, 5)])
+@pytest.mark.parametrize("no,nv,seed", [(1, 5, 1), (2, 3, 2), (3, 17, 3), (5, 24, 4), (6, 33, 5), (10, 12, 6)])
 def test_triples_matches_oracle(no, nv, seed):
     _need_gpu()
     import pycc_amd
@@ -85,10 +85,10 @@ def test_triples_matches_oracle(no, nv, seed):
     assert abs(et - ref) < 1e-12 * max(abs(ref), 1e-3), (et, ref)
 
 
-def test_triples_density_fitted_and_mixed_contexts():
+@pytest.mark.parametrize("no,nv,seed", [(4, 21, 7), (9, 13, 8)])   # o >= 9: several k chunks per (i, j)
+def test_triples_density_fitted_and_mixed_contexts(no, nv, seed):
     _need_gpu()
     import pycc_amd
-    no, nv, seed = 4, 21, 7
     print("triples DF / mixed no=%d nv=%d" % (no, nv), flush=True)
     sd = synthetic_df(no, nv, seed=seed, t2max=0.1)
     H = DenseHamiltonian(sd.fock, dirac_eri(sd.B))
