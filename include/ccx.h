@@ -160,8 +160,8 @@ int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const d
  * :311), with the connected and disconnected T3 numerators of cctriples.py:27-72 and :108-146 and
  * the orbital-energy denominators from the diagonal of `fock`. FP64, except in CCX_MODE_MIXED_F16X3,
  * whose three GEMMs per occupied triple are FP16x3 split products with FP64 output (E(T) within
- * ~1e-9 relative of FP64). Synchronises `stream`; holds o v^2 (v + o) + 7 v^3 doubles of device
- * scratch (half the first term in MIXED_F16X3) for the duration of the call. */
+ * ~1e-9 relative of FP64). Synchronises `stream`; holds o v^2 (v + o) + 49 v^3 doubles of device
+ * scratch (the first term as FP16 hi/lo in MIXED_F16X3) for the duration of the call. */
 int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
                 void* stream);
 
