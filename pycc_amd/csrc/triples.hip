@@ -8,9 +8,8 @@
 // whose B operand (t2 and -(mq|rc) columns of the two (q, r) orders side by side) is packed per
 // triple. For fixed (i, j), up to T3_KC values of k share one GEMM per slot (wider N for p = i and
 // p = j, taller M for p = k); the three v^2 x 2v blocks of a triple are then added into W's [a][b][c]
-// order through 16^3 LDS tiles.
-// A tiled kernel then adds the disconnected part (t3d_ijk, cctriples.py:108-146: <ij|ab> t1_kc + t2_ijab f_kc and their
-// permutations) to form V and reduces
+// order through 16^3 LDS tiles. A tiled kernel then adds the disconnected part (t3d_ijk,
+// cctriples.py:108-146: <ij|ab> t1_kc + t2_ijab f_kc and their permutations) to form V and reduces
 //
 //   E_ijk = sum_abc V_abc (4 W_abc + W_bca + W_cab - 2 W_acb - 2 W_bac - 2 W_cba) / D_ijk^abc,
 //   E(T)  = sum_{i>=j>=k} (2 - d_ij - d_ik - d_jk) E_ijk,
