@@ -397,7 +397,8 @@ def integral_element_checks(cc, sd, t1, t2, c1, c2, rng, n_fock=16, n_eri=32, h=
 
 def triples_bench(args):
     """The (T) correction (pycc/cctriples.py:177-240) at o=32, v=256 by default (seed 3): converge CCSD in FP64,
-    time E(T) once in FP64, then twice in MIXED_F16X3 (FP16x3 slot GEMMs) on the same amplitudes;
+    time E(T) once in FP64, then twice in MIXED_F16X3 (FP16x3 slot GEMMs) on the same amplitudes,
+    each in ranges of 8 values of i (ccx_triples_range) with a progress line per range;
     `value` is the MIXED_F16X3 time and E_T_error its difference from the FP64 E(T). Algorithmic
     work: the reference's twelve t3c_ijk contractions per triple i >= j >= k, 12 v^3 (v + o) flops,
     priced at the FP16x3 peak (dense FP16 MFMA / 3).
@@ -415,7 +416,8 @@ def triples_bench(args):
     log("[triples] %s CCSD o=%d v=%d: E = %.12f in %d iterations (%.1f s)" % (prec, o, v, ecc, cc.niter, solve_ms / 1e3))
     dp_ms, et_dp = None, None
     if prec == "DP":
-        dp_ms, et_dp = _ms(lambda: cc.triples_energy())
+        dp_ms, et_dp = _ms(lambda: cc.triples_energy(
+            i_chunk=8, progress=lambda d, n: log("[triples] FP64 E(T): i < %d of %d done" % (d, n))))
         log("[triples] FP64 E(T) = %.15f in %.1f ms" % (et_dp, dp_ms))
     t1, t2 = cc.t1.clone(), cc.t2.clone()
     del cc
@@ -423,7 +425,8 @@ def triples_bench(args):
     mx = pycc_amd.CCwfn(sd, no=o, precision="MIXED_F16X3", verbose=False)
     runs = []
     for rep in range(2):
-        ms, et = _ms(lambda: mx.triples_energy(t1, t2))
+        ms, et = _ms(lambda: mx.triples_energy(
+            t1, t2, i_chunk=8, progress=lambda d, n: log("[triples] MIXED_F16X3 E(T): i < %d of %d done" % (d, n))))
         runs.append(ms)
         log("[triples] MIXED_F16X3 E(T) = %.15f in %.1f ms (error %s Eh)" % (
             et, ms, "%.2e" % (et - et_dp) if et_dp is not None else "n/a"))

@@ -165,6 +165,14 @@ int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const d
 int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
                 void* stream);
 
+/* e_dev[0] += the part of ccx_triples' E(T) from the occupied triples i >= j >= k with
+ * i0 <= i < i1 (0 <= i0 <= i1 <= no). Zeroing e_dev[0] and calling it over consecutive ranges that
+ * cover [0, no) in ascending order gives ccx_triples' value bit for bit. A long (T) can then report
+ * progress, or stop and resume, between ranges. Same modes, scratch and synchronisation as
+ * ccx_triples; no counterpart in the reference, whose t_tjl is one call (pycc/cctriples.py:177). */
+int ccx_triples_range(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, int i0, int i1,
+                      double* e_dev, void* stream);
+
 /* Measurement hooks (no counterpart in the reference, which has only wall-clock prints,
  * pycc/utils.py:229-238): HIP events recorded on the residual's stream around the ladder GEMMs,
  * the ring GEMMs and each whole residual.

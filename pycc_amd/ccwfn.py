@@ -393,15 +393,30 @@ class CCwfn(object):
         _lib.call("ccx_pseudoenergy", self._ctx.h, dptr(l2), dptr(self._e_dev[:1]), self._stream())
         return float(self._e_dev[0].item())
 
-    def triples_energy(self, t1=None, t2=None, F=None):
+    def triples_energy(self, t1=None, t2=None, F=None, i_chunk=None, progress=None):
         """The (T) correction E(T) at (F, t1, t2) on the GPU (``pycc/cctriples.py:177-240``,
         ``t_tjl``): FP64, except that MIXED_F16X3 runs its GEMMs as FP16x3 split products (E(T)
-        within ~1e-9 relative of FP64)."""
+        within ~1e-9 relative of FP64).
+
+        With ``i_chunk`` the occupied triples run in ranges of ``i_chunk`` values of the largest
+        index (``ccx_triples_range``), the same value bit for bit; ``progress(i_done, no)`` is called
+        after each range."""
         F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
         t1 = self.t1 if t1 is None else self._as_dev(t1, (self.no, self.nv))
         t2 = self.t2 if t2 is None else self._as_dev(t2, (self.no, self.no, self.nv, self.nv))
-        _lib.call("ccx_triples", self._ctx.h, dptr(F), dptr(t1), dptr(t2), dptr(self._e_dev[:1]),
-                  self._stream())
+        e = self._e_dev[:1]
+        if i_chunk is None:
+            _lib.call("ccx_triples", self._ctx.h, dptr(F), dptr(t1), dptr(t2), dptr(e), self._stream())
+        else:
+            if int(i_chunk) < 1:
+                raise PyCCError("i_chunk must be >= 1")
+            e.zero_()
+            for i0 in range(0, self.no, int(i_chunk)):
+                i1 = min(self.no, i0 + int(i_chunk))
+                _lib.call("ccx_triples_range", self._ctx.h, dptr(F), dptr(t1), dptr(t2), int(i0), int(i1),
+                          dptr(e), self._stream())
+                if progress is not None:
+                    progress(i1, self.no)
         return float(self._e_dev[0].item())
 
     # ---- measurement hooks -----------------------------------------------------------------

@@ -257,7 +257,15 @@ int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double
                 void* stream) {
   return guarded([&] {
     CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
-    ccx::triples_energy(ctx->c, fock, t1, t2, e_dev, st(stream));
+    ccx::triples_energy(ctx->c, fock, t1, t2, 0, ctx->c.o, false, e_dev, st(stream));
+  });
+}
+
+int ccx_triples_range(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, int i0, int i1,
+                      double* e_dev, void* stream) {
+  return guarded([&] {
+    CCX_CHECK(ctx, ccx::ERR_ARG, "null context");
+    ccx::triples_energy(ctx->c, fock, t1, t2, i0, i1, true, e_dev, st(stream));
   });
 }
 

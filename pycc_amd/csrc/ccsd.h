@@ -183,8 +183,10 @@ void density_energy(Ctx& c, const double* t1, const double* t2, const double* l2
                     const double* Doovv, const double* Dvvvo, double* e_dev, hipStream_t s);
 // j1, j2 = the CCSD Jacobian applied to (x1, x2) on the current HBAR (x2 pair-symmetric)
 void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2, hipStream_t s);
-// *e_dev = the closed-shell (T) correction at (F, t1, t2) (triples.hip); synchronises the stream
-void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2, double* e_dev, hipStream_t s);
+// the closed-shell (T) correction at (F, t1, t2) over the triples i >= j >= k with i0 <= i < i1, added to
+// *e_dev (accumulate) or written there (triples.hip); synchronises the stream
+void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2, int i0, int i1, bool accumulate,
+                    double* e_dev, hipStream_t s);
 void timing_start(Ctx& c);
 void timing_read(Ctx& c, double* out, int n);
 void diis_reset(Ctx& c, const double* t1, const double* t2, int max_diis, int cap, hipStream_t s);

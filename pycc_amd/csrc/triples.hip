@@ -274,9 +274,11 @@ __global__ void t3_accumulate_kernel(const double* __restrict__ partials, int64_
 
 }  // namespace
 
-void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2, double* e_dev, hipStream_t s) {
+void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2, int i0, int i1, bool accumulate,
+                    double* e_dev, hipStream_t s) {
   CCX_CHECK(c.have_ints, ERR_STATE, "triples called before the integrals were set");
   CCX_CHECK(F && t1 && t2 && e_dev, ERR_ARG, "null argument");
+  CCX_CHECK(0 <= i0 && i0 <= i1 && i1 <= c.o, ERR_ARG, "triples: need 0 <= i0 <= i1 <= no");
   const int64_t o = c.o, v = c.v, nmo = c.nmo, vv = v * v, vvv = vv * v;
   CCX_CHECK(c.Covvv && c.Cooov && c.Ioovv, ERR_STATE, "the context holds no FP64 <ov|vv>/<oo|ov> blocks");
   const int64_t nt = ceil_div(v, TT), tiles = nt * nt * nt;
@@ -324,10 +326,12 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
     t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A);
     CCX_HIP(hipGetLastError());
   }
-  t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
-  CCX_HIP(hipGetLastError());
+  if (!accumulate) {
+    t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
+    CCX_HIP(hipGetLastError());
+  }
   const int64_t ldx = 2 * v * KC;   // row length of XI, XJ
-  for (int i = 0; i < o; ++i)
+  for (int i = i0; i < i1; ++i)
     for (int j = 0; j <= i; ++j)
       for (int k0 = 0; k0 <= j; k0 += (int)KC) {
         const int nk = (int)std::min<int64_t>(KC, j + 1 - k0);

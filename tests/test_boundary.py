@@ -47,6 +47,7 @@ def test_invalid_arguments_return_error_codes_without_a_gpu():
     assert lib.ccx_create(2, 5, 7, ctypes.byref(h)) == -1
     assert lib.ccx_residual(None, None, None, None, None, None, None) == -1
     assert lib.ccx_triples(None, None, None, None, None, None) == -1
+    assert lib.ccx_triples_range(None, None, None, None, 0, 1, None, None) == -1
     assert b"null context" in lib.ccx_last_error()
     assert lib.ccx_contract(b"ab,bc", 1.0, None, None, None, None, None, None, 0.0, None, None, None, None) == -1
 

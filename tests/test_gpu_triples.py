@@ -103,6 +103,22 @@ def test_triples_density_fitted_and_mixed_contexts(no, nv, seed):
     assert abs(x3 - dp) < 1e-8 * abs(dp), (x3, dp)
 
 
+@pytest.mark.parametrize("precision", ["DP", "MIXED_F16X3"])
+def test_triples_in_ranges_of_i_equal_one_call(precision):
+    _need_gpu()
+    import pycc_amd
+    no, nv, seed = 9, 13, 9
+    print("triples by ranges of i, %s no=%d nv=%d" % (precision, no, nv), flush=True)
+    _, H = _ham(no, nv, seed)
+    t1, t2 = random_amplitudes(no, nv, seed, scale=0.1)
+    cc = pycc_amd.CCwfn(H, no=no, precision=precision, verbose=False)
+    whole = cc.triples_energy(t1, t2)
+    seen = []
+    parts = cc.triples_energy(t1, t2, i_chunk=4, progress=lambda d, n: seen.append((d, n)))
+    assert parts == whole, (parts, whole)
+    assert seen == [(4, 9), (8, 9), (9, 9)]
+
+
 def test_water_ccsd_t_mixed_f16x3_within_one_microhartree(water_dz_path):
     _need_gpu()
     import pycc_amd
