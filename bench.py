@@ -9,9 +9,16 @@ one iteration of the CCSD solve loop (residual, Jacobi update, energy, DIIS), ex
 Legs (each in its own context, run one after the other):
   mixed  -- precision ``--precision`` (default MIXED_F16X3): W warmup + K timed iterations,
             then iterated (untimed) to convergence;
-  fp64   -- precision DP, the same; its converged energy is the FP64 reference at this size.
+  fp64   -- precision DP, the same; its converged energy is the FP64 reference at this size. The
+            iteration and time at which it first meets the mixed leg's thresholds give the time to
+            solution at equal thresholds;
+  others -- ``--secondary`` (default MIXED, MIXED_BF16X3: the FP32 form and the BF16x3-ladder form),
+            the same as the mixed leg, reported under ``other_mixed_forms``.
 ``value`` is the mixed leg's iterations/s; the FP64 leg, the speed-up and the energy / amplitude
-errors of the mixed solution against the FP64 one are reported beside it.
+errors of the mixed solutions against the FP64 one are reported beside it. Each leg reports its
+reference-equivalent rate (pycc's contraction count F_ref per second), its executed rate (the flops
+its kernels run, each product counted once) and the fraction of the peak-bound time (each kernel
+class's flops at the MFMA peak of the precision it runs in).
 
 The roofline object describes the dominant kernel family of the mixed leg (the ring GEMMs), timed
 with HIP events on the residual's stream by the library over the timed iterations; ``traffic``
@@ -36,6 +43,11 @@ if ROOT not in sys.path:
 PEAK = {"DP": 78.6e12, "MIXED": 157.3e12, "MIXED_BF16X3": 157.3e12, "MIXED_F16X3": 2.5e15 / 3}
 PEAK_FP64, PEAK_FP32, PEAK_BF16 = 78.6e12, 157.3e12, 2.5e15
 HBM_PEAK = 8.0e12
+DTYPES = {"MIXED": "f32 (ladder+rings+hole-hole), f64 (rest)",
+          "MIXED_BF16X3": "bf16x3 ladder, f32 rings and hole-hole, f64 rest",
+          "MIXED_F16X3": "f16x3 (ladder, rings and the other large contractions; scaled hi+lo, f32 accumulation), "
+                         "f64 rest",
+          "DP": "f64"}
 
 
 def log(*a):
@@ -69,7 +81,10 @@ def backward_pass(cc, store_vvvv=False):
     return out
 
 
-def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv, backward=False):
+def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv, backward=False, mark=None):
+    """One precision leg: W warm-up and K timed iterations of the solve loop, then (untimed) on to
+    convergence. ``mark = (e_conv2, r_conv2)`` records the iteration and elapsed solve time at which
+    the loop first meets that second pair of thresholds (its convergence test, ccwfn.py:295)."""
     import torch
     import pycc_amd
     t0 = time.time()
@@ -77,12 +92,22 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
     torch.cuda.synchronize()
     setup = time.time() - t0
     log("[%s] setup %.1f s, device memory %.1f GB" % (precision, setup, cc.device_bytes() / 1e9))
+    marked = {}
+
+    def step():
+        r = next(it, None)
+        if r is not None and mark and not marked and abs(r["ediff"]) < mark[0] and r["rms"] < mark[1]:
+            torch.cuda.synchronize()
+            marked.update(iterations=r["niter"], time_s=time.perf_counter() - t_solve,
+                          e_conv=mark[0], r_conv=mark[1])
+        return r
+
     it = cc.iterate(e_conv, r_conv, maxiter, 8, 1)
     last = None
     done = False
     t_solve = time.perf_counter()
     for _ in range(warmup):
-        last = next(it, None)
+        last = step()
         if last is None or last["converged"]:
             done = True
             break
@@ -91,7 +116,7 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
     t1 = time.perf_counter()
     n = 0
     while n < steps and not done:
-        last = next(it, None)
+        last = step()
         if last is None:
             done = True
             break
@@ -105,7 +130,7 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
         precision, n, wall / max(n, 1), last["ecc"] if last else float("nan"), last["rms"] if last else 0))
     if converge:
         while not done:
-            last = next(it, None)
+            last = step()
             if last is None:
                 break
             if last["niter"] % 5 == 0:
@@ -121,7 +146,7 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
         "timing": timing, "converged": bool(last and last["converged"]),
         "iterations": cc.niter, "energy": cc.ecc if (last and last["converged"]) else (last["ecc"] if last else None),
         "device_GB": cc.device_bytes() / 1e9, "solve_s": solve_s if converge else None,
-        "e_conv": e_conv, "r_conv": r_conv,
+        "e_conv": e_conv, "r_conv": r_conv, "at_mark": marked or None,
     }
     if backward and out["converged"]:
         out["backward"] = backward_pass(cc)
@@ -132,6 +157,45 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
     del cc, it
     torch.cuda.empty_cache()
     return out, t1c, t2c
+
+
+def cpu_info():
+    """CPU model, logical CPU counts and the BLAS numpy calls (threadpoolctl), for the CPU baseline."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = None
+    try:
+        from threadpoolctl import threadpool_info
+        for lib in threadpool_info():
+            if lib.get("user_api") == "blas":
+                blas = "%s %s (%s, %s)" % (lib.get("internal_api"), lib.get("version"),
+                                           os.path.basename(lib.get("filepath", "")), lib.get("threading_layer"))
+                break
+    except Exception:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count()
+    return {"model": model, "os_cpu_count": os.cpu_count(), "affinity": affinity, "blas": blas}
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: the CPU share this process is given. On a shared GPU host that
+    share is OMP_NUM_THREADS (one GPU's cores), which can be far below os.cpu_count(); otherwise
+    the CPUs this process may run on."""
+    info = cpu_info()
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        return min(int(share), info["affinity"] or int(share))
+    return info["affinity"] or 1
 
 
 def cpu_baseline(threads):
@@ -159,6 +223,30 @@ def cpu_baseline(threads):
     return so, sv, s_per_iter
 
 
+# Peaks by kernel class and precision mode. FP16x3 / BF16x3 products count once against a third of
+# the dense 16-bit MFMA rate (three products per algorithmic product).
+P_X3 = PEAK_BF16 / 3
+CLASS_PEAKS = {
+    "DP": {"ladder": PEAK_FP64, "ring": PEAK_FP64, "hole_hole": PEAK_FP64, "rest": PEAK_FP64},
+    "MIXED": {"ladder": PEAK_FP32, "ring": PEAK_FP32, "hole_hole": PEAK_FP32, "rest": PEAK_FP64},
+    "MIXED_BF16X3": {"ladder": P_X3, "ring": PEAK_FP32, "hole_hole": PEAK_FP32, "rest": PEAK_FP64},
+    "MIXED_F16X3": {"ladder": P_X3, "ring": P_X3, "hole_hole": P_X3, "rest": P_X3},
+}
+
+
+def leg_rates(precision, o, v, ms_per_step):
+    """Reference-equivalent and executed TFLOP/s of one leg, and the fraction of the peak-bound
+    time: sum over kernel classes of (class flops / class peak), divided by the measured time."""
+    from pycc_amd.flops import f_ref, f_executed, f_ladder_kernel, f_ring_kernel, f_rest, f_hole_hole
+    t = ms_per_step * 1e-3
+    parts = {"ladder": f_ladder_kernel(o, v), "ring": f_ring_kernel(o, v), "hole_hole": f_hole_hole(o, v),
+             "rest": f_rest(o, v) - f_hole_hole(o, v)}
+    t_peak = sum(parts[k] / CLASS_PEAKS[precision][k] for k in parts)
+    return {"tflops_reference_equivalent": f_ref(o, v) / t / 1e12,
+            "tflops_executed": f_executed(o, v) / t / 1e12,
+            "peak_bound_ms": 1e3 * t_peak, "frac_of_peak": t_peak / t}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +267,9 @@ def main():
     ap.add_argument("--no-dp", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-backward", action="store_true", help="skip the FP64 backward pass")
+    ap.add_argument("--secondary", default="MIXED,MIXED_BF16X3",
+                    help="comma-separated further mixed forms timed and converged after the FP64 leg "
+                         "(empty: none)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only copy a GIB-sized FP64 device buffer 3 times (calibrates HBM byte counters)")
     ap.add_argument("--backward", action="store_true",
@@ -208,9 +299,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}))
         return
 
-    import numpy as np
     import torch
-    from pycc_amd.flops import f_ref, f_ladder_kernel, f_ring_kernel
+    from pycc_amd.flops import f_ref, f_ladder_kernel, f_executed
     from pycc_amd.synth import synthetic_df
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -220,16 +310,36 @@ def main():
     log("synthetic Hamiltonian o=%d v=%d naux=%d seed=%d generated in %.1f s" % (o, v, sd.naux, args.seed, time.time() - t0))
 
     converge = not args.no_converge
+    mixed_thr = (args.e_conv_mixed, args.r_conv_mixed)
     mixed, mt1, mt2 = run_leg(sd, o, args.precision, args.warmup, args.steps, converge, args.maxiter,
-                              args.e_conv_mixed if args.precision != "DP" else args.e_conv,
-                              args.r_conv_mixed if args.precision != "DP" else args.r_conv_dp)
+                              *(mixed_thr if args.precision != "DP" else (args.e_conv, args.r_conv_dp)))
     mt1, mt2 = mt1.cpu(), mt2.cpu()
     dp = None
     if not args.no_dp and args.precision != "DP":
         torch.cuda.empty_cache()
         dp, dt1, dt2 = run_leg(sd, o, "DP", args.warmup, args.steps, converge, args.maxiter,
-                               args.e_conv, args.r_conv_dp, backward=not args.no_backward)
+                               args.e_conv, args.r_conv_dp, backward=not args.no_backward, mark=mixed_thr)
         dt1, dt2 = dt1.cpu(), dt2.cpu()
+    # the other mixed forms, each in its own context after the ones above are freed
+    others = {}
+    for prec in [p for p in args.secondary.split(",") if p and p != args.precision and p != "DP"]:
+        torch.cuda.empty_cache()
+        leg, t1x, t2x = run_leg(sd, o, prec, args.warmup, args.steps, converge, args.maxiter, *mixed_thr)
+        others[prec] = (leg, t1x.cpu(), t2x.cpu())
+        del t1x, t2x
+
+    def versus_fp64(leg, t1x, t2x):
+        out = {}
+        if dp and dp["iters_per_s"]:
+            out["speedup_vs_fp64"] = leg["iters_per_s"] / dp["iters_per_s"]
+        if dp and leg["converged"] and dp["converged"]:
+            out["energy_error_uEh"] = 1e6 * (leg["energy"] - dp["energy"])
+            out["max_abs_dT1"] = float((t1x - dt1).abs().max())
+            out["max_abs_dT2"] = float((t2x - dt2).abs().max())
+            mk = dp.get("at_mark")
+            if mk and leg["solve_s"]:
+                out["time_to_solution_vs_fp64_same_thresholds"] = mk["time_s"] / leg["solve_s"]
+        return out
 
     F = f_ref(o, v)
     res = {}
@@ -243,25 +353,21 @@ def main():
     res["higher_is_better"] = True
     res["scaling"] = "weak"
     res["vs_baseline"] = None
-    res["dtype"] = {"MIXED": "f32 (ladder+rings), f64 (rest)", "MIXED_BF16X3": "bf16x3 ladder, f32 rings, f64 rest",
-                    "MIXED_F16X3": "f16x3 (ladder+rings, scaled hi+lo, f32 accumulation), f64 rest",
-                    "DP": "f64"}[args.precision]
+    res["dtype"] = DTYPES[args.precision]
     res["data"] = "synthetic (seeded DF Hamiltonian, no external data)"
     res["config"] = {"workload": "closed-shell CCSD iteration, synthetic DF Hamiltonian",
                      "no": o, "nv": v, "nmo": o + v, "naux": sd.naux, "seed": args.seed,
                      "precision": args.precision, "diis": 8}
-    res["tflops"] = F / (mixed["ms_per_step"] * 1e-3) / 1e12
+    res.update(leg_rates(args.precision, o, v, mixed["ms_per_step"]))
     res["f_ref_per_iter"] = F
+    res["f_executed_per_iter"] = f_executed(o, v)
     if dp:
         res["fp64"] = {"iters_per_s": dp["iters_per_s"], "ms_per_step": dp["ms_per_step"],
-                       "tflops": F / (dp["ms_per_step"] * 1e-3) / 1e12,
                        "iterations": dp["iterations"], "energy": dp["energy"], "converged": dp["converged"],
-                       "time_to_solution_s": dp["solve_s"], "e_conv": dp["e_conv"], "r_conv": dp["r_conv"]}
-        res["speedup_vs_fp64"] = mixed["iters_per_s"] / dp["iters_per_s"] if dp["iters_per_s"] else None
-        if mixed["energy"] is not None and dp["energy"] is not None:
-            res["energy_error_uEh"] = 1e6 * (mixed["energy"] - dp["energy"])
-            res["max_abs_dT1"] = float((mt1 - dt1).abs().max())
-            res["max_abs_dT2"] = float((mt2 - dt2).abs().max())
+                       "time_to_solution_s": dp["solve_s"], "e_conv": dp["e_conv"], "r_conv": dp["r_conv"],
+                       "at_mixed_thresholds": dp["at_mark"]}
+        res["fp64"].update(leg_rates("DP", o, v, dp["ms_per_step"]))
+        res.update(versus_fp64(mixed, mt1, mt2))
     if dp and "backward" in dp:
         b = dp["backward"]
         b["workload"] = ("FP64 Lagrangian backward (factored HBAR + Lambda residual = vJp with the energy "
@@ -270,6 +376,18 @@ def main():
     res["mixed"] = {"iterations": mixed["iterations"], "energy": mixed["energy"], "converged": mixed["converged"],
                     "time_to_solution_s": mixed["solve_s"], "e_conv": mixed["e_conv"], "r_conv": mixed["r_conv"],
                     "setup_s": mixed["setup_s"], "device_GB": mixed["device_GB"]}
+    for prec, (leg, t1x, t2x) in others.items():
+        n_r, ms_r, fl_r = leg["timing"]["ring"]
+        n_l, ms_l, fl_l = leg["timing"]["ladder"]
+        entry = {"dtype": DTYPES[prec], "iters_per_s": leg["iters_per_s"], "ms_per_step": leg["ms_per_step"],
+                 "iterations": leg["iterations"], "energy": leg["energy"], "converged": leg["converged"],
+                 "time_to_solution_s": leg["solve_s"], "e_conv": leg["e_conv"], "r_conv": leg["r_conv"],
+                 "ring_tflops": fl_r / (ms_r * 1e-3) / 1e12 if ms_r > 0 else None,
+                 "ladder_tflops": fl_l / (ms_l * 1e-3) / 1e12 if ms_l > 0 else None,
+                 "device_GB": leg["device_GB"]}
+        entry.update(leg_rates(prec, o, v, leg["ms_per_step"]))
+        entry.update(versus_fp64(leg, t1x, t2x))
+        res.setdefault("other_mixed_forms", {})[prec] = entry
 
     # roofline of the dominant kernel family (ring GEMMs) over the timed iterations
     n_r, ms_r, fl_r = mixed["timing"]["ring"]
@@ -295,11 +413,13 @@ def main():
     res["ring_ms_per_step"] = ms_r / max(mixed["timed_steps"], 1)
 
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
+        info = cpu_info()
+        log("[cpu] numpy oracle on %d threads (%s; %s)" % (threads, info["model"], info["blas"]))
         so, sv, s_it = cpu_baseline(threads)
         scale = f_ref(o, v) / f_ref(so, sv)
         res["cpu_baseline"] = {"value": 1.0 / (s_it * scale), "unit": "iters/s", "cores": threads,
-                               "kind": "port",
+                               "kind": "port", "cpu": info,
                                "sample": "numpy oracle (restated pycc numpy path), median of 3 iterations at "
                                          "o=16,v=128 seed 0 = %.3f s/iter, scaled by F_ref ratio %.1f" % (s_it, scale)}
     print(json.dumps(res), flush=True)

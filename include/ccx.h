@@ -41,8 +41,9 @@ typedef struct ccx_ctx ccx_ctx;
 #define CCX_MODE_MIXED 1        /* ladder + ring GEMMs in FP32 (v_mfma_f32_16x16x4f32, FP32 accum.); */
                                 /* every other term, the update, DIIS, norms and energy in FP64      */
 #define CCX_MODE_MIXED_BF16X3 2 /* as MIXED, with the ladder as BF16x3 split products                */
-#define CCX_MODE_MIXED_F16X3 3  /* BF16x3 ladder; ring GEMMs as FP16x3 split products (hi + lo, 22  */
-                                /* bits, power-of-two operand scales, FP32 accumulation)             */
+#define CCX_MODE_MIXED_F16X3 3  /* ladder, rings and every other large contraction as FP16x3 split  */
+                                /* products (hi + lo, 22 bits, power-of-two operand scales, FP32     */
+                                /* accumulation); update, DIIS, norms, energy and Fmi's tau~.L FP64  */
 
 #define CCX_OK 0
 #define CCX_ERR_ARG (-1)

@@ -360,9 +360,10 @@ def ao_integrals_shells(atoms, basis=CCPVDZ, spherical=True):
     return S, T, V, eri, enuc
 
 
-def water_hamiltonian(basis="cc-pvdz"):
-    """(F_mo, ERI_mo Dirac, E_scf, nocc, h_mo, eri_mo chemist, E_nuc) for pycc's "H2O" molecule."""
-    atoms = water_geometry()
+def water_hamiltonian(basis="cc-pvdz", r_oh_ang=1.1, angle_deg=104.0):
+    """(F_mo, ERI_mo Dirac, E_scf, nocc, h_mo, eri_mo chemist, E_nuc) for water; the default geometry
+    is pycc's "H2O" molecule (O-H 1.1 A, H-O-H 104 degrees)."""
+    atoms = water_geometry(r_oh_ang, angle_deg)
     S, T, V, eri, enuc = ao_integrals_shells(atoms, BASIS_SETS[basis.lower()])
     nocc = 5
     E, C, eps, F_ao = rhf(S, T + V, eri, nocc)

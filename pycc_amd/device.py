@@ -8,8 +8,8 @@ resolves the compute device. It differs from the reference in two deliberate way
 * ``precision`` selects the MI355X precision mode of the contraction kernels: ``'DP'`` (FP64
   everywhere), ``'MIXED'`` (FP32 MFMA ladder + ring GEMMs, everything else FP64; ``'SP'`` is
   accepted as its alias), ``'MIXED_BF16X3'`` (as MIXED, ladder as BF16x3 split products) or
-  ``'MIXED_F16X3'`` (BF16x3 ladder, ring GEMMs as FP16x3 split products with power-of-two
-  operand scales).
+  ``'MIXED_F16X3'`` (the ladder, the ring GEMMs and the other large contractions as FP16x3 split
+  products with power-of-two operand scales).
 
 ``ContractionBackend.__call__(subscripts, A, B)`` mirrors ``pycc/device.py:64-86`` for two torch
 operands on the GPU and runs on the library's MFMA contraction kernel (``ccx_contract``).

@@ -63,3 +63,28 @@ def f_ring_kernel(o: int, v: int) -> float:
 
 def f_kernel(o: int, v: int) -> float:
     return f_ladder_kernel(o, v) + f_ring_kernel(o, v)
+
+
+# Sites of F_ref that the kernels evaluate differently: the ladder (:931) and Zmbij (:715) as the
+# packed ladder GEMM, the six o^3 v^3 W / ring contractions (:644, :645, :683, :933-935) as the ring
+# GEMM units, and the t1 t1 <ov|vo> / <ov|ov> terms (:937-938), factored as t_ma (t_ie <mb|ej>)
+# (4 o^3 v^2 each).
+_REPLACED = {"ijef,abef->ijab", "mbef,ijef->mbij", "jnfb,mnef->mbej", "njfb,mnef->mbej", "jnfb,mnfe->mbje",
+             "imae,mbej->ijab", "mjae,mbie->ijab", "imea,mbej->ijab", "imeb,maje->ijab"}
+# the two o^4 v^2 hole-hole terms (:603, :930), which run in the ring / ladder precision
+_HOLE_HOLE = {"ijef,mnef->mnij", "mnab,mnij->ijab"}
+
+
+def f_rest(o: int, v: int) -> float:
+    """Flops of every contraction besides the ladder, Z and the ring units, as the kernels run them."""
+    rest = sum(n * _site_flops(s, o, v) for s, n in REFERENCE_SITES if s not in _REPLACED)
+    return float(rest + 2 * 4.0 * o ** 3 * v ** 2)
+
+
+def f_hole_hole(o: int, v: int) -> float:
+    return float(sum(n * _site_flops(s, o, v) for s, n in REFERENCE_SITES if s in _HOLE_HOLE))
+
+
+def f_executed(o: int, v: int) -> float:
+    """Flops one residual evaluation executes (each product counted once, whatever its precision)."""
+    return f_kernel(o, v) + f_rest(o, v)

@@ -78,3 +78,42 @@ def test_ccwfn_rejects_unknown_kwargs_and_models():
     with pytest.raises(InvalidKeywordError):
         CCwfn(None, no=1, model='CC3')
     assert 'CCSD(T)' in CCwfn.VALID_MODELS
+
+
+class _RecordingLib:
+    """Stands in for libccx in the pycc binding: records every call, computes nothing."""
+
+    def __init__(self):
+        self.calls = []
+
+    def __getattr__(self, name):
+        def fn(*args):
+            self.calls.append(name)
+            return 0
+        return fn
+
+
+def test_pycc_binding_refuses_bad_operands_before_calling_the_library():
+    """pycc's 'SP' hands float32 tensors to the binding (pycc/device.py:147-151); the library reads
+    FP64 only. The binding converts float32 / float64 CUDA tensors and refuses anything else
+    (host memory, integer dtypes, wrong shapes) before any library call."""
+    import torch
+    from pycc_amd import ccx_backend as cb
+    ctx = object.__new__(cb.CCXContext)
+    ctx.lib, ctx.h = _RecordingLib(), ctypes.c_void_p(1)
+    ctx.no, ctx.nv, ctx.nmo = 2, 3, 5
+    F = torch.zeros(5, 5, dtype=torch.float32)
+    t1 = torch.zeros(2, 3, dtype=torch.float32)
+    t2 = torch.zeros(2, 2, 3, 3, dtype=torch.float32)
+    for call in (ctx.residuals, ctx.energy, ctx.triples):
+        with pytest.raises(TypeError, match="on the GPU"):
+            call(F, t1, t2)                        # host tensors
+    with pytest.raises(TypeError, match="float32 or float64"):
+        cb.as_fp64(torch.zeros(2, 3, dtype=torch.int32), "t1")
+    with pytest.raises(TypeError, match="torch tensor"):
+        cb.as_fp64([[0.0] * 3] * 2, "t1")
+    assert ctx.lib.calls == []
+    ctx.h = None
+    with pytest.raises(ValueError, match="no libccx mode"):
+        cb.CCXContext(2, 3, "HALF", None, None, lib=_RecordingLib())
+    assert set(cb.CCXContext.MODES) == {"DP", "SP", "F16X3"}
