@@ -77,21 +77,64 @@ void gather_chem(const double* eri, int64_t nmo, int64_t p0, int64_t np, int64_t
 }
 
 // ---------------------------------------------------------------------------------------------
-__global__ void tau_kernel(int64_t o, int64_t v, double f1, double f2, const double* __restrict__ t1,
-                           const double* __restrict__ t2, double* __restrict__ tau) {
-  int64_t total = o * o * v * v;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t b = e % v, t = e / v;
-    int64_t a = t % v; t /= v;
-    int64_t j = t % o, i = t / o;
-    tau[e] = f1 * t2[e] + f2 * (t1[i * v + a] * t1[j * v + b]);
+// tau = f1 t2 + f2 t1 (x) t1. One block per (pair ij, chunk of 256 VEC TAU_STEPS plane elements):
+// the pair and the plane position come from 32-bit arithmetic on the block index, and each thread
+// moves VEC consecutive doubles per step (16-byte loads and stores when the v x v plane has an even
+// size, so that every plane starts on a 16-byte boundary); all loads of a thread are issued before
+// its first store.
+constexpr int TAU_STEPS = 4;
+template <int VEC>
+__global__ __launch_bounds__(256) void tau_kernel(int o, int v, int nchunk, double f1, double f2,
+                                                  const double* __restrict__ t1, const double* __restrict__ t2,
+                                                  double* __restrict__ tau) {
+  const int ij = (int)(blockIdx.x / (unsigned)nchunk), chunk = (int)blockIdx.x - ij * nchunk;
+  const int i = ij / o, j = ij - i * o;
+  const uint32_t plane = (uint32_t)v * (uint32_t)v;
+  const double* __restrict__ ti = t1 + (int64_t)i * v;
+  const double* __restrict__ tj = t1 + (int64_t)j * v;
+  const int64_t base = (int64_t)ij * plane;
+  const uint32_t e0 = (uint32_t)chunk * (256u * VEC * TAU_STEPS) + threadIdx.x * VEC;
+  if (VEC == 2) {
+    double2 x[TAU_STEPS];
+#pragma unroll
+    for (int k = 0; k < TAU_STEPS; ++k) {
+      const uint32_t e = e0 + k * 256u * VEC;
+      if (e < plane) x[k] = *(const double2*)(t2 + base + e);
+    }
+#pragma unroll
+    for (int k = 0; k < TAU_STEPS; ++k) {
+      const uint32_t e = e0 + k * 256u * VEC;
+      if (e < plane) {
+        const uint32_t a0 = e / (uint32_t)v, b0 = e - a0 * v;
+        const uint32_t a1 = (e + 1) / (uint32_t)v, b1 = e + 1 - a1 * v;
+        double2 y;
+        y.x = f1 * x[k].x + f2 * (ti[a0] * tj[b0]);
+        y.y = f1 * x[k].y + f2 * (ti[a1] * tj[b1]);
+        *(double2*)(tau + base + e) = y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < TAU_STEPS; ++k) {
+      const uint32_t e = e0 + k * 256u;
+      if (e < plane) {
+        const uint32_t a = e / (uint32_t)v, b = e - a * v;
+        tau[base + e] = f1 * t2[base + e] + f2 * (ti[a] * tj[b]);
+      }
+    }
   }
 }
 
 void build_tau(int64_t o, int64_t v, double f1, double f2, const double* t1, const double* t2,
                double* tau, hipStream_t s) {
-  tau_kernel<<<grid_for(o * o * v * v), 256, 0, s>>>(o, v, f1, f2, t1, t2, tau);
+  if (o == 0 || v == 0) return;
+  CCX_CHECK(v * v < (1ll << 31), ERR_ARG, "build_tau: v x v plane too large");
+  const int vec = (v % 2 == 0) ? 2 : 1;
+  const int64_t nchunk = ceil_div(v * v, 256 * vec * TAU_STEPS);
+  CCX_CHECK(o * o * nchunk < (1ll << 31), ERR_ARG, "build_tau: grid too large");
+  const unsigned grid = (unsigned)(o * o * nchunk);
+  if (vec == 2) tau_kernel<2><<<grid, 256, 0, s>>>((int)o, (int)v, (int)nchunk, f1, f2, t1, t2, tau);
+  else tau_kernel<1><<<grid, 256, 0, s>>>((int)o, (int)v, (int)nchunk, f1, f2, t1, t2, tau);
   CCX_HIP(hipGetLastError());
 }
 
@@ -632,36 +675,62 @@ void accumulate(int64_t n, double alpha, const float* in, double* out, hipStream
 
 // ---- bandwidth-bound weighted sums (the GEMV-shaped FP64 contractions of the residual) ----
 // out[p ldo + q] = beta out + alpha sum_{n < N, f < F} w[n ldw + f] X[p sp + q sq + n sn + f]:
-// one wave per output, lanes along the contiguous f, a fixed-order butterfly at the end.
-__global__ __launch_bounds__(256) void wsum_kernel(int64_t P, int64_t Q, int64_t N, int64_t F, double alpha,
+// one wave per output. The (n, f) range is walked as segments of 64 VEC contiguous elements (one
+// load per lane, VEC = 2 for 16-byte loads); WSUM_U segments are loaded before any is summed, so
+// each wave keeps WSUM_U independent loads of X in flight. Lanes sum in a fixed order, then a
+// fixed butterfly.
+constexpr int WSUM_U = 8;
+template <int VEC>
+__global__ __launch_bounds__(256) void wsum_kernel(int64_t P, int64_t Q, int N, int F, double alpha,
                                                    const double* __restrict__ w, int64_t ldw,
                                                    const double* __restrict__ X, int64_t sp, int64_t sq, int64_t sn,
                                                    double beta, double* __restrict__ out, int64_t ldo) {
   const int lane = threadIdx.x & 63;
-  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; t < P * Q; t += nw) {
-    const int64_t p = t / Q, q = t - p * Q;
-    const double* x = X + p * sp + q * sq;
-    double a0 = 0.0, a1 = 0.0;
-    int64_t n = 0;
-    for (; n + 1 < N; n += 2) {
-      const double *x0 = x + n * sn, *x1 = x0 + sn, *w0 = w + n * ldw, *w1 = w0 + ldw;
-      for (int64_t f = lane; f < F; f += 64) {
-        a0 = fma(w0[f], x0[f], a0);
-        a1 = fma(w1[f], x1[f], a1);
+  const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (t >= P * Q) return;
+  const int64_t p = t / Q, q = t - p * Q;
+  const double* __restrict__ x = X + p * sp + q * sq;
+  constexpr int SEG = 64 * VEC;
+  const int fc = (F + SEG - 1) / SEG;   // segments per n
+  const int nseg = N * fc;
+  double acc[WSUM_U];
+#pragma unroll
+  for (int u = 0; u < WSUM_U; ++u) acc[u] = 0.0;
+  for (int c0 = 0; c0 < nseg; c0 += WSUM_U) {
+    double xv[WSUM_U][VEC], wv[WSUM_U][VEC];
+#pragma unroll
+    for (int u = 0; u < WSUM_U; ++u) {
+      const int c = c0 + u;
+      const int n = c / fc;
+      const int f = (c - n * fc) * SEG + lane * VEC;
+      if (c < nseg && f < F) {
+        if (VEC == 2) {
+          const double2 a = *(const double2*)(x + n * sn + f);
+          const double2 b = *(const double2*)(w + n * ldw + f);
+          xv[u][0] = a.x; xv[u][VEC - 1] = a.y;
+          wv[u][0] = b.x; wv[u][VEC - 1] = b.y;
+        } else {
+          xv[u][0] = x[n * sn + f];
+          wv[u][0] = w[n * ldw + f];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) xv[u][k] = wv[u][k] = 0.0;
       }
     }
-    if (n < N) {
-      const double *x0 = x + n * sn, *w0 = w + n * ldw;
-      for (int64_t f = lane; f < F; f += 64) a0 = fma(w0[f], x0[f], a0);
-    }
-    double acc = a0 + a1;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) {
-      double* d = out + p * ldo + q;
-      *d = (beta == 0.0 ? 0.0 : beta * *d) + alpha * acc;
-    }
+    for (int u = 0; u < WSUM_U; ++u)
+#pragma unroll
+      for (int k = 0; k < VEC; ++k) acc[u] = fma(wv[u][k], xv[u][k], acc[u]);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < WSUM_U; ++u) s += acc[u];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) {
+    double* d = out + p * ldo + q;
+    *d = (beta == 0.0 ? 0.0 : beta * *d) + alpha * s;
   }
 }
 
@@ -696,8 +765,16 @@ __global__ __launch_bounds__(256) void colsum_finish_kernel(int64_t C, int nrc, 
 void wsum(int64_t P, int64_t Q, int64_t N, int64_t F, double alpha, const double* w, int64_t ldw, const double* X,
           int64_t sp, int64_t sq, int64_t sn, double beta, double* out, int64_t ldo, hipStream_t s) {
   if (P * Q == 0) return;
-  const int64_t blocks = std::min<int64_t>(ceil_div(P * Q, 4), 65536);   // 4 waves per block
-  wsum_kernel<<<(unsigned)blocks, 256, 0, s>>>(P, Q, N, F, alpha, w, ldw, X, sp, sq, sn, beta, out, ldo);
+  CCX_CHECK(N < (1ll << 31) / std::max<int64_t>(1, F) && ceil_div(P * Q, 4) < (1ll << 31), ERR_ARG,
+            "wsum: extents too large");
+  const int64_t blocks = ceil_div(P * Q, 4);   // 4 waves per block, one output per wave
+  // 16-byte loads need even strides, an even F and 16-byte aligned bases
+  const bool vec2 = F % 2 == 0 && ldw % 2 == 0 && sp % 2 == 0 && sq % 2 == 0 && sn % 2 == 0 &&
+                    ((uintptr_t)w & 15) == 0 && ((uintptr_t)X & 15) == 0;
+  if (vec2)
+    wsum_kernel<2><<<(unsigned)blocks, 256, 0, s>>>(P, Q, (int)N, (int)F, alpha, w, ldw, X, sp, sq, sn, beta, out, ldo);
+  else
+    wsum_kernel<1><<<(unsigned)blocks, 256, 0, s>>>(P, Q, (int)N, (int)F, alpha, w, ldw, X, sp, sq, sn, beta, out, ldo);
   CCX_HIP(hipGetLastError());
 }
 

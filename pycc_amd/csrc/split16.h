@@ -45,6 +45,9 @@ void x3_absmax(const T* x, int64_t n, float* part, hipStream_t s);
 int x3_partials();
 // *s_out = 2^k such that (ca max(pa) + cb max(pb)^2) * 2^k < 2^15 (pb may be null).
 void x3_scale(const float* pa, float ca, const float* pb, float cb, float* s_out, hipStream_t s);
+// *s_out = 2^k such that max(pa[0 .. n)) * 2^k < 2^15: the scale of the union of the operands whose
+// partial maxima fill pa (n a multiple of x3_partials()).
+void x3_scale_union(const float* pa, int n, float* s_out, hipStream_t s);
 
 // hi/lo[r * ldd + c] = split(src[r * lds + c] * sc), r < rows, c < cols. il: interleaved layout
 // (lo = hi + 32, ldd a multiple of 32; k-block q of a row at 64 q, the row at 2 ldd r).

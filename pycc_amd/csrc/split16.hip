@@ -205,6 +205,185 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
   }
 }
 
+// Skinny split GEMM for M <= 64 (the t1 products of <ov|vv> and the :759 term): HBM-bound on B.
+// One workgroup of 4 waves covers 64 rows (all of A) x 64 NBN columns; wave w owns the 16 NBN
+// columns [16 NBN w, 16 NBN (w + 1)) of the tile. K goes in steps of 64: A's 64 x 64 k-tile (hi and
+// lo) is staged through LDS (144-byte rows: conflict-free 16-byte reads), and B goes from global
+// memory straight into registers, the next step's loads in flight during the current step's MFMAs.
+// Lane l (row l % 16 of a 16-row block, group g = l / 16) holds the 16 k values 16 g .. 16 g + 15
+// of the step as two 8-value halves u = 0, 1: one 32-byte run, so each row's 128-byte line of a
+// 64-k step is read whole. MFMA u of a step sums k = 16 g + 8 u + [0, 8) over the four groups;
+// A's fragments use the same k assignment, so the two MFMAs together cover the step's 64 k.
+constexpr int SK_STEP = 64, SK_PITCH = SK_STEP + 8;
+template <typename E, int NBN>
+__global__ __launch_bounds__(256) void x3_skinny_kernel(X3Gemm p) {
+  typedef typename X3<E>::v8 v8;
+  __shared__ __attribute__((aligned(16))) E sA[2][2][64 * SK_PITCH];   // [stage][hi/lo]
+  const E* __restrict__ Ah = (const E*)p.Ah;
+  const E* __restrict__ Al = (const E*)p.Al;
+  const E* __restrict__ Bh = (const E*)p.Bh;
+  const E* __restrict__ Bl = (const E*)p.Bl;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int fr = lane >> 4, fc = lane & 15;
+  const int64_t n0 = (int64_t)blockIdx.x * (64 * NBN) + wave * (16 * NBN);
+  const int64_t kbeg = p.work ? (int64_t)blockIdx.y * p.kchunk : 0;
+  const int64_t kend = p.work ? (kbeg + p.kchunk < p.Kp ? kbeg + p.kchunk : p.Kp) : p.Kp;
+  const int nbm = (int)((p.M + 15) / 16);   // valid 16-row blocks of A
+  // offset of k in a row (blocked K: k = q kb + r at q sk + r; kb is a multiple of 16, so a run of
+  // 16 k starting at a multiple of 16 stays in one block)
+  auto koff = [&](int64_t k, int64_t sk) -> int64_t {
+    if (!p.kb) return k;
+    const int64_t q = k / p.kb;
+    return q * sk + (k - q * p.kb);
+  };
+  // B rows of this lane (columns past N read row N - 1 and are not stored)
+  int64_t offB[NBN];
+#pragma unroll
+  for (int j = 0; j < NBN; ++j) {
+    const int64_t n = n0 + 16 * j + fc;
+    offB[j] = (n < p.N ? n : p.N - 1) * p.ldb;
+  }
+  // A: thread t stages 16 bytes (k chunk t % 8 of the step) of rows t / 8 and t / 8 + 32
+  const int ar = tid >> 3, ac = tid & 7;
+  int64_t offA[2];
+  bool okA[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t m = ar + 32 * h;
+    okA[h] = m < p.M;
+    offA[h] = (okA[h] ? m : 0) * p.lda;
+  }
+  const v8 zero8 = v8{};
+  v8 ra[2][2];   // [row half][hi/lo]
+  auto aload = [&](int64_t k0) {
+    const int64_t k = k0 + 8 * ac;
+    const bool kok = k < kend;
+    const int64_t o = kok ? koff(k, p.ska) : 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const bool ok = okA[h] && kok;
+      ra[h][0] = ok ? *(const v8*)(Ah + offA[h] + o) : zero8;
+      ra[h][1] = ok ? *(const v8*)(Al + offA[h] + o) : zero8;
+    }
+  };
+  auto astore = [&](int st) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      *(v8*)&sA[st][0][(ar + 32 * h) * SK_PITCH + 8 * ac] = ra[h][0];
+      *(v8*)&sA[st][1][(ar + 32 * h) * SK_PITCH + 8 * ac] = ra[h][1];
+    }
+  };
+  v8 rb0[NBN][2][2], rb1[NBN][2][2];   // two register stages: [block][hi/lo][half]
+  auto bload = [&](v8 (&rb)[NBN][2][2], int64_t k0) {
+    const int64_t k = k0 + 16 * fr;
+    const bool kok = k < kend;
+    const int64_t o = kok ? koff(k, p.skb) : 0;
+#pragma unroll
+    for (int j = 0; j < NBN; ++j)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        rb[j][0][u] = kok ? *(const v8*)(Bh + offB[j] + o + 8 * u) : zero8;
+        rb[j][1][u] = kok ? *(const v8*)(Bl + offB[j] + o + 8 * u) : zero8;
+      }
+  };
+  f4 acc[4][NBN];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  auto compute = [&](int st, const v8 (&rb)[NBN][2][2]) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i >= nbm) break;   // wave-uniform
+        const int at = (16 * i + fc) * SK_PITCH + 16 * fr + 8 * u;
+        const v8 ah = *(const v8*)&sA[st][0][at];
+        const v8 al = *(const v8*)&sA[st][1][at];
+#pragma unroll
+        for (int j = 0; j < NBN; ++j) {
+          acc[i][j] = X3<E>::mfma(al, rb[j][0][u], acc[i][j]);
+          acc[i][j] = X3<E>::mfma(ah, rb[j][0][u], acc[i][j]);
+          acc[i][j] = X3<E>::mfma(ah, rb[j][1][u], acc[i][j]);
+        }
+      }
+  };
+  aload(kbeg);
+  bload(rb0, kbeg);
+  astore(0);
+  __syncthreads();
+  for (int64_t k0 = kbeg; k0 < kend; k0 += 2 * SK_STEP) {
+    {
+      const bool nxt = k0 + SK_STEP < kend;
+      if (nxt) {
+        aload(k0 + SK_STEP);
+        bload(rb1, k0 + SK_STEP);
+      }
+      compute(0, rb0);
+      if (nxt) astore(1);
+      __syncthreads();
+      if (!nxt) break;
+    }
+    {
+      const bool nxt = k0 + 2 * SK_STEP < kend;
+      if (nxt) {
+        aload(k0 + 2 * SK_STEP);
+        bload(rb0, k0 + 2 * SK_STEP);
+      }
+      compute(1, rb1);
+      if (nxt) astore(0);
+      __syncthreads();
+    }
+  }
+
+  // C register r of lane l in block (i, j) is C[16 i + 4 (l / 16) + r][n0 + 16 j + l % 16]
+  if (p.work) {   // split-K: raw partial sums
+    float* Wb = p.work + (int64_t)blockIdx.y * p.M * p.N;
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) {
+      const int64_t n = n0 + 16 * j + fc;
+      if (n >= p.N) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t m = 16 * i + 4 * fr + r;
+          if (m < p.M) Wb[m * p.N + n] = acc[i][j][r];
+        }
+    }
+    return;
+  }
+  float f0 = p.alpha;
+  if (p.sa) f0 = f0 / p.sa[0];
+  if (p.sb) f0 = f0 / p.sb[0];
+#pragma unroll
+  for (int j = 0; j < NBN; ++j) {
+    const int64_t n = n0 + 16 * j + fc;
+    if (n >= p.N) continue;
+    const float f = p.sbn ? f0 / p.sbn[n] : f0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = 16 * i + 4 * fr + r;
+        if (m < p.M) {
+          if (p.C64) {
+            const int64_t at = m * p.ldc + n;
+            double val = (double)(f * acc[i][j][r]);
+            if (p.beta != 0.f) val += (double)p.beta * (p.C64in ? p.C64in[at] : p.C64[at]);
+            if (p.D64) val += p.D64[at] * p.dvec[n];
+            p.C64[at] = val;
+          } else {
+            float* d = p.C + m * p.ldc + n;
+            float val = f * acc[i][j][r];
+            if (p.beta != 0.f) val += p.beta * *d;
+            *d = val;
+          }
+        }
+      }
+  }
+}
+
 // 256 x 256 output tiles for large GEMMs, 8 waves (2 per SIMD) in 2 x 4, each computing 128 x 64.
 // Operand k-tiles (256 rows x 32 k, hi and lo) go from global memory straight into LDS
 // (global_load_lds_dwordx4, issued from inline assembly so that the only waits are the explicit
@@ -712,6 +891,32 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
     scale<float>(p.beta, tvs(p.C, "mn", {p.M, p.N}, {p.ldc, 1}), s);
     return;
   }
+  // two 16-column blocks per wave (128 columns per workgroup): 158 registers, three waves per SIMD
+  constexpr int nbn = 2;
+  const int64_t sk_tiles = ceil_div(p.N, 64 * nbn);
+  if (p.M <= 64 && (p.kb == 0 || p.kb % 16 == 0) && !p.C64in && !p.D64 && (!p.C64 || ws || sk_tiles >= 512)) {
+    // skinny: all of A in one 64-row tile
+    const int64_t tiles = sk_tiles;
+    CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
+    int64_t splits = 1;
+    p.work = nullptr;
+    if (ws && tiles < 512) {   // split K (at least 8 steps of 64 each) to about 1024 workgroups
+      splits = std::max<int64_t>(1, std::min<int64_t>(1024 / tiles, p.Kp / (8 * SK_STEP)));
+      if (splits > 1 || p.C64) {
+        p.kchunk = ceil_div(ceil_div(p.Kp, splits), SK_STEP) * SK_STEP;
+        splits = ceil_div(p.Kp, p.kchunk);
+        p.work = (float*)ws->get((size_t)splits * p.M * p.N * sizeof(float));
+      }
+    }
+    dim3 grid((unsigned)tiles, (unsigned)splits);
+    x3_skinny_kernel<E, nbn><<<grid, 256, 0, s>>>(p);
+    CCX_HIP(hipGetLastError());
+    if (p.work) {
+      x3_reduce_kernel<<<(unsigned)std::min<int64_t>(ceil_div(p.M * p.N, 256), 4096), 256, 0, s>>>(p, (int)splits);
+      CCX_HIP(hipGetLastError());
+    }
+    return;
+  }
   const bool big = p.M >= 256;
   const bool wide = big && p.N >= 8192;   // 320-column tiles: fewer operand panel fetches per output
   const int64_t bt = big ? 256 : 128, bn = wide ? 320 : bt;
@@ -757,6 +962,10 @@ template void x3_absmax<float>(const float*, int64_t, float*, hipStream_t);
 
 void x3_scale(const float* pa, float ca, const float* pb, float cb, float* s_out, hipStream_t s) {
   x3_scale_kernel<<<1, 256, 0, s>>>(pa, X3_RED_BLOCKS, ca, pb ? pb : pa, pb ? X3_RED_BLOCKS : 0, cb, s_out);
+  CCX_HIP(hipGetLastError());
+}
+void x3_scale_union(const float* pa, int n, float* s_out, hipStream_t s) {
+  x3_scale_kernel<<<1, 256, 0, s>>>(pa, n, 1.f, pa, 0, 0.f, s_out);
   CCX_HIP(hipGetLastError());
 }
 int x3_partials() { return X3_RED_BLOCKS; }

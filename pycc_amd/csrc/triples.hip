@@ -92,9 +92,9 @@ __device__ __forceinline__ void t3_tile_origin(int64_t v, int64_t* org) {
 
 // A[p][(x, y)][kk] = Covvv[p][x][y][kk] for kk < v, t2[p][kk - v][x][y] for v <= kk < v + o, 0 above:
 // the two t3c_ijk products that share p as one GEMM operand with K = v + o (padded to Kp).
-__global__ void t3_operand_a_kernel(int64_t o, int64_t v, int64_t Kp, const double* __restrict__ Covvv,
+__global__ void t3_operand_a_kernel(int64_t o, int64_t v, int64_t Kp, int64_t nrow_p, const double* __restrict__ Covvv,
                                     const double* __restrict__ t2, double* __restrict__ A) {
-  const int64_t total = o * v * v * Kp;
+  const int64_t total = nrow_p * v * v * Kp;   // rows p < nrow_p
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t kk = e % Kp, row = e / Kp;   // row = (p, x, y)
@@ -105,6 +105,23 @@ __global__ void t3_operand_a_kernel(int64_t o, int64_t v, int64_t Kp, const doub
     A[e] = val;
   }
 }
+
+// Device buffer owned by one call: freed on every exit, including an exception thrown after it was
+// allocated (hipFree waits for the work queued on the device before it releases the memory).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(int64_t n) { alloc(n); }
+  void alloc(int64_t n) {
+    CCX_HIP(hipMalloc((void**)&p, (size_t)std::max<int64_t>(n, 1) * sizeof(T)));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
 
 // Up to T3_KC (q, r) entries; entry e packs the two orders (q_e, r_e) and (r_e, q_e).
 constexpr int T3_KC = 8;
@@ -294,27 +311,32 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   // scratch: XI, XJ (v^2 x 2v KC each), XK (KC v^2 x 2v), W (v^3), B (rows of the i, j and k entries,
   // (2 KC + 1) 2v x Kp), tile partials, and A (o v^2 x Kp) in FP64 -- or, in MIXED_F16X3, A and B
   // split (hi/lo interleaved, rows of 2 Kp halves) plus absmax partials and the operand scales
-  const int64_t nA = o * vv * Kp, nXc = vv * 2 * v * KC, nB = (2 * KC + 1) * 2 * v * Kp;
+  // A: only the rows p < i1 are read by this range (p is one of i >= j >= k)
+  const int64_t nA = (int64_t)i1 * vv * Kp, nXc = vv * 2 * v * KC, nB = (2 * KC + 1) * 2 * v * Kp;
   const int64_t np = x3_partials();
-  f16* Ahl = nullptr;    // A split, [o v^2][2 Kp]
-  float* scA = nullptr;  // scA[0] = scale of A, then absmax partials
+  DevBuf<f16> Ahl;     // A split, [i1 v^2][2 Kp]
+  DevBuf<float> scA;   // scA[0] = scale of A, then absmax partials
   if (x3) {   // A split first; its FP64 form lives only until then
-    CCX_HIP(hipMalloc(&Ahl, nA * 2 * sizeof(f16)));
-    CCX_HIP(hipMalloc(&scA, (8 + np) * sizeof(float)));
-    double* A64 = nullptr;
-    CCX_HIP(hipMalloc(&A64, nA * sizeof(double)));
-    t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A64);
-    CCX_HIP(hipGetLastError());
-    x3_absmax<double>(A64, nA, scA + 8, s);
-    x3_scale(scA + 8, 1.f, nullptr, 0.f, scA, s);
-    x3_split_rows<double>(A64, o * vv, Kp, Kp, scA, Ahl, Ahl + 32, Kp, s, true);
-    CCX_HIP(hipStreamSynchronize(s));
-    CCX_HIP(hipFree(A64));
+    Ahl.alloc(nA * 2);
+    scA.alloc(8 + 2 * np);
+    // The entries of A over every p are exactly those of Covvv and of t2 (plus zero padding), so its
+    // scale is that of their union: the same for every range of i, whatever rows are built.
+    x3_absmax<double>(c.Covvv, o * vv * v, scA.p + 8, s);
+    x3_absmax<double>(t2, o * o * vv, scA.p + 8 + np, s);
+    x3_scale_union(scA.p + 8, (int)(2 * np), scA.p, s);
+    if (nA > 0) {
+      DevBuf<double> A64(nA);
+      t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv,
+                                                                                           t2, A64.p);
+      CCX_HIP(hipGetLastError());
+      x3_split_rows<double>(A64.p, (int64_t)i1 * vv, Kp, Kp, scA.p, Ahl.p, Ahl.p + 32, Kp, s, true);
+      CCX_HIP(hipStreamSynchronize(s));
+    }
   }
   const int64_t n64 = 3 * nXc + vvv + nB + tiles + (x3 ? 0 : nA);
   const int64_t nx3 = x3 ? nB * 2 * (int64_t)sizeof(f16) / 8 + (np + 8) / 2 + 1 : 0;   // in doubles
-  double* base = nullptr;
-  CCX_HIP(hipMalloc(&base, (n64 + nx3) * sizeof(double)));
+  DevBuf<double> buf(n64 + nx3);
+  double* base = buf.p;
   double *XI = base, *XJ = XI + nXc, *XK = XJ + nXc, *W = XK + nXc, *B = W + vvv, *partials = B + nB;
   double* A = partials + tiles;
   f16* Bhl = x3 ? (f16*)(base + n64) : nullptr;       // B split, [(2 KC + 1) 2v][2 Kp]
@@ -322,8 +344,8 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   float* part = x3 ? sc + 8 : nullptr;
   Workspace& ws = c.ws;
 
-  if (!x3) {
-    t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, c.Covvv, t2, A);
+  if (!x3 && nA > 0) {
+    t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv, t2, A);
     CCX_HIP(hipGetLastError());
   }
   if (!accumulate) {
@@ -368,12 +390,12 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
             g.M = M; g.N = N; g.Kp = Kp; g.lda = 2 * Kp; g.ldb = 2 * Kp; g.ldc = ldc;
             g.kb = 32; g.ska = 64; g.skb = 64;
             g.Ah = Ah; g.Al = Ah + 32; g.Bh = Bh; g.Bl = Bh + 32;
-            g.sa = scA; g.sb = sc + 1; g.C64 = C;
+            g.sa = scA.p; g.sb = sc + 1; g.C64 = C;
             x3_gemm<_Float16>(g, s, &ws);
           };
-          run(Ahl + (int64_t)i * vv * 2 * Kp, vv, BhI, 2 * v * nk, XI, ldx);
-          run(Ahl + (int64_t)j * vv * 2 * Kp, vv, BhJ, 2 * v * nk, XJ, ldx);
-          run(Ahl + (int64_t)k0 * vv * 2 * Kp, nk * vv, BhK, 2 * v, XK, 2 * v);
+          run(Ahl.p + (int64_t)i * vv * 2 * Kp, vv, BhI, 2 * v * nk, XI, ldx);
+          run(Ahl.p + (int64_t)j * vv * 2 * Kp, vv, BhJ, 2 * v * nk, XJ, ldx);
+          run(Ahl.p + (int64_t)k0 * vv * 2 * Kp, nk * vv, BhK, 2 * v, XK, 2 * v);
         } else {
           const std::vector<int64_t> VVK{v, v, Kp};
           const TV XIv = tvs(XI, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
@@ -401,9 +423,6 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
         }
       }
   CCX_HIP(hipStreamSynchronize(s));
-  CCX_HIP(hipFree(base));
-  if (Ahl) CCX_HIP(hipFree(Ahl));
-  if (scA) CCX_HIP(hipFree(scA));
 }
 
 }  // namespace ccx
