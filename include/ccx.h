@@ -28,6 +28,7 @@
 #ifndef CCX_H
 #define CCX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -206,6 +207,19 @@ int ccx_contract_f32(const char* subscripts, float alpha, const float* A, const 
  * of CCX_MODE_MIXED_F16X3 exposed on its own. Uses a per-thread device workspace. */
 int ccx_gemm_f16x3(int transa, int transb, int64_t M, int64_t N, int64_t K, float alpha, const float* A,
                    int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc, void* stream);
+
+/* ccx_contract without any device allocation: the same contraction, with its split-K partial sums in
+ * a fixed device array of CCX_NOALLOC_SCRATCH_BYTES per GPU inside the library (no hipMalloc, no
+ * hipFree, no allocating library call; the split count is chosen to fit, and a request beyond the
+ * array is refused with CCX_ERR_ARG). No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it). Every call on a GPU shares that GPU's array: calls that overlap on one GPU (other threads or
+ * other streams) race. The array's device address is looked up on the first call on each device. */
+#define CCX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int ccx_contract_noalloc(const char* subscripts, double alpha, const double* A, const int64_t* a_shape,
+                         const int64_t* a_strides, const double* B, const int64_t* b_shape,
+                         const int64_t* b_strides, double beta, double* C, const int64_t* c_shape,
+                         const int64_t* c_strides, void* stream);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* ccx_last_error(void);

@@ -55,6 +55,8 @@ SIGNATURES = {
     "ccx_device_bytes": (ctypes.c_int64, [_vp]),
     "ccx_contract": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, _vp, c_i64p, c_i64p, _vp, c_i64p, c_i64p,
                                     ctypes.c_double, _vp, c_i64p, c_i64p, _vp]),
+    "ccx_contract_noalloc": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, _vp, c_i64p, c_i64p, _vp, c_i64p,
+                                            c_i64p, ctypes.c_double, _vp, c_i64p, c_i64p, _vp]),
     "ccx_contract_f32": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, _vp, c_i64p, c_i64p, _vp, c_i64p,
                                         c_i64p, ctypes.c_float, _vp, c_i64p, c_i64p, _vp]),
     "ccx_gemm_f16x3": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,

@@ -3,11 +3,15 @@
 #include "ccsd.h"
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 struct ccx_ctx {
   ccx::Ctx c;
 };
+
+// Scratch of ccx_contract_noalloc (split-K partial sums), CCX_NOALLOC_SCRATCH_BYTES per GPU.
+static __device__ __align__(256) double scratch[CCX_NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 namespace {
 
@@ -62,14 +66,38 @@ ccx::TV make_view(const void* p, const std::string& lab, const int64_t* shape, c
 template <typename T>
 int contract_any(const char* sub, T alpha, const T* A, const int64_t* as, const int64_t* ast, const T* B,
                  const int64_t* bs, const int64_t* bst, T beta, T* C, const int64_t* cs, const int64_t* cst,
-                 void* stream) {
+                 void* stream, ccx::Workspace* fixed_ws = nullptr) {
   static thread_local ccx::Workspace ws;
   return guarded([&] {
     std::string la, lb, lc;
     parse_subscripts(sub, la, lb, lc);
     ccx::contract<T>(alpha, make_view(A, la, as, ast), make_view(B, lb, bs, bst), beta,
-                     make_view(C, lc, cs, cst), ws, st(stream));
+                     make_view(C, lc, cs, cst), fixed_ws ? *fixed_ws : ws, st(stream));
   });
+}
+
+// ---- the allocation-free contraction's scratch ----------------------------------------------------
+// Split-K partial sums of ccx_contract_noalloc live in the file-scope device array `scratch` (one copy
+// per GPU, held until the process ends). Host code never passes `scratch` itself to a kernel: on the
+// host the name is only a stand-in symbol. Its device address is looked up on the first call on each
+// device, with that device current, and kept in scratch_dev[]; kernels get that address.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of the current device's scratch (looked up on the first call on that device).
+void* noalloc_scratch() {
+  int dev = -1;
+  CCX_HIP(hipGetDevice(&dev));
+  CCX_CHECK(dev >= 0 && dev < NOALLOC_MAX_DEVICES, ccx::ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    CCX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    CCX_CHECK(p != nullptr, ccx::ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    scratch_dev[dev] = p;
+  }
+  return scratch_dev[dev];
 }
 
 }  // namespace
@@ -292,6 +320,18 @@ int ccx_contract(const char* subscripts, double alpha, const double* A, const in
                  double beta, double* C, const int64_t* c_shape, const int64_t* c_strides, void* stream) {
   return contract_any<double>(subscripts, alpha, A, a_shape, a_strides, B, b_shape, b_strides, beta, C,
                               c_shape, c_strides, stream);
+}
+
+int ccx_contract_noalloc(const char* subscripts, double alpha, const double* A, const int64_t* a_shape,
+                         const int64_t* a_strides, const double* B, const int64_t* b_shape,
+                         const int64_t* b_strides, double beta, double* C, const int64_t* c_shape,
+                         const int64_t* c_strides, void* stream) {
+  void* p = nullptr;
+  const int rc = guarded([&] { p = noalloc_scratch(); });
+  if (rc != CCX_OK) return rc;
+  ccx::Workspace ws = ccx::Workspace::wrap(p, CCX_NOALLOC_SCRATCH_BYTES);
+  return contract_any<double>(subscripts, alpha, A, a_shape, a_strides, B, b_shape, b_strides, beta, C,
+                              c_shape, c_strides, stream, &ws);
 }
 
 int ccx_contract_f32(const char* subscripts, float alpha, const float* A, const int64_t* a_shape,

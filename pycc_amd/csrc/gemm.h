@@ -6,13 +6,26 @@
 
 namespace ccx {
 
-// Grow-only device scratch owned by a context; `limit` bounds what split-K may request.
+// Grow-only device scratch owned by a context; `limit` bounds what split-K may request. A fixed
+// workspace wraps memory it does not own (size = limit): it never allocates or frees, and a request
+// beyond its size is an error.
 struct Workspace {
   void* ptr = nullptr;
   size_t size = 0;
   size_t limit = size_t(1) << 30;
+  bool fixed = false;
+  static Workspace wrap(void* p, size_t bytes) {
+    Workspace w;
+    w.ptr = p;
+    w.size = w.limit = bytes;
+    w.fixed = true;
+    return w;
+  }
   void* get(size_t bytes) {
     if (bytes > size) {
+      CCX_CHECK(!fixed, ERR_ARG,
+                "workspace request of " + std::to_string(bytes) + " bytes exceeds the fixed scratch of " +
+                    std::to_string(size) + " bytes");
       if (ptr) CCX_HIP(hipFree(ptr));
       ptr = nullptr;
       CCX_HIP(hipMalloc(&ptr, bytes));
@@ -21,7 +34,7 @@ struct Workspace {
     return ptr;
   }
   void release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !fixed) (void)hipFree(ptr);
     ptr = nullptr;
     size = 0;
   }
