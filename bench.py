@@ -81,6 +81,16 @@ def backward_pass(cc, store_vvvv=False):
     return out
 
 
+def free_device():
+    """Release every context no longer referenced (its device memory goes with it) before the next
+    leg builds its own."""
+    import gc
+    import torch
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
 def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv, backward=False, mark=None):
     """One precision leg: W warm-up and K timed iterations of the solve loop, then (untimed) on to
     convergence. ``mark = (e_conv2, r_conv2)`` records the iteration and elapsed solve time at which
@@ -155,7 +165,7 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
             out["backward"]["device_GB"]))
     t1c, t2c = cc.t1.clone(), cc.t2.clone()
     del cc, it
-    torch.cuda.empty_cache()
+    free_device()
     return out, t1c, t2c
 
 
@@ -316,14 +326,14 @@ def main():
     mt1, mt2 = mt1.cpu(), mt2.cpu()
     dp = None
     if not args.no_dp and args.precision != "DP":
-        torch.cuda.empty_cache()
+        free_device()
         dp, dt1, dt2 = run_leg(sd, o, "DP", args.warmup, args.steps, converge, args.maxiter,
                                args.e_conv, args.r_conv_dp, backward=not args.no_backward, mark=mixed_thr)
         dt1, dt2 = dt1.cpu(), dt2.cpu()
     # the other mixed forms, each in its own context after the ones above are freed
     others = {}
     for prec in [p for p in args.secondary.split(",") if p and p != args.precision and p != "DP"]:
-        torch.cuda.empty_cache()
+        free_device()
         leg, t1x, t2x = run_leg(sd, o, prec, args.warmup, args.steps, converge, args.maxiter, *mixed_thr)
         others[prec] = (leg, t1x.cpu(), t2x.cpu())
         del t1x, t2x

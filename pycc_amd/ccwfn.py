@@ -22,6 +22,7 @@ every precision mode; ``precision`` only changes the arithmetic of the ladder an
 from __future__ import annotations
 
 import ctypes
+import weakref
 import math
 import time
 
@@ -282,12 +283,16 @@ class CCwfn(object):
         self._require_dp("HBAR")
         key = tuple((x.data_ptr(), x._version, tuple(x.shape)) for x in (F, t1, t2))
         cur = getattr(self, "_hbar_src", None)
-        if cur is not None and cur["key"] == key and (cur["stored"] or not stored) and cur["owner"] is owner:
+        # the owner (a cchbar, which refers back to this object) is held weakly: no reference cycle
+        # keeps the context, and its device memory, alive after the last user reference goes
+        cur_owner = cur["owner"]() if cur is not None and cur["owner"] is not None else None
+        if cur is not None and cur["key"] == key and (cur["stored"] or not stored) and cur_owner is owner:
             return
         self._hbar_src = None
         _lib.call("ccx_hbar_build", self._ctx.h, dptr(F), dptr(t1), dptr(t2),
                   _lib.HBAR_STORED if stored else 0, self._stream())
-        self._hbar_src = {"key": key, "stored": bool(stored), "owner": owner, "refs": (F, t1, t2)}
+        self._hbar_src = {"key": key, "stored": bool(stored),
+                          "owner": weakref.ref(owner) if owner is not None else None, "refs": (F, t1, t2)}
 
     def _lambda_residual(self, l1, l2, with_source=True):
         """r_L1, r_L2 (cclambda.py:345-369, :468-496) on the context's current HBAR."""

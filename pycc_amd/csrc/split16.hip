@@ -933,6 +933,22 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
   if (ws && tiles < 512) {
     splits = std::min<int64_t>({1024, ceil_div(1024, tiles), p.Kp / (32 * BK)});
     splits = std::max<int64_t>(splits, 1);
+    if (big) {
+      // one 512-thread workgroup per CU: among up to twice as many splits (shorter FP32 partial
+      // sums), take the count whose last round of 256 workgroups is fullest
+      const int64_t smax = std::min<int64_t>(2 * splits, std::max<int64_t>(splits, p.Kp / (32 * BK)));
+      int64_t best = splits;
+      double best_eff = 0.0;
+      for (int64_t sc = splits; sc <= smax; ++sc) {
+        const int64_t wg = tiles * sc;
+        const double eff = (double)wg / (256.0 * (double)ceil_div(wg, 256));
+        if (eff > best_eff + 1e-9) {
+          best_eff = eff;
+          best = sc;
+        }
+      }
+      splits = best;
+    }
     if (splits > 1 || p.C64) {
       p.kchunk = ceil_div(ceil_div(p.Kp, splits), BK) * BK;
       splits = ceil_div(p.Kp, p.kchunk);
@@ -1068,50 +1084,56 @@ __global__ __launch_bounds__(256) void split_udt_kernel(const double* __restrict
 // The FP32 ring W in one pass (ccwfn.py:641-643 Wmbej, :680-682 Wmbje), W[(m,e)][x][(j,b)]:
 //   x = 0: W0 + PT[j][(m,e,b)]  - sum_n t_nb (nj|me)
 //   x = 1: W0 - P2T[j][(m,b,e)] + sum_n t_nb (mj|ne)
-// summed in FP64 and rounded once. One workgroup per (m, j, block of WE e's); thread = b. The two
-// Cooov slices of a block of n go to LDS (read as broadcasts); the P2T tile [b][e] is transposed
+// summed in FP64 and rounded once. One workgroup per (m, j, block of WE e's); thread = b, in
+// chunks of WB. The Cooov coefficients of a block are the same for all its threads: they are read
+// through scalar loads (no LDS broadcast traffic). The W0 and PT operands of a chunk are loaded
+// before the n loop, so their latency hides behind the FMAs; the P2T tile [b][e] is transposed
 // through LDS so that every global access is along b or e.
-constexpr int WE = 16, WN = 64, WB = 256;
+constexpr int WE = 16, WB = 128;
+template <bool FULL>   // FULL: every block holds WE valid e's (v a multiple of WE)
 __global__ __launch_bounds__(WB) void w_build_x3_kernel(int64_t o, int64_t v, const double* __restrict__ t1,
                                                         const double* __restrict__ Cooov,
                                                         const float* __restrict__ W0, const float* __restrict__ PT,
                                                         const float* __restrict__ P2T, float* __restrict__ W) {
-  __shared__ double c1[WN][WE], c2[WN][WE];
   __shared__ float p2[WB][WE + 1];
   const int64_t ov = o * v, neb = (v + WE - 1) / WE;
   const int64_t eb = blockIdx.x % neb, mj = blockIdx.x / neb;
   const int64_t m = mj / o, j = mj - m * o, e0 = eb * WE;
-  const int ne = (int)(v - e0 < WE ? v - e0 : WE);
+  const int ne = FULL ? WE : (int)(v - e0 < WE ? v - e0 : WE);
   const int tid = threadIdx.x;
+  // e's past v (a partial last block) repeat the last valid one; their sums are not stored
+  auto ecl = [&](int e) -> int { return FULL ? e : (e < ne ? e : ne - 1); };
+  const double* __restrict__ c1 = Cooov + (j * o + m) * v + e0;       // (nj|me) = c1[n o^2 v + e]
+  const double* __restrict__ c2 = Cooov + (m * o + j) * o * v + e0;   // (mj|ne) = c2[n v + e]
+  const float* __restrict__ p2src = P2T + (j * o + m) * v * v;
   for (int64_t b0 = 0; b0 < v; b0 += WB) {
     const int64_t b = b0 + tid;
     const bool bok = b < v;
+    const int64_t bc = bok ? b : v - 1;
+    float w0[WE], w1[WE], pt[WE];
+#pragma unroll
+    for (int e = 0; e < WE; ++e) {
+      const int64_t me = m * v + e0 + ecl(e);
+      const int64_t at0 = me * 2 * ov + j * v + bc;
+      w0[e] = W0[at0];
+      w1[e] = W0[at0 + ov];
+      pt[e] = PT[(j * ov + me) * v + bc];
+    }
     double a0[WE], a1[WE];
 #pragma unroll
     for (int e = 0; e < WE; ++e) a0[e] = a1[e] = 0.0;
-    for (int64_t n0 = 0; n0 < o; n0 += WN) {
-      const int nn = (int)(o - n0 < WN ? o - n0 : WN);
-      __syncthreads();
-      for (int q = tid; q < WN * WE; q += WB) {
-        const int n = q / WE, e = q - n * WE;
-        const bool ok = n < nn && e < ne;
-        c1[n][e] = ok ? Cooov[(((n0 + n) * o + j) * o + m) * v + e0 + e] : 0.0;   // (nj|me)
-        c2[n][e] = ok ? Cooov[((m * o + j) * o + n0 + n) * v + e0 + e] : 0.0;     // (mj|ne)
-      }
-      __syncthreads();
-      if (bok) {
-        for (int n = 0; n < nn; ++n) {
-          const double tb = t1[(n0 + n) * v + b];
+    for (int64_t n = 0; n < o; ++n) {
+      const double tb = t1[n * v + bc];
+      const double* __restrict__ r1 = c1 + n * o * o * v;
+      const double* __restrict__ r2 = c2 + n * v;
 #pragma unroll
-          for (int e = 0; e < WE; ++e) {
-            a0[e] = fma(c1[n][e], tb, a0[e]);
-            a1[e] = fma(c2[n][e], tb, a1[e]);
-          }
-        }
+      for (int e = 0; e < WE; ++e) {
+        a0[e] = fma(r1[ecl(e)], tb, a0[e]);
+        a1[e] = fma(r2[ecl(e)], tb, a1[e]);
       }
     }
     // P2T[j][m][b][e0 .. e0 + WE) for the WB b's of this chunk, through LDS
-    const float* p2src = P2T + (j * o + m) * v * v;
+    __syncthreads();
     for (int q = tid; q < WB * WE; q += WB) {
       const int bb = q / WE, e = q - bb * WE;
       p2[bb][e] = (b0 + bb < v && e < ne) ? p2src[(b0 + bb) * v + e0 + e] : 0.f;
@@ -1123,8 +1145,8 @@ __global__ __launch_bounds__(WB) void w_build_x3_kernel(int64_t o, int64_t v, co
         if (e >= ne) break;
         const int64_t me = m * v + e0 + e;
         const int64_t at0 = me * 2 * ov + j * v + b, at1 = at0 + ov;
-        W[at0] = (float)((double)W0[at0] + (double)PT[(j * ov + me) * v + b] - a0[e]);
-        W[at1] = (float)((double)W0[at1] - (double)p2[tid][e] + a1[e]);
+        W[at0] = (float)((double)w0[e] + (double)pt[e] - a0[e]);
+        W[at1] = (float)((double)w1[e] - (double)p2[tid][e] + a1[e]);
       }
     }
   }
@@ -1245,7 +1267,8 @@ void w_build_x3(Ctx& c, const double* t1, float* W, hipStream_t s) {
   const int64_t o = c.o, v = c.v;
   const int64_t blocks = o * o * ceil_div(v, WE);
   CCX_CHECK(blocks < (1ll << 31), ERR_ARG, "W build grid too large");
-  w_build_x3_kernel<<<(unsigned)blocks, WB, 0, s>>>(o, v, t1, c.Cooov, c.W0, c.PT, c.P2T, W);
+  if (v % WE == 0) w_build_x3_kernel<true><<<(unsigned)blocks, WB, 0, s>>>(o, v, t1, c.Cooov, c.W0, c.PT, c.P2T, W);
+  else w_build_x3_kernel<false><<<(unsigned)blocks, WB, 0, s>>>(o, v, t1, c.Cooov, c.W0, c.PT, c.P2T, W);
   CCX_HIP(hipGetLastError());
 }
 
