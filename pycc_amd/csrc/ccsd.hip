@@ -564,8 +564,7 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
     contract<float>(0.5f, tv(c.tau32, "mnab", OOVV), tv(c.W32, "mnij", OOOO), 0.0f, tv(X32, "ijab", OOVV), ws, s);
     accumulate(ov * ov, 1.0, X32, c.Hh, s);
   }
-  // :932, :937, :938, :940 through Zt
-  cd(-1.0, T1("ma"), Zt, 1.0, Hh);
+  // :932, :937, :938, :940 through Zt: -t_ma Zt_mijb, summed inside assemble_r2
   // :939  t_ie <ab|ej> = t_ie (jb|ae)
   if (x3) covvv_x3_hh(c, c.Hh, s);
   else cd(1.0, T1("ie"), tv(c.Covvv, "jbae", OVVV), 1.0, Hh);
@@ -573,10 +572,10 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   // ---- r2 = Hh + Hh^T + ladder + rings (ccwfn.py:790) --------------------------------------------
   if (c.mode == MODE_DP)
     assemble_r2<double>(o, v, c.Hh, (const double*)c.CP, (const double*)c.CM, (const double*)P,
-                        (const double*)Q, r2, s);
+                        (const double*)Q, t1, c.Zt, r2, s);
   else
     assemble_r2<float>(o, v, c.Hh, (const float*)c.CP, (const float*)c.CM, (const float*)P,
-                       (const float*)Q, r2, s);
+                       (const float*)Q, t1, c.Zt, r2, s);
 
   // ---- r1: ccwfn.py:754-760 ------------------------------------------------------------------
   const TV R1 = tv(r1, "ia", OV);

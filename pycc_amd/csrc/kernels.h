@@ -57,9 +57,11 @@ template <typename T>
 void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStream_t s);
 
 // r2[i][j][a][b] = H[ijab] + H[jiba] + ladder[ijab] + P[ia,jb] + P[jb,ia] + Q[ja,ib] + Q[ib,ja]
+//                  - sum_m (t1[m][a] Zt[m][i][j][b] + t1[m][b] Zt[m][j][i][a])
+// computed once per pair-symmetric couple and written to both (ij)(ab) partners
 template <typename T>
 void assemble_r2(int64_t o, int64_t v, const double* H, const T* CP, const T* CM, const T* P,
-                 const T* Q, double* r2, hipStream_t s);
+                 const T* Q, const double* t1, const double* Zt, double* r2, hipStream_t s);
 
 // E = 2 f_ia t_ia + (t2_ijab + t_ia t_jb) L_ijab, written to e_out (device scalar).
 void energy(int64_t o, int64_t v, const double* fock, int64_t nmo, const double* t1, const double* t2,

@@ -413,20 +413,32 @@ void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStrea
 // One block per (i, j) pair and 32 x 32 tile of (a, b). The three transposed operands
 // (H[j][i][b][a], P[(jb),(ia)], Q[(ib),(ja)]) are read along their contiguous index and transposed
 // through LDS, so every global access is coalesced.
+// r2 is symmetric under (ij)(ab): one workgroup per pair i <= j and 32 x 32 tile (a, b) computes
+//   r2[i][j][a][b] = H[ijab] + H[jiba] + ladder[ijab] + P[(ia),(jb)] + P[(jb),(ia)] + Q[(ja),(ib)]
+//                    + Q[(ib),(ja)] - sum_m (t_ma Zt[m][i][j][b] + t_mb Zt[m][j][i][a])
+// once and writes it to both r2[i][j][a][b] and r2[j][i][b][a] (the second through an LDS transpose),
+// so every input element is read once. For i = j only the tiles with a-tile <= b-tile run (the
+// diagonal tile writes itself only). The Zt term is ccwfn.py:932, :937, :938, :940 (the -t_ma Zt
+// contraction of both halves of the symmetrised residual), summed over m in chunks through LDS.
+constexpr int ASM_MC = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, const double* __restrict__ H,
                                                           const T* __restrict__ CP, const T* __restrict__ CM,
                                                           const T* __restrict__ P, const T* __restrict__ Q,
-                                                          double* __restrict__ r2) {
+                                                          const double* __restrict__ t1,
+                                                          const double* __restrict__ Zt, double* __restrict__ r2) {
   __shared__ double sh[32][33];
   __shared__ T sp[32][33];
   __shared__ T sq[32][33];
+  __shared__ double sta[ASM_MC][32], sz2[ASM_MC][32];
   const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
   const int64_t ldP = npP + o * v, ldM = npM + o * v, ov = o * v;
-  const int64_t ij = blockIdx.y;
-  const int64_t i = ij / o, j = ij % o;
+  int64_t i, j;
+  decodeP(blockIdx.y, i, j);   // i <= j
   const int64_t tiles_b = (v + 31) / 32;
-  const int64_t a0 = (blockIdx.x / tiles_b) * 32, b0 = (blockIdx.x % tiles_b) * 32;
+  const int64_t at = blockIdx.x / tiles_b, bt = blockIdx.x % tiles_b;
+  if (i == j && at > bt) return;   // covered by the transposed write of tile (bt, at)
+  const int64_t a0 = at * 32, b0 = bt * 32;
   const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
   // transposed operands: element (a = a0 + x, b = b0 + y) read with x (a) fastest
   for (int y = y0; y < 32; y += 8) {
@@ -437,33 +449,83 @@ __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, 
     sq[y][x] = ok ? Q[(i * v + b) * ov + j * v + a] : T(0);
   }
   __syncthreads();
-  const int64_t il = i < j ? i : j, ih = i < j ? j : i;
-  const T* cpr = CP + pairP(il, ih) * ldP;
-  const T* cmr = (i != j) ? CM + pairM(il, ih) * ldM : nullptr;
+  const T* cpr = CP + pairP(i, j) * ldP;
+  const T* cmr = (i != j) ? CM + pairM(i, j) * ldM : nullptr;
   const bool sij = i < j;
-  for (int y = y0; y < 32; y += 8) {
-    const int64_t a = a0 + y, b = b0 + x;
-    if (a >= v || b >= v) continue;
-    const int64_t e = ((i * o + j) * v + a) * v + b;
-    double r = H[e] + sh[x][y];
-    const int64_t al = a < b ? a : b, ah = a < b ? b : a;
-    r += (double)cpr[pairP(al, ah)];
-    if (cmr && a != b) {
-      double m = (double)cmr[pairM(al, ah)];
-      r += (sij == (a < b)) ? m : -m;
+  const int64_t b = b0 + x;
+  const bool bok = b < v;
+  double r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int y = y0 + 8 * k;
+    const int64_t a = a0 + y;
+    double acc = 0.0;
+    if (a < v && bok) {
+      acc = H[((i * o + j) * v + a) * v + b] + sh[x][y];
+      const int64_t al = a < b ? a : b, ah = a < b ? b : a;
+      acc += (double)cpr[pairP(al, ah)];
+      if (cmr && a != b) {
+        const double m = (double)cmr[pairM(al, ah)];
+        acc += (sij == (a < b)) ? m : -m;
+      }
+      acc += (double)P[(i * v + a) * ov + j * v + b] + (double)sp[x][y];
+      acc += (double)Q[(j * v + a) * ov + i * v + b] + (double)sq[x][y];
     }
-    r += (double)P[(i * v + a) * ov + j * v + b] + (double)sp[x][y];
-    r += (double)Q[(j * v + a) * ov + i * v + b] + (double)sq[x][y];
-    r2[e] = r;
+    r[k] = acc;
+  }
+  // - sum_m (t_ma Zt[m][i][j][b] + t_mb Zt[m][j][i][a]): this thread's b terms from global memory,
+  // the tile's a terms through LDS
+  double z[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t m0 = 0; m0 < o; m0 += ASM_MC) {
+    const int mc = (int)(o - m0 < ASM_MC ? o - m0 : ASM_MC);
+    __syncthreads();
+    for (int q = threadIdx.x; q < ASM_MC * 32; q += 256) {
+      const int mm = q >> 5, xx = q & 31;
+      const int64_t a = a0 + xx;
+      const bool ok = mm < mc && a < v;
+      sta[mm][xx] = ok ? t1[(m0 + mm) * v + a] : 0.0;
+      sz2[mm][xx] = ok ? Zt[(((m0 + mm) * o + j) * o + i) * v + a] : 0.0;
+    }
+    __syncthreads();
+    for (int mm = 0; mm < mc; ++mm) {
+      const int64_t m = m0 + mm;
+      const double z1 = bok ? Zt[((m * o + i) * o + j) * v + b] : 0.0;
+      const double tb = bok ? t1[m * v + b] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int y = y0 + 8 * k;
+        z[k] = fma(sta[mm][y], z1, z[k]);
+        z[k] = fma(tb, sz2[mm][y], z[k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] -= z[k];
+  // r2[i][j][a][b] directly (b fastest)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t a = a0 + y0 + 8 * k;
+    if (a < v && bok) r2[((i * o + j) * v + a) * v + b] = r[k];
+  }
+  if (i == j && at == bt) return;   // the diagonal tile of a diagonal pair is its own transpose
+  // r2[j][i][b][a] = the same values, through an LDS transpose (a fastest)
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) sh[y0 + 8 * k][x] = r[k];   // sh[a - a0][b - b0]
+  __syncthreads();
+  for (int y = y0; y < 32; y += 8) {
+    const int64_t bb = b0 + y, a = a0 + x;
+    if (bb < v && a < v) r2[((j * o + i) * v + bb) * v + a] = sh[x][y];
   }
 }
 
 template <typename T>
 void assemble_r2(int64_t o, int64_t v, const double* H, const T* CP, const T* CM, const T* P,
-                 const T* Q, double* r2, hipStream_t s) {
-  int64_t tiles = ceil_div(v, 32);
-  dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * o));
-  assemble_r2_kernel<T><<<grid, 256, 0, s>>>(o, v, H, CP, CM, P, Q, r2);
+                 const T* Q, const double* t1, const double* Zt, double* r2, hipStream_t s) {
+  const int64_t tiles = ceil_div(v, 32);
+  CCX_CHECK(o * (o + 1) / 2 < 65536 && tiles * tiles < (1ll << 31), ERR_ARG, "assemble_r2: grid too large");
+  dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * (o + 1) / 2));
+  assemble_r2_kernel<T><<<grid, 256, 0, s>>>(o, v, H, CP, CM, P, Q, t1, Zt, r2);
   CCX_HIP(hipGetLastError());
 }
 
@@ -806,10 +868,10 @@ template void ladder_pack<float>(int64_t, int64_t, int64_t, int64_t, VVVVSrc, co
                                  float*, hipStream_t);
 template void zt_add<double>(int64_t, int64_t, const double*, const double*, double*, hipStream_t);
 template void zt_add<float>(int64_t, int64_t, const float*, const float*, double*, hipStream_t);
-template void assemble_r2<double>(int64_t, int64_t, const double*, const double*, const double*,
-                                  const double*, const double*, double*, hipStream_t);
+template void assemble_r2<double>(int64_t, int64_t, const double*, const double*, const double*, const double*,
+                                  const double*, const double*, const double*, double*, hipStream_t);
 template void assemble_r2<float>(int64_t, int64_t, const double*, const float*, const float*, const float*,
-                                 const float*, double*, hipStream_t);
+                                 const float*, const double*, const double*, double*, hipStream_t);
 template void cast<double, float>(int64_t, const double*, float*, hipStream_t);
 template void cast<float, double>(int64_t, const float*, double*, hipStream_t);
 
