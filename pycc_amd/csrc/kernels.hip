@@ -419,8 +419,11 @@ void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStrea
 // once and writes it to both r2[i][j][a][b] and r2[j][i][b][a] (the second through an LDS transpose),
 // so every input element is read once. For i = j only the tiles with a-tile <= b-tile run (the
 // diagonal tile writes itself only). The Zt term is ccwfn.py:932, :937, :938, :940 (the -t_ma Zt
-// contraction of both halves of the symmetrised residual), summed over m in chunks through LDS.
-constexpr int ASM_MC = 16;
+// contraction of both halves of the symmetrised residual): one FP64 MFMA product per wave, the 16 x 16
+// block (a, b) of wave w = 2 (a-block) + (b-block), K = the two m sums side by side (2o), its operands
+// read straight from global memory in the MFMA lane layout (A: lane l holds A[l % 16][l / 16], B: lane
+// l holds B[l / 16][l % 16], C: lane l register r holds C[4 r + l / 16][l % 16]).
+typedef double asm_d4 __attribute__((ext_vector_type(4)));
 template <typename T>
 __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, const double* __restrict__ H,
                                                           const T* __restrict__ CP, const T* __restrict__ CM,
@@ -430,7 +433,7 @@ __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, 
   __shared__ double sh[32][33];
   __shared__ T sp[32][33];
   __shared__ T sq[32][33];
-  __shared__ double sta[ASM_MC][32], sz2[ASM_MC][32];
+  __shared__ double sz[32][33];
   const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
   const int64_t ldP = npP + o * v, ldM = npM + o * v, ov = o * v;
   int64_t i, j;
@@ -447,6 +450,30 @@ __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, 
     sh[y][x] = ok ? H[((j * o + i) * v + b) * v + a] : 0.0;
     sp[y][x] = ok ? P[(j * v + b) * ov + i * v + a] : T(0);
     sq[y][x] = ok ? Q[(i * v + b) * ov + j * v + a] : T(0);
+  }
+  {
+    // Zt term of this wave's 16 x 16 block: sum over k < 2o of A[a][k] B[k][b] with
+    // A[a][m] = t1[m][a], B[m][b] = Zt[m][i][j][b] for k = m < o, and
+    // A[a][o + m] = Zt[m][j][i][a], B[o + m][b] = t1[m][b]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t am = a0 + 16 * (w >> 1) + (lane & 15), bn = b0 + 16 * (w & 1) + (lane & 15);
+    const bool aok = am < v, bok = bn < v;
+    asm_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t k0 = 0; k0 < 2 * o; k0 += 4) {
+      const int64_t k = k0 + (lane >> 4);
+      double av = 0.0, bv = 0.0;
+      if (k < o) {
+        av = aok ? t1[k * v + am] : 0.0;
+        bv = bok ? Zt[((k * o + i) * o + j) * v + bn] : 0.0;
+      } else if (k < 2 * o) {
+        const int64_t m = k - o;
+        av = aok ? Zt[((m * o + j) * o + i) * v + am] : 0.0;
+        bv = bok ? t1[m * v + bn] : 0.0;
+      }
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sz[16 * (w >> 1) + 4 * r + (lane >> 4)][16 * (w & 1) + (lane & 15)] = acc[r];
   }
   __syncthreads();
   const T* cpr = CP + pairP(i, j) * ldP;
@@ -470,37 +497,10 @@ __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, 
       }
       acc += (double)P[(i * v + a) * ov + j * v + b] + (double)sp[x][y];
       acc += (double)Q[(j * v + a) * ov + i * v + b] + (double)sq[x][y];
+      acc -= sz[y][x];
     }
     r[k] = acc;
   }
-  // - sum_m (t_ma Zt[m][i][j][b] + t_mb Zt[m][j][i][a]): this thread's b terms from global memory,
-  // the tile's a terms through LDS
-  double z[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t m0 = 0; m0 < o; m0 += ASM_MC) {
-    const int mc = (int)(o - m0 < ASM_MC ? o - m0 : ASM_MC);
-    __syncthreads();
-    for (int q = threadIdx.x; q < ASM_MC * 32; q += 256) {
-      const int mm = q >> 5, xx = q & 31;
-      const int64_t a = a0 + xx;
-      const bool ok = mm < mc && a < v;
-      sta[mm][xx] = ok ? t1[(m0 + mm) * v + a] : 0.0;
-      sz2[mm][xx] = ok ? Zt[(((m0 + mm) * o + j) * o + i) * v + a] : 0.0;
-    }
-    __syncthreads();
-    for (int mm = 0; mm < mc; ++mm) {
-      const int64_t m = m0 + mm;
-      const double z1 = bok ? Zt[((m * o + i) * o + j) * v + b] : 0.0;
-      const double tb = bok ? t1[m * v + b] : 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int y = y0 + 8 * k;
-        z[k] = fma(sta[mm][y], z1, z[k]);
-        z[k] = fma(tb, sz2[mm][y], z[k]);
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) r[k] -= z[k];
   // r2[i][j][a][b] directly (b fastest)
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
