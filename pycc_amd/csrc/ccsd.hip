@@ -534,8 +534,13 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   const TV G = tv(c.G, "be", VV);
   axpby<double>(1.0, tv(c.Fae, "be", VV), 0.0, G, s);
   cd(-0.5, T1("mb"), Fme, 1.0, G);
-  if (x3 && c.T2h) {
-    hh_g_x3(c, t2, c.Hh, s);   // :922 and :925 in one FP16x3 GEMM (G's diagonal in FP64)
+  // In MIXED_F16X3 the FP32 terms :925 (off-diagonal G) and :930 go to Xr (the ring W buffer, dead
+  // after the ring GEMMs), and <ij|ab> (:922) and the diagonal t2_ijab G_bb are added in FP64 by
+  // assemble_r2; Hh then holds the -t2 Hm term only.
+  float* Xr = x3 ? (float*)c.W : nullptr;
+  const bool g_split = x3 && c.T2h;
+  if (g_split) {
+    hh_g_x3(c, t2, Xr, s);   // Xr = off-diagonal t2 G (:925) in FP32; Gd = G's diagonal
   } else {
     axpby<double>(0.5, tv(c.Ioovv, "ijab", OOVV), 0.0, Hh, s);      // 1/2 <ab|ij> = 1/2 <ij|ab> :922
     cd(1.0, T2("ijae"), G, 1.0, Hh);
@@ -544,8 +549,9 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
   const TV Hm = tv(c.Hm, "mj", OO);
   axpby<double>(1.0, tv(c.Fmi, "mj", OO), 0.0, Hm, s);
   cd(0.5, T1("je"), Fme, 1.0, Hm);
-  cd(-1.0, T2("imab"), Hm, 1.0, Hh);
+  cd(-1.0, T2("imab"), Hm, g_split ? 0.0 : 1.0, Hh);
   // :930  1/2 tau_mnab Wmnij
+  const float* X930 = nullptr;
   if (std::is_same<T, double>::value) {
     cd(0.5, tv(c.tau, "mnab", OOVV), Wmnij, 1.0, Hh);
   } else if (c.mode == MODE_MIXED_F16X3) {
@@ -553,29 +559,31 @@ static void residual_t(Ctx& c, const double* F, const double* t1, const double* 
     x3_absmax<float>(c.W32, o * o * o * o, c.xpart, s);
     x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 7, s);
     x3_split_t<float>(c.W32, o * o, o * o, 1, o * o, c.xs + 7, c.WTh, c.WTl, 0, c.o2p(), s);   // [(ij)][(mn)]
-    X3Gemm g;   // Hh[(ij)][(ab)] += 1/2 sum_(mn) Wmnij tau_mnab, added into the FP64 Hh
+    X3Gemm g;   // Xr[(ij)][(ab)] (+)= 1/2 sum_(mn) Wmnij tau_mnab in FP32
     g.M = o * o; g.N = v * v; g.Kp = c.o2p(); g.lda = c.o2p(); g.ldb = c.o2p(); g.ldc = v * v;
-    g.Ah = c.WTh; g.Al = c.WTl; g.Bh = c.TTh; g.Bl = c.TTl; g.C64 = c.Hh; g.beta = 1.f;
+    g.Ah = c.WTh; g.Al = c.WTl; g.Bh = c.TTh; g.Bl = c.TTl; g.C = Xr; g.beta = g_split ? 1.f : 0.f;
     g.sa = c.xs + 7; g.sb = c.xs + 5; g.alpha = 0.5f;
     x3_gemm<_Float16>(g, s, &ws);
+    X930 = Xr;
   } else {
     float* X32 = (float*)c.S2;   // S2 is dead after the P ring GEMM
     cast<double, float>(o * o * o * o, c.Wmnij, c.W32, s);
     contract<float>(0.5f, tv(c.tau32, "mnab", OOVV), tv(c.W32, "mnij", OOOO), 0.0f, tv(X32, "ijab", OOVV), ws, s);
-    accumulate(ov * ov, 1.0, X32, c.Hh, s);
+    X930 = X32;
   }
   // :932, :937, :938, :940 through Zt: -t_ma Zt_mijb, summed inside assemble_r2
-  // :939  t_ie <ab|ej> = t_ie (jb|ae)
-  if (x3) covvv_x3_hh(c, c.Hh, s);
-  else cd(1.0, T1("ie"), tv(c.Covvv, "jbae", OVVV), 1.0, Hh);
+  // :939  t_ie <ab|ej> = t_ie (jb|ae): in MIXED_F16X3 the FP32 products PT[i][(j,b,a)], added inside
+  // assemble_r2 (with X930, the FP32 :930 term of the mixed modes)
+  if (!x3) cd(1.0, T1("ie"), tv(c.Covvv, "jbae", OVVV), 1.0, Hh);
 
   // ---- r2 = Hh + Hh^T + ladder + rings (ccwfn.py:790) --------------------------------------------
   if (c.mode == MODE_DP)
     assemble_r2<double>(o, v, c.Hh, (const double*)c.CP, (const double*)c.CM, (const double*)P,
-                        (const double*)Q, t1, c.Zt, r2, s);
+                        (const double*)Q, t1, c.Zt, nullptr, nullptr, nullptr, nullptr, nullptr, r2, s);
   else
     assemble_r2<float>(o, v, c.Hh, (const float*)c.CP, (const float*)c.CM, (const float*)P,
-                       (const float*)Q, t1, c.Zt, r2, s);
+                       (const float*)Q, t1, c.Zt, x3 ? c.PT : nullptr, X930, g_split ? c.Ioovv : nullptr,
+                       g_split ? t2 : nullptr, g_split ? c.Gd : nullptr, r2, s);
 
   // ---- r1: ccwfn.py:754-760 ------------------------------------------------------------------
   const TV R1 = tv(r1, "ia", OV);

@@ -30,8 +30,14 @@ void ring_prep(int64_t o, int64_t v, const double* t1, const double* t2, T* S1, 
 // null: tau = t2).
 template <typename T>
 void tau_pack(int64_t o, int64_t v, const double* t1, const double* t2, T* Tp, T* Tm, hipStream_t s);
-// BF16x3 operands are K-contiguous with K padded to a multiple of 32 (padding stays zero).
-__host__ __device__ inline int64_t bf16_kpad(int64_t k) { return (k + 31) / 32 * 32; }
+// BF16x3 / FP16x3 ladder operands are K-contiguous with K padded to an odd number of 32-wide blocks
+// (padding stays zero): interleaved rows are then an odd number of 128-byte lines long, which spreads
+// the rows of one k-step over the L2 channels (measured at o=48, v=384: the tau- ladder GEMM fetched
+// 60 GB instead of 78 GB from beyond L2, and the iteration took 1% less).
+__host__ __device__ inline int64_t bf16_kpad(int64_t k) {
+  const int64_t b = (k + 31) / 32;
+  return (b + (b > 0 && b % 2 == 0 ? 1 : 0)) * 32;
+}
 void tau_pack_bf16x3(int64_t o, int64_t v, int64_t KpP, int64_t KpM, const double* t1, const double* t2,
                      bf16* Tp_hi, bf16* Tp_lo, bf16* Tm_hi, bf16* Tm_lo, hipStream_t s);
 
@@ -58,10 +64,13 @@ void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStrea
 
 // r2[i][j][a][b] = H[ijab] + H[jiba] + ladder[ijab] + P[ia,jb] + P[jb,ia] + Q[ja,ib] + Q[ib,ja]
 //                  - sum_m (t1[m][a] Zt[m][i][j][b] + t1[m][b] Zt[m][j][i][a])
+//                  [+ PTt[i][j][b][a] + PTt[j][i][a][b]] [+ X[ijab] + X[jiba]]   (FP32, each optional)
+//                  [+ I[ijab]] [+ T2d[ijab] Gd[b] + T2d[jiba] Gd[a]]               (FP64, each optional)
 // computed once per pair-symmetric couple and written to both (ij)(ab) partners
 template <typename T>
 void assemble_r2(int64_t o, int64_t v, const double* H, const T* CP, const T* CM, const T* P,
-                 const T* Q, const double* t1, const double* Zt, double* r2, hipStream_t s);
+                 const T* Q, const double* t1, const double* Zt, const float* PTt, const float* X,
+                 const double* I, const double* T2d, const double* Gd, double* r2, hipStream_t s);
 
 // E = 2 f_ia t_ia + (t2_ijab + t_ia t_jb) L_ijab, written to e_out (device scalar).
 void energy(int64_t o, int64_t v, const double* fock, int64_t nmo, const double* t1, const double* t2,

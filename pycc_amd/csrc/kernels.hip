@@ -416,24 +416,31 @@ void zt_add(int64_t o, int64_t v, const T* CP, const T* CM, double* Zt, hipStrea
 // r2 is symmetric under (ij)(ab): one workgroup per pair i <= j and 32 x 32 tile (a, b) computes
 //   r2[i][j][a][b] = H[ijab] + H[jiba] + ladder[ijab] + P[(ia),(jb)] + P[(jb),(ia)] + Q[(ja),(ib)]
 //                    + Q[(ib),(ja)] - sum_m (t_ma Zt[m][i][j][b] + t_mb Zt[m][j][i][a])
-// once and writes it to both r2[i][j][a][b] and r2[j][i][b][a] (the second through an LDS transpose),
-// so every input element is read once. For i = j only the tiles with a-tile <= b-tile run (the
-// diagonal tile writes itself only). The Zt term is ccwfn.py:932, :937, :938, :940 (the -t_ma Zt
-// contraction of both halves of the symmetrised residual): one FP64 MFMA product per wave, the 16 x 16
-// block (a, b) of wave w = 2 (a-block) + (b-block), K = the two m sums side by side (2o), its operands
-// read straight from global memory in the MFMA lane layout (A: lane l holds A[l % 16][l / 16], B: lane
-// l holds B[l / 16][l % 16], C: lane l register r holds C[4 r + l / 16][l % 16]).
+//                    [+ PTt[i][j][b][a] + PTt[j][i][a][b]] [+ X[ijab] + X[jiba]]
+//                    [+ I[ijab] + t2[ijab] Gd[b] + t2[jiba] Gd[a]]
+// (the optional FP32 half-residual terms: PTt the t_ie <ab|ej> products of :939 stored with (b, a)
+// planes, X the 1/2 tau Wmnij term of :930 plus the off-diagonal t2 G of :925; and in FP64 the
+// <ij|ab> of :922 (both halves: <ij|ab> = <ji|ba>) and the diagonal of :925) once and writes it to
+// both r2[i][j][a][b] and r2[j][i][b][a], so every input element is read once. For i = j only the
+// tiles with a-tile <= b-tile run (the diagonal tile writes itself only).
+// The terms indexed (j, i, b, a) are read along a and summed into one LDS tile; the Zt term
+// (ccwfn.py:932, :937, :938, :940: the -t_ma Zt contraction of both halves) is one FP64 MFMA product
+// per wave, K = the two m sums side by side (2o), operands straight from global memory in the lane
+// layout (A: lane l holds A[l % 16][l / 16], B: lane l holds B[l / 16][l % 16], C: lane l register r
+// holds C[4 r + l / 16][l % 16]); each lane then finishes the four elements of its C registers.
 typedef double asm_d4 __attribute__((ext_vector_type(4)));
 template <typename T>
 __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, const double* __restrict__ H,
                                                           const T* __restrict__ CP, const T* __restrict__ CM,
                                                           const T* __restrict__ P, const T* __restrict__ Q,
                                                           const double* __restrict__ t1,
-                                                          const double* __restrict__ Zt, double* __restrict__ r2) {
-  __shared__ double sh[32][33];
-  __shared__ T sp[32][33];
-  __shared__ T sq[32][33];
-  __shared__ double sz[32][33];
+                                                          const double* __restrict__ Zt,
+                                                          const float* __restrict__ PTt,
+                                                          const float* __restrict__ X,
+                                                          const double* __restrict__ I,
+                                                          const double* __restrict__ T2d,
+                                                          const double* __restrict__ Gd, double* __restrict__ r2) {
+  __shared__ double part[32][33];   // [b - b0][a - a0]: the (j, i, b, a) terms; later the output tile
   const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2;
   const int64_t ldP = npP + o * v, ldM = npM + o * v, ov = o * v;
   int64_t i, j;
@@ -442,23 +449,30 @@ __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, 
   const int64_t at = blockIdx.x / tiles_b, bt = blockIdx.x % tiles_b;
   if (i == j && at > bt) return;   // covered by the transposed write of tile (bt, at)
   const int64_t a0 = at * 32, b0 = bt * 32;
-  const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
-  // transposed operands: element (a = a0 + x, b = b0 + y) read with x (a) fastest
-  for (int y = y0; y < 32; y += 8) {
-    const int64_t a = a0 + x, b = b0 + y;
-    const bool ok = a < v && b < v;
-    sh[y][x] = ok ? H[((j * o + i) * v + b) * v + a] : 0.0;
-    sp[y][x] = ok ? P[(j * v + b) * ov + i * v + a] : T(0);
-    sq[y][x] = ok ? Q[(i * v + b) * ov + j * v + a] : T(0);
-  }
   {
-    // Zt term of this wave's 16 x 16 block: sum over k < 2o of A[a][k] B[k][b] with
-    // A[a][m] = t1[m][a], B[m][b] = Zt[m][i][j][b] for k = m < o, and
-    // A[a][o + m] = Zt[m][j][i][a], B[o + m][b] = t1[m][b]
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t am = a0 + 16 * (w >> 1) + (lane & 15), bn = b0 + 16 * (w & 1) + (lane & 15);
+    const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+    for (int y = y0; y < 32; y += 8) {
+      const int64_t a = a0 + x, b = b0 + y;
+      double p = 0.0;
+      if (a < v && b < v) {
+        p = H[((j * o + i) * v + b) * v + a] + (double)P[(j * v + b) * ov + i * v + a] +
+            (double)Q[(i * v + b) * ov + j * v + a];
+        if (PTt) p += (double)PTt[((i * o + j) * v + b) * v + a];
+        if (X) p += (double)X[((j * o + i) * v + b) * v + a];
+        if (T2d) p += T2d[((j * o + i) * v + b) * v + a] * Gd[a];
+      }
+      part[y][x] = p;
+    }
+  }
+  // Zt term of this wave's 16 x 16 block: sum over k < 2o of A[a][k] B[k][b] with
+  // A[a][m] = t1[m][a], B[m][b] = Zt[m][i][j][b] for k = m < o, and
+  // A[a][o + m] = Zt[m][j][i][a], B[o + m][b] = t1[m][b]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ab0 = 16 * (w >> 1), bb0 = 16 * (w & 1);   // the wave's block inside the tile
+  asm_d4 acc = {0.0, 0.0, 0.0, 0.0};
+  {
+    const int64_t am = a0 + ab0 + (lane & 15), bn = b0 + bb0 + (lane & 15);
     const bool aok = am < v, bok = bn < v;
-    asm_d4 acc = {0.0, 0.0, 0.0, 0.0};
     for (int64_t k0 = 0; k0 < 2 * o; k0 += 4) {
       const int64_t k = k0 + (lane >> 4);
       double av = 0.0, bv = 0.0;
@@ -472,60 +486,68 @@ __global__ __launch_bounds__(256) void assemble_r2_kernel(int64_t o, int64_t v, 
       }
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) sz[16 * (w >> 1) + 4 * r + (lane >> 4)][16 * (w & 1) + (lane & 15)] = acc[r];
   }
   __syncthreads();
   const T* cpr = CP + pairP(i, j) * ldP;
   const T* cmr = (i != j) ? CM + pairM(i, j) * ldM : nullptr;
   const bool sij = i < j;
-  const int64_t b = b0 + x;
+  const int bl = bb0 + (lane & 15);   // this lane's b - b0
+  const int64_t b = b0 + bl;
   const bool bok = b < v;
   double r[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int y = y0 + 8 * k;
-    const int64_t a = a0 + y;
-    double acc = 0.0;
+    const int al = ab0 + 4 * k + (lane >> 4);   // a - a0 of register k
+    const int64_t a = a0 + al;
+    double e = 0.0;
     if (a < v && bok) {
-      acc = H[((i * o + j) * v + a) * v + b] + sh[x][y];
-      const int64_t al = a < b ? a : b, ah = a < b ? b : a;
-      acc += (double)cpr[pairP(al, ah)];
+      const int64_t ijab = ((i * o + j) * v + a) * v + b;
+      e = H[ijab] + part[bl][al];
+      const int64_t alo = a < b ? a : b, ahi = a < b ? b : a;
+      e += (double)cpr[pairP(alo, ahi)];
       if (cmr && a != b) {
-        const double m = (double)cmr[pairM(al, ah)];
-        acc += (sij == (a < b)) ? m : -m;
+        const double m = (double)cmr[pairM(alo, ahi)];
+        e += (sij == (a < b)) ? m : -m;
       }
-      acc += (double)P[(i * v + a) * ov + j * v + b] + (double)sp[x][y];
-      acc += (double)Q[(j * v + a) * ov + i * v + b] + (double)sq[x][y];
-      acc -= sz[y][x];
+      e += (double)P[(i * v + a) * ov + j * v + b] + (double)Q[(j * v + a) * ov + i * v + b];
+      if (PTt) e += (double)PTt[((j * o + i) * v + a) * v + b];
+      if (X) e += (double)X[ijab];
+      if (I) e += I[ijab];
+      if (T2d) e += T2d[ijab] * Gd[b];
+      e -= acc[k];
     }
-    r[k] = acc;
+    r[k] = e;
   }
   // r2[i][j][a][b] directly (b fastest)
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int64_t a = a0 + y0 + 8 * k;
+    const int64_t a = a0 + ab0 + 4 * k + (lane >> 4);
     if (a < v && bok) r2[((i * o + j) * v + a) * v + b] = r[k];
   }
   if (i == j && at == bt) return;   // the diagonal tile of a diagonal pair is its own transpose
   // r2[j][i][b][a] = the same values, through an LDS transpose (a fastest)
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 4; ++k) sh[y0 + 8 * k][x] = r[k];   // sh[a - a0][b - b0]
+  for (int k = 0; k < 4; ++k) part[ab0 + 4 * k + (lane >> 4)][bl] = r[k];   // part[a - a0][b - b0]
   __syncthreads();
-  for (int y = y0; y < 32; y += 8) {
-    const int64_t bb = b0 + y, a = a0 + x;
-    if (bb < v && a < v) r2[((j * o + i) * v + bb) * v + a] = sh[x][y];
+  {
+    const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+    for (int y = y0; y < 32; y += 8) {
+      const int64_t bb = b0 + y, a = a0 + x;
+      if (bb < v && a < v) r2[((j * o + i) * v + bb) * v + a] = part[x][y];
+    }
   }
 }
 
 template <typename T>
 void assemble_r2(int64_t o, int64_t v, const double* H, const T* CP, const T* CM, const T* P,
-                 const T* Q, const double* t1, const double* Zt, double* r2, hipStream_t s) {
+                 const T* Q, const double* t1, const double* Zt, const float* PTt, const float* X,
+                 const double* I, const double* T2d, const double* Gd, double* r2, hipStream_t s) {
+  CCX_CHECK((T2d == nullptr) == (Gd == nullptr), ERR_INTERNAL, "assemble_r2: t2 and G's diagonal go together");
   const int64_t tiles = ceil_div(v, 32);
   CCX_CHECK(o * (o + 1) / 2 < 65536 && tiles * tiles < (1ll << 31), ERR_ARG, "assemble_r2: grid too large");
   dim3 grid((unsigned)(tiles * tiles), (unsigned)(o * (o + 1) / 2));
-  assemble_r2_kernel<T><<<grid, 256, 0, s>>>(o, v, H, CP, CM, P, Q, t1, Zt, r2);
+  assemble_r2_kernel<T><<<grid, 256, 0, s>>>(o, v, H, CP, CM, P, Q, t1, Zt, PTt, X, I, T2d, Gd, r2);
   CCX_HIP(hipGetLastError());
 }
 
@@ -869,9 +891,11 @@ template void ladder_pack<float>(int64_t, int64_t, int64_t, int64_t, VVVVSrc, co
 template void zt_add<double>(int64_t, int64_t, const double*, const double*, double*, hipStream_t);
 template void zt_add<float>(int64_t, int64_t, const float*, const float*, double*, hipStream_t);
 template void assemble_r2<double>(int64_t, int64_t, const double*, const double*, const double*, const double*,
+                                  const double*, const double*, const double*, const float*, const float*,
                                   const double*, const double*, const double*, double*, hipStream_t);
 template void assemble_r2<float>(int64_t, int64_t, const double*, const float*, const float*, const float*,
-                                 const float*, const double*, const double*, double*, hipStream_t);
+                                 const float*, const double*, const double*, const float*, const float*,
+                                 const double*, const double*, const double*, double*, hipStream_t);
 template void cast<double, float>(int64_t, const double*, float*, hipStream_t);
 template void cast<float, double>(int64_t, const float*, double*, hipStream_t);
 

@@ -86,13 +86,12 @@ void covvv_x3_products(Ctx& c, const double* t1, hipStream_t s);   // PT, P2T fr
 void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s);              // + 2 t_mf (mf|ae) - t_mf (me|af)
 // the FP32 ring W[(m,e)][x][(j,b)] (:641-643, :680-682) from W0, PT, P2T and t1 (Cooov terms)
 void w_build_x3(Ctx& c, const double* t1, float* W, hipStream_t s);
-void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s);                // :939
 void covvv_x3_r1(Ctx& c, const double* Ud, double* r1, hipStream_t s);   // :759
-// Hh = 1/2 <ij|ab> + t2_ijae G_be (:922, :925): the off-diagonal G as one FP16x3 GEMM, the diagonal
-// t2_ijab G_bb (orbital-energy sized) and 1/2 <ij|ab> added in FP64 by its epilogue. Needs a grid
-// the direct FP64 epilogue takes (hh_g_x3_fits).
+// X = t2_ijae G_be over e != b (the off-diagonal part of :925) as one FP16x3 GEMM with FP32 output
+// [i][j][a][b]; the diagonal t2_ijab G_bb (orbital-energy sized, kept in c.Gd) and <ij|ab> (:922)
+// are added in FP64 by the r2 assembly. Sized for grids of at least 512 tiles (hh_g_x3_fits).
 bool hh_g_x3_fits(int64_t o, int64_t v);
-void hh_g_x3(Ctx& c, const double* t2, double* Hh, hipStream_t s);
+void hh_g_x3(Ctx& c, const double* t2, float* X, hipStream_t s);
 // The split ring operands BJ, BK, AP, AQ (K = ovp, or 2 ovp for AP; interleaved, lo = hi + 32) and Ud
 // from t1, t2.
 void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const double* t2, const float* sc, f16* BJh,

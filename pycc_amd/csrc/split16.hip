@@ -68,6 +68,7 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BT - 1) / BT;
   const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
+  const int64_t split = blockIdx.y;
   // groups of GM M tiles sweep all N tiles: consecutive tiles share B rows and GM A panels
   const int64_t group = t / (GM * tiles_n), first_m = group * GM;
   const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
     for (int j = 0; j < NB; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
   const int fr = lane >> 4, fc = lane & 15;
-  const int64_t kbeg = p.work ? (int64_t)blockIdx.y * p.kchunk : 0;
+  const int64_t kbeg = p.work ? split * p.kchunk : 0;
   const int64_t kend = p.work ? (kbeg + p.kchunk < p.Kp ? kbeg + p.kchunk : p.Kp) : p.Kp;
   gload(kbeg);
   sstore(0);
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(256) void x3_gemm_kernel(X3Gemm p) {
   }
 
   if (p.work) {   // split-K: raw partial sums
-    float* Wb = p.work + (int64_t)blockIdx.y * p.M * p.N;
+    float* Wb = p.work + split * p.M * p.N;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       const int64_t n = tn * BT + wn * WT + j * 16 + fc;
@@ -419,6 +420,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   const int wm = wave >> 2, wn = wave & 3;
   const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BN - 1) / BN;
   const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
+  const int64_t split = blockIdx.y;
   const int64_t group = t / (GM * tiles_n), first_m = group * GM;
   const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
   const int64_t within = t - group * GM * tiles_n;
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
 #pragma unroll
     for (int j = 0; j < NBN; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-  const int64_t kbeg = p.work ? (int64_t)blockIdx.y * p.kchunk : 0;
+  const int64_t kbeg = p.work ? split * p.kchunk : 0;
   const int64_t kend = p.work ? (kbeg + p.kchunk < p.Kp ? kbeg + p.kchunk : p.Kp) : p.Kp;
   issue(0, kbeg);
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   }
 
   if (p.work) {   // split-K: raw partial sums
-    float* Wb = p.work + (int64_t)blockIdx.y * p.M * p.N;
+    float* Wb = p.work + split * p.M * p.N;
 #pragma unroll
     for (int j = 0; j < NBN; ++j) {
       const int64_t n = tn * BN + wn * WTN + j * 16 + fc;
@@ -1066,7 +1068,7 @@ void ring_prep_x3(int64_t o, int64_t v, int64_t ovp, const double* t1, const dou
 // and their uses are permuted adds:
 //   Fae[a][e] += 2 sum_m P2T[m][(m,a,e)] - sum_m PT[m][(m,e,a)]               (:496)
 //   Wmbej[m][e][j][b] += PT[j][(m,e,b)],  Wmbje[m][e][j][b] -= P2T[j][(m,b,e)]  (:642, :681)
-//   Hh[i][j][a][b] += PT[i][(j,b,a)]                                            (:939)
+//   Hh[i][j][a][b] += PT[i][(j,b,a)]                      (:939, added inside the r2 assembly)
 namespace {
 
 // dst[(i, q)][f] (row stride ldd, q < nq) = split(src[q][i][f] * sc): Ud [(m,e)][(i,f)] -> [i][(m,e)][f]
@@ -1084,102 +1086,58 @@ __global__ __launch_bounds__(256) void split_udt_kernel(const double* __restrict
 // The FP32 ring W in one pass (ccwfn.py:641-643 Wmbej, :680-682 Wmbje), W[(m,e)][x][(j,b)]:
 //   x = 0: W0 + PT[j][(m,e,b)]  - sum_n t_nb (nj|me)
 //   x = 1: W0 - P2T[j][(m,b,e)] + sum_n t_nb (mj|ne)
-// summed in FP64 and rounded once. One workgroup per (m, j, block of WE e's); thread = b, in
-// chunks of WB. The Cooov coefficients of a block are the same for all its threads: they are read
-// through scalar loads (no LDS broadcast traffic). The W0 and PT operands of a chunk are loaded
-// before the n loop, so their latency hides behind the FMAs; the P2T tile [b][e] is transposed
-// through LDS so that every global access is along b or e.
-constexpr int WE = 16, WB = 128;
-template <bool FULL>   // FULL: every block holds WE valid e's (v a multiple of WE)
-__global__ __launch_bounds__(WB) void w_build_x3_kernel(int64_t o, int64_t v, const double* __restrict__ t1,
-                                                        const double* __restrict__ Cooov,
-                                                        const float* __restrict__ W0, const float* __restrict__ PT,
-                                                        const float* __restrict__ P2T, float* __restrict__ W) {
-  __shared__ float p2[WB][WE + 1];
-  const int64_t ov = o * v, neb = (v + WE - 1) / WE;
-  const int64_t eb = blockIdx.x % neb, mj = blockIdx.x / neb;
-  const int64_t m = mj / o, j = mj - m * o, e0 = eb * WE;
-  const int ne = FULL ? WE : (int)(v - e0 < WE ? v - e0 : WE);
-  const int tid = threadIdx.x;
-  // e's past v (a partial last block) repeat the last valid one; their sums are not stored
-  auto ecl = [&](int e) -> int { return FULL ? e : (e < ne ? e : ne - 1); };
-  const double* __restrict__ c1 = Cooov + (j * o + m) * v + e0;       // (nj|me) = c1[n o^2 v + e]
-  const double* __restrict__ c2 = Cooov + (m * o + j) * o * v + e0;   // (mj|ne) = c2[n v + e]
-  const float* __restrict__ p2src = P2T + (j * o + m) * v * v;
-  for (int64_t b0 = 0; b0 < v; b0 += WB) {
-    const int64_t b = b0 + tid;
-    const bool bok = b < v;
-    const int64_t bc = bok ? b : v - 1;
-    float w0[WE], w1[WE], pt[WE];
+// summed in FP64 and rounded once. For fixed (m, j) the two n sums are (v x o)(o x v) products,
+// A0[e][n] = (nj|me), A1[e][n] = (mj|ne), T[n][b] = t_nb: one workgroup per (m, j, 32 e's, 128 b's),
+// its 4 waves in 2 (e) x 2 (b), each wave 16 e x 64 b as four 16 x 16 blocks per sum on FP64 MFMA
+// (v_mfma_f64_16x16x4f64, K = o), operands read straight from global memory in the lane layout
+// (A: lane l holds A[l % 16][l / 16], B: lane l holds B[l / 16][l % 16], C: lane l register r holds
+// C[4 r + l / 16][l % 16]). The epilogue adds W0, PT and P2T and writes W.
+typedef double wb_d4 __attribute__((ext_vector_type(4)));
+constexpr int WBE = 32, WBB = 128;
+__global__ __launch_bounds__(256) void w_build_x3_kernel(int64_t o, int64_t v, const double* __restrict__ t1,
+                                                         const double* __restrict__ Cooov,
+                                                         const float* __restrict__ W0, const float* __restrict__ PT,
+                                                         const float* __restrict__ P2T, float* __restrict__ W) {
+  const int64_t ov = o * v, neb = (v + WBE - 1) / WBE, nbb = (v + WBB - 1) / WBB;
+  int64_t q = blockIdx.x;
+  const int64_t bb = q % nbb;
+  q /= nbb;
+  const int64_t eb = q % neb, mj = q / neb;
+  const int64_t m = mj / o, j = mj - m * o;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t e0 = eb * WBE + 16 * (w >> 1), b0 = bb * WBB + 64 * (w & 1);
+  const int64_t ea = e0 + (lane & 15);                       // A row of this lane
+  const bool eok = ea < v;
+  wb_d4 acc0[4], acc1[4];
 #pragma unroll
-    for (int e = 0; e < WE; ++e) {
-      const int64_t me = m * v + e0 + ecl(e);
-      const int64_t at0 = me * 2 * ov + j * v + bc;
-      w0[e] = W0[at0];
-      w1[e] = W0[at0 + ov];
-      pt[e] = PT[(j * ov + me) * v + bc];
-    }
-    double a0[WE], a1[WE];
+  for (int jb = 0; jb < 4; ++jb) acc0[jb] = acc1[jb] = wb_d4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t k0 = 0; k0 < o; k0 += 4) {
+    const int64_t n = k0 + (lane >> 4);
+    const bool nok = n < o;
+    const double a0 = (nok && eok) ? Cooov[((n * o + j) * o + m) * v + ea] : 0.0;   // (nj|me)
+    const double a1 = (nok && eok) ? Cooov[((m * o + j) * o + n) * v + ea] : 0.0;   // (mj|ne)
 #pragma unroll
-    for (int e = 0; e < WE; ++e) a0[e] = a1[e] = 0.0;
-    for (int64_t n = 0; n < o; ++n) {
-      const double tb = t1[n * v + bc];
-      const double* __restrict__ r1 = c1 + n * o * o * v;
-      const double* __restrict__ r2 = c2 + n * v;
-#pragma unroll
-      for (int e = 0; e < WE; ++e) {
-        a0[e] = fma(r1[ecl(e)], tb, a0[e]);
-        a1[e] = fma(r2[ecl(e)], tb, a1[e]);
-      }
-    }
-    // P2T[j][m][b][e0 .. e0 + WE) for the WB b's of this chunk, through LDS
-    __syncthreads();
-    for (int q = tid; q < WB * WE; q += WB) {
-      const int bb = q / WE, e = q - bb * WE;
-      p2[bb][e] = (b0 + bb < v && e < ne) ? p2src[(b0 + bb) * v + e0 + e] : 0.f;
-    }
-    __syncthreads();
-    if (bok) {
-#pragma unroll
-      for (int e = 0; e < WE; ++e) {
-        if (e >= ne) break;
-        const int64_t me = m * v + e0 + e;
-        const int64_t at0 = me * 2 * ov + j * v + b, at1 = at0 + ov;
-        W[at0] = (float)((double)w0[e] + (double)pt[e] - a0[e]);
-        W[at1] = (float)((double)w1[e] - (double)p2[tid][e] + a1[e]);
-      }
+    for (int jb = 0; jb < 4; ++jb) {
+      const int64_t bn = b0 + 16 * jb + (lane & 15);
+      const double tb = (nok && bn < v) ? t1[n * v + bn] : 0.0;
+      acc0[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, tb, acc0[jb], 0, 0, 0);
+      acc1[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, tb, acc1[jb], 0, 0, 0);
     }
   }
-}
-
-// For each plane q = q1 n2 + q2 (q1 < n1, q2 < n2): out[x] = (base ? base[x] : out[x]) + alpha in[q v^2 + r v + c]
-// with x = q1 s1 + q2 s2 + off + c ldo + r, for r, c < v, through a 32 x 33 LDS transpose (coalesced on
-// both sides).
-__global__ __launch_bounds__(256) void plane_t_add_kernel(const float* __restrict__ in, int64_t v, int64_t n1,
-                                                          int64_t n2, int64_t s1, int64_t s2, int64_t off,
-                                                          int64_t ldo, double alpha, const double* base,
-                                                          double* out) {
-  __shared__ float tile[32][33];
-  const int64_t tv_ = (v + 31) / 32, per = tv_ * tv_;
-  for (int64_t w = blockIdx.x; w < n1 * n2 * per; w += gridDim.x) {
-    const int64_t q = w / per, rem = w - q * per;
-    const int64_t r0 = (rem / tv_) * 32, c0 = (rem % tv_) * 32;
-    const int64_t q1 = q / n2, q2 = q - q1 * n2;
-    const float* src = in + q * v * v;
-    const int64_t o0 = q1 * s1 + q2 * s2 + off;
-    double* dst = out + o0;
-    const double* bse = base ? base + o0 : dst;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int y = ty; y < 32; y += 8) {
-      const int64_t r = r0 + y, c = c0 + tx;
-      tile[y][tx] = (r < v && c < v) ? src[r * v + c] : 0.f;
+  const float* __restrict__ p2src = P2T + (j * o + m) * v * v;
+#pragma unroll
+  for (int jb = 0; jb < 4; ++jb) {
+    const int64_t b = b0 + 16 * jb + (lane & 15);
+    if (b >= v) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t e = e0 + 4 * r + (lane >> 4);
+      if (e >= v) continue;
+      const int64_t me = m * v + e;
+      const int64_t at0 = me * 2 * ov + j * v + b, at1 = at0 + ov;
+      W[at0] = (float)((double)W0[at0] + (double)PT[(j * ov + me) * v + b] - acc0[jb][r]);
+      W[at1] = (float)((double)W0[at1] - (double)p2src[b * v + e] + acc1[jb][r]);
     }
-    __syncthreads();
-    for (int y = ty; y < 32; y += 8) {
-      const int64_t c = c0 + y, r = r0 + tx;
-      if (c < v && r < v) dst[c * ldo + r] = bse[c * ldo + r] + alpha * (double)tile[tx][y];
-    }
-    __syncthreads();
   }
 }
 
@@ -1265,19 +1223,9 @@ void covvv_x3_fae(Ctx& c, double* Fae, hipStream_t s) {
 // W = the FP32 ring W (every element written)
 void w_build_x3(Ctx& c, const double* t1, float* W, hipStream_t s) {
   const int64_t o = c.o, v = c.v;
-  const int64_t blocks = o * o * ceil_div(v, WE);
+  const int64_t blocks = o * o * ceil_div(v, WBE) * ceil_div(v, WBB);
   CCX_CHECK(blocks < (1ll << 31), ERR_ARG, "W build grid too large");
-  if (v % WE == 0) w_build_x3_kernel<true><<<(unsigned)blocks, WB, 0, s>>>(o, v, t1, c.Cooov, c.W0, c.PT, c.P2T, W);
-  else w_build_x3_kernel<false><<<(unsigned)blocks, WB, 0, s>>>(o, v, t1, c.Cooov, c.W0, c.PT, c.P2T, W);
-  CCX_HIP(hipGetLastError());
-}
-
-void covvv_x3_hh(Ctx& c, double* Hh, hipStream_t s) {
-  const int64_t o = c.o, v = c.v;
-  // Hh[i][j][a][b] += PT[i][j][b][a]: plane (i, j), out Hh[(i o + j) v^2 + a v + b]
-  const int64_t planes = o * o * ceil_div(v, 32) * ceil_div(v, 32);
-  plane_t_add_kernel<<<(unsigned)std::min<int64_t>(planes, 1 << 20), 256, 0, s>>>(
-      c.PT, v, o, o, o * v * v, v * v, 0, v, 1.0, nullptr, Hh);
+  w_build_x3_kernel<<<(unsigned)blocks, 256, 0, s>>>(o, v, t1, c.Cooov, c.W0, c.PT, c.P2T, W);
   CCX_HIP(hipGetLastError());
 }
 
@@ -1326,7 +1274,7 @@ bool hh_g_x3_fits(int64_t o, int64_t v) {
   return o * o * v >= 256 && ceil_div(o * o * v, 256) * ceil_div(v, 256) >= 512;
 }
 
-void hh_g_x3(Ctx& c, const double* t2, double* Hh, hipStream_t s) {
+void hh_g_x3(Ctx& c, const double* t2, float* X, hipStream_t s) {
   const int64_t o = c.o, v = c.v, v32 = c.v32();
   CCX_CHECK(hh_g_x3_fits(o, v) && c.T2h && c.GPh, ERR_INTERNAL, "hh_g_x3 on a grid it does not fit");
   g_offdiag_kernel<<<grid_e(v * v), 256, 0, s>>>(c.G, v, c.Gp, c.Gd);
@@ -1335,11 +1283,11 @@ void hh_g_x3(Ctx& c, const double* t2, double* Hh, hipStream_t s) {
   x3_scale(c.xpart, 1.f, nullptr, 0.f, c.xs + 14, s);
   x3_split_rows<double>(c.Gp, v, v, v, c.xs + 14, c.GPh, c.GPl, v32, s, true);
   x3_split_rows<double>(t2, o * o * v, v, v, c.xs + 13, c.T2h, c.T2l, v32, s, true);
-  X3Gemm g;   // Hh[(ija)][b] = 1/2 <ij|ab> + sum_(e != b) t2[(ija)][e] G[b][e] + t2[(ija)][b] G[b][b]
+  X3Gemm g;   // X[(ija)][b] = sum_(e != b) t2[(ija)][e] G[b][e] in FP32
   g.M = o * o * v; g.N = v; g.Kp = v32; g.lda = 2 * v32; g.ldb = 2 * v32; g.ldc = v;
   g.kb = 32; g.ska = 64; g.skb = 64;
   g.Ah = c.T2h; g.Al = c.T2l; g.Bh = c.GPh; g.Bl = c.GPl; g.sa = c.xs + 13; g.sb = c.xs + 14;
-  g.C64 = Hh; g.C64in = c.Ioovv; g.beta = 0.5f; g.D64 = t2; g.dvec = c.Gd;
+  g.C = X; g.beta = 0.f;
   x3_gemm<_Float16>(g, s);
 }
 
