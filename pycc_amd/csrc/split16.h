@@ -33,12 +33,21 @@ struct X3Gemm {
   // split-K (set by x3_gemm): grid.y = splits K ranges of kchunk, raw partials in work[s][M][N]
   int64_t kchunk = 0;
   float* work = nullptr;
+  // x3_gemm_kcat only (interleaved operands, FP32 C): k in [K1, Kp) reads A from Ah2 (lo = Ah2 +
+  // 32) at k - K1 and row (m mod sw) sw + m / sw, the (x, y) -> (y, x) swap of a row m = x sw + y
+  // (M = sw^2); batch b of nbatch adds b sA2 to Ah2, b sB to Bh and Bl, and b sC to C.
+  const void* Ah2 = nullptr;
+  int64_t K1 = 0, sw = 0, nbatch = 1, sA2 = 0, sB = 0, sC = 0;
 };
 struct Workspace;
 // ws (optional) enables split-K when the tile grid cannot fill the chip.
 template <typename E>   // __bf16 or _Float16
 void x3_gemm(const X3Gemm& p, hipStream_t s, Workspace* ws = nullptr);
 
+// nbatch FP16x3 GEMMs C_b = alpha / (sa sb) [A | A2_b swapped] [B_b] with K = Kp concatenated
+// from two operands (fields Ah2 .. sC above): the connected (T) numerators, two slot products per
+// GEMM. Output tiles of 256 x (192 or 256), whichever pads N less.
+void x3_gemm_kcat(const X3Gemm& p, hipStream_t s);
 // max |x| over n elements as x3_partials() per-block partial maxima in `part`.
 template <typename T>
 void x3_absmax(const T* x, int64_t n, float* part, hipStream_t s);

@@ -403,30 +403,37 @@ __device__ __forceinline__ void x3_dma16(const E* g, const E* lds) {
                :: "s"(__builtin_amdgcn_readfirstlane(base)), "v"(g) : "memory");
 }
 
-// BN = 256 or 320 output columns per tile (320: 128 x 80 per wave, 144 KiB of LDS; B's 20 row
-// blocks of 16 go to wave q % 8, so waves 0-3 load three and 4-7 two).
-template <typename E, int BN>
+// BN = 192, 256 or 320 output columns per tile (320: 128 x 80 per wave, 144 KiB of LDS; B's 20 row
+// blocks of 16 go to wave q % 8, so waves 0-3 load three and 4-7 two). KCAT: the K-concatenated,
+// batched form of x3_gemm_kcat (interleaved operands, FP32 C, no split K).
+template <typename E, int BN, bool KCAT = false>
 __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   typedef typename X3<E>::v8 v8;
   constexpr int BT = 256, WTM = 128, WTN = BN / 4, NBM = WTM / 16, NBN = WTN / 16, IPW = 2;
   constexpr int NIB = BN / 16, IPB = (NIB + 7) / 8;   // B row blocks, per wave (rounded up)
   __shared__ __attribute__((aligned(1024))) E sA[2][2][BT * 32];   // [stage][hi/lo]
   __shared__ __attribute__((aligned(1024))) E sB[2][2][BN * 32];
-  const E* __restrict__ Ah = (const E*)p.Ah;
-  const E* __restrict__ Al = (const E*)p.Al;
-  const E* __restrict__ Bh = (const E*)p.Bh;
-  const E* __restrict__ Bl = (const E*)p.Bl;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 2, wn = wave & 3;
   const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BN - 1) / BN;
-  const int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n);
+  int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n * (KCAT ? p.nbatch : 1));
+  int64_t batch = 0;
+  if constexpr (KCAT) {
+    batch = t / (tiles_m * tiles_n);
+    t -= batch * tiles_m * tiles_n;
+  }
+  const E* __restrict__ Ah = (const E*)p.Ah;
+  const E* __restrict__ Al = (const E*)p.Al;
+  const E* __restrict__ Bh = (const E*)p.Bh + batch * p.sB;
+  const E* __restrict__ Bl = (const E*)p.Bl + batch * p.sB;
+  const E* __restrict__ Ah2 = KCAT ? (const E*)p.Ah2 + batch * p.sA2 : nullptr;
   const int64_t split = blockIdx.y;
   const int64_t group = t / (GM * tiles_n), first_m = group * GM;
   const int64_t gsize = tiles_m - first_m < GM ? tiles_m - first_m : GM;
   const int64_t within = t - group * GM * tiles_n;
   const int64_t tm = first_m + within % gsize, tn = within / gsize;
 
-  int64_t gA[IPW], gB[IPB];
+  int64_t gA[IPW], gB[IPB], gA2[KCAT ? IPW : 1];
   int ldso[IPW], ldsb[IPB];
 #pragma unroll
   for (int i = 0; i < IPW; ++i) {
@@ -435,6 +442,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
     int64_t m = tm * BT + row;
     m = m < p.M ? m : p.M - 1;
     gA[i] = m * p.lda + c * 8;
+    if constexpr (KCAT) gA2[i] = ((m % p.sw) * p.sw + m / p.sw) * p.lda + c * 8;
     ldso[i] = R0 * 32;
   }
 #pragma unroll
@@ -457,10 +465,19 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
       ka = q * p.ska + r;
       kb = q * p.skb + r;
     }
+    if (KCAT && k0 >= p.K1) {   // second operand of the concatenation (wave-uniform)
+      const int64_t ka2 = ((k0 - p.K1) >> 5) * p.ska;
 #pragma unroll
-    for (int i = 0; i < IPW; ++i) {
-      x3_dma16<E>(Ah + gA[i] + ka, &sA[st][0][ldso[i]]);
-      x3_dma16<E>(Al + gA[i] + ka, &sA[st][1][ldso[i]]);
+      for (int i = 0; i < IPW; ++i) {
+        x3_dma16<E>(Ah2 + gA2[i] + ka2, &sA[st][0][ldso[i]]);
+        x3_dma16<E>(Ah2 + 32 + gA2[i] + ka2, &sA[st][1][ldso[i]]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < IPW; ++i) {
+        x3_dma16<E>(Ah + gA[i] + ka, &sA[st][0][ldso[i]]);
+        x3_dma16<E>(Al + gA[i] + ka, &sA[st][1][ldso[i]]);
+      }
     }
 #pragma unroll
     for (int i = 0; i < IPB; ++i) {
@@ -554,7 +571,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
             if (p.D64) val += p.D64[at] * p.dvec[n];
             p.C64[at] = val;
           } else {
-            float* d = p.C + m * p.ldc + n;
+            float* d = p.C + batch * p.sC + m * p.ldc + n;
             float val = f * acc[i][j][r];
             if (p.beta != 0.f) val += p.beta * *d;
             *d = val;
@@ -969,6 +986,21 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
 }
 template void x3_gemm<__bf16>(const X3Gemm&, hipStream_t, Workspace*);
 template void x3_gemm<_Float16>(const X3Gemm&, hipStream_t, Workspace*);
+
+void x3_gemm_kcat(const X3Gemm& p, hipStream_t s) {
+  if (p.M == 0 || p.N == 0 || p.nbatch == 0) return;
+  CCX_CHECK(p.kb == BK && p.ska == 2 * BK && p.skb == 2 * BK && p.Kp % BK == 0 && p.K1 % BK == 0 && p.K1 > 0 &&
+                p.K1 < p.Kp && p.lda % 8 == 0 && p.ldb % 8 == 0 && p.Ah2 && p.C && !p.C64 && !p.work &&
+                p.sw > 0 && p.sw * p.sw == p.M && p.beta == 0.f && !p.sbn,
+            ERR_INTERNAL, "x3 K-concatenated GEMM layout");
+  // the output tile width that pads N less (256 on a tie; the 320-column tile spills registers here)
+  const int bn = ceil_div(p.N, 192) * 192 < ceil_div(p.N, 256) * 256 ? 192 : 256;
+  const int64_t tiles = ceil_div(p.M, 256) * ceil_div(p.N, bn) * p.nbatch;
+  CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
+  if (bn == 256) x3_gemm_dma_kernel<_Float16, 256, true><<<(unsigned)tiles, 512, 0, s>>>(p);
+  else x3_gemm_dma_kernel<_Float16, 192, true><<<(unsigned)tiles, 512, 0, s>>>(p);
+  CCX_HIP(hipGetLastError());
+}
 
 template <typename T>
 void x3_absmax(const T* x, int64_t n, float* part, hipStream_t s) {

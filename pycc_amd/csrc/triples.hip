@@ -23,12 +23,14 @@
 #include "contract.h"
 #include "split16.h"
 
+#include <algorithm>
+
 namespace ccx {
 
 namespace {
 
-__device__ __forceinline__ double t3_block_sum(double x) {
-  __shared__ double sh[256];
+__device__ __forceinline__ double t3_block_sum(double x) {   // blockDim.x <= 512, a power of two
+  __shared__ double sh[512];
   sh[threadIdx.x] = x;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -82,9 +84,8 @@ __device__ __forceinline__ void t3_accumulate(double (*sh)[TT][TP], double coef,
   }
 }
 
-__device__ __forceinline__ void t3_tile_origin(int64_t v, int64_t* org) {
+__device__ __forceinline__ void t3_tile_origin(int64_t v, int64_t* org, int64_t t = blockIdx.x) {
   const int64_t nt = (v + TT - 1) / TT;
-  const int64_t t = blockIdx.x;
   org[0] = t / (nt * nt) * TT;
   org[1] = t / nt % nt * TT;
   org[2] = t % nt * TT;
@@ -214,10 +215,11 @@ __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, in
                                                         const double* __restrict__ t1,
                                                         const double* __restrict__ t2,
                                                         const double* __restrict__ Ioovv,
+                                                        const int* __restrict__ tile_of_block,
                                                         double* __restrict__ partials) {
   __shared__ double sh[TT][TT][TP];
   int64_t org[3];
-  t3_tile_origin(v, org);
+  t3_tile_origin(v, org, tile_of_block[blockIdx.x]);
   const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
   double q[TT], w[TT], r[TT];
 #pragma unroll
@@ -281,6 +283,263 @@ __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, in
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// ---- MIXED_F16X3: two slot products per GEMM, W never stored ------------------------------------
+// With A_p^s the rows of A_p with (x, y) swapped and B_(q,r) the packed operand of t3_pack_kernel,
+// the six slot products pair up along K into three FP16x3 GEMMs (K = 2 Kp, FP32 output [x][y][vp]):
+//   Y1[a][b][c] = X_ijk[a][b][c] + X_jik[b][a][c] = [A_i | A_j^s](a,b) . [B_(j,k) | B_(i,k)](c)
+//   Y2[a][c][b] = X_ikj[a][c][b] + X_kij[c][a][b] = [A_i | A_k^s](a,c) . [B_(k,j) | B_(i,j)](b)
+//   Y3[b][c][a] = X_jki[b][c][a] + X_kji[c][b][a] = [A_j | A_k^s](b,c) . [B_(k,i) | B_(j,i)](a)
+// so that W[a][b][c] = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a].
+//
+// B rows of the chunk (i, j, k0 .. k0 + nk): block 3 e + s (s = 0, 1, 2 for Y1, Y2, Y3 of k = k0 + e)
+// holds, for z < v, [B_(q1,r1)[z] | B_(q2,r2)[z]] (2 Kp wide) with the pairs named above.
+__global__ void t3_pack_kcat_kernel(int64_t o, int64_t v, int64_t Kp, int i, int j, int k0, int nk,
+                                    const double* __restrict__ t2, const double* __restrict__ Cooov,
+                                    double* __restrict__ B) {
+  const int64_t L = 2 * Kp, total = 3 * (int64_t)nk * v * L;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t kk2 = e % L, row = e / L, blk = row / v, z = row - blk * v;
+    const int e3 = (int)(blk / 3), sl = (int)(blk - 3 * e3), k = k0 + e3;
+    const bool h = kk2 >= Kp;
+    const int64_t kk = h ? kk2 - Kp : kk2;
+    int q, r;
+    if (sl == 0) { q = h ? i : j; r = k; }        // (j, k) | (i, k)
+    else if (sl == 1) { q = h ? i : k; r = j; }   // (k, j) | (i, j)
+    else { q = h ? j : k; r = i; }                // (k, i) | (j, i)
+    double val = 0.0;
+    if (kk < v) val = t2[(((int64_t)r * o + q) * v + z) * v + kk];
+    else if (kk < v + o) val = -Cooov[(((kk - v) * o + q) * o + r) * v + z];
+    B[e] = val;
+  }
+}
+
+// One work-group per (orbit {A >= B >= C} of 16^3 tiles, triple k0 + blockIdx.y): W of the orbit's
+// distinct tiles (one, three or six permutations of (A, B, C)) from Y1, Y2, Y3 into LDS (FP32),
+// then the energy sum of t3_energy_kernel over those tiles, every permuted W read from LDS. HBM
+// traffic is Y read once; W is never written.
+constexpr int OX = 273, OY = 17, OTS = TT * OX;   // LDS tile [x][y][z] at x OX + y OY + z
+// slot x holds the tile at the permutation (A, B, C), (A, C, B), (B, A, C), (B, C, A), (C, A, B),
+// (C, B, A) of the orbit.
+
+constexpr int OT_THREADS = 512;   // 2 waves per SIMD at one work-group per CU
+__device__ __forceinline__ double t3_rcp(double x) {   // 1/x: the hardware estimate and two Newton steps
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+
+__global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
+    int64_t o, int64_t v, int64_t vp, int i, int j, int k0, const float* __restrict__ Y1,
+    const float* __restrict__ Y2, const float* __restrict__ Y3, const int* __restrict__ orbits, int64_t norb,
+    const double* __restrict__ F, int64_t nmo, const double* __restrict__ t1, const double* __restrict__ t2,
+    const double* __restrict__ Ioovv, double* __restrict__ partials) {
+  // for the tile (P0, P1, P2) of slot x, the slots of (P1, P2, P0), (P2, P0, P1), (P0, P2, P1),
+  // (P1, P0, P2) and (P2, P1, P0)
+  constexpr int t3_cyc[6] = {3, 5, 1, 4, 0, 2};
+  constexpr int t3_cyc2[6] = {4, 2, 5, 0, 3, 1};
+  constexpr int t3_sw23[6] = {1, 0, 3, 2, 5, 4};
+  constexpr int t3_sw12[6] = {2, 4, 0, 5, 1, 3};
+  constexpr int t3_rev[6] = {5, 3, 4, 1, 2, 0};
+  __shared__ float sW[6 * OTS];
+  const int e = blockIdx.y, k = k0 + e;
+  const int64_t nt = (v + TT - 1) / TT, vv = v * v, ys = vv * vp;
+  const int code = orbits[blockIdx.x];
+  const int A = (int)(code / (nt * nt)), B = (int)(code / nt % nt), C = (int)(code % nt);
+  const int T[6][3] = {{A, B, C}, {A, C, B}, {B, A, C}, {B, C, A}, {C, A, B}, {C, B, A}};
+  // rep[x]: the first slot with slot x's tile (A >= B >= C: only A == B or B == C repeat tiles)
+  int rep[6];
+  {
+    const bool ab = A == B, bc = B == C;
+    rep[0] = 0;
+    rep[1] = bc ? 0 : 1;
+    rep[2] = ab ? 0 : 2;
+    rep[3] = ab ? 1 : 3;
+    rep[4] = ab ? 4 : (bc ? 2 : 4);
+    rep[5] = ab ? 4 : (bc ? 3 : 5);
+    if (ab && bc) rep[1] = rep[2] = rep[3] = rep[4] = rep[5] = 0;
+  }
+  const float* y1 = Y1 + e * ys;
+  const float* y2 = Y2 + e * ys;
+  const float* y3 = Y3 + e * ys;
+  // box loads: thread t reads 4 floats at z = 4 (t % 4) of the rows (x, y) = (t / 64 + 8 h, t / 4 % 16)
+  constexpr int NH = 4096 / (4 * OT_THREADS), XS = 16 / NH;
+  const int r0 = threadIdx.x >> 2, c4 = threadIdx.x & 3;
+  const int xl0 = r0 >> 4, yl = r0 & 15, zl = 4 * c4;
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  auto box = [&](const float* Y, int gx, int gy, int gz, f4v* out) {   // tile origins in tiles
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      const int64_t x = 16 * gx + xl0 + XS * h, y = 16 * gy + yl, z = 16 * gz + zl;
+      f4v val = f4v{0.f, 0.f, 0.f, 0.f};
+      if (x < v && y < v && z < v) {
+        val = *(const f4v*)(Y + (x * v + y) * vp + z);
+        if (z + 3 >= v) {   // the row's padding past v
+          if (z + 1 >= v) val[1] = 0.f;
+          if (z + 2 >= v) val[2] = 0.f;
+          val[3] = 0.f;
+        }
+      }
+      out[h] = val;
+    }
+  };
+  f4v ra[6][NH], rb[6][NH];
+#pragma unroll
+  for (int x = 0; x < 6; ++x)
+    if (rep[x] == x) box(y1, T[x][0], T[x][1], T[x][2], ra[x]);   // Y1[a][b][c]
+#pragma unroll
+  for (int x = 0; x < 6; ++x)
+    if (rep[x] == x) box(y2, T[x][0], T[x][2], T[x][1], rb[x]);   // Y2[a][c][b]
+#pragma unroll
+  for (int x = 0; x < 6; ++x)
+    if (rep[x] == x) {
+      float* w = sW + x * OTS;
+#pragma unroll
+      for (int h = 0; h < NH; ++h)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[(xl0 + XS * h) * OX + yl * OY + zl + u] = ra[x][h][u];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 6; ++x)
+    if (rep[x] == x) box(y3, T[x][1], T[x][2], T[x][0], ra[x]);   // Y3[b][c][a]
+#pragma unroll
+  for (int x = 0; x < 6; ++x)
+    if (rep[x] == x) {   // Y2 element (a, c, b) = (xl, yl, zl + u) -> W[a][b][c]
+      float* w = sW + x * OTS;
+#pragma unroll
+      for (int h = 0; h < NH; ++h)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[(xl0 + XS * h) * OX + (zl + u) * OY + yl] += rb[x][h][u];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 6; ++x)
+    if (rep[x] == x) {   // Y3 element (b, c, a) = (xl, yl, zl + u) -> W[a][b][c]
+      float* w = sW + x * OTS;
+#pragma unroll
+      for (int h = 0; h < NH; ++h)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[(zl + u) * OX + (xl0 + XS * h) * OY + yl] += ra[x][h][u];
+    }
+  __syncthreads();
+
+  // The disconnected numerator's factors of a tile go through LDS: the six 16 x 16 blocks Iij[a][b],
+  // Tij[a][b], Iik[a][c], Tik[a][c], Ijk[b][c], Tjk[b][c] and the vectors t1_ia, f_ia, eps_a (and
+  // those of j, b and k, c), loaded once per tile with every load in flight together.
+  __shared__ double sD[6][TT][TT + 1];
+  __shared__ double sV[9][TT];
+  // energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) of the a half t / 256
+  const int half = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, cl = threadIdx.x & 15;
+  const double* I6[6] = {Ioovv + ((int64_t)i * o + j) * vv, t2 + ((int64_t)i * o + j) * vv,
+                         Ioovv + ((int64_t)i * o + k) * vv, t2 + ((int64_t)i * o + k) * vv,
+                         Ioovv + ((int64_t)j * o + k) * vv, t2 + ((int64_t)j * o + k) * vv};
+  const int occ3[3] = {i, j, k};
+  const double dijk = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k];
+  double s = 0.0;
+#pragma unroll
+  for (int x = 0; x < 6; ++x) {
+    if (rep[x] != x) continue;   // block-uniform
+    const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
+    {
+      // block d of sD: rows along (a, a, a, a, b, b), columns along (b, b, c, c, c, c); the half h of
+      // the threads loads the blocks 3 h .. 3 h + 2
+      double ld[3];
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) {
+        const int d = 3 * half + dd;
+        const int64_t r = org[d < 4 ? 0 : 1] + bl, cc = org[d < 2 ? 1 : 2] + cl;
+        ld[dd] = (r < v && cc < v) ? I6[d][r * v + cc] : 0.0;
+      }
+      double lv = 0.0;
+      if (threadIdx.x < 9 * TT) {   // vector q = 3 n + u: u = 0 t1, 1 f_(occ) row, 2 eps, for n = a, b, c
+        const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
+        const int64_t g = org[n] + cl;
+        if (g < v) {
+          const int occ = occ3[n];
+          lv = u == 0 ? t1[occ * v + g] : (u == 1 ? F[occ * nmo + o + g] : F[(o + g) * nmo + o + g]);
+        }
+      }
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) sD[3 * half + dd][bl][cl] = ld[dd];
+      if (threadIdx.x < 9 * TT) sV[threadIdx.x / TT][cl] = lv;
+    }
+    __syncthreads();
+    const int64_t a0 = org[0], b = org[1] + bl, c = org[2] + cl;
+    if (b < v && c < v) {
+      const float* w0 = sW + x * OTS;
+      const float* wc = sW + rep[t3_cyc[x]] * OTS;
+      const float* wc2 = sW + rep[t3_cyc2[x]] * OTS;
+      const float* w23 = sW + rep[t3_sw23[x]] * OTS;
+      const float* w12 = sW + rep[t3_sw12[x]] * OTS;
+      const float* wr = sW + rep[t3_rev[x]] * OTS;
+      const double dbc = dijk - sV[5][bl] - sV[8][cl];
+      const double ijbc = sD[4][bl][cl], tjbc = sD[5][bl][cl];
+      const double t1jb = sV[3][bl], fjb = sV[4][bl], t1kc = sV[6][cl], fkc = sV[7][cl];
+#pragma unroll 4
+      for (int al = 8 * half; al < 8 * half + 8; ++al) {
+        if (a0 + al < v) {   // wave-uniform
+          const double w = w0[al * OX + bl * OY + cl];
+          const double q = (double)wc[bl * OX + cl * OY + al] + (double)wc2[cl * OX + al * OY + bl] -
+                           2.0 * ((double)w23[al * OX + cl * OY + bl] + (double)w12[bl * OX + al * OY + cl] +
+                                  (double)wr[cl * OX + bl * OY + al]);
+          const double vabc = w + sD[0][al][bl] * t1kc + sD[2][al][cl] * t1jb + ijbc * sV[0][al] +
+                              sD[1][al][bl] * fkc + sD[3][al][cl] * fjb + tjbc * sV[1][al];
+          s += vabc * (4.0 * w + q) * t3_rcp(dbc - sV[2][al]);
+        }
+      }
+    }
+    __syncthreads();   // sD, sV are reloaded for the next tile
+  }
+  s = t3_block_sum(s);
+  if (threadIdx.x == 0) partials[(int64_t)e * norb + blockIdx.x] = s;
+}
+
+// e[0] += sum over the chunk's triples (i, j, k0 + x), in order, of weight * sum(partials[x][0 .. norb))
+__global__ void t3_accumulate_chunk_kernel(const double* __restrict__ partials, int64_t norb, int nk, int i, int j,
+                                           int k0, double* e) {
+  for (int x = 0; x < nk; ++x) {
+    double s = 0.0;
+    for (int64_t b = threadIdx.x; b < norb; b += blockDim.x) s += partials[(int64_t)x * norb + b];
+    s = t3_block_sum(s);
+    const int k = k0 + x;
+    if (threadIdx.x == 0) e[0] += (2.0 - (double)((i == j) + (i == k) + (j == k))) * s;   // cctriples.py:237
+  }
+}
+
+// orbits {A >= B >= C} of the nt^3 tiles as (A nt + B) nt + C
+std::vector<int> t3_orbits(int64_t nt) {
+  std::vector<int> out;
+  for (int64_t A = 0; A < nt; ++A)
+    for (int64_t B = 0; B <= A; ++B)
+      for (int64_t C = 0; C <= B; ++C) out.push_back((int)((A * nt + B) * nt + C));
+  return out;
+}
+
+// Work-group -> tile table of the energy kernel. The energy of a tile (A, B, C) reads W at the six
+// permutations of (A, B, C), so the tiles of one orbit {(A, B, C) permuted} are given to consecutive
+// work-groups of one XCD (work-group w runs on XCD w mod 8): the orbit's W tiles are fetched from HBM
+// about once and the other members read them from that XCD's L2.
+std::vector<int> t3_orbit_order(int64_t nt) {
+  std::vector<int> seq;
+  seq.reserve((size_t)(nt * nt * nt));
+  for (int64_t A = 0; A < nt; ++A)
+    for (int64_t B = 0; B <= A; ++B)
+      for (int64_t C = 0; C <= B; ++C) {
+        const int64_t P[6][3] = {{A, B, C}, {A, C, B}, {B, A, C}, {B, C, A}, {C, A, B}, {C, B, A}};
+        int got[6], n = 0;
+        for (const auto& p : P) {
+          const int t = (int)((p[0] * nt + p[1]) * nt + p[2]);
+          if (std::find(got, got + n, t) == got + n) got[n++] = t;
+        }
+        seq.insert(seq.end(), got, got + n);
+      }
+  const int64_t n = (int64_t)seq.size(), q = n / 8;
+  std::vector<int> out((size_t)n);
+  for (int64_t w = 0; w < n; ++w) out[(size_t)w] = w < 8 * q ? seq[(size_t)((w % 8) * q + w / 8)] : seq[(size_t)w];
+  return out;
+}
+
 // e[0] += weight * sum(partials[0 .. n)), in a fixed order
 __global__ void t3_accumulate_kernel(const double* __restrict__ partials, int64_t n, double weight, double* e) {
   double s = 0.0;
@@ -290,6 +549,81 @@ __global__ void t3_accumulate_kernel(const double* __restrict__ partials, int64_
 }
 
 }  // namespace
+
+// MIXED_F16X3: for fixed (i, j) the k values run in chunks of T3_KC; per chunk the K-concatenated
+// B rows (FP64, then split with one scale), the batched Y1, Y2, Y3 GEMMs and one orbit-energy launch.
+static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const double* t2, int i0, int i1, bool accumulate,
+                              double* e_dev, hipStream_t s) {
+  const int64_t o = c.o, v = c.v, nmo = c.nmo, vv = v * v;
+  const int64_t nt = ceil_div(v, TT), KC = T3_KC;
+  const int64_t Kp = (v + o + 31) / 32 * 32, L = 2 * Kp, vp = (v + 3) / 4 * 4;
+  const int64_t nA = (int64_t)i1 * vv * Kp, nBr = 3 * KC * v, np = x3_partials();
+  CCX_CHECK(vv * vp * KC < (int64_t(1) << 40) && ceil_div(vv, 256) < (int64_t(1) << 24), ERR_ARG,
+            "too many virtual orbitals for the (T) scratch");
+  DevBuf<f16> Ahl(nA * 2);   // A split, [i1 v^2][2 Kp]
+  DevBuf<float> sc(16 + 2 * np);   // sc[0] = scale of A, sc[1] = scale of the chunk's B, then partials
+  // The entries of A over every p are exactly those of Covvv and of t2 (plus zero padding), so its
+  // scale is that of their union: the same for every range of i, whatever rows are built.
+  x3_absmax<double>(c.Covvv, o * vv * v, sc.p + 16, s);
+  x3_absmax<double>(t2, o * o * vv, sc.p + 16 + np, s);
+  x3_scale_union(sc.p + 16, (int)(2 * np), sc.p, s);
+  if (nA > 0) {   // A split; its FP64 form lives only until then
+    DevBuf<double> A64(nA);
+    t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv, t2,
+                                                                                         A64.p);
+    CCX_HIP(hipGetLastError());
+    x3_split_rows<double>(A64.p, (int64_t)i1 * vv, Kp, Kp, sc.p, Ahl.p, Ahl.p + 32, Kp, s, true);
+    CCX_HIP(hipStreamSynchronize(s));
+  }
+  const std::vector<int> orb = t3_orbits(nt);
+  const int64_t norb = (int64_t)orb.size();
+  DevBuf<int> dorb(norb);
+  CCX_HIP(hipMemcpy(dorb.p, orb.data(), (size_t)norb * sizeof(int), hipMemcpyHostToDevice));
+  DevBuf<double> Bp(nBr * L);        // K-concatenated B rows of a chunk
+  DevBuf<f16> Bhl(nBr * 2 * L);      // the same split, rows of 2 L halves
+  DevBuf<float> Y(3 * KC * vv * vp);   // Y1, Y2, Y3 of the chunk's triples, [e][x][y][vp]
+  DevBuf<double> part(KC * norb);
+  float *Y1 = Y.p, *Y2 = Y1 + KC * vv * vp, *Y3 = Y2 + KC * vv * vp;
+  if (!accumulate) {
+    t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
+    CCX_HIP(hipGetLastError());
+  }
+  auto Ap = [&](int p) { return Ahl.p + (int64_t)p * vv * 2 * Kp; };
+  for (int i = i0; i < i1; ++i)
+    for (int j = 0; j <= i; ++j)
+      for (int k0 = 0; k0 <= j; k0 += (int)KC) {
+        const int nk = (int)std::min<int64_t>(KC, j + 1 - k0);
+        const int64_t nrows = 3 * (int64_t)nk * v;
+        t3_pack_kcat_kernel<<<(int)std::min<int64_t>(4096, ceil_div(nrows * L, 256)), 256, 0, s>>>(
+            o, v, Kp, i, j, k0, nk, t2, c.Cooov, Bp.p);
+        CCX_HIP(hipGetLastError());
+        x3_absmax<double>(Bp.p, nrows * L, sc.p + 16, s);
+        x3_scale(sc.p + 16, 1.f, nullptr, 0.f, sc.p + 1, s);
+        x3_split_rows<double>(Bp.p, nrows, L, L, sc.p + 1, Bhl.p, Bhl.p + 32, L, s, true);
+        X3Gemm g;   // C_e[m][n] = sum_kk [A | A2_e swapped][m][kk] B_e[n][kk]
+        g.M = vv; g.N = v; g.Kp = 2 * Kp; g.K1 = Kp; g.lda = 2 * Kp; g.ldb = 2 * L; g.ldc = vp;
+        g.kb = 32; g.ska = 64; g.skb = 64; g.sw = v; g.nbatch = nk;
+        g.sa = sc.p; g.sb = sc.p + 1; g.sB = 3 * v * 2 * L; g.sC = vv * vp;
+        // Y1 = [A_i | A_j^s] [B_(j,k) | B_(i,k)]
+        g.Ah = Ap(i); g.Al = Ap(i) + 32; g.Ah2 = Ap(j); g.sA2 = 0;
+        g.Bh = Bhl.p; g.Bl = Bhl.p + 32; g.C = Y1;
+        x3_gemm_kcat(g, s);
+        // Y2 = [A_i | A_k^s] [B_(k,j) | B_(i,j)]
+        g.Ah2 = Ap(k0); g.sA2 = vv * 2 * Kp;
+        g.Bh = Bhl.p + v * 2 * L; g.Bl = (const f16*)g.Bh + 32; g.C = Y2;
+        x3_gemm_kcat(g, s);
+        // Y3 = [A_j | A_k^s] [B_(k,i) | B_(j,i)]
+        g.Ah = Ap(j); g.Al = Ap(j) + 32;
+        g.Bh = Bhl.p + 2 * v * 2 * L; g.Bl = (const f16*)g.Bh + 32; g.C = Y3;
+        x3_gemm_kcat(g, s);
+        t3_orbit_energy_kernel<<<dim3((unsigned)norb, (unsigned)nk), OT_THREADS, 0, s>>>(
+            o, v, vp, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
+        CCX_HIP(hipGetLastError());
+        t3_accumulate_chunk_kernel<<<1, 256, 0, s>>>(part.p, norb, nk, i, j, k0, e_dev);
+        CCX_HIP(hipGetLastError());
+      }
+  CCX_HIP(hipStreamSynchronize(s));
+}
 
 void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2, int i0, int i1, bool accumulate,
                     double* e_dev, hipStream_t s) {
@@ -301,50 +635,34 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   const int64_t nt = ceil_div(v, TT), tiles = nt * nt * nt;
   CCX_CHECK(tiles < (int64_t(1) << 31), ERR_ARG, "too many virtual orbitals for the (T) tiles");
   // In MIXED_F16X3 the slot GEMMs are FP16x3 split products (interleaved hi/lo operands, K padded
-  // to 32, FP32 accumulation, FP64 output); every other mode runs them in FP64.
+  // to 32, FP32 accumulation and output, two slot products per GEMM); every other mode runs them
+  // in FP64.
   const bool x3 = c.mode == MODE_MIXED_F16X3;
   const int64_t Kp = x3 ? (v + o + 31) / 32 * 32 : (v + o + 7) / 8 * 8;
   const int64_t KC = T3_KC;
+  // A: only the rows p < i1 are read by this range (p is one of i >= j >= k)
+  const int64_t nA = (int64_t)i1 * vv * Kp;
+  if (x3) {
+    triples_energy_x3(c, F, t1, t2, i0, i1, accumulate, e_dev, s);
+    return;
+  }
   // For fixed (i, j) the k values run in chunks of KC: the p = i and p = j GEMMs of a chunk share
   // their A operand and become one GEMM each with N = 2v nk, and the p = k GEMMs (one B operand,
   // consecutive A_k rows) one GEMM with M = nk v^2.
   // scratch: XI, XJ (v^2 x 2v KC each), XK (KC v^2 x 2v), W (v^3), B (rows of the i, j and k entries,
-  // (2 KC + 1) 2v x Kp), tile partials, and A (o v^2 x Kp) in FP64 -- or, in MIXED_F16X3, A and B
-  // split (hi/lo interleaved, rows of 2 Kp halves) plus absmax partials and the operand scales
-  // A: only the rows p < i1 are read by this range (p is one of i >= j >= k)
-  const int64_t nA = (int64_t)i1 * vv * Kp, nXc = vv * 2 * v * KC, nB = (2 * KC + 1) * 2 * v * Kp;
-  const int64_t np = x3_partials();
-  DevBuf<f16> Ahl;     // A split, [i1 v^2][2 Kp]
-  DevBuf<float> scA;   // scA[0] = scale of A, then absmax partials
-  if (x3) {   // A split first; its FP64 form lives only until then
-    Ahl.alloc(nA * 2);
-    scA.alloc(8 + 2 * np);
-    // The entries of A over every p are exactly those of Covvv and of t2 (plus zero padding), so its
-    // scale is that of their union: the same for every range of i, whatever rows are built.
-    x3_absmax<double>(c.Covvv, o * vv * v, scA.p + 8, s);
-    x3_absmax<double>(t2, o * o * vv, scA.p + 8 + np, s);
-    x3_scale_union(scA.p + 8, (int)(2 * np), scA.p, s);
-    if (nA > 0) {
-      DevBuf<double> A64(nA);
-      t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv,
-                                                                                           t2, A64.p);
-      CCX_HIP(hipGetLastError());
-      x3_split_rows<double>(A64.p, (int64_t)i1 * vv, Kp, Kp, scA.p, Ahl.p, Ahl.p + 32, Kp, s, true);
-      CCX_HIP(hipStreamSynchronize(s));
-    }
+  // (2 KC + 1) 2v x Kp), tile partials, and A (i1 v^2 x Kp)
+  const int64_t nXc = vv * 2 * v * KC, nB = (2 * KC + 1) * 2 * v * Kp;
+  DevBuf<int> order(tiles);
+  {
+    const std::vector<int> h = t3_orbit_order(nt);
+    CCX_HIP(hipMemcpy(order.p, h.data(), (size_t)tiles * sizeof(int), hipMemcpyHostToDevice));
   }
-  const int64_t n64 = 3 * nXc + vvv + nB + tiles + (x3 ? 0 : nA);
-  const int64_t nx3 = x3 ? nB * 2 * (int64_t)sizeof(f16) / 8 + (np + 8) / 2 + 1 : 0;   // in doubles
-  DevBuf<double> buf(n64 + nx3);
-  double* base = buf.p;
-  double *XI = base, *XJ = XI + nXc, *XK = XJ + nXc, *W = XK + nXc, *B = W + vvv, *partials = B + nB;
+  DevBuf<double> buf(3 * nXc + vvv + nB + tiles + nA);
+  double *XI = buf.p, *XJ = XI + nXc, *XK = XJ + nXc, *W = XK + nXc, *B = W + vvv, *partials = B + nB;
   double* A = partials + tiles;
-  f16* Bhl = x3 ? (f16*)(base + n64) : nullptr;       // B split, [(2 KC + 1) 2v][2 Kp]
-  float* sc = x3 ? (float*)(Bhl + nB * 2) : nullptr;  // sc[1] = scale of B
-  float* part = x3 ? sc + 8 : nullptr;
   Workspace& ws = c.ws;
 
-  if (!x3 && nA > 0) {
+  if (nA > 0) {
     t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv, t2, A);
     CCX_HIP(hipGetLastError());
   }
@@ -369,7 +687,6 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
         double* BI = B;
         double* BJ = BI + (int64_t)nk * 2 * v * Kp;
         double* BK = BJ + (int64_t)nk * 2 * v * Kp;
-        const int64_t nrows = (2 * (int64_t)nk + 1) * 2 * v;
         const int pg = (int)std::min<int64_t>(1024, ceil_div((int64_t)nk * 2 * v * Kp, 256));
         t3_pack_kernel<<<pg, 256, 0, s>>>(o, v, Kp, pi, t2, c.Cooov, BI);
         CCX_HIP(hipGetLastError());
@@ -378,35 +695,15 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
         t3_pack_kernel<<<(int)std::min<int64_t>(1024, ceil_div(2 * v * Kp, 256)), 256, 0, s>>>(o, v, Kp, pk, t2,
                                                                                            c.Cooov, BK);
         CCX_HIP(hipGetLastError());
-        if (x3) {
-          x3_absmax<double>(B, nrows * Kp, part, s);
-          x3_scale(part, 1.f, nullptr, 0.f, sc + 1, s);
-          x3_split_rows<double>(B, nrows, Kp, Kp, sc + 1, Bhl, Bhl + 32, Kp, s, true);
-          const f16* BhI = Bhl;
-          const f16* BhJ = Bhl + (int64_t)nk * 2 * v * 2 * Kp;
-          const f16* BhK = BhJ + (int64_t)nk * 2 * v * 2 * Kp;
-          auto run = [&](const f16* Ah, int64_t M, const f16* Bh, int64_t N, double* C, int64_t ldc) {
-            X3Gemm g;   // C[m][n] = sum_kk A[m][kk] B[n][kk]
-            g.M = M; g.N = N; g.Kp = Kp; g.lda = 2 * Kp; g.ldb = 2 * Kp; g.ldc = ldc;
-            g.kb = 32; g.ska = 64; g.skb = 64;
-            g.Ah = Ah; g.Al = Ah + 32; g.Bh = Bh; g.Bl = Bh + 32;
-            g.sa = scA.p; g.sb = sc + 1; g.C64 = C;
-            x3_gemm<_Float16>(g, s, &ws);
-          };
-          run(Ahl.p + (int64_t)i * vv * 2 * Kp, vv, BhI, 2 * v * nk, XI, ldx);
-          run(Ahl.p + (int64_t)j * vv * 2 * Kp, vv, BhJ, 2 * v * nk, XJ, ldx);
-          run(Ahl.p + (int64_t)k0 * vv * 2 * Kp, nk * vv, BhK, 2 * v, XK, 2 * v);
-        } else {
-          const std::vector<int64_t> VVK{v, v, Kp};
-          const TV XIv = tvs(XI, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
-          const TV XJv = tvs(XJ, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
-          contract<double>(1.0, tv(A + (int64_t)i * vv * Kp, "xyk", VVK), tv(BI, "zk", {2 * v * nk, Kp}), 0.0, XIv,
-                           ws, s);
-          contract<double>(1.0, tv(A + (int64_t)j * vv * Kp, "xyk", VVK), tv(BJ, "zk", {2 * v * nk, Kp}), 0.0, XJv,
-                           ws, s);
-          contract<double>(1.0, tv(A + (int64_t)k0 * vv * Kp, "mk", {nk * vv, Kp}), tv(BK, "zk", {2 * v, Kp}), 0.0,
-                           tv(XK, "mz", {nk * vv, 2 * v}), ws, s);
-        }
+        const std::vector<int64_t> VVK{v, v, Kp};
+        const TV XIv = tvs(XI, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
+        const TV XJv = tvs(XJ, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
+        contract<double>(1.0, tv(A + (int64_t)i * vv * Kp, "xyk", VVK), tv(BI, "zk", {2 * v * nk, Kp}), 0.0, XIv,
+                         ws, s);
+        contract<double>(1.0, tv(A + (int64_t)j * vv * Kp, "xyk", VVK), tv(BJ, "zk", {2 * v * nk, Kp}), 0.0, XJv,
+                         ws, s);
+        contract<double>(1.0, tv(A + (int64_t)k0 * vv * Kp, "mk", {nk * vv, Kp}), tv(BK, "zk", {2 * v, Kp}), 0.0,
+                         tv(XK, "mz", {nk * vv, 2 * v}), ws, s);
         for (int e = 0; e < nk; ++e) {
           const int k = k0 + e;
           T3Slots xs;
@@ -415,7 +712,8 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
           xs.X[2] = XK + (int64_t)e * vv * 2 * v; xs.s0[2] = 2 * vv; xs.s1[2] = 2 * v;
           t3_assemble_kernel<<<(int)tiles, 256, 0, s>>>(v, xs, W);
           CCX_HIP(hipGetLastError());
-          t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, partials);
+          t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, order.p,
+                                                      partials);
           CCX_HIP(hipGetLastError());
           const double weight = 2.0 - (double)((i == j) + (i == k) + (j == k));   // cctriples.py:237
           t3_accumulate_kernel<<<1, 256, 0, s>>>(partials, tiles, weight, e_dev);
