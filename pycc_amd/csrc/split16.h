@@ -46,7 +46,7 @@ void x3_gemm(const X3Gemm& p, hipStream_t s, Workspace* ws = nullptr);
 
 // nbatch FP16x3 GEMMs C_b = alpha / (sa sb) [A | A2_b swapped] [B_b] with K = Kp concatenated
 // from two operands (fields Ah2 .. sC above): the connected (T) numerators, two slot products per
-// GEMM. Output tiles of 256 x (192 or 256), whichever pads N less.
+// GEMM. Output tiles of 384 x 192, 256 x 256 or 256 x 192, whichever pads M x N least.
 void x3_gemm_kcat(const X3Gemm& p, hipStream_t s);
 // max |x| over n elements as x3_partials() per-block partial maxima in `part`.
 template <typename T>

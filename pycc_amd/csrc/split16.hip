@@ -404,12 +404,13 @@ __device__ __forceinline__ void x3_dma16(const E* g, const E* lds) {
 }
 
 // BN = 192, 256 or 320 output columns per tile (320: 128 x 80 per wave, 144 KiB of LDS; B's 20 row
-// blocks of 16 go to wave q % 8, so waves 0-3 load three and 4-7 two). KCAT: the K-concatenated,
-// batched form of x3_gemm_kcat (interleaved operands, FP32 C, no split K).
-template <typename E, int BN, bool KCAT = false>
+// blocks of 16 go to wave q % 8, so waves 0-3 load three and 4-7 two); BT = 256 or 384 rows (384
+// with BN = 192: 192 x 48 per wave, 144 KiB). KCAT: the K-concatenated, batched form of
+// x3_gemm_kcat (interleaved operands, FP32 C, no split K).
+template <typename E, int BN, bool KCAT = false, int BT = 256>
 __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   typedef typename X3<E>::v8 v8;
-  constexpr int BT = 256, WTM = 128, WTN = BN / 4, NBM = WTM / 16, NBN = WTN / 16, IPW = 2;
+  constexpr int WTM = BT / 2, WTN = BN / 4, NBM = WTM / 16, NBN = WTN / 16, IPW = BT / 128;
   constexpr int NIB = BN / 16, IPB = (NIB + 7) / 8;   // B row blocks, per wave (rounded up)
   __shared__ __attribute__((aligned(1024))) E sA[2][2][BT * 32];   // [stage][hi/lo]
   __shared__ __attribute__((aligned(1024))) E sB[2][2][BN * 32];
@@ -993,11 +994,23 @@ void x3_gemm_kcat(const X3Gemm& p, hipStream_t s) {
                 p.K1 < p.Kp && p.lda % 8 == 0 && p.ldb % 8 == 0 && p.Ah2 && p.C && !p.C64 && !p.work &&
                 p.sw > 0 && p.sw * p.sw == p.M && p.beta == 0.f && !p.sbn,
             ERR_INTERNAL, "x3 K-concatenated GEMM layout");
-  // the output tile width that pads N less (256 on a tie; the 320-column tile spills registers here)
-  const int bn = ceil_div(p.N, 192) * 192 < ceil_div(p.N, 256) * 256 ? 192 : 256;
-  const int64_t tiles = ceil_div(p.M, 256) * ceil_div(p.N, bn) * p.nbatch;
+  // the output tile (384 x 192, 256 x 256 or 256 x 192) that pads M x N least, in that order on a
+  // tie (the 320-column tile spills registers in this form)
+  const int shapes[3][2] = {{384, 192}, {256, 256}, {256, 192}};
+  int pick = 0;
+  double best = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double area = (double)(ceil_div(p.M, shapes[c][0]) * shapes[c][0]) *
+                        (double)(ceil_div(p.N, shapes[c][1]) * shapes[c][1]);
+    if (c == 0 || area < best) {
+      best = area;
+      pick = c;
+    }
+  }
+  const int64_t tiles = ceil_div(p.M, shapes[pick][0]) * ceil_div(p.N, shapes[pick][1]) * p.nbatch;
   CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
-  if (bn == 256) x3_gemm_dma_kernel<_Float16, 256, true><<<(unsigned)tiles, 512, 0, s>>>(p);
+  if (pick == 0) x3_gemm_dma_kernel<_Float16, 192, true, 384><<<(unsigned)tiles, 512, 0, s>>>(p);
+  else if (pick == 1) x3_gemm_dma_kernel<_Float16, 256, true><<<(unsigned)tiles, 512, 0, s>>>(p);
   else x3_gemm_dma_kernel<_Float16, 192, true><<<(unsigned)tiles, 512, 0, s>>>(p);
   CCX_HIP(hipGetLastError());
 }
