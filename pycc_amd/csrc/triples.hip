@@ -1,5 +1,6 @@
 // (T) correction of closed-shell CCSD(T) (pycc/cctriples.py:177-240, t_tjl): FP64, except that the
-// MIXED_F16X3 mode runs its three GEMMs per triple as FP16x3 split products with FP64 output.
+// MIXED_F16X3 mode pairs the slot products along K into three FP16x3 GEMMs with FP32 output and
+// sums the energy of each orbit of tiles from LDS, never storing W (triples_energy_x3 below).
 //
 // For every occupied triple i >= j >= k the connected numerator W[a][b][c] (cctriples.py:27-72,
 // t3c_ijk without the denominator) comes from GEMMs and one tiled permute-and-add: for each p in
