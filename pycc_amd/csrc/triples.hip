@@ -384,13 +384,46 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
       out[h] = val;
     }
   };
-  f4v ra[6][NH], rb[6][NH];
+  // every load of the work-group is issued here: the Y boxes of the kept tiles, then the
+  // disconnected factors of each kept tile (held in registers until its energy sum)
+  f4v ra[6][NH], rb[6][NH], rc[6][NH];
 #pragma unroll
   for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) box(y1, T[x][0], T[x][1], T[x][2], ra[x]);   // Y1[a][b][c]
+    if (rep[x] == x) {
+      box(y1, T[x][0], T[x][1], T[x][2], ra[x]);   // Y1[a][b][c]
+      box(y2, T[x][0], T[x][2], T[x][1], rb[x]);   // Y2[a][c][b]
+      box(y3, T[x][1], T[x][2], T[x][0], rc[x]);   // Y3[b][c][a]
+    }
+  // energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) of the a half t / 256
+  const int half = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, cl = threadIdx.x & 15;
+  const double* I6[6] = {Ioovv + ((int64_t)i * o + j) * vv, t2 + ((int64_t)i * o + j) * vv,
+                         Ioovv + ((int64_t)i * o + k) * vv, t2 + ((int64_t)i * o + k) * vv,
+                         Ioovv + ((int64_t)j * o + k) * vv, t2 + ((int64_t)j * o + k) * vv};
+  const int occ3[3] = {i, j, k};
+  // block d of sD: rows along (a, a, a, a, b, b), columns along (b, b, c, c, c, c); the half h of
+  // the threads loads the blocks 3 h .. 3 h + 2, and threads below 144 one element of the vectors
+  // q = 3 n + u (u = 0 t1, 1 f_(occ) row, 2 eps, for n = a, b, c)
+  double pd[6][3], pv[6];
 #pragma unroll
-  for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) box(y2, T[x][0], T[x][2], T[x][1], rb[x]);   // Y2[a][c][b]
+  for (int x = 0; x < 6; ++x) {
+    pv[x] = 0.0;
+    if (rep[x] != x) continue;
+    const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd) {
+      const int d = 3 * half + dd;
+      const int64_t r = org[d < 4 ? 0 : 1] + bl, cc = org[d < 2 ? 1 : 2] + cl;
+      pd[x][dd] = (r < v && cc < v) ? I6[d][r * v + cc] : 0.0;
+    }
+    if (threadIdx.x < 9 * TT) {
+      const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
+      const int64_t g = org[n] + cl;
+      if (g < v) {
+        const int occ = occ3[n];
+        pv[x] = u == 0 ? t1[occ * v + g] : (u == 1 ? F[occ * nmo + o + g] : F[(o + g) * nmo + o + g]);
+      }
+    }
+  }
 #pragma unroll
   for (int x = 0; x < 6; ++x)
     if (rep[x] == x) {
@@ -401,9 +434,6 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
         for (int u = 0; u < 4; ++u) w[(xl0 + XS * h) * OX + yl * OY + zl + u] = ra[x][h][u];
     }
   __syncthreads();
-#pragma unroll
-  for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) box(y3, T[x][1], T[x][2], T[x][0], ra[x]);   // Y3[b][c][a]
 #pragma unroll
   for (int x = 0; x < 6; ++x)
     if (rep[x] == x) {   // Y2 element (a, c, b) = (xl, yl, zl + u) -> W[a][b][c]
@@ -421,50 +451,24 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
 #pragma unroll
       for (int h = 0; h < NH; ++h)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) w[(zl + u) * OX + (xl0 + XS * h) * OY + yl] += ra[x][h][u];
+        for (int u = 0; u < 4; ++u) w[(zl + u) * OX + (xl0 + XS * h) * OY + yl] += rc[x][h][u];
     }
   __syncthreads();
 
-  // The disconnected numerator's factors of a tile go through LDS: the six 16 x 16 blocks Iij[a][b],
-  // Tij[a][b], Iik[a][c], Tik[a][c], Ijk[b][c], Tjk[b][c] and the vectors t1_ia, f_ia, eps_a (and
-  // those of j, b and k, c), loaded once per tile with every load in flight together.
+  // The disconnected factors of a tile go through LDS: the six 16 x 16 blocks Iij[a][b], Tij[a][b],
+  // Iik[a][c], Tik[a][c], Ijk[b][c], Tjk[b][c] and the vectors t1_ia, f_ia, eps_a (and those of j,
+  // b and k, c).
   __shared__ double sD[6][TT][TT + 1];
   __shared__ double sV[9][TT];
-  // energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) of the a half t / 256
-  const int half = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, cl = threadIdx.x & 15;
-  const double* I6[6] = {Ioovv + ((int64_t)i * o + j) * vv, t2 + ((int64_t)i * o + j) * vv,
-                         Ioovv + ((int64_t)i * o + k) * vv, t2 + ((int64_t)i * o + k) * vv,
-                         Ioovv + ((int64_t)j * o + k) * vv, t2 + ((int64_t)j * o + k) * vv};
-  const int occ3[3] = {i, j, k};
   const double dijk = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k];
   double s = 0.0;
 #pragma unroll
   for (int x = 0; x < 6; ++x) {
     if (rep[x] != x) continue;   // block-uniform
     const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
-    {
-      // block d of sD: rows along (a, a, a, a, b, b), columns along (b, b, c, c, c, c); the half h of
-      // the threads loads the blocks 3 h .. 3 h + 2
-      double ld[3];
 #pragma unroll
-      for (int dd = 0; dd < 3; ++dd) {
-        const int d = 3 * half + dd;
-        const int64_t r = org[d < 4 ? 0 : 1] + bl, cc = org[d < 2 ? 1 : 2] + cl;
-        ld[dd] = (r < v && cc < v) ? I6[d][r * v + cc] : 0.0;
-      }
-      double lv = 0.0;
-      if (threadIdx.x < 9 * TT) {   // vector q = 3 n + u: u = 0 t1, 1 f_(occ) row, 2 eps, for n = a, b, c
-        const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
-        const int64_t g = org[n] + cl;
-        if (g < v) {
-          const int occ = occ3[n];
-          lv = u == 0 ? t1[occ * v + g] : (u == 1 ? F[occ * nmo + o + g] : F[(o + g) * nmo + o + g]);
-        }
-      }
-#pragma unroll
-      for (int dd = 0; dd < 3; ++dd) sD[3 * half + dd][bl][cl] = ld[dd];
-      if (threadIdx.x < 9 * TT) sV[threadIdx.x / TT][cl] = lv;
-    }
+    for (int dd = 0; dd < 3; ++dd) sD[3 * half + dd][bl][cl] = pd[x][dd];
+    if (threadIdx.x < 9 * TT) sV[threadIdx.x / TT][cl] = pv[x];
     __syncthreads();
     const int64_t a0 = org[0], b = org[1] + bl, c = org[2] + cl;
     if (b < v && c < v) {

@@ -85,7 +85,9 @@ def test_triples_matches_oracle(no, nv, seed):
     assert abs(et - ref) < 1e-12 * max(abs(ref), 1e-3), (et, ref)
 
 
-@pytest.mark.parametrize("no,nv,seed", [(4, 21, 7), (9, 13, 8)])   # o >= 9: several k chunks per (i, j)
+# o >= 9: several k chunks per (i, j); v = 50: 4^3 tiles of 16 (orbits of one, three and six tiles,
+# the last tile partly past v)
+@pytest.mark.parametrize("no,nv,seed", [(4, 21, 7), (9, 13, 8), (6, 50, 10)])
 def test_triples_density_fitted_and_mixed_contexts(no, nv, seed):
     _need_gpu()
     import pycc_amd
