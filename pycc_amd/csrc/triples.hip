@@ -30,8 +30,8 @@ namespace ccx {
 
 namespace {
 
-__device__ __forceinline__ double t3_block_sum(double x) {   // blockDim.x <= 512, a power of two
-  __shared__ double sh[512];
+__device__ __forceinline__ double t3_block_sum(double x) {   // blockDim.x <= 1024, a power of two
+  __shared__ double sh[1024];
   sh[threadIdx.x] = x;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -323,7 +323,7 @@ constexpr int OX = 273, OY = 17, OTS = TT * OX;   // LDS tile [x][y][z] at x OX 
 // slot x holds the tile at the permutation (A, B, C), (A, C, B), (B, A, C), (B, C, A), (C, A, B),
 // (C, B, A) of the orbit.
 
-constexpr int OT_THREADS = 512;   // 2 waves per SIMD at one work-group per CU
+constexpr int OT_THREADS = 1024;   // 4 waves per SIMD at one work-group per CU
 __device__ __forceinline__ double t3_rcp(double x) {   // 1/x: the hardware estimate and two Newton steps
   double r = __builtin_amdgcn_rcp(x);
   r = fma(fma(-x, r, 1.0), r, r);
@@ -394,26 +394,29 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
       box(y2, T[x][0], T[x][2], T[x][1], rb[x]);   // Y2[a][c][b]
       box(y3, T[x][1], T[x][2], T[x][0], rc[x]);   // Y3[b][c][a]
     }
-  // energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) of the a half t / 256
-  const int half = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, cl = threadIdx.x & 15;
+  // energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) and the AL values of a from
+  // AL (t / 256)
+  constexpr int AL = 16 * 256 / OT_THREADS;
+  const int part = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, cl = threadIdx.x & 15;
   const double* I6[6] = {Ioovv + ((int64_t)i * o + j) * vv, t2 + ((int64_t)i * o + j) * vv,
                          Ioovv + ((int64_t)i * o + k) * vv, t2 + ((int64_t)i * o + k) * vv,
                          Ioovv + ((int64_t)j * o + k) * vv, t2 + ((int64_t)j * o + k) * vv};
   const int occ3[3] = {i, j, k};
-  // block d of sD: rows along (a, a, a, a, b, b), columns along (b, b, c, c, c, c); the half h of
-  // the threads loads the blocks 3 h .. 3 h + 2, and threads below 144 one element of the vectors
-  // q = 3 n + u (u = 0 t1, 1 f_(occ) row, 2 eps, for n = a, b, c)
-  double pd[6][3], pv[6];
+  // block d of sD: rows along (a, a, a, a, b, b), columns along (b, b, c, c, c, c); thread t loads
+  // the elements t + OT_THREADS r of the six blocks, and threads below 144 one element of the
+  // vectors q = 3 n + u (u = 0 t1, 1 f_(occ) row, 2 eps, for n = a, b, c)
+  constexpr int NR = (6 * 256 + OT_THREADS - 1) / OT_THREADS;
+  double pd[6][NR], pv[6];
 #pragma unroll
   for (int x = 0; x < 6; ++x) {
     pv[x] = 0.0;
     if (rep[x] != x) continue;
     const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
 #pragma unroll
-    for (int dd = 0; dd < 3; ++dd) {
-      const int d = 3 * half + dd;
-      const int64_t r = org[d < 4 ? 0 : 1] + bl, cc = org[d < 2 ? 1 : 2] + cl;
-      pd[x][dd] = (r < v && cc < v) ? I6[d][r * v + cc] : 0.0;
+    for (int rr = 0; rr < NR; ++rr) {
+      const int el = threadIdx.x + OT_THREADS * rr, d = el >> 8;
+      const int64_t r = org[d < 4 ? 0 : 1] + ((el >> 4) & 15), cc = org[d < 2 ? 1 : 2] + (el & 15);
+      pd[x][rr] = (el < 6 * 256 && r < v && cc < v) ? I6[d < 6 ? d : 0][r * v + cc] : 0.0;
     }
     if (threadIdx.x < 9 * TT) {
       const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
@@ -467,7 +470,10 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
     if (rep[x] != x) continue;   // block-uniform
     const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
 #pragma unroll
-    for (int dd = 0; dd < 3; ++dd) sD[3 * half + dd][bl][cl] = pd[x][dd];
+    for (int rr = 0; rr < NR; ++rr) {
+      const int el = threadIdx.x + OT_THREADS * rr;
+      if (el < 6 * 256) sD[el >> 8][(el >> 4) & 15][el & 15] = pd[x][rr];
+    }
     if (threadIdx.x < 9 * TT) sV[threadIdx.x / TT][cl] = pv[x];
     __syncthreads();
     const int64_t a0 = org[0], b = org[1] + bl, c = org[2] + cl;
@@ -482,7 +488,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
       const double ijbc = sD[4][bl][cl], tjbc = sD[5][bl][cl];
       const double t1jb = sV[3][bl], fjb = sV[4][bl], t1kc = sV[6][cl], fkc = sV[7][cl];
 #pragma unroll 4
-      for (int al = 8 * half; al < 8 * half + 8; ++al) {
+      for (int al = AL * part; al < AL * part + AL; ++al) {
         if (a0 + al < v) {   // wave-uniform
           const double w = w0[al * OX + bl * OY + cl];
           const double q = (double)wc[bl * OX + cl * OY + al] + (double)wc2[cl * OX + al * OY + bl] -
