@@ -86,8 +86,8 @@ def test_triples_matches_oracle(no, nv, seed):
 
 
 # o >= 9: several k chunks per (i, j); v = 50: 4^3 tiles of 16 (orbits of one, three and six tiles,
-# the last tile partly past v)
-@pytest.mark.parametrize("no,nv,seed", [(4, 21, 7), (9, 13, 8), (6, 50, 10)])
+# the last tile partly past v); v = 3, 2: one partial tile, rows shorter than a 4-float vector
+@pytest.mark.parametrize("no,nv,seed", [(4, 21, 7), (9, 13, 8), (6, 50, 10), (2, 3, 11), (3, 2, 12)])
 def test_triples_density_fitted_and_mixed_contexts(no, nv, seed):
     _need_gpu()
     import pycc_amd
@@ -102,7 +102,9 @@ def test_triples_density_fitted_and_mixed_contexts(no, nv, seed):
     mx = pycc_amd.CCwfn(H, no=no, precision="MIXED", verbose=False).triples_energy(t1, t2)
     assert abs(mx - dp) < 1e-14 * abs(dp), (mx, dp)
     x3 = pycc_amd.CCwfn(H, no=no, precision="MIXED_F16X3", verbose=False).triples_energy(t1, t2)
-    assert abs(x3 - dp) < 1e-8 * abs(dp), (x3, dp)
+    # FP16x3 products carry ~2^-22 relative error each; over a few dozen terms (v < 8) it does not
+    # average out (3e-8 to 5e-8 measured at v = 2, 3, before and after the K pairing)
+    assert abs(x3 - dp) < (1e-7 if nv < 8 else 1e-8) * abs(dp), (x3, dp)
 
 
 @pytest.mark.parametrize("precision", ["DP", "MIXED_F16X3"])
