@@ -363,7 +363,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   const float* y1 = Y1 + e * ys;
   const float* y2 = Y2 + e * ys;
   const float* y3 = Y3 + e * ys;
-  // box loads: thread t reads 4 floats at z = 4 (t % 4) of the rows (x, y) = (t / 64 + 8 h, t / 4 % 16)
+  // box loads: thread t reads 4 floats at z = 4 (t % 4) of the rows (x, y) = (t / 64 + XS h, t / 4 % 16)
   constexpr int NH = 4096 / (4 * OT_THREADS), XS = 16 / NH;
   const int r0 = threadIdx.x >> 2, c4 = threadIdx.x & 3;
   const int xl0 = r0 >> 4, yl = r0 & 15, zl = 4 * c4;
