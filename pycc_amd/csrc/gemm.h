@@ -58,10 +58,18 @@ struct PlainGemm {
   // split-K: when ktot > 0, grid.y runs over K chunks of length K (the last one shorter), each
   // chunk is a "batch" entry (a_b / b_b advance by one chunk) and ktot is the total K.
   int64_t ktot = 0;
+  // K concatenation (plain_gemm_kcat; A k-contiguous, FP64): k in [K1, K) reads A2 at k - K1 and
+  // row (m mod sw) sw + m / sw, the (x, y) -> (y, x) swap of a row m = x sw + y (M = sw^2); batch b
+  // adds b a2_b to A2.
+  const T* A2 = nullptr;
+  int64_t K1 = 0, sw = 0, a2_b = 0;
 };
 template <typename T>
 bool plain_gemm_ok(const PlainGemm<T>& p);
 template <typename T>
 void plain_gemm(const PlainGemm<T>& p, hipStream_t s);
+// C_b = alpha [A | A2_b swapped] B_b^T + beta C_b over K = K concatenated at K1 (FP64, A and B
+// k-contiguous, fields A2 .. a2_b above; K1 a multiple of 16).
+void plain_gemm_kcat(const PlainGemm<double>& p, hipStream_t s);
 
 }  // namespace ccx

@@ -102,6 +102,27 @@ struct PLoader {
       v[i] = x.u;
     }
   }
+  // row r of the tile at element offset roff[r] (an LDS table, relative to base) instead of r * ld;
+  // guarded like load_safe unless FASTL (rows beyond nrows / k beyond nk are zero)
+  template <bool FASTL>
+  __device__ __forceinline__ void load_off(const T* __restrict__ base, const int* roff, int64_t nrows, int64_t nk) {
+    if constexpr (!KC) return;   // row offsets are for k-contiguous tiles only
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int64_t o = roff[row[i]];
+      if constexpr (FASTL) {
+        v[i] = *(const uint4*)(base + o + seg[i] * VEC);
+      } else {
+        union { uint4 u; T e[VEC]; } x;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          const int64_t k = (int64_t)seg[i] * VEC + e;
+          x.e[e] = (row[i] < nrows && k < nk) ? base[o + k] : T(0);
+        }
+        v[i] = x.u;
+      }
+    }
+  }
   __device__ __forceinline__ void store(PTile<T, R, BK, KC>& t) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) *(uint4*)&t.d[row[i]][seg[i] * VEC] = v[i];
@@ -110,7 +131,8 @@ struct PLoader {
 
 // BT = 128: 4 waves of 64 x 64, two workgroups per CU. BT = 256 (FP32 only): 4 waves of 128 x 128
 // (256 accumulator registers each), one workgroup per CU, 4x the MFMAs per LDS read and per barrier.
-template <typename T, int BK, bool A_KC, bool B_NC, int BT>
+// KCAT: the K-concatenated form of plain_gemm_kcat (A k-contiguous).
+template <typename T, int BK, bool A_KC, bool B_NC, int BT, bool KCAT = false>
 __global__ __launch_bounds__(256, BT == 128 ? 2 : 1) void plain_gemm_kernel(PlainGemm<T> p) {
   constexpr int BM = BT, BN = BT, WM = BT / 2, WN = BT / 2, NBI = WM / 16, NBJ = WN / 16;
   typedef typename PAcc<T>::type acc_t;
@@ -148,6 +170,21 @@ __global__ __launch_bounds__(256, BT == 128 ? 2 : 1) void plain_gemm_kernel(Plai
   PLoader<T, BN, BK, !B_NC> lb;
   la.init(tid);
   lb.init(tid);
+  // KCAT: the second operand's row offsets, one per tile row, in LDS (sw^2 lda < 2^31, checked on
+  // the host; rows past M clamp to row M - 1 and are not stored). The barrier before the k loop
+  // publishes them.
+  // The first operand's rows get the same treatment (offsets r lda from the tile origin), so that
+  // one load path serves both.
+  const T* A2 = KCAT ? p.A2 + blockIdx.y * p.a2_b : nullptr;
+  __shared__ int roff[2][KCAT ? BM : 1];
+  if constexpr (KCAT) {
+    if (tid < BM) {
+      const int64_t m = m0 + tid < p.M ? m0 + tid : p.M - 1;
+      roff[0][tid] = (int)(tid * p.lda);
+      roff[1][tid] = (int)(((m % p.sw) * p.sw + m / p.sw) * p.lda);
+    }
+    __syncthreads();
+  }
 
   acc_t acc[NBI][NBJ];
 #pragma unroll
@@ -162,14 +199,17 @@ __global__ __launch_bounds__(256, BT == 128 ? 2 : 1) void plain_gemm_kernel(Plai
   auto kloop = [&](auto fast_tag) {
     constexpr bool FAST = decltype(fast_tag)::value;
     auto load = [&](int64_t k0) {
-      if constexpr (FAST) {
+      const int64_t nk = Kb - k0;
+      if constexpr (KCAT) {   // the k-tiles past K1 read the second operand (block-uniform)
+        const bool second = k0 >= p.K1;
+        la.template load_off<FAST>(second ? A2 + (k0 - p.K1) : a_at(k0), roff[second ? 1 : 0], a_rows, nk);
+      } else if constexpr (FAST) {
         la.load_fast(a_at(k0), p.lda);
-        lb.load_fast(b_at(k0), p.ldb);
       } else {
-        const int64_t nk = Kb - k0;
         la.load_safe(a_at(k0), p.lda, a_rows, nk);
-        lb.load_safe(b_at(k0), p.ldb, b_rows, nk);
       }
+      if constexpr (FAST) lb.load_fast(b_at(k0), p.ldb);
+      else lb.load_safe(b_at(k0), p.ldb, b_rows, nk);
     };
     load(0);
     la.store(As[0]);
@@ -271,6 +311,17 @@ void plain_gemm(const PlainGemm<T>& p, hipStream_t s) {
   else if (p.a_kc) launch_plain<T, true, false>(p, s);
   else if (p.b_nc) launch_plain<T, false, true>(p, s);
   else launch_plain<T, false, false>(p, s);
+}
+
+void plain_gemm_kcat(const PlainGemm<double>& p, hipStream_t s) {
+  if (p.M == 0 || p.N == 0 || p.batch == 0) return;
+  CCX_CHECK(plain_gemm_ok(p) && p.a_kc && !p.b_nc && p.ktot == 0 && p.A2 && p.K1 > 0 && p.K1 % 16 == 0 &&
+                p.K1 < p.K && p.sw > 0 && p.sw * p.sw == p.M && p.M * p.lda < (int64_t(1) << 31) &&
+                ((uintptr_t)p.A2 % 16) == 0 && p.a2_b % 2 == 0,
+            ERR_INTERNAL, "plain_gemm_kcat layout");
+  const int64_t tiles = ceil_div(p.M, 128) * ceil_div(p.N, 128);
+  plain_gemm_kernel<double, 16, true, false, 128, true><<<dim3((unsigned)tiles, (unsigned)p.batch), 256, 0, s>>>(p);
+  CCX_HIP(hipGetLastError());
 }
 
 template bool plain_gemm_ok<double>(const PlainGemm<double>&);

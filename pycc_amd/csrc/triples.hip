@@ -1,15 +1,13 @@
-// (T) correction of closed-shell CCSD(T) (pycc/cctriples.py:177-240, t_tjl): FP64, except that the
-// MIXED_F16X3 mode pairs the slot products along K into three FP16x3 GEMMs with FP32 output and
-// sums the energy of each orbit of tiles from LDS, never storing W (triples_energy_x3 below).
+// (T) correction of closed-shell CCSD(T) (pycc/cctriples.py:177-240, t_tjl): FP64, except in the
+// MIXED_F16X3 mode, whose slot GEMMs are FP16x3 with FP32 output and which sums the energy of each
+// orbit of tiles from LDS, never storing W (triples_energy_x3 below).
 //
 // For every occupied triple i >= j >= k the connected numerator W[a][b][c] (cctriples.py:27-72,
-// t3c_ijk without the denominator) comes from GEMMs and one tiled permute-and-add: for each p in
-// (i, j, k) the four terms that share p (two <ab|ep> t2 and two <mc|qr> t2 products) are one GEMM
-// row block of v^2 x 2v x (v + o), whose A operand [Covvv[p] | t2[p]^T] is built once per call and
-// whose B operand (t2 and -(mq|rc) columns of the two (q, r) orders side by side) is packed per
-// triple. For fixed (i, j), up to T3_KC values of k share one GEMM per slot (wider N for p = i and
-// p = j, taller M for p = k); the three v^2 x 2v blocks of a triple are then added into W's [a][b][c]
-// order through 16^3 LDS tiles. A tiled kernel then adds the disconnected part (t3d_ijk,
+// t3c_ijk without the denominator) comes from GEMMs against A_p = [Covvv[p] | t2[p]^T] (built once
+// per call): the six slot products, each holding two of the twelve t3c_ijk terms, pair up along K
+// into three GEMMs per triple (Y1, Y2, Y3 below), batched over up to T3_KC values of k for fixed
+// (i, j); a tiled kernel adds their three v^3 blocks into W's [a][b][c] order through 16^3 LDS
+// tiles. A tiled kernel then adds the disconnected part (t3d_ijk,
 // cctriples.py:108-146: <ij|ab> t1_kc + t2_ijab f_kc and their permutations) to form V and reduces
 //
 //   E_ijk = sum_abc V_abc (4 W_abc + W_bca + W_cab - 2 W_acb - 2 W_bac - 2 W_cba) / D_ijk^abc,
@@ -125,81 +123,36 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
-// Up to T3_KC (q, r) entries; entry e packs the two orders (q_e, r_e) and (r_e, q_e).
+// k values per chunk of a fixed (i, j)
 constexpr int T3_KC = 8;
-struct T3Pairs {
-  int n;
-  int q[T3_KC], r[T3_KC];
-};
 
-// B[e][z'][kk] = t2[r][q][z][kk] for kk < v and -(m q|r z) = -<m z|q r> with m = kk - v below v + o,
-// where (q, r) is entry e's first order for z' < v and its second order for z' >= v, z = z' mod v.
-// For the triple (i, j, k) the slot of p = i takes the entry (j, k), that of p = j the entry (i, k)
-// and that of p = k the entry (i, j).
-__global__ void t3_pack_kernel(int64_t o, int64_t v, int64_t Kp, T3Pairs pr, const double* __restrict__ t2,
-                               const double* __restrict__ Cooov, double* __restrict__ B) {
-  const int64_t total = (int64_t)pr.n * 2 * v * Kp;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t kk = e % Kp, zz = e / Kp % (2 * v), en = e / (2 * v * Kp);
-    const int h = zz >= v, z = (int)(zz - h * v);
-    int q0 = pr.q[0], r0 = pr.r[0];
-#pragma unroll
-    for (int x = 1; x < T3_KC; ++x)
-      if (en == x) { q0 = pr.q[x]; r0 = pr.r[x]; }
-    const int q = h ? r0 : q0, r = h ? q0 : r0;
-    double val = 0.0;
-    if (kk < v) val = t2[(((int64_t)r * o + q) * v + z) * v + kk];
-    else if (kk < v + o) val = -Cooov[(((kk - v) * o + q) * o + r) * v + z];
-    B[e] = val;
-  }
-}
-
-// W[a][b][c] = X_ijk[a,b,c] + X_ikj[a,c,b] + X_jik[b,a,c] + X_jki[b,c,a] + X_kij[c,a,b] + X_kji[c,b,a]
-// from the slot products X_s[x][y][z'] (X_s[x * s0 + y * s1 + z'], z' in [0, 2v); z' >= v holds the
-// slot's second order).
-struct T3Slots {
-  const double* X[3];
-  int64_t s0[3], s1[3];
-};
-__global__ __launch_bounds__(256) void t3_assemble_kernel(int64_t v, T3Slots xs, double* __restrict__ W) {
+// W[a][b][c] = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a] (the K-paired slot products below, FP64,
+// rows [x][y] of v elements)
+__global__ __launch_bounds__(256) void t3_assemble3_kernel(int64_t v, const double* __restrict__ Y1,
+                                                           const double* __restrict__ Y2,
+                                                           const double* __restrict__ Y3, double* __restrict__ W) {
   __shared__ double sh[TT][TT][TP];
   int64_t org[3];
   t3_tile_origin(v, org);
   const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
-  const double *X0 = xs.X[0], *X1 = xs.X[1], *X2 = xs.X[2];
+  const int64_t vv = v * v;
   double acc[TT], r[TT];
 #pragma unroll
   for (int al = 0; al < TT; ++al) acc[al] = 0.0;
-  t3_load<0, 1, 2>(X0, xs.s0[0], xs.s1[0], 0, v, org, r);          // X_ijk[a][b][c]
+  t3_load<0, 1, 2>(Y1, vv, v, 0, v, org, r);
   t3_store(sh, r);
   __syncthreads();
-  t3_load<0, 2, 1>(X0, xs.s0[0], xs.s1[0], v, v, org, r);          // X_ikj[a][c][b]
+  t3_load<0, 2, 1>(Y2, vv, v, 0, v, org, r);
   t3_accumulate<0, 1, 2>(sh, 1.0, acc);
   __syncthreads();
   t3_store(sh, r);
   __syncthreads();
-  t3_load<1, 0, 2>(X1, xs.s0[1], xs.s1[1], 0, v, org, r);          // X_jik[b][a][c]
+  t3_load<1, 2, 0>(Y3, vv, v, 0, v, org, r);
   t3_accumulate<0, 2, 1>(sh, 1.0, acc);
   __syncthreads();
   t3_store(sh, r);
   __syncthreads();
-  t3_load<1, 2, 0>(X1, xs.s0[1], xs.s1[1], v, v, org, r);          // X_jki[b][c][a]
-  t3_accumulate<1, 0, 2>(sh, 1.0, acc);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<2, 0, 1>(X2, xs.s0[2], xs.s1[2], 0, v, org, r);          // X_kij[c][a][b]
   t3_accumulate<1, 2, 0>(sh, 1.0, acc);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<2, 1, 0>(X2, xs.s0[2], xs.s1[2], v, v, org, r);          // X_kji[c][b][a]
-  t3_accumulate<2, 0, 1>(sh, 1.0, acc);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_accumulate<2, 1, 0>(sh, 1.0, acc);
   const int64_t b = org[1] + bl, c = org[2] + cl;
   if (b < v && c < v)
     for (int al = 0; al < TT; ++al) {
@@ -284,13 +237,17 @@ __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, in
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// ---- MIXED_F16X3: two slot products per GEMM, W never stored ------------------------------------
-// With A_p^s the rows of A_p with (x, y) swapped and B_(q,r) the packed operand of t3_pack_kernel,
-// the six slot products pair up along K into three FP16x3 GEMMs (K = 2 Kp, FP32 output [x][y][vp]):
+// ---- slot products paired along K ----------------------------------------------------------------
+// The slot products X_pqr[x][y][z] = sum_kk A_p[(x, y)][kk] B_(q,r)[z][kk] hold the twelve t3c_ijk
+// terms (cctriples.py:50-62), with B_(q,r)[z][kk] = t2[r][q][z][kk] for kk < v and -(m q|r z) =
+// -<m z|q r> (m = kk - v) below v + o, and W[a][b][c] = X_ijk[a,b,c] + X_ikj[a,c,b] + X_jik[b,a,c] +
+// X_jki[b,c,a] + X_kij[c,a,b] + X_kji[c,b,a]. With A_p^s the rows of A_p with (x, y) swapped, they
+// pair up along K into three GEMMs (K = 2 Kp):
 //   Y1[a][b][c] = X_ijk[a][b][c] + X_jik[b][a][c] = [A_i | A_j^s](a,b) . [B_(j,k) | B_(i,k)](c)
 //   Y2[a][c][b] = X_ikj[a][c][b] + X_kij[c][a][b] = [A_i | A_k^s](a,c) . [B_(k,j) | B_(i,j)](b)
 //   Y3[b][c][a] = X_jki[b][c][a] + X_kji[c][b][a] = [A_j | A_k^s](b,c) . [B_(k,i) | B_(j,i)](a)
-// so that W[a][b][c] = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a].
+// so that W[a][b][c] = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a]: FP64 GEMMs (plain_gemm_kcat) in
+// every mode but MIXED_F16X3, whose GEMMs are FP16x3 with FP32 output [x][y][vp] (x3_gemm_kcat).
 //
 // B rows of the chunk (i, j, k0 .. k0 + nk): block 3 e + s (s = 0, 1, 2 for Y1, Y2, Y3 of k = k0 + e)
 // holds, for z < v, [B_(q1,r1)[z] | B_(q2,r2)[z]] (2 Kp wide) with the pairs named above.
@@ -649,7 +606,7 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   // to 32, FP32 accumulation and output, two slot products per GEMM); every other mode runs them
   // in FP64.
   const bool x3 = c.mode == MODE_MIXED_F16X3;
-  const int64_t Kp = x3 ? (v + o + 31) / 32 * 32 : (v + o + 7) / 8 * 8;
+  const int64_t Kp = x3 ? (v + o + 31) / 32 * 32 : (v + o + 15) / 16 * 16;
   const int64_t KC = T3_KC;
   // A: only the rows p < i1 are read by this range (p is one of i >= j >= k)
   const int64_t nA = (int64_t)i1 * vv * Kp;
@@ -657,22 +614,22 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
     triples_energy_x3(c, F, t1, t2, i0, i1, accumulate, e_dev, s);
     return;
   }
-  // For fixed (i, j) the k values run in chunks of KC: the p = i and p = j GEMMs of a chunk share
-  // their A operand and become one GEMM each with N = 2v nk, and the p = k GEMMs (one B operand,
-  // consecutive A_k rows) one GEMM with M = nk v^2.
-  // scratch: XI, XJ (v^2 x 2v KC each), XK (KC v^2 x 2v), W (v^3), B (rows of the i, j and k entries,
-  // (2 KC + 1) 2v x Kp), tile partials, and A (i1 v^2 x Kp)
-  const int64_t nXc = vv * 2 * v * KC, nB = (2 * KC + 1) * 2 * v * Kp;
+  // For fixed (i, j) the k values run in chunks of KC. The six slot products pair up along K into
+  // the three products Y1, Y2, Y3 of triples_energy_x3 (FP64 here); per chunk they are three
+  // batched GEMMs (plain_gemm_kcat, a batch entry per k), then per triple W is assembled from them
+  // and its energy summed. Scratch: Y1, Y2, Y3 (KC v^3 each), W (v^3), the chunk's B rows
+  // (3 KC v x 2 Kp), tile partials and A (i1 v^2 x Kp).
+  // (every region starts on a 256-byte boundary: the GEMM loads its operands 16 bytes at a time)
+  auto r32 = [](int64_t n) { return (n + 31) / 32 * 32; };
+  const int64_t L = 2 * Kp, nY = r32(KC * vvv), nW = r32(vvv), nB = r32(3 * KC * v * L), nP = r32(tiles);
   DevBuf<int> order(tiles);
   {
     const std::vector<int> h = t3_orbit_order(nt);
     CCX_HIP(hipMemcpy(order.p, h.data(), (size_t)tiles * sizeof(int), hipMemcpyHostToDevice));
   }
-  DevBuf<double> buf(3 * nXc + vvv + nB + tiles + nA);
-  double *XI = buf.p, *XJ = XI + nXc, *XK = XJ + nXc, *W = XK + nXc, *B = W + vvv, *partials = B + nB;
-  double* A = partials + tiles;
-  Workspace& ws = c.ws;
-
+  DevBuf<double> buf(3 * nY + nW + nB + nP + nA);
+  double *Y1 = buf.p, *Y2 = Y1 + nY, *Y3 = Y2 + nY, *W = Y3 + nY, *B = W + nW, *partials = B + nB;
+  double* A = partials + nP;
   if (nA > 0) {
     t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv, t2, A);
     CCX_HIP(hipGetLastError());
@@ -681,47 +638,28 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
     t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
     CCX_HIP(hipGetLastError());
   }
-  const int64_t ldx = 2 * v * KC;   // row length of XI, XJ
+  auto Ap = [&](int p) { return A + (int64_t)p * vv * Kp; };
   for (int i = i0; i < i1; ++i)
     for (int j = 0; j <= i; ++j)
       for (int k0 = 0; k0 <= j; k0 += (int)KC) {
         const int nk = (int)std::min<int64_t>(KC, j + 1 - k0);
-        // B rows: [0, nk 2v) the p = i entries (j, k), [nk 2v, 2 nk 2v) the p = j entries (i, k), then
-        // the p = k entry (i, j); X_s[x][y][z'] = sum_e <yx|ep> t_rq^{ze} - sum_m <mz|qr> t_pm^{xy} with
-        // <yx|ep> = (px|ye) = Covvv[p][x][y][e]: the twelve t3c_ijk terms of cctriples.py:50-62
-        T3Pairs pi{nk, {}, {}}, pj{nk, {}, {}}, pk{1, {}, {}};
-        for (int e = 0; e < nk; ++e) {
-          pi.q[e] = j; pi.r[e] = k0 + e;
-          pj.q[e] = i; pj.r[e] = k0 + e;
-        }
-        pk.q[0] = i; pk.r[0] = j;
-        double* BI = B;
-        double* BJ = BI + (int64_t)nk * 2 * v * Kp;
-        double* BK = BJ + (int64_t)nk * 2 * v * Kp;
-        const int pg = (int)std::min<int64_t>(1024, ceil_div((int64_t)nk * 2 * v * Kp, 256));
-        t3_pack_kernel<<<pg, 256, 0, s>>>(o, v, Kp, pi, t2, c.Cooov, BI);
+        const int64_t nrows = 3 * (int64_t)nk * v;
+        t3_pack_kcat_kernel<<<(int)std::min<int64_t>(4096, ceil_div(nrows * L, 256)), 256, 0, s>>>(
+            o, v, Kp, i, j, k0, nk, t2, c.Cooov, B);
         CCX_HIP(hipGetLastError());
-        t3_pack_kernel<<<pg, 256, 0, s>>>(o, v, Kp, pj, t2, c.Cooov, BJ);
-        CCX_HIP(hipGetLastError());
-        t3_pack_kernel<<<(int)std::min<int64_t>(1024, ceil_div(2 * v * Kp, 256)), 256, 0, s>>>(o, v, Kp, pk, t2,
-                                                                                           c.Cooov, BK);
-        CCX_HIP(hipGetLastError());
-        const std::vector<int64_t> VVK{v, v, Kp};
-        const TV XIv = tvs(XI, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
-        const TV XJv = tvs(XJ, "xyz", {v, v, 2 * v * nk}, {v * ldx, ldx, 1});
-        contract<double>(1.0, tv(A + (int64_t)i * vv * Kp, "xyk", VVK), tv(BI, "zk", {2 * v * nk, Kp}), 0.0, XIv,
-                         ws, s);
-        contract<double>(1.0, tv(A + (int64_t)j * vv * Kp, "xyk", VVK), tv(BJ, "zk", {2 * v * nk, Kp}), 0.0, XJv,
-                         ws, s);
-        contract<double>(1.0, tv(A + (int64_t)k0 * vv * Kp, "mk", {nk * vv, Kp}), tv(BK, "zk", {2 * v, Kp}), 0.0,
-                         tv(XK, "mz", {nk * vv, 2 * v}), ws, s);
+        PlainGemm<double> g;   // C_e[m][n] = sum_kk [A | A2_e swapped][m][kk] B_e[n][kk]
+        g.M = vv; g.N = v; g.K = L; g.K1 = Kp; g.sw = v; g.batch = nk;
+        g.lda = Kp; g.ldb = L; g.ldc = v; g.a_kc = true; g.b_nc = false;
+        g.b_b = 3 * v * L; g.c_b = vvv;
+        g.A = Ap(i); g.A2 = Ap(j); g.a2_b = 0; g.B = B; g.C = Y1;                   // [A_i | A_j^s]
+        plain_gemm_kcat(g, s);
+        g.A2 = Ap(k0); g.a2_b = vv * Kp; g.B = B + v * L; g.C = Y2;                // [A_i | A_k^s]
+        plain_gemm_kcat(g, s);
+        g.A = Ap(j); g.B = B + 2 * v * L; g.C = Y3;                                // [A_j | A_k^s]
+        plain_gemm_kcat(g, s);
         for (int e = 0; e < nk; ++e) {
           const int k = k0 + e;
-          T3Slots xs;
-          xs.X[0] = XI + (int64_t)e * 2 * v; xs.s0[0] = v * ldx; xs.s1[0] = ldx;
-          xs.X[1] = XJ + (int64_t)e * 2 * v; xs.s0[1] = v * ldx; xs.s1[1] = ldx;
-          xs.X[2] = XK + (int64_t)e * vv * 2 * v; xs.s0[2] = 2 * vv; xs.s1[2] = 2 * v;
-          t3_assemble_kernel<<<(int)tiles, 256, 0, s>>>(v, xs, W);
+          t3_assemble3_kernel<<<(int)tiles, 256, 0, s>>>(v, Y1 + e * vvv, Y2 + e * vvv, Y3 + e * vvv, W);
           CCX_HIP(hipGetLastError());
           t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, order.p,
                                                       partials);
