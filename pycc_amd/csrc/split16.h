@@ -38,6 +38,10 @@ struct X3Gemm {
   // (M = sw^2); batch b of nbatch adds b sA2 to Ah2, b sB to Bh and Bl, and b sC to C.
   const void* Ah2 = nullptr;
   int64_t K1 = 0, sw = 0, nbatch = 1, sA2 = 0, sB = 0, sC = 0;
+  // ytile (x3_gemm_kcat only, = sw = N, a multiple of 16): C element (m = x sw + y, n) goes to the
+  // tiled layout [x][y / 16][n / 16][y % 16][n % 16] (ldc unused), so that a 16 x 16 (y, n) block
+  // is one run of 1 KiB: the offset is (m - m % 16) sw + 16 (n - n % 16) + 16 (m % 16) + n % 16
+  int64_t ytile = 0;
 };
 struct Workspace;
 // ws (optional) enables split-K when the tile grid cannot fill the chip.

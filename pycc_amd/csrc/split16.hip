@@ -407,15 +407,18 @@ __device__ __forceinline__ void x3_dma16(const E* g, const E* lds) {
 // blocks of 16 go to wave q % 8, so waves 0-3 load three and 4-7 two); BT = 256 or 384 rows (384
 // with BN = 192: 192 x 48 per wave, 144 KiB). KCAT: the K-concatenated, batched form of
 // x3_gemm_kcat (interleaved operands, FP32 C, no split K).
-template <typename E, int BN, bool KCAT = false, int BT = 256>
+// WGM: waves along M (2: 2 x 4 waves; 4: 4 x 2, e.g. 96 x 96 per wave for 384 x 192 tiles, which
+// holds fewer fragment registers than 192 x 48).
+template <typename E, int BN, bool KCAT = false, int BT = 256, bool YT = false, int WGM = 2>
 __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   typedef typename X3<E>::v8 v8;
-  constexpr int WTM = BT / 2, WTN = BN / 4, NBM = WTM / 16, NBN = WTN / 16, IPW = BT / 128;
+  constexpr int WGN = 8 / WGM;
+  constexpr int WTM = BT / WGM, WTN = BN / WGN, NBM = WTM / 16, NBN = WTN / 16, IPW = BT / 128;
   constexpr int NIB = BN / 16, IPB = (NIB + 7) / 8;   // B row blocks, per wave (rounded up)
   __shared__ __attribute__((aligned(1024))) E sA[2][2][BT * 32];   // [stage][hi/lo]
   __shared__ __attribute__((aligned(1024))) E sB[2][2][BN * 32];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 2, wn = wave & 3;
+  const int wm = wave / WGN, wn = wave % WGN;
   const int64_t tiles_m = (p.M + BT - 1) / BT, tiles_n = (p.N + BN - 1) / BN;
   int64_t t = xcd_remap_x3(blockIdx.x, tiles_m * tiles_n * (KCAT ? p.nbatch : 1));
   int64_t batch = 0;
@@ -434,7 +437,8 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   const int64_t within = t - group * GM * tiles_n;
   const int64_t tm = first_m + within % gsize, tn = within / gsize;
 
-  int64_t gA[IPW], gB[IPB], gA2[KCAT ? IPW : 1];
+  int64_t gA[IPW], gB[IPB];
+  int gA2[KCAT ? IPW : 1];   // M lda < 2^31 (checked by x3_gemm_kcat)
   int ldso[IPW], ldsb[IPB];
 #pragma unroll
   for (int i = 0; i < IPW; ++i) {
@@ -443,7 +447,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
     int64_t m = tm * BT + row;
     m = m < p.M ? m : p.M - 1;
     gA[i] = m * p.lda + c * 8;
-    if constexpr (KCAT) gA2[i] = ((m % p.sw) * p.sw + m / p.sw) * p.lda + c * 8;
+    if constexpr (KCAT) gA2[i] = (int)(((m % p.sw) * p.sw + m / p.sw) * p.lda + c * 8);
     ldso[i] = R0 * 32;
   }
 #pragma unroll
@@ -554,6 +558,23 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
   float f0 = p.alpha;
   if (p.sa) f0 = f0 / p.sa[0];
   if (p.sb) f0 = f0 / p.sb[0];
+  if constexpr (KCAT) {   // FP32 C, beta = 0; YT: the tiled layout of x3_gemm_kcat's ytile
+    float* Cb = p.C + batch * p.sC;
+#pragma unroll
+    for (int j = 0; j < NBN; ++j) {
+      const int64_t nb = tn * BN + wn * WTN + j * 16;   // 16-aligned block of n
+      if (nb + fc >= p.N) continue;
+#pragma unroll
+      for (int i = 0; i < NBM; ++i) {
+        const int64_t mb = tm * BT + wm * WTM + i * 16;   // 16-aligned block of m
+        float* d = Cb + (YT ? mb * p.ytile + nb * 16 + fc : mb * p.ldc + nb + fc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (mb + fr * 4 + r < p.M) d[(fr * 4 + r) * (YT ? 16 : p.ldc)] = f0 * acc[i][j][r];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < NBN; ++j) {
     const int64_t n = tn * BN + wn * WTN + j * 16 + fc;
@@ -572,7 +593,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
             if (p.D64) val += p.D64[at] * p.dvec[n];
             p.C64[at] = val;
           } else {
-            float* d = p.C + batch * p.sC + m * p.ldc + n;
+            float* d = p.C + m * p.ldc + n;
             float val = f * acc[i][j][r];
             if (p.beta != 0.f) val += p.beta * *d;
             *d = val;
@@ -992,7 +1013,8 @@ void x3_gemm_kcat(const X3Gemm& p, hipStream_t s) {
   if (p.M == 0 || p.N == 0 || p.nbatch == 0) return;
   CCX_CHECK(p.kb == BK && p.ska == 2 * BK && p.skb == 2 * BK && p.Kp % BK == 0 && p.K1 % BK == 0 && p.K1 > 0 &&
                 p.K1 < p.Kp && p.lda % 8 == 0 && p.ldb % 8 == 0 && p.Ah2 && p.C && !p.C64 && !p.work &&
-                p.sw > 0 && p.sw * p.sw == p.M && p.beta == 0.f && !p.sbn,
+                p.sw > 0 && p.sw * p.sw == p.M && p.M * p.lda < (int64_t(1) << 31) && p.beta == 0.f && !p.sbn &&
+                (p.ytile == 0 || (p.ytile % 16 == 0 && p.ytile == p.sw && p.ytile == p.N)),
             ERR_INTERNAL, "x3 K-concatenated GEMM layout");
   // the output tile (384 x 192, 256 x 256 or 256 x 192) that pads M x N least, in that order on a
   // tie (the 320-column tile spills registers in this form)
@@ -1009,8 +1031,12 @@ void x3_gemm_kcat(const X3Gemm& p, hipStream_t s) {
   }
   const int64_t tiles = ceil_div(p.M, shapes[pick][0]) * ceil_div(p.N, shapes[pick][1]) * p.nbatch;
   CCX_CHECK(tiles < (1ll << 31), ERR_ARG, "x3 GEMM grid too large");
-  if (pick == 0) x3_gemm_dma_kernel<_Float16, 192, true, 384><<<(unsigned)tiles, 512, 0, s>>>(p);
+  const bool yt = p.ytile > 0;
+  if (pick == 0 && yt) x3_gemm_dma_kernel<_Float16, 192, true, 384, true, 4><<<(unsigned)tiles, 512, 0, s>>>(p);
+  else if (pick == 0) x3_gemm_dma_kernel<_Float16, 192, true, 384, false, 4><<<(unsigned)tiles, 512, 0, s>>>(p);
+  else if (pick == 1 && yt) x3_gemm_dma_kernel<_Float16, 256, true, 256, true><<<(unsigned)tiles, 512, 0, s>>>(p);
   else if (pick == 1) x3_gemm_dma_kernel<_Float16, 256, true><<<(unsigned)tiles, 512, 0, s>>>(p);
+  else if (yt) x3_gemm_dma_kernel<_Float16, 192, true, 256, true><<<(unsigned)tiles, 512, 0, s>>>(p);
   else x3_gemm_dma_kernel<_Float16, 192, true><<<(unsigned)tiles, 512, 0, s>>>(p);
   CCX_HIP(hipGetLastError());
 }

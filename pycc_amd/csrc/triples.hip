@@ -247,7 +247,8 @@ __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, in
 //   Y2[a][c][b] = X_ikj[a][c][b] + X_kij[c][a][b] = [A_i | A_k^s](a,c) . [B_(k,j) | B_(i,j)](b)
 //   Y3[b][c][a] = X_jki[b][c][a] + X_kji[c][b][a] = [A_j | A_k^s](b,c) . [B_(k,i) | B_(j,i)](a)
 // so that W[a][b][c] = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a]: FP64 GEMMs (plain_gemm_kcat) in
-// every mode but MIXED_F16X3, whose GEMMs are FP16x3 with FP32 output [x][y][vp] (x3_gemm_kcat).
+// every mode but MIXED_F16X3, whose GEMMs are FP16x3 with FP32 output, in 16 x 16 (y, z) blocks
+// when v is a multiple of 16 (x3_gemm_kcat with ytile).
 //
 // B rows of the chunk (i, j, k0 .. k0 + nk): block 3 e + s (s = 0, 1, 2 for Y1, Y2, Y3 of k = k0 + e)
 // holds, for z < v, [B_(q1,r1)[z] | B_(q2,r2)[z]] (2 Kp wide) with the pairs named above.
@@ -288,7 +289,7 @@ __device__ __forceinline__ double t3_rcp(double x) {   // 1/x: the hardware esti
 }
 
 __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
-    int64_t o, int64_t v, int64_t vp, int i, int j, int k0, const float* __restrict__ Y1,
+    int64_t o, int64_t v, int64_t ldy, bool tiled, int i, int j, int k0, const float* __restrict__ Y1,
     const float* __restrict__ Y2, const float* __restrict__ Y3, const int* __restrict__ orbits, int64_t norb,
     const double* __restrict__ F, int64_t nmo, const double* __restrict__ t1, const double* __restrict__ t2,
     const double* __restrict__ Ioovv, double* __restrict__ partials) {
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   constexpr int t3_rev[6] = {5, 3, 4, 1, 2, 0};
   __shared__ float sW[6 * OTS];
   const int e = blockIdx.y, k = k0 + e;
-  const int64_t nt = (v + TT - 1) / TT, vv = v * v, ys = vv * vp;
+  const int64_t nt = (v + TT - 1) / TT, vv = v * v, ys = vv * ldy;
   const int code = orbits[blockIdx.x];
   const int A = (int)(code / (nt * nt)), B = (int)(code / nt % nt), C = (int)(code % nt);
   const int T[6][3] = {{A, B, C}, {A, C, B}, {B, A, C}, {B, C, A}, {C, A, B}, {C, B, A}};
@@ -320,7 +321,9 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   const float* y1 = Y1 + e * ys;
   const float* y2 = Y2 + e * ys;
   const float* y3 = Y3 + e * ys;
-  // box loads: thread t reads 4 floats at z = 4 (t % 4) of the rows (x, y) = (t / 64 + XS h, t / 4 % 16)
+  // Y is [x][y][z] with rows of ldy floats, or (tiled, v a multiple of 16) [x][y / 16][z / 16][y % 16]
+  // [z % 16] (x3_gemm_kcat's ytile), where the (y, z) block of a box is one run of 1 KiB per x. Box
+  // loads: thread t reads 4 floats at z = 4 (t % 4) of the rows (x, y) = (t / 64 + XS h, t / 4 % 16).
   constexpr int NH = 4096 / (4 * OT_THREADS), XS = 16 / NH;
   const int r0 = threadIdx.x >> 2, c4 = threadIdx.x & 3;
   const int xl0 = r0 >> 4, yl = r0 & 15, zl = 4 * c4;
@@ -331,7 +334,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
       const int64_t x = 16 * gx + xl0 + XS * h, y = 16 * gy + yl, z = 16 * gz + zl;
       f4v val = f4v{0.f, 0.f, 0.f, 0.f};
       if (x < v && y < v && z < v) {
-        val = *(const f4v*)(Y + (x * v + y) * vp + z);
+        val = *(const f4v*)(Y + (tiled ? (x * v + 16 * gy) * v + 256 * gz + yl * 16 + zl : (x * v + y) * ldy + z));
         if (z + 3 >= v) {   // the row's padding past v
           if (z + 1 >= v) val[1] = 0.f;
           if (z + 2 >= v) val[2] = 0.f;
@@ -524,9 +527,12 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
                               double* e_dev, hipStream_t s) {
   const int64_t o = c.o, v = c.v, nmo = c.nmo, vv = v * v;
   const int64_t nt = ceil_div(v, TT), KC = T3_KC;
-  const int64_t Kp = (v + o + 31) / 32 * 32, L = 2 * Kp, vp = (v + 3) / 4 * 4;
+  const int64_t Kp = (v + o + 31) / 32 * 32, L = 2 * Kp;
+  // Y in 16 x 16 (y, z) blocks when v is a multiple of 16, else rows padded to 4 floats
+  const bool tiled = v % 16 == 0;
+  const int64_t ldy = tiled ? v : (v + 3) / 4 * 4;
   const int64_t nA = (int64_t)i1 * vv * Kp, nBr = 3 * KC * v, np = x3_partials();
-  CCX_CHECK(vv * vp * KC < (int64_t(1) << 40) && ceil_div(vv, 256) < (int64_t(1) << 24), ERR_ARG,
+  CCX_CHECK(vv * ldy * KC < (int64_t(1) << 40) && ceil_div(vv, 256) < (int64_t(1) << 24), ERR_ARG,
             "too many virtual orbitals for the (T) scratch");
   DevBuf<f16> Ahl(nA * 2);   // A split, [i1 v^2][2 Kp]
   DevBuf<float> sc(16 + 2 * np);   // sc[0] = scale of A, sc[1] = scale of the chunk's B, then partials
@@ -549,9 +555,10 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
   CCX_HIP(hipMemcpy(dorb.p, orb.data(), (size_t)norb * sizeof(int), hipMemcpyHostToDevice));
   DevBuf<double> Bp(nBr * L);        // K-concatenated B rows of a chunk
   DevBuf<f16> Bhl(nBr * 2 * L);      // the same split, rows of 2 L halves
-  DevBuf<float> Y(3 * KC * vv * vp);   // Y1, Y2, Y3 of the chunk's triples, [e][x][y][vp]
+  const int64_t ys = vv * ldy;          // one triple's Y block (see t3_orbit_energy_kernel)
+  DevBuf<float> Y(3 * KC * ys);        // Y1, Y2, Y3 of the chunk's triples
   DevBuf<double> part(KC * norb);
-  float *Y1 = Y.p, *Y2 = Y1 + KC * vv * vp, *Y3 = Y2 + KC * vv * vp;
+  float *Y1 = Y.p, *Y2 = Y1 + KC * ys, *Y3 = Y2 + KC * ys;
   if (!accumulate) {
     t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
     CCX_HIP(hipGetLastError());
@@ -569,9 +576,9 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
         x3_scale(sc.p + 16, 1.f, nullptr, 0.f, sc.p + 1, s);
         x3_split_rows<double>(Bp.p, nrows, L, L, sc.p + 1, Bhl.p, Bhl.p + 32, L, s, true);
         X3Gemm g;   // C_e[m][n] = sum_kk [A | A2_e swapped][m][kk] B_e[n][kk]
-        g.M = vv; g.N = v; g.Kp = 2 * Kp; g.K1 = Kp; g.lda = 2 * Kp; g.ldb = 2 * L; g.ldc = vp;
-        g.kb = 32; g.ska = 64; g.skb = 64; g.sw = v; g.nbatch = nk;
-        g.sa = sc.p; g.sb = sc.p + 1; g.sB = 3 * v * 2 * L; g.sC = vv * vp;
+        g.M = vv; g.N = v; g.Kp = 2 * Kp; g.K1 = Kp; g.lda = 2 * Kp; g.ldb = 2 * L; g.ldc = ldy;
+        g.kb = 32; g.ska = 64; g.skb = 64; g.sw = v; g.nbatch = nk; g.ytile = tiled ? v : 0;
+        g.sa = sc.p; g.sb = sc.p + 1; g.sB = 3 * v * 2 * L; g.sC = ys;
         // Y1 = [A_i | A_j^s] [B_(j,k) | B_(i,k)]
         g.Ah = Ap(i); g.Al = Ap(i) + 32; g.Ah2 = Ap(j); g.sA2 = 0;
         g.Bh = Bhl.p; g.Bl = Bhl.p + 32; g.C = Y1;
@@ -585,7 +592,7 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
         g.Bh = Bhl.p + 2 * v * 2 * L; g.Bl = (const f16*)g.Bh + 32; g.C = Y3;
         x3_gemm_kcat(g, s);
         t3_orbit_energy_kernel<<<dim3((unsigned)norb, (unsigned)nk), OT_THREADS, 0, s>>>(
-            o, v, vp, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
+            o, v, ldy, tiled, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
         CCX_HIP(hipGetLastError());
         t3_accumulate_chunk_kernel<<<1, 256, 0, s>>>(part.p, norb, nk, i, j, k0, e_dev);
         CCX_HIP(hipGetLastError());
