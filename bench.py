@@ -91,10 +91,13 @@ def free_device():
     torch.cuda.empty_cache()
 
 
-def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv, backward=False, mark=None):
+def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv, backward=False, mark=None,
+            refine_from=None):
     """One precision leg: W warm-up and K timed iterations of the solve loop, then (untimed) on to
     convergence. ``mark = (e_conv2, r_conv2)`` records the iteration and elapsed solve time at which
-    the loop first meets that second pair of thresholds (its convergence test, ccwfn.py:295)."""
+    the loop first meets that second pair of thresholds (its convergence test, ccwfn.py:295).
+    ``refine_from = (t1, t2)``: after the converged solve (and the backward), the same context solves
+    again from those amplitudes to the same thresholds (FP64 refinement of a mixed solution)."""
     import torch
     import pycc_amd
     t0 = time.time()
@@ -164,6 +167,20 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
             precision, out["backward"]["hbar_build_ms"], out["backward"]["lambda_residual_ms"],
             out["backward"]["device_GB"]))
     t1c, t2c = cc.t1.clone(), cc.t2.clone()
+    if refine_from is not None and out["converged"]:
+        cc.t1.copy_(refine_from[0].to(cc.t1.device))
+        cc.t2.copy_(refine_from[1].to(cc.t2.device))
+        torch.cuda.synchronize()
+        t_ref = time.perf_counter()
+        e_ref = cc.solve_cc(e_conv, r_conv, maxiter)
+        torch.cuda.synchronize()
+        ref_s = time.perf_counter() - t_ref
+        out["refine"] = {"iterations": cc.niter, "time_s": ref_s, "converged": e_ref is not None,
+                         "energy": e_ref, "e_conv": e_conv, "r_conv": r_conv,
+                         "energy_minus_solve_uEh": 1e6 * (e_ref - out["energy"]) if e_ref is not None else None,
+                         "max_abs_dT2_vs_solve": float((cc.t2 - t2c).abs().max())}
+        log("[%s] refinement from the mixed amplitudes: %d iterations, %.1f s, E - E(solve) = %.2e uEh" % (
+            precision, cc.niter, ref_s, out["refine"]["energy_minus_solve_uEh"] or float("nan")))
     del cc, it
     free_device()
     return out, t1c, t2c
@@ -328,7 +345,8 @@ def main():
     if not args.no_dp and args.precision != "DP":
         free_device()
         dp, dt1, dt2 = run_leg(sd, o, "DP", args.warmup, args.steps, converge, args.maxiter,
-                               args.e_conv, args.r_conv_dp, backward=not args.no_backward, mark=mixed_thr)
+                               args.e_conv, args.r_conv_dp, backward=not args.no_backward, mark=mixed_thr,
+                               refine_from=(mt1, mt2) if converge and mixed["converged"] else None)
         dt1, dt2 = dt1.cpu(), dt2.cpu()
     # the other mixed forms, each in its own context after the ones above are freed
     others = {}
@@ -377,6 +395,13 @@ def main():
                        "time_to_solution_s": dp["solve_s"], "e_conv": dp["e_conv"], "r_conv": dp["r_conv"],
                        "at_mixed_thresholds": dp["at_mark"]}
         res["fp64"].update(leg_rates("DP", o, v, dp["ms_per_step"]))
+        if "refine" in dp:
+            # the FP64 solution reached by a mixed solve then FP64 iterations from its amplitudes
+            r = dict(dp["refine"])
+            if mixed["solve_s"] and dp["solve_s"] and r["converged"]:
+                r["mixed_plus_refine_s"] = mixed["solve_s"] + r["time_s"]
+                r["speedup_vs_fp64_solve"] = dp["solve_s"] / r["mixed_plus_refine_s"]
+            res["fp64"]["refined_from_mixed"] = r
         res.update(versus_fp64(mixed, mt1, mt2))
     if dp and "backward" in dp:
         b = dp["backward"]
