@@ -567,7 +567,7 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
 #pragma unroll
       for (int i = 0; i < NBM; ++i) {
         const int64_t mb = tm * BT + wm * WTM + i * 16;   // 16-aligned block of m
-        float* d = Cb + (YT ? mb * p.ytile + nb * 16 + fc : mb * p.ldc + nb + fc);
+        float* d = Cb + (YT ? mb * p.ytile + nb * 16 + 4 * (fc & 3) + (fc >> 2) : mb * p.ldc + nb + fc);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (mb + fr * 4 + r < p.M) d[(fr * 4 + r) * (YT ? 16 : p.ldc)] = f0 * acc[i][j][r];

@@ -277,7 +277,24 @@ __global__ void t3_pack_kcat_kernel(int64_t o, int64_t v, int64_t Kp, int i, int
 // distinct tiles (one, three or six permutations of (A, B, C)) from Y1, Y2, Y3 into LDS (FP32),
 // then the energy sum of t3_energy_kernel over those tiles, every permuted W read from LDS. HBM
 // traffic is Y read once; W is never written.
-constexpr int OX = 273, OY = 17, OTS = TT * OX;   // LDS tile [x][y][z] at x OX + y OY + z
+//
+// LDS holds a W tile (x, y, z) at word 256 x + 64 (y / 4) + L(x, y, z): L maps the twelve index bits
+// linearly over GF(2) (XOR of the columns below) onto the six bank bits, so that the address is
+// the XOR of one term per coordinate. Every LDS access of W here is made by waves whose 64 lanes
+// run over all 16 values of one coordinate and the two low bits of another; for each of the six
+// such (coordinate, coordinate) pairs the 64 lanes hit 64 distinct banks, and each half wave 32
+// distinct banks of 32, so that no access has a bank conflict. Column of x bit k: 17, 34, 4, 8;
+// of y bit k: 20, 40, 1, 2; of z bit k: 26, 35, 4, 2.
+__device__ __forceinline__ constexpr int t3_ax(int x) {   // words
+  return (x << 8) ^ (x & 12) ^ ((x & 1) * 17) ^ (((x >> 1) & 1) * 34);
+}
+__device__ __forceinline__ constexpr int t3_ay(int y) {
+  return ((y >> 2) << 6) ^ ((y >> 2) & 3) ^ ((y & 1) * 20) ^ (((y >> 1) & 1) * 40);
+}
+__device__ __forceinline__ constexpr int t3_az(int z) {
+  return ((z & 1) * 26) ^ (((z >> 1) & 1) * 35) ^ (((z >> 2) & 1) * 4) ^ (((z >> 3) & 1) * 2);
+}
+constexpr int OTW = TT * TT * TT;   // words per LDS tile
 // slot x holds the tile at the permutation (A, B, C), (A, C, B), (B, A, C), (B, C, A), (C, A, B),
 // (C, B, A) of the orbit.
 
@@ -288,8 +305,11 @@ __device__ __forceinline__ double t3_rcp(double x) {   // 1/x: the hardware esti
   return fma(fma(-x, r, 1.0), r, r);
 }
 
+// FULL: Y in the tiled layout of x3_gemm_kcat's ytile (v a multiple of 16, every tile whole); else
+// Y is [x][y][z] with rows of ldy floats and the tiles at the edge are partial.
+template <bool FULL>
 __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
-    int64_t o, int64_t v, int64_t ldy, bool tiled, int i, int j, int k0, const float* __restrict__ Y1,
+    int64_t o, int64_t v, int64_t ldy, int i, int j, int k0, const float* __restrict__ Y1,
     const float* __restrict__ Y2, const float* __restrict__ Y3, const int* __restrict__ orbits, int64_t norb,
     const double* __restrict__ F, int64_t nmo, const double* __restrict__ t1, const double* __restrict__ t2,
     const double* __restrict__ Ioovv, double* __restrict__ partials) {
@@ -300,7 +320,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   constexpr int t3_sw23[6] = {1, 0, 3, 2, 5, 4};
   constexpr int t3_sw12[6] = {2, 4, 0, 5, 1, 3};
   constexpr int t3_rev[6] = {5, 3, 4, 1, 2, 0};
-  __shared__ float sW[6 * OTS];
+  __shared__ float sW[6 * OTW];
   const int e = blockIdx.y, k = k0 + e;
   const int64_t nt = (v + TT - 1) / TT, vv = v * v, ys = vv * ldy;
   const int code = orbits[blockIdx.x];
@@ -321,51 +341,61 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   const float* y1 = Y1 + e * ys;
   const float* y2 = Y2 + e * ys;
   const float* y3 = Y3 + e * ys;
-  // Y is [x][y][z] with rows of ldy floats, or (tiled, v a multiple of 16) [x][y / 16][z / 16][y % 16]
-  // [z % 16] (x3_gemm_kcat's ytile), where the (y, z) block of a box is one run of 1 KiB per x. Box
-  // loads: thread t reads 4 floats at z = 4 (t % 4) of the rows (x, y) = (t / 64 + XS h, t / 4 % 16).
-  constexpr int NH = 4096 / (4 * OT_THREADS), XS = 16 / NH;
-  const int r0 = threadIdx.x >> 2, c4 = threadIdx.x & 3;
-  const int xl0 = r0 >> 4, yl = r0 & 15, zl = 4 * c4;
+  // Box loads: wave w takes the row x = w of the box; lane (yl, cq) = (l % 16, l / 16) the four
+  // elements z = cq, cq + 4, cq + 8, cq + 12 of the row y = yl (in the tiled layout the 16 bytes at
+  // 64 yl + 16 cq of the box's 1 KiB run for that x).
   typedef float f4v __attribute__((ext_vector_type(4)));
-  auto box = [&](const float* Y, int gx, int gy, int gz, f4v* out) {   // tile origins in tiles
-#pragma unroll
-    for (int h = 0; h < NH; ++h) {
-      const int64_t x = 16 * gx + xl0 + XS * h, y = 16 * gy + yl, z = 16 * gz + zl;
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const int wv = threadIdx.x >> 6, yl = threadIdx.x & 15, cq = (threadIdx.x >> 4) & 3;
+  auto box = [&](const float* Y, int gx, int gy, int gz) -> f4v {   // tile origins in tiles
+    const int64_t x = 16 * gx + wv, y = 16 * gy + yl;
+    if constexpr (FULL) {
+      return *(const f4v*)(Y + (x * v + 16 * gy) * v + 256 * gz + yl * 16 + 4 * cq);
+    } else {
       f4v val = f4v{0.f, 0.f, 0.f, 0.f};
-      if (x < v && y < v && z < v) {
-        val = *(const f4v*)(Y + (tiled ? (x * v + 16 * gy) * v + 256 * gz + yl * 16 + zl : (x * v + y) * ldy + z));
-        if (z + 3 >= v) {   // the row's padding past v
-          if (z + 1 >= v) val[1] = 0.f;
-          if (z + 2 >= v) val[2] = 0.f;
-          val[3] = 0.f;
+      if (x < v && y < v) {
+        const float* row = Y + (x * v + y) * ldy;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t z = 16 * gz + cq + 4 * u;
+          if (z < v) val[u] = row[z];
         }
       }
-      out[h] = val;
+      return val;
     }
   };
-  // every load of the work-group is issued here: the Y boxes of the kept tiles, then the
-  // disconnected factors of each kept tile (held in registers until its energy sum)
-  f4v ra[6][NH], rb[6][NH], rc[6][NH];
+  // the Y boxes of the kept tiles, all loaded here
+  f4v ra[6], rb[6], rc[6];
 #pragma unroll
   for (int x = 0; x < 6; ++x)
     if (rep[x] == x) {
-      box(y1, T[x][0], T[x][1], T[x][2], ra[x]);   // Y1[a][b][c]
-      box(y2, T[x][0], T[x][2], T[x][1], rb[x]);   // Y2[a][c][b]
-      box(y3, T[x][1], T[x][2], T[x][0], rc[x]);   // Y3[b][c][a]
+      ra[x] = box(y1, T[x][0], T[x][1], T[x][2]);   // Y1[a][b][c]
+      rb[x] = box(y2, T[x][0], T[x][2], T[x][1]);   // Y2[a][c][b]
+      rc[x] = box(y3, T[x][1], T[x][2], T[x][0]);   // Y3[b][c][a]
     }
-  // energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) and the AL values of a from
-  // AL (t / 256)
-  constexpr int AL = 16 * 256 / OT_THREADS;
-  const int part = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, cl = threadIdx.x & 15;
+  // W = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a] in three passes; lane (yl, cq) of wave wv holds
+  // Y1 (a, b, c) = (wv, yl, cq + 4 u), Y2 (a, c, b) = (wv, yl, cq + 4 u), Y3 (b, c, a) = (wv, yl, cq + 4 u)
+  const int a1 = t3_ax(wv) ^ t3_ay(yl) ^ t3_az(cq);   // W (wv, yl, cq + 4 u)
+  const int a2 = t3_ax(wv) ^ t3_ay(cq) ^ t3_az(yl);   // W (wv, cq + 4 u, yl)
+  const int a3 = t3_ax(cq) ^ t3_ay(wv) ^ t3_az(yl);   // W (cq + 4 u, wv, yl)
+#pragma unroll
+  for (int x = 0; x < 6; ++x)
+    if (rep[x] == x) {
+      float* w = sW + x * OTW;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[a1 ^ t3_az(4 * u)] = ra[x][u];
+    }
+  // then the loads of the disconnected factors of each kept tile (held in registers until its
+  // energy sum): the 16 x 16 blocks <ij|ab>, t2_ijab (d = 0, 1), <ik|ac>, t2_ikac (2, 3), <jk|bc>,
+  // t2_jkbc (4, 5), element t + OT_THREADS r of the six for thread t; and, for threads below 144,
+  // one element of the vectors q = 3 n + u (u = 0 t1, 1 the f row of the occupied orbital, 2 eps,
+  // for n = a, b, c)
   const double* I6[6] = {Ioovv + ((int64_t)i * o + j) * vv, t2 + ((int64_t)i * o + j) * vv,
                          Ioovv + ((int64_t)i * o + k) * vv, t2 + ((int64_t)i * o + k) * vv,
                          Ioovv + ((int64_t)j * o + k) * vv, t2 + ((int64_t)j * o + k) * vv};
   const int occ3[3] = {i, j, k};
-  // block d of sD: rows along (a, a, a, a, b, b), columns along (b, b, c, c, c, c); thread t loads
-  // the elements t + OT_THREADS r of the six blocks, and threads below 144 one element of the
-  // vectors q = 3 n + u (u = 0 t1, 1 f_(occ) row, 2 eps, for n = a, b, c)
   constexpr int NR = (6 * 256 + OT_THREADS - 1) / OT_THREADS;
+  const int cl = threadIdx.x & 15;
   double pd[6][NR], pv[6];
 #pragma unroll
   for (int x = 0; x < 6; ++x) {
@@ -376,53 +406,50 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
     for (int rr = 0; rr < NR; ++rr) {
       const int el = threadIdx.x + OT_THREADS * rr, d = el >> 8;
       const int64_t r = org[d < 4 ? 0 : 1] + ((el >> 4) & 15), cc = org[d < 2 ? 1 : 2] + (el & 15);
-      pd[x][rr] = (el < 6 * 256 && r < v && cc < v) ? I6[d < 6 ? d : 0][r * v + cc] : 0.0;
+      pd[x][rr] = (el < 6 * 256 && (FULL || (r < v && cc < v))) ? I6[d < 6 ? d : 0][r * v + cc] : 0.0;
     }
     if (threadIdx.x < 9 * TT) {
       const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
       const int64_t g = org[n] + cl;
-      if (g < v) {
+      if (FULL || g < v) {
         const int occ = occ3[n];
         pv[x] = u == 0 ? t1[occ * v + g] : (u == 1 ? F[occ * nmo + o + g] : F[(o + g) * nmo + o + g]);
       }
     }
   }
+  __syncthreads();
 #pragma unroll
   for (int x = 0; x < 6; ++x)
     if (rep[x] == x) {
-      float* w = sW + x * OTS;
+      float* w = sW + x * OTW;
 #pragma unroll
-      for (int h = 0; h < NH; ++h)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) w[(xl0 + XS * h) * OX + yl * OY + zl + u] = ra[x][h][u];
+      for (int u = 0; u < 4; ++u) w[a2 ^ t3_ay(4 * u)] += rb[x][u];
     }
   __syncthreads();
 #pragma unroll
   for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) {   // Y2 element (a, c, b) = (xl, yl, zl + u) -> W[a][b][c]
-      float* w = sW + x * OTS;
+    if (rep[x] == x) {
+      float* w = sW + x * OTW;
 #pragma unroll
-      for (int h = 0; h < NH; ++h)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) w[(xl0 + XS * h) * OX + (zl + u) * OY + yl] += rb[x][h][u];
-    }
-  __syncthreads();
-#pragma unroll
-  for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) {   // Y3 element (b, c, a) = (xl, yl, zl + u) -> W[a][b][c]
-      float* w = sW + x * OTS;
-#pragma unroll
-      for (int h = 0; h < NH; ++h)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) w[(zl + u) * OX + (xl0 + XS * h) * OY + yl] += rc[x][h][u];
+      for (int u = 0; u < 4; ++u) w[a3 ^ t3_ax(4 * u)] += rc[x][u];
     }
   __syncthreads();
 
-  // The disconnected factors of a tile go through LDS: the six 16 x 16 blocks Iij[a][b], Tij[a][b],
-  // Iik[a][c], Tik[a][c], Ijk[b][c], Tjk[b][c] and the vectors t1_ia, f_ia, eps_a (and those of j,
-  // b and k, c).
-  __shared__ double sD[6][TT][TT + 1];
-  __shared__ double sV[9][TT];
+  // Energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) of the tile and a = 4 (t / 256)
+  // + ii, ii = 0..3. The disconnected factors go through LDS as pairs: sP[0][a][b] = (<ij|ab>,
+  // t2_ijab), sP[1][a][c] = (<ik|ac>, t2_ikac), sP[2][b][c] = (<jk|bc>, t2_jkbc), sTF[n][.] = (t1, f
+  // row) and sE[n][.] = eps for n = a, b, c.
+  __shared__ d2v sP[3][TT][TT];
+  __shared__ d2v sTF[3][TT];
+  __shared__ double sE[3][TT];
+  const int part = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, al0 = 4 * part;
+  // word addresses of W at (a, b, c) permuted, for ii = 0 (the XOR of t3_a*(ii) adds ii)
+  const int e0 = t3_ax(al0) ^ t3_ay(bl) ^ t3_az(cl);   // (a, b, c)
+  const int e1 = t3_ax(bl) ^ t3_ay(cl) ^ t3_az(al0);   // (b, c, a)
+  const int e2 = t3_ax(cl) ^ t3_ay(al0) ^ t3_az(bl);   // (c, a, b)
+  const int e3 = t3_ax(al0) ^ t3_ay(cl) ^ t3_az(bl);   // (a, c, b)
+  const int e4 = t3_ax(bl) ^ t3_ay(al0) ^ t3_az(cl);   // (b, a, c)
+  const int e5 = t3_ax(cl) ^ t3_ay(bl) ^ t3_az(al0);   // (c, b, a)
   const double dijk = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k];
   double s = 0.0;
 #pragma unroll
@@ -431,36 +458,38 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
     const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
 #pragma unroll
     for (int rr = 0; rr < NR; ++rr) {
-      const int el = threadIdx.x + OT_THREADS * rr;
-      if (el < 6 * 256) sD[el >> 8][(el >> 4) & 15][el & 15] = pd[x][rr];
+      const int el = threadIdx.x + OT_THREADS * rr, d = el >> 8;
+      if (el < 6 * 256) ((double*)&sP[d >> 1][(el >> 4) & 15][el & 15])[d & 1] = pd[x][rr];
     }
-    if (threadIdx.x < 9 * TT) sV[threadIdx.x / TT][cl] = pv[x];
+    if (threadIdx.x < 9 * TT) {
+      const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
+      if (u < 2) ((double*)&sTF[n][cl])[u] = pv[x];
+      else sE[n][cl] = pv[x];
+    }
     __syncthreads();
     const int64_t a0 = org[0], b = org[1] + bl, c = org[2] + cl;
-    if (b < v && c < v) {
-      const float* w0 = sW + x * OTS;
-      const float* wc = sW + rep[t3_cyc[x]] * OTS;
-      const float* wc2 = sW + rep[t3_cyc2[x]] * OTS;
-      const float* w23 = sW + rep[t3_sw23[x]] * OTS;
-      const float* w12 = sW + rep[t3_sw12[x]] * OTS;
-      const float* wr = sW + rep[t3_rev[x]] * OTS;
-      const double dbc = dijk - sV[5][bl] - sV[8][cl];
-      const double ijbc = sD[4][bl][cl], tjbc = sD[5][bl][cl];
-      const double t1jb = sV[3][bl], fjb = sV[4][bl], t1kc = sV[6][cl], fkc = sV[7][cl];
-#pragma unroll 4
-      for (int al = AL * part; al < AL * part + AL; ++al) {
-        if (a0 + al < v) {   // wave-uniform
-          const double w = w0[al * OX + bl * OY + cl];
-          const double q = (double)wc[bl * OX + cl * OY + al] + (double)wc2[cl * OX + al * OY + bl] -
-                           2.0 * ((double)w23[al * OX + cl * OY + bl] + (double)w12[bl * OX + al * OY + cl] +
-                                  (double)wr[cl * OX + bl * OY + al]);
-          const double vabc = w + sD[0][al][bl] * t1kc + sD[2][al][cl] * t1jb + ijbc * sV[0][al] +
-                              sD[1][al][bl] * fkc + sD[3][al][cl] * fjb + tjbc * sV[1][al];
-          s += vabc * (4.0 * w + q) * t3_rcp(dbc - sV[2][al]);
-        }
+    if (FULL || (b < v && c < v)) {
+      // byte offsets in sW of the six permuted reads: slot (bits 14 and up) | word address * 4
+      const int q0 = (x * OTW + e0) * 4, q1 = (rep[t3_cyc[x]] * OTW + e1) * 4;
+      const int q2 = (rep[t3_cyc2[x]] * OTW + e2) * 4, q3 = (rep[t3_sw23[x]] * OTW + e3) * 4;
+      const int q4 = (rep[t3_sw12[x]] * OTW + e4) * 4, q5 = (rep[t3_rev[x]] * OTW + e5) * 4;
+      auto rd = [&](int off) { return (double)*(const float*)((const char*)sW + off); };
+      const double dbc = dijk - sE[1][bl] - sE[2][cl];
+      const d2v pbc = sP[2][bl][cl], tfb = sTF[1][bl], tfc = sTF[2][cl];
+#pragma unroll
+      for (int ii = 0; ii < 4; ++ii) {
+        const int al = al0 + ii;
+        if (!FULL && a0 + al >= v) break;   // wave-uniform
+        const double w = rd(q0 ^ 4 * t3_ax(ii));
+        const double q = rd(q1 ^ 4 * t3_az(ii)) + rd(q2 ^ 4 * t3_ay(ii)) -
+                         2.0 * (rd(q3 ^ 4 * t3_ax(ii)) + rd(q4 ^ 4 * t3_ay(ii)) + rd(q5 ^ 4 * t3_az(ii)));
+        const d2v pab = sP[0][al][bl], pac = sP[1][al][cl], tfa = sTF[0][al];
+        const double vabc = w + pab.x * tfc.x + pac.x * tfb.x + pbc.x * tfa.x + pab.y * tfc.y + pac.y * tfb.y +
+                            pbc.y * tfa.y;
+        s += vabc * (4.0 * w + q) * t3_rcp(dbc - sE[0][al]);
       }
     }
-    __syncthreads();   // sD, sV are reloaded for the next tile
+    __syncthreads();   // sP, sTF, sE are reloaded for the next tile
   }
   s = t3_block_sum(s);
   if (threadIdx.x == 0) partials[(int64_t)e * norb + blockIdx.x] = s;
@@ -591,8 +620,12 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
         g.Ah = Ap(j); g.Al = Ap(j) + 32;
         g.Bh = Bhl.p + 2 * v * 2 * L; g.Bl = (const f16*)g.Bh + 32; g.C = Y3;
         x3_gemm_kcat(g, s);
-        t3_orbit_energy_kernel<<<dim3((unsigned)norb, (unsigned)nk), OT_THREADS, 0, s>>>(
-            o, v, ldy, tiled, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
+        if (tiled)
+          t3_orbit_energy_kernel<true><<<dim3((unsigned)norb, (unsigned)nk), OT_THREADS, 0, s>>>(
+              o, v, ldy, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
+        else
+          t3_orbit_energy_kernel<false><<<dim3((unsigned)norb, (unsigned)nk), OT_THREADS, 0, s>>>(
+              o, v, ldy, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
         CCX_HIP(hipGetLastError());
         t3_accumulate_chunk_kernel<<<1, 256, 0, s>>>(part.p, norb, nk, i, j, k0, e_dev);
         CCX_HIP(hipGetLastError());
