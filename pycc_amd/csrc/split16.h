@@ -39,10 +39,9 @@ struct X3Gemm {
   const void* Ah2 = nullptr;
   int64_t K1 = 0, sw = 0, nbatch = 1, sA2 = 0, sB = 0, sC = 0;
   // ytile (x3_gemm_kcat only, = sw = N, a multiple of 16): C element (m = x sw + y, n) goes to the
-  // tiled layout [x][y / 16][n / 16][y % 16][16] (ldc unused), so that a 16 x 16 (y, n) block is
-  // one run of 1 KiB, with the columns of a block row in the order n % 4, then n % 16 / 4 (the 16
-  // bytes at 4 c hold n % 16 = c, c + 4, c + 8, c + 12): the offset is (m - m % 16) sw +
-  // 16 (n - n % 16) + 16 (m % 16) + 4 (n % 4) + n % 16 / 4
+  // tiled layout [x][y / 16][n / 16][n % 16][y % 16] (ldc unused), so that a 16 x 16 (y, n) block is
+  // one run of 1 KiB, stored column by column: the offset is (m - m % 16) sw + 16 (n - n % 16) +
+  // 16 (n % 16) + m % 16
   int64_t ytile = 0;
 };
 struct Workspace;

@@ -278,21 +278,21 @@ __global__ void t3_pack_kcat_kernel(int64_t o, int64_t v, int64_t Kp, int i, int
 // then the energy sum of t3_energy_kernel over those tiles, every permuted W read from LDS. HBM
 // traffic is Y read once; W is never written.
 //
-// LDS holds a W tile (x, y, z) at word 256 x + 64 (y / 4) + L(x, y, z): L maps the twelve index bits
-// linearly over GF(2) (XOR of the columns below) onto the six bank bits, so that the address is
-// the XOR of one term per coordinate. Every LDS access of W here is made by waves whose 64 lanes
-// run over all 16 values of one coordinate and the two low bits of another; for each of the six
-// such (coordinate, coordinate) pairs the 64 lanes hit 64 distinct banks, and each half wave 32
-// distinct banks of 32, so that no access has a bank conflict. Column of x bit k: 17, 34, 4, 8;
-// of y bit k: 20, 40, 1, 2; of z bit k: 26, 35, 4, 2.
+// LDS holds a W tile (x, y, z) at a word address that is linear over GF(2) in the twelve index
+// bits: the XOR, over the set bits, of x bit k: 1028, 2056, 273, 546; y bit k: 65, 130, 20, 40;
+// z bit k: 4, 2, 26, 35 (k = 0..3), a bijection onto 0..4095. Every LDS access of W here is made by
+// waves whose 64 lanes run over all 16 values of one coordinate and the two high bits of another
+// (bit 2 within each half wave); for each of the six such (coordinate, coordinate) pairs the 64
+// lanes hit 64 distinct banks (address mod 64), and each half wave 32 distinct banks mod 32, so
+// that no access has a bank conflict.
 __device__ __forceinline__ constexpr int t3_ax(int x) {   // words
-  return (x << 8) ^ (x & 12) ^ ((x & 1) * 17) ^ (((x >> 1) & 1) * 34);
+  return ((x & 1) * 1028) ^ (((x >> 1) & 1) * 2056) ^ (((x >> 2) & 1) * 273) ^ (((x >> 3) & 1) * 546);
 }
 __device__ __forceinline__ constexpr int t3_ay(int y) {
-  return ((y >> 2) << 6) ^ ((y >> 2) & 3) ^ ((y & 1) * 20) ^ (((y >> 1) & 1) * 40);
+  return ((y & 1) * 65) ^ (((y >> 1) & 1) * 130) ^ (((y >> 2) & 1) * 20) ^ (((y >> 3) & 1) * 40);
 }
 __device__ __forceinline__ constexpr int t3_az(int z) {
-  return ((z & 1) * 26) ^ (((z >> 1) & 1) * 35) ^ (((z >> 2) & 1) * 4) ^ (((z >> 3) & 1) * 2);
+  return ((z & 1) * 4) ^ (((z >> 1) & 1) * 2) ^ (((z >> 2) & 1) * 26) ^ (((z >> 3) & 1) * 35);
 }
 constexpr int OTW = TT * TT * TT;   // words per LDS tile
 // slot x holds the tile at the permutation (A, B, C), (A, C, B), (B, A, C), (B, C, A), (C, A, B),
@@ -341,24 +341,23 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   const float* y1 = Y1 + e * ys;
   const float* y2 = Y2 + e * ys;
   const float* y3 = Y3 + e * ys;
-  // Box loads: wave w takes the row x = w of the box; lane (yl, cq) = (l % 16, l / 16) the four
-  // elements z = cq, cq + 4, cq + 8, cq + 12 of the row y = yl (in the tiled layout the 16 bytes at
-  // 64 yl + 16 cq of the box's 1 KiB run for that x).
+  // Box loads: wave w takes the row x = w of the box; lane (zl, cq) = (l % 16, l / 16) the four
+  // elements y = 4 cq + u, u = 0..3, of the column z = zl (in the tiled layout the 16 bytes at
+  // 64 zl + 16 cq of the box's 1 KiB run for that x).
   typedef float f4v __attribute__((ext_vector_type(4)));
   typedef double d2v __attribute__((ext_vector_type(2)));
-  const int wv = threadIdx.x >> 6, yl = threadIdx.x & 15, cq = (threadIdx.x >> 4) & 3;
+  const int wv = threadIdx.x >> 6, zl = threadIdx.x & 15, cq = (threadIdx.x >> 4) & 3;
   auto box = [&](const float* Y, int gx, int gy, int gz) -> f4v {   // tile origins in tiles
-    const int64_t x = 16 * gx + wv, y = 16 * gy + yl;
+    const int64_t x = 16 * gx + wv, z = 16 * gz + zl;
     if constexpr (FULL) {
-      return *(const f4v*)(Y + (x * v + 16 * gy) * v + 256 * gz + yl * 16 + 4 * cq);
+      return *(const f4v*)(Y + (x * v + 16 * gy) * v + 256 * gz + zl * 16 + 4 * cq);
     } else {
       f4v val = f4v{0.f, 0.f, 0.f, 0.f};
-      if (x < v && y < v) {
-        const float* row = Y + (x * v + y) * ldy;
+      if (x < v && z < v) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int64_t z = 16 * gz + cq + 4 * u;
-          if (z < v) val[u] = row[z];
+          const int64_t y = 16 * gy + 4 * cq + u;
+          if (y < v) val[u] = Y[(x * v + y) * ldy + z];
         }
       }
       return val;
@@ -373,17 +372,17 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
       rb[x] = box(y2, T[x][0], T[x][2], T[x][1]);   // Y2[a][c][b]
       rc[x] = box(y3, T[x][1], T[x][2], T[x][0]);   // Y3[b][c][a]
     }
-  // W = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a] in three passes; lane (yl, cq) of wave wv holds
-  // Y1 (a, b, c) = (wv, yl, cq + 4 u), Y2 (a, c, b) = (wv, yl, cq + 4 u), Y3 (b, c, a) = (wv, yl, cq + 4 u)
-  const int a1 = t3_ax(wv) ^ t3_ay(yl) ^ t3_az(cq);   // W (wv, yl, cq + 4 u)
-  const int a2 = t3_ax(wv) ^ t3_ay(cq) ^ t3_az(yl);   // W (wv, cq + 4 u, yl)
-  const int a3 = t3_ax(cq) ^ t3_ay(wv) ^ t3_az(yl);   // W (cq + 4 u, wv, yl)
+  // W = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a] in three passes; lane (zl, cq) of wave wv holds
+  // Y1 (a, b, c) = (wv, 4 cq + u, zl), Y2 (a, c, b) = (wv, 4 cq + u, zl), Y3 (b, c, a) = (wv, 4 cq + u, zl)
+  const int a1 = t3_ax(wv) ^ t3_ay(4 * cq) ^ t3_az(zl);   // W (wv, 4 cq + u, zl)
+  const int a2 = t3_ax(wv) ^ t3_ay(zl) ^ t3_az(4 * cq);   // W (wv, zl, 4 cq + u)
+  const int a3 = t3_ax(zl) ^ t3_ay(wv) ^ t3_az(4 * cq);   // W (zl, wv, 4 cq + u)
 #pragma unroll
   for (int x = 0; x < 6; ++x)
     if (rep[x] == x) {
       float* w = sW + x * OTW;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) w[a1 ^ t3_az(4 * u)] = ra[x][u];
+      for (int u = 0; u < 4; ++u) w[a1 ^ t3_ay(u)] = ra[x][u];
     }
   // then the loads of the disconnected factors of each kept tile (held in registers until its
   // energy sum): the 16 x 16 blocks <ij|ab>, t2_ijab (d = 0, 1), <ik|ac>, t2_ikac (2, 3), <jk|bc>,
@@ -423,7 +422,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
     if (rep[x] == x) {
       float* w = sW + x * OTW;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) w[a2 ^ t3_ay(4 * u)] += rb[x][u];
+      for (int u = 0; u < 4; ++u) w[a2 ^ t3_az(u)] += rb[x][u];
     }
   __syncthreads();
 #pragma unroll
@@ -431,18 +430,19 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
     if (rep[x] == x) {
       float* w = sW + x * OTW;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) w[a3 ^ t3_ax(4 * u)] += rc[x][u];
+      for (int u = 0; u < 4; ++u) w[a3 ^ t3_az(u)] += rc[x][u];
     }
   __syncthreads();
 
-  // Energy: thread t takes the column (b, c) = (t / 16 % 16, t % 16) of the tile and a = 4 (t / 256)
-  // + ii, ii = 0..3. The disconnected factors go through LDS as pairs: sP[0][a][b] = (<ij|ab>,
+  // Energy: thread t takes the column (b, c) = (4 (t / 16 % 4) + t / 64 % 4, t % 16) of the tile and
+  // a = 4 (t / 256) + ii, ii = 0..3. The disconnected factors go through LDS as pairs: sP[0][a][b] = (<ij|ab>,
   // t2_ijab), sP[1][a][c] = (<ik|ac>, t2_ikac), sP[2][b][c] = (<jk|bc>, t2_jkbc), sTF[n][.] = (t1, f
   // row) and sE[n][.] = eps for n = a, b, c.
   __shared__ d2v sP[3][TT][TT];
   __shared__ d2v sTF[3][TT];
   __shared__ double sE[3][TT];
-  const int part = threadIdx.x >> 8, bl = (threadIdx.x >> 4) & 15, al0 = 4 * part;
+  const int part = threadIdx.x >> 8, bl = (((threadIdx.x >> 4) & 3) << 2) | ((threadIdx.x >> 6) & 3);
+  const int al0 = 4 * part;
   // word addresses of W at (a, b, c) permuted, for ii = 0 (the XOR of t3_a*(ii) adds ii)
   const int e0 = t3_ax(al0) ^ t3_ay(bl) ^ t3_az(cl);   // (a, b, c)
   const int e1 = t3_ax(bl) ^ t3_ay(cl) ^ t3_az(al0);   // (b, c, a)
