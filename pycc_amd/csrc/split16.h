@@ -38,10 +38,9 @@ struct X3Gemm {
   // (M = sw^2); batch b of nbatch adds b sA2 to Ah2, b sB to Bh and Bl, and b sC to C.
   const void* Ah2 = nullptr;
   int64_t K1 = 0, sw = 0, nbatch = 1, sA2 = 0, sB = 0, sC = 0;
-  // ytile (x3_gemm_kcat only, = sw = N, a multiple of 16): C element (m = x sw + y, n) goes to the
-  // tiled layout [x][y / 16][n / 16][n % 16][y % 16] (ldc unused), so that a 16 x 16 (y, n) block is
-  // one run of 1 KiB, stored column by column: the offset is (m - m % 16) sw + 16 (n - n % 16) +
-  // 16 (n % 16) + m % 16
+  // ytile (x3_gemm_kcat only, = sw = N, a multiple of 8): C element (m = x sw + y, n) goes to the
+  // tiled layout [x][y / 8][n / 8][n % 8][y % 8] (ldc unused), so that an 8 x 8 (y, n) block is one
+  // run of 256 bytes, stored column by column: the offset is (m - m % 8) sw + 8 n + m % 8
   int64_t ytile = 0;
 };
 struct Workspace;

@@ -567,11 +567,11 @@ __global__ __launch_bounds__(512) void x3_gemm_dma_kernel(X3Gemm p) {
 #pragma unroll
       for (int i = 0; i < NBM; ++i) {
         const int64_t mb = tm * BT + wm * WTM + i * 16;   // 16-aligned block of m
-        if constexpr (YT) {   // x3_gemm_kcat's ytile layout (M, N multiples of 16): the lane's four
-          // rows m % 16 = 4 fr .. 4 fr + 3 of column n are 16 contiguous bytes
+        if constexpr (YT) {   // x3_gemm_kcat's ytile layout (M, N multiples of 8): the lane's four
+          // rows m = mb + 4 fr .. + 3 of column n are 16 contiguous bytes
           if (mb >= p.M) continue;
           typedef float f4v __attribute__((ext_vector_type(4)));
-          *(f4v*)(Cb + mb * p.ytile + nb * 16 + fc * 16 + fr * 4) =
+          *(f4v*)(Cb + (mb + 8 * (fr >> 1)) * p.ytile + 8 * (nb + fc) + 4 * (fr & 1)) =
               f4v{f0 * acc[i][j][0], f0 * acc[i][j][1], f0 * acc[i][j][2], f0 * acc[i][j][3]};
         } else {
           float* d = Cb + mb * p.ldc + nb + fc;
@@ -1022,7 +1022,7 @@ void x3_gemm_kcat(const X3Gemm& p, hipStream_t s) {
   CCX_CHECK(p.kb == BK && p.ska == 2 * BK && p.skb == 2 * BK && p.Kp % BK == 0 && p.K1 % BK == 0 && p.K1 > 0 &&
                 p.K1 < p.Kp && p.lda % 8 == 0 && p.ldb % 8 == 0 && p.Ah2 && p.C && !p.C64 && !p.work &&
                 p.sw > 0 && p.sw * p.sw == p.M && p.M * p.lda < (int64_t(1) << 31) && p.beta == 0.f && !p.sbn &&
-                (p.ytile == 0 || (p.ytile % 16 == 0 && p.ytile == p.sw && p.ytile == p.N &&
+                (p.ytile == 0 || (p.ytile % 8 == 0 && p.ytile == p.sw && p.ytile == p.N &&
                                    (uintptr_t)p.C % 16 == 0 && p.sC % 4 == 0)),
             ERR_INTERNAL, "x3 K-concatenated GEMM layout");
   // the output tile (384 x 192, 256 x 256 or 256 x 192) that pads M x N least, in that order on a

@@ -247,8 +247,8 @@ __global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, in
 //   Y2[a][c][b] = X_ikj[a][c][b] + X_kij[c][a][b] = [A_i | A_k^s](a,c) . [B_(k,j) | B_(i,j)](b)
 //   Y3[b][c][a] = X_jki[b][c][a] + X_kji[c][b][a] = [A_j | A_k^s](b,c) . [B_(k,i) | B_(j,i)](a)
 // so that W[a][b][c] = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a]: FP64 GEMMs (plain_gemm_kcat) in
-// every mode but MIXED_F16X3, whose GEMMs are FP16x3 with FP32 output, in 16 x 16 (y, z) blocks
-// when v is a multiple of 16 (x3_gemm_kcat with ytile).
+// every mode but MIXED_F16X3, whose GEMMs are FP16x3 with FP32 output, in 8 x 8 (y, z) blocks
+// when v is a multiple of 8 (x3_gemm_kcat with ytile).
 //
 // B rows of the chunk (i, j, k0 .. k0 + nk): block 3 e + s (s = 0, 1, 2 for Y1, Y2, Y3 of k = k0 + e)
 // holds, for z < v, [B_(q1,r1)[z] | B_(q2,r2)[z]] (2 Kp wide) with the pairs named above.
@@ -273,46 +273,59 @@ __global__ void t3_pack_kcat_kernel(int64_t o, int64_t v, int64_t Kp, int i, int
   }
 }
 
-// One work-group per (orbit {A >= B >= C} of 16^3 tiles, triple k0 + blockIdx.y): W of the orbit's
+// One work-group per (orbit {A >= B >= C} of 8^3 tiles, triple k0 + blockIdx.y): W of the orbit's
 // distinct tiles (one, three or six permutations of (A, B, C)) from Y1, Y2, Y3 into LDS (FP32),
 // then the energy sum of t3_energy_kernel over those tiles, every permuted W read from LDS. HBM
-// traffic is Y read once; W is never written.
-//
-// LDS holds a W tile (x, y, z) at a word address that is linear over GF(2) in the twelve index
-// bits: the XOR, over the set bits, of x bit k: 1028, 2056, 273, 546; y bit k: 65, 130, 20, 40;
-// z bit k: 4, 2, 26, 35 (k = 0..3), a bijection onto 0..4095. Every LDS access of W here is made by
-// waves whose 64 lanes run over all 16 values of one coordinate and the two high bits of another
-// (bit 2 within each half wave); for each of the six such (coordinate, coordinate) pairs the 64
-// lanes hit 64 distinct banks (address mod 64), and each half wave 32 distinct banks mod 32, so
-// that no access has a bank conflict.
+// traffic is Y read once; W is never written. A work-group needs 36 KiB of LDS, so that four are
+// resident on a CU and the loads of one overlap the sums of the others.
+constexpr int OE = 8;                // tile edge
+constexpr int OTW = OE * OE * OE;    // words per LDS tile
+constexpr int OT_THREADS = 256;
+// LDS holds a W tile (x, y, z) at a word address that is linear over GF(2) in the nine index bits:
+// the XOR, over the set bits, of x bit k: 8, 6, 89; y bit k: 11, 34, 155; z bit k: 1, 52, 260
+// (k = 0..2), a bijection onto 0..511. Every LDS access of W here is made by waves whose 64 lanes
+// run over two whole coordinates, or over x bits 0-1 (lane bits 4-5), one whole coordinate and
+// bit 2 of the third; for each such pattern the 64 lanes hit 64 distinct banks (address mod 64)
+// and each half wave 32 distinct banks mod 32, so that no access has a bank conflict.
 __device__ __forceinline__ constexpr int t3_ax(int x) {   // words
-  return ((x & 1) * 1028) ^ (((x >> 1) & 1) * 2056) ^ (((x >> 2) & 1) * 273) ^ (((x >> 3) & 1) * 546);
+  return ((x & 1) * 8) ^ (((x >> 1) & 1) * 6) ^ (((x >> 2) & 1) * 89);
 }
 __device__ __forceinline__ constexpr int t3_ay(int y) {
-  return ((y & 1) * 65) ^ (((y >> 1) & 1) * 130) ^ (((y >> 2) & 1) * 20) ^ (((y >> 3) & 1) * 40);
+  return ((y & 1) * 11) ^ (((y >> 1) & 1) * 34) ^ (((y >> 2) & 1) * 155);
 }
 __device__ __forceinline__ constexpr int t3_az(int z) {
-  return ((z & 1) * 4) ^ (((z >> 1) & 1) * 2) ^ (((z >> 2) & 1) * 26) ^ (((z >> 3) & 1) * 35);
+  return ((z & 1) * 1) ^ (((z >> 1) & 1) * 52) ^ (((z >> 2) & 1) * 260);
 }
-constexpr int OTW = TT * TT * TT;   // words per LDS tile
 // slot x holds the tile at the permutation (A, B, C), (A, C, B), (B, A, C), (B, C, A), (C, A, B),
 // (C, B, A) of the orbit.
 
-constexpr int OT_THREADS = 1024;   // 4 waves per SIMD at one work-group per CU
-__device__ __forceinline__ double t3_rcp(double x) {   // 1/x: the hardware estimate and two Newton steps
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(fma(-x, r, 1.0), r, r);
+// which of (A, B, C) sits at position n of slot x's permutation: 2 bits at 6 x + 2 n of 0x192261624
+// (the slots' permutations 012, 021, 102, 120, 201, 210), so that a run-time slot needs no array
+__device__ __forceinline__ constexpr int t3_slot_src(int x, int n) {
+  return (int)((0x192261624ull >> (6 * x + 2 * n)) & 3);
+}
+__device__ __forceinline__ int t3_slot_tile(int x, int n, int A, int B, int C) {
+  const int src = t3_slot_src(x, n);
+  return src == 0 ? A : (src == 1 ? B : C);
+}
+
+// 1/x: the hardware estimate (relative error below 2^-23) and one Newton step (below 2^-45)
+__device__ __forceinline__ double t3_rcp(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
   return fma(fma(-x, r, 1.0), r, r);
 }
 
-// FULL: Y in the tiled layout of x3_gemm_kcat's ytile (v a multiple of 16, every tile whole); else
+// FULL: Y in the tiled layout of x3_gemm_kcat's ytile (v a multiple of 8, every tile whole); else
 // Y is [x][y][z] with rows of ldy floats and the tiles at the edge are partial.
+// The disconnected blocks come from PB (t3_pair_blocks_kernel) and t1, F. One work-group per (orbit,
+// triple) pair t = orbit + norb e of the 1-D grid, taken in an order that gives each XCD a
+// contiguous range of t (consecutive orbits share tiles, so their blocks meet in that XCD's L2).
 template <bool FULL>
 __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
-    int64_t o, int64_t v, int64_t ldy, int i, int j, int k0, const float* __restrict__ Y1,
+    int64_t o, int64_t v, int64_t ldy, int i, int j, int k0, int nk, const float* __restrict__ Y1,
     const float* __restrict__ Y2, const float* __restrict__ Y3, const int* __restrict__ orbits, int64_t norb,
-    const double* __restrict__ F, int64_t nmo, const double* __restrict__ t1, const double* __restrict__ t2,
-    const double* __restrict__ Ioovv, double* __restrict__ partials) {
+    const double* __restrict__ F, int64_t nmo, const double* __restrict__ t1, const double* __restrict__ PB,
+    double* __restrict__ partials) {
   // for the tile (P0, P1, P2) of slot x, the slots of (P1, P2, P0), (P2, P0, P1), (P0, P2, P1),
   // (P1, P0, P2) and (P2, P1, P0)
   constexpr int t3_cyc[6] = {3, 5, 1, 4, 0, 2};
@@ -320,13 +333,28 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   constexpr int t3_sw23[6] = {1, 0, 3, 2, 5, 4};
   constexpr int t3_sw12[6] = {2, 4, 0, 5, 1, 3};
   constexpr int t3_rev[6] = {5, 3, 4, 1, 2, 0};
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  typedef double d2v __attribute__((ext_vector_type(2)));
   __shared__ float sW[6 * OTW];
-  const int e = blockIdx.y, k = k0 + e;
-  const int64_t nt = (v + TT - 1) / TT, vv = v * v, ys = vv * ldy;
-  const int code = orbits[blockIdx.x];
+  // the disconnected factors of the tiles, in pairs: sP[x][0][a][b] = (<ij|ab>, t2_ijab),
+  // sP[x][1][a][c] = (<ik|ac>, t2_ikac), sP[x][2][b][c] = (<jk|bc>, t2_jkbc); sTF[n][s][.] = (t1,
+  // f row) of the occupied orbital (i, j, k)[n] over tile (A, B, C)[s] and sE[s][.] = eps over it
+  __shared__ d2v sP[6][3][OE][OE];
+  __shared__ d2v sTF[3][3][OE];
+  __shared__ double sE[3][OE];
+  int64_t tw;
+  {   // hardware work-group id (round-robin over the 8 XCDs) -> t
+    const int64_t n = norb * nk, q = n / 8, r = n % 8, xcd = blockIdx.x % 8, w = blockIdx.x / 8;
+    tw = xcd * q + (xcd < r ? xcd : r) + w;
+  }
+  const int e = (int)(tw / norb), k = k0 + e;
+  const int64_t ob = tw - (int64_t)e * norb;
+  const int64_t nt = (v + OE - 1) / OE, vv = v * v, ys = vv * ldy;
+  const int code = orbits[ob];
   const int A = (int)(code / (nt * nt)), B = (int)(code / nt % nt), C = (int)(code % nt);
   const int T[6][3] = {{A, B, C}, {A, C, B}, {B, A, C}, {B, C, A}, {C, A, B}, {C, B, A}};
-  // rep[x]: the first slot with slot x's tile (A >= B >= C: only A == B or B == C repeat tiles)
+  // rep[x]: the first slot with slot x's tile (A >= B >= C: only A == B or B == C repeat tiles);
+  // bit x of kept: rep[x] == x
   int rep[6];
   {
     const bool ab = A == B, bc = B == C;
@@ -338,161 +366,192 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
     rep[5] = ab ? 4 : (bc ? 3 : 5);
     if (ab && bc) rep[1] = rep[2] = rep[3] = rep[4] = rep[5] = 0;
   }
+  int kept = 0;
+#pragma unroll
+  for (int x = 0; x < 6; ++x) kept |= (rep[x] == x) << x;
   const float* y1 = Y1 + e * ys;
   const float* y2 = Y2 + e * ys;
   const float* y3 = Y3 + e * ys;
-  // Box loads: wave w takes the row x = w of the box; lane (zl, cq) = (l % 16, l / 16) the four
-  // elements y = 4 cq + u, u = 0..3, of the column z = zl (in the tiled layout the 16 bytes at
-  // 64 zl + 16 cq of the box's 1 KiB run for that x).
-  typedef float f4v __attribute__((ext_vector_type(4)));
-  typedef double d2v __attribute__((ext_vector_type(2)));
-  const int wv = threadIdx.x >> 6, zl = threadIdx.x & 15, cq = (threadIdx.x >> 4) & 3;
+  // Box loads: the threads of half hf = t / 128 take the boxes of the slots 2 p + hf; thread (x, z,
+  // yg) = (t / 16 % 8, t / 2 % 8, t % 2) the four elements y = 4 yg + u, u = 0..3, of the row x and
+  // column z (in the tiled layout the 16 bytes at 32 z + 16 yg of the box's 256-byte run for x).
+  const int hf = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);   // wave-uniform
+  const int bx = (threadIdx.x >> 4) & 7, bz = (threadIdx.x >> 1) & 7, yg = threadIdx.x & 1;
+  const int lbox = bx * (int)vv + 8 * bz + 4 * yg;   // the lane's offset in a whole box (v^2 < 2^28)
   auto box = [&](const float* Y, int gx, int gy, int gz) -> f4v {   // tile origins in tiles
-    const int64_t x = 16 * gx + wv, z = 16 * gz + zl;
     if constexpr (FULL) {
-      return *(const f4v*)(Y + (x * v + 16 * gy) * v + 256 * gz + zl * 16 + 4 * cq);
+      return *(const f4v*)(Y + ((OE * (int64_t)gx * v + OE * gy) * v + 64 * gz) + lbox);
     } else {
+      const int64_t x = OE * gx + bx, z = OE * gz + bz;
       f4v val = f4v{0.f, 0.f, 0.f, 0.f};
       if (x < v && z < v) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int64_t y = 16 * gy + 4 * cq + u;
+          const int64_t y = OE * gy + 4 * yg + u;
           if (y < v) val[u] = Y[(x * v + y) * ldy + z];
         }
       }
       return val;
     }
   };
-  // the Y boxes of the kept tiles, all loaded here
-  f4v ra[6], rb[6], rc[6];
+  f4v ra[3], rb[3], rc[3];   // slot 2 p + hf
 #pragma unroll
-  for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) {
-      ra[x] = box(y1, T[x][0], T[x][1], T[x][2]);   // Y1[a][b][c]
-      rb[x] = box(y2, T[x][0], T[x][2], T[x][1]);   // Y2[a][c][b]
-      rc[x] = box(y3, T[x][1], T[x][2], T[x][0]);   // Y3[b][c][a]
+  for (int p = 0; p < 3; ++p) {
+    const int x = 2 * p + hf;
+    if ((kept >> x) & 1) {   // wave-uniform
+      const int P0 = t3_slot_tile(x, 0, A, B, C), P1 = t3_slot_tile(x, 1, A, B, C), P2 = t3_slot_tile(x, 2, A, B, C);
+      ra[p] = box(y1, P0, P1, P2);   // Y1[a][b][c]
+      rb[p] = box(y2, P0, P2, P1);   // Y2[a][c][b]
+      rc[p] = box(y3, P1, P2, P0);   // Y3[b][c][a]
     }
-  // W = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a] in three passes; lane (zl, cq) of wave wv holds
-  // Y1 (a, b, c) = (wv, 4 cq + u, zl), Y2 (a, c, b) = (wv, 4 cq + u, zl), Y3 (b, c, a) = (wv, 4 cq + u, zl)
-  const int a1 = t3_ax(wv) ^ t3_ay(4 * cq) ^ t3_az(zl);   // W (wv, 4 cq + u, zl)
-  const int a2 = t3_ax(wv) ^ t3_ay(zl) ^ t3_az(4 * cq);   // W (wv, zl, 4 cq + u)
-  const int a3 = t3_ax(zl) ^ t3_ay(wv) ^ t3_az(4 * cq);   // W (zl, wv, 4 cq + u)
+  }
+  // the disconnected factors: element t + OT_THREADS r of the 6 x 6 x 64 block elements (slot x =
+  // el / 384, block d = el / 64 % 6: <ij|ab>, t2_ijab, <ik|ac>, t2_ikac, <jk|bc>, t2_jkbc) and of the
+  // 6 x 9 x 8 vector elements (slot x, q = 3 n + u: u = 0 t1, 1 f row, 2 eps; n = a, b, c)
+  // wave w of round rr takes block bi = 4 rr + w < 18: pair d = bi % 3 ((i, j), (i, k), (j, k)) of
+  // slot x = bi / 3, whose 1 KiB in PB lane l reads as the element (l / 8, l % 8). Every wave loads
+  // (bi clamped to 17; slots that repeat a tile load blocks that are never read), so that no load
+  // sits in a branch.
+  constexpr int NB = (18 + 3) / 4;
+  const int lane = threadIdx.x & 63, w4 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  d2v pd[NB];
+  double pv = 0.0;
 #pragma unroll
-  for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) {
+  for (int rr = 0; rr < NB; ++rr) {
+    const int bi = min(4 * rr + w4, 17), x = bi / 3, d = bi - 3 * x;   // wave-uniform
+    const int pp = d == 0 ? 0 : (d == 1 ? 1 + e : 1 + nk + e);
+    const int rt = t3_slot_tile(x, d < 2 ? 0 : 1, A, B, C), ct = t3_slot_tile(x, d == 0 ? 1 : 2, A, B, C);
+    pd[rr] = *(const d2v*)(PB + 2 * ((((int64_t)pp * nt + rt) * nt + ct) * 64 + lane));
+  }
+  // the vectors: row R = t / 8 < 21 (t1 of the occupied (i, j, k)[R / 3] over tile (A, B, C)[R % 3]
+  // for R < 9, then the f rows for R < 18, then eps over tile R - 18), element t % 8
+  {
+    const int R = threadIdx.x >> 3;
+    if (R < 21) {
+      const int n = (R < 18 ? R % 9 : 0) / 3, sidx = R < 18 ? R % 3 : R - 18;
+      const int64_t g = OE * (int64_t)(sidx == 0 ? A : (sidx == 1 ? B : C)) + (threadIdx.x & 7);
+      if (FULL || g < v) {
+        const int occ = n == 0 ? i : (n == 1 ? j : k);
+        pv = R < 9 ? t1[occ * v + g] : (R < 18 ? F[occ * nmo + o + g] : F[(o + g) * nmo + o + g]);
+      }
+    }
+  }
+  // W = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a] in three passes; thread (bx, bz, yg) holds Y1 (a, b,
+  // c) = (bx, 4 yg + u, bz), Y2 (a, c, b) = (bx, 4 yg + u, bz), Y3 (b, c, a) = (bx, 4 yg + u, bz)
+  const int a1 = t3_ax(bx) ^ t3_ay(4 * yg) ^ t3_az(bz);   // W (bx, 4 yg + u, bz)
+  const int a2 = t3_ax(bx) ^ t3_ay(bz) ^ t3_az(4 * yg);   // W (bx, bz, 4 yg + u)
+  const int a3 = t3_ax(bz) ^ t3_ay(bx) ^ t3_az(4 * yg);   // W (bz, bx, 4 yg + u)
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const int x = 2 * p + hf;
+    if ((kept >> x) & 1) {
       float* w = sW + x * OTW;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) w[a1 ^ t3_ay(u)] = ra[x][u];
+      for (int u = 0; u < 4; ++u) w[a1 ^ t3_ay(u)] = ra[p][u];
     }
-  // then the loads of the disconnected factors of each kept tile (held in registers until its
-  // energy sum): the 16 x 16 blocks <ij|ab>, t2_ijab (d = 0, 1), <ik|ac>, t2_ikac (2, 3), <jk|bc>,
-  // t2_jkbc (4, 5), element t + OT_THREADS r of the six for thread t; and, for threads below 144,
-  // one element of the vectors q = 3 n + u (u = 0 t1, 1 the f row of the occupied orbital, 2 eps,
-  // for n = a, b, c)
-  const double* I6[6] = {Ioovv + ((int64_t)i * o + j) * vv, t2 + ((int64_t)i * o + j) * vv,
-                         Ioovv + ((int64_t)i * o + k) * vv, t2 + ((int64_t)i * o + k) * vv,
-                         Ioovv + ((int64_t)j * o + k) * vv, t2 + ((int64_t)j * o + k) * vv};
-  const int occ3[3] = {i, j, k};
-  constexpr int NR = (6 * 256 + OT_THREADS - 1) / OT_THREADS;
-  const int cl = threadIdx.x & 15;
-  double pd[6][NR], pv[6];
+  }
 #pragma unroll
-  for (int x = 0; x < 6; ++x) {
-    pv[x] = 0.0;
-    if (rep[x] != x) continue;
-    const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
+  for (int rr = 0; rr < NB; ++rr) {
+    const int bi = 4 * rr + w4, x = bi / 3, d = bi - 3 * x;
+    if (bi < 18) sP[x][d][lane >> 3][lane & 7] = pd[rr];
+  }
+  {
+    const int R = threadIdx.x >> 3;
+    if (R < 18) ((double*)&sTF[(R % 9) / 3][R % 3][threadIdx.x & 7])[R < 9 ? 0 : 1] = pv;
+    else if (R < 21) sE[R - 18][threadIdx.x & 7] = pv;
+  }
+  __syncthreads();
 #pragma unroll
-    for (int rr = 0; rr < NR; ++rr) {
-      const int el = threadIdx.x + OT_THREADS * rr, d = el >> 8;
-      const int64_t r = org[d < 4 ? 0 : 1] + ((el >> 4) & 15), cc = org[d < 2 ? 1 : 2] + (el & 15);
-      pd[x][rr] = (el < 6 * 256 && (FULL || (r < v && cc < v))) ? I6[d < 6 ? d : 0][r * v + cc] : 0.0;
-    }
-    if (threadIdx.x < 9 * TT) {
-      const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
-      const int64_t g = org[n] + cl;
-      if (FULL || g < v) {
-        const int occ = occ3[n];
-        pv[x] = u == 0 ? t1[occ * v + g] : (u == 1 ? F[occ * nmo + o + g] : F[(o + g) * nmo + o + g]);
-      }
+  for (int p = 0; p < 3; ++p) {
+    const int x = 2 * p + hf;
+    if ((kept >> x) & 1) {
+      float* w = sW + x * OTW;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[a2 ^ t3_az(u)] += rb[p][u];
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) {
+  for (int p = 0; p < 3; ++p) {
+    const int x = 2 * p + hf;
+    if ((kept >> x) & 1) {
       float* w = sW + x * OTW;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) w[a2 ^ t3_az(u)] += rb[x][u];
+      for (int u = 0; u < 4; ++u) w[a3 ^ t3_az(u)] += rc[p][u];
     }
-  __syncthreads();
-#pragma unroll
-  for (int x = 0; x < 6; ++x)
-    if (rep[x] == x) {
-      float* w = sW + x * OTW;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) w[a3 ^ t3_az(u)] += rc[x][u];
-    }
+  }
   __syncthreads();
 
-  // Energy: thread t takes the column (b, c) = (4 (t / 16 % 4) + t / 64 % 4, t % 16) of the tile and
-  // a = 4 (t / 256) + ii, ii = 0..3. The disconnected factors go through LDS as pairs: sP[0][a][b] = (<ij|ab>,
-  // t2_ijab), sP[1][a][c] = (<ik|ac>, t2_ikac), sP[2][b][c] = (<jk|bc>, t2_jkbc), sTF[n][.] = (t1, f
-  // row) and sE[n][.] = eps for n = a, b, c.
-  __shared__ d2v sP[3][TT][TT];
-  __shared__ d2v sTF[3][TT];
-  __shared__ double sE[3][TT];
-  const int part = threadIdx.x >> 8, bl = (((threadIdx.x >> 4) & 3) << 2) | ((threadIdx.x >> 6) & 3);
-  const int al0 = 4 * part;
-  // word addresses of W at (a, b, c) permuted, for ii = 0 (the XOR of t3_a*(ii) adds ii)
+  // Energy: thread t takes the column (b, c) = (t / 8 % 8, t % 8) of each kept tile and a = t / 64 +
+  // 4 ii, ii = 0, 1.
+  const int bl = (threadIdx.x >> 3) & 7, cl = threadIdx.x & 7, al0 = threadIdx.x >> 6;
+  // word addresses of W at (a, b, c) permuted, for ii = 0 (the XOR of t3_a*(4 ii) adds 4 ii)
   const int e0 = t3_ax(al0) ^ t3_ay(bl) ^ t3_az(cl);   // (a, b, c)
   const int e1 = t3_ax(bl) ^ t3_ay(cl) ^ t3_az(al0);   // (b, c, a)
   const int e2 = t3_ax(cl) ^ t3_ay(al0) ^ t3_az(bl);   // (c, a, b)
   const int e3 = t3_ax(al0) ^ t3_ay(cl) ^ t3_az(bl);   // (a, c, b)
   const int e4 = t3_ax(bl) ^ t3_ay(al0) ^ t3_az(cl);   // (b, a, c)
   const int e5 = t3_ax(cl) ^ t3_ay(bl) ^ t3_az(al0);   // (c, b, a)
+  auto rd = [&](int off) { return (double)*(const float*)((const char*)sW + off); };
   const double dijk = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k];
   double s = 0.0;
 #pragma unroll
   for (int x = 0; x < 6; ++x) {
     if (rep[x] != x) continue;   // block-uniform
-    const int64_t org[3] = {16 * (int64_t)T[x][0], 16 * (int64_t)T[x][1], 16 * (int64_t)T[x][2]};
-#pragma unroll
-    for (int rr = 0; rr < NR; ++rr) {
-      const int el = threadIdx.x + OT_THREADS * rr, d = el >> 8;
-      if (el < 6 * 256) ((double*)&sP[d >> 1][(el >> 4) & 15][el & 15])[d & 1] = pd[x][rr];
-    }
-    if (threadIdx.x < 9 * TT) {
-      const int q = threadIdx.x / TT, n = q / 3, u = q - 3 * n;
-      if (u < 2) ((double*)&sTF[n][cl])[u] = pv[x];
-      else sE[n][cl] = pv[x];
-    }
-    __syncthreads();
-    const int64_t a0 = org[0], b = org[1] + bl, c = org[2] + cl;
+    const int64_t a0 = OE * (int64_t)T[x][0], b = OE * (int64_t)T[x][1] + bl, c = OE * (int64_t)T[x][2] + cl;
     if (FULL || (b < v && c < v)) {
-      // byte offsets in sW of the six permuted reads: slot (bits 14 and up) | word address * 4
+      // byte offsets in sW of the six permuted reads: slot (bits 11 and up) | word address * 4
       const int q0 = (x * OTW + e0) * 4, q1 = (rep[t3_cyc[x]] * OTW + e1) * 4;
       const int q2 = (rep[t3_cyc2[x]] * OTW + e2) * 4, q3 = (rep[t3_sw23[x]] * OTW + e3) * 4;
       const int q4 = (rep[t3_sw12[x]] * OTW + e4) * 4, q5 = (rep[t3_rev[x]] * OTW + e5) * 4;
-      auto rd = [&](int off) { return (double)*(const float*)((const char*)sW + off); };
-      const double dbc = dijk - sE[1][bl] - sE[2][cl];
-      const d2v pbc = sP[2][bl][cl], tfb = sTF[1][bl], tfc = sTF[2][cl];
+      const int s0 = t3_slot_src(x, 0), s1 = t3_slot_src(x, 1), s2 = t3_slot_src(x, 2);
+      const double dbc = dijk - sE[s1][bl] - sE[s2][cl];
+      const d2v pbc = sP[x][2][bl][cl], tfb = sTF[1][s1][bl], tfc = sTF[2][s2][cl];
 #pragma unroll
-      for (int ii = 0; ii < 4; ++ii) {
-        const int al = al0 + ii;
+      for (int ii = 0; ii < 2; ++ii) {
+        const int al = al0 + 4 * ii;
         if (!FULL && a0 + al >= v) break;   // wave-uniform
-        const double w = rd(q0 ^ 4 * t3_ax(ii));
-        const double q = rd(q1 ^ 4 * t3_az(ii)) + rd(q2 ^ 4 * t3_ay(ii)) -
-                         2.0 * (rd(q3 ^ 4 * t3_ax(ii)) + rd(q4 ^ 4 * t3_ay(ii)) + rd(q5 ^ 4 * t3_az(ii)));
-        const d2v pab = sP[0][al][bl], pac = sP[1][al][cl], tfa = sTF[0][al];
+        // 4 W_abc + W_bca + W_cab - 2 (W_acb + W_bac + W_cba)
+        const double w = rd(q0 ^ 4 * t3_ax(4 * ii));
+        const double z = fma(-2.0, rd(q3 ^ 4 * t3_ax(4 * ii)) + rd(q4 ^ 4 * t3_ay(4 * ii)) + rd(q5 ^ 4 * t3_az(4 * ii)),
+                             fma(4.0, w, rd(q1 ^ 4 * t3_az(4 * ii)) + rd(q2 ^ 4 * t3_ay(4 * ii))));
+        const d2v pab = sP[x][0][al][bl], pac = sP[x][1][al][cl], tfa = sTF[0][s0][al];
         const double vabc = w + pab.x * tfc.x + pac.x * tfb.x + pbc.x * tfa.x + pab.y * tfc.y + pac.y * tfb.y +
                             pbc.y * tfa.y;
-        s += vabc * (4.0 * w + q) * t3_rcp(dbc - sE[0][al]);
+        s += vabc * z * t3_rcp(dbc - sE[s0][al]);
       }
     }
-    __syncthreads();   // sP, sTF, sE are reloaded for the next tile
   }
-  s = t3_block_sum(s);
-  if (threadIdx.x == 0) partials[(int64_t)e * norb + blockIdx.x] = s;
+  // work-group sum: within each wave, then over the four waves
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  __shared__ double sw4[OT_THREADS / 64];
+  if ((threadIdx.x & 63) == 0) sw4[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[tw] = (sw4[0] + sw4[1]) + (sw4[2] + sw4[3]);
+}
+
+// PB[pp][rt][ct][r][c] = (<p q|a b>, t2_pqab) at a = 8 rt + r, b = 8 ct + c (zero past v), for the pairs
+// pp = 0: (i, j), 1 + e: (i, k0 + e), 1 + nk + e: (j, k0 + e), e < nk: the disconnected factors of a
+// chunk as contiguous 8 x 8 blocks of (integral, amplitude) pairs.
+__global__ void t3_pair_blocks_kernel(int64_t o, int64_t v, int64_t nt, int i, int j, int k0, int nk,
+                                      const double* __restrict__ Ioovv, const double* __restrict__ t2,
+                                      double* __restrict__ PB) {
+  const int64_t per = nt * nt * 64, total = (1 + 2 * (int64_t)nk) * per;
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+    const int pp = (int)(x / per);
+    const int64_t rem = x - pp * per, blk = rem >> 6, rt = blk / nt, ct = blk - rt * nt;
+    const int64_t a = 8 * rt + ((rem >> 3) & 7), b = 8 * ct + (rem & 7);
+    const int p = pp <= nk ? i : j, q = pp == 0 ? j : k0 + (pp <= nk ? pp - 1 : pp - 1 - nk);
+    double iv = 0.0, tv = 0.0;
+    if (a < v && b < v) {
+      const int64_t off = ((int64_t)p * o + q) * v * v + a * v + b;
+      iv = Ioovv[off];
+      tv = t2[off];
+    }
+    PB[2 * x] = iv;
+    PB[2 * x + 1] = tv;
+  }
 }
 
 // e[0] += sum over the chunk's triples (i, j, k0 + x), in order, of weight * sum(partials[x][0 .. norb))
@@ -555,10 +614,10 @@ __global__ void t3_accumulate_kernel(const double* __restrict__ partials, int64_
 static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const double* t2, int i0, int i1, bool accumulate,
                               double* e_dev, hipStream_t s) {
   const int64_t o = c.o, v = c.v, nmo = c.nmo, vv = v * v;
-  const int64_t nt = ceil_div(v, TT), KC = T3_KC;
+  const int64_t nt = ceil_div(v, OE), KC = T3_KC;
   const int64_t Kp = (v + o + 31) / 32 * 32, L = 2 * Kp;
-  // Y in 16 x 16 (y, z) blocks when v is a multiple of 16, else rows padded to 4 floats
-  const bool tiled = v % 16 == 0;
+  // Y in 8 x 8 (y, z) blocks when v is a multiple of 8, else rows padded to 4 floats
+  const bool tiled = v % 8 == 0;
   const int64_t ldy = tiled ? v : (v + 3) / 4 * 4;
   const int64_t nA = (int64_t)i1 * vv * Kp, nBr = 3 * KC * v, np = x3_partials();
   CCX_CHECK(vv * ldy * KC < (int64_t(1) << 40) && ceil_div(vv, 256) < (int64_t(1) << 24), ERR_ARG,
@@ -587,6 +646,7 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
   const int64_t ys = vv * ldy;          // one triple's Y block (see t3_orbit_energy_kernel)
   DevBuf<float> Y(3 * KC * ys);        // Y1, Y2, Y3 of the chunk's triples
   DevBuf<double> part(KC * norb);
+  DevBuf<double> PB((1 + 2 * KC) * nt * nt * 64 * 2);   // the chunk's disconnected blocks
   float *Y1 = Y.p, *Y2 = Y1 + KC * ys, *Y3 = Y2 + KC * ys;
   if (!accumulate) {
     t3_zero_kernel<<<1, 64, 0, s>>>(e_dev);
@@ -620,12 +680,18 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
         g.Ah = Ap(j); g.Al = Ap(j) + 32;
         g.Bh = Bhl.p + 2 * v * 2 * L; g.Bl = (const f16*)g.Bh + 32; g.C = Y3;
         x3_gemm_kcat(g, s);
+        {
+          const int64_t nb = (1 + 2 * (int64_t)nk) * nt * nt * 64;
+          t3_pair_blocks_kernel<<<(int)std::min<int64_t>(8192, ceil_div(nb, 256)), 256, 0, s>>>(
+              o, v, nt, i, j, k0, nk, c.Ioovv, t2, PB.p);
+          CCX_HIP(hipGetLastError());
+        }
         if (tiled)
-          t3_orbit_energy_kernel<true><<<dim3((unsigned)norb, (unsigned)nk), OT_THREADS, 0, s>>>(
-              o, v, ldy, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
+          t3_orbit_energy_kernel<true><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
+              o, v, ldy, i, j, k0, nk, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, PB.p, part.p);
         else
-          t3_orbit_energy_kernel<false><<<dim3((unsigned)norb, (unsigned)nk), OT_THREADS, 0, s>>>(
-              o, v, ldy, i, j, k0, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, t2, c.Ioovv, part.p);
+          t3_orbit_energy_kernel<false><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
+              o, v, ldy, i, j, k0, nk, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, PB.p, part.p);
         CCX_HIP(hipGetLastError());
         t3_accumulate_chunk_kernel<<<1, 256, 0, s>>>(part.p, norb, nk, i, j, k0, e_dev);
         CCX_HIP(hipGetLastError());
