@@ -273,6 +273,33 @@ __global__ void t3_pack_kcat_kernel(int64_t o, int64_t v, int64_t Kp, int i, int
   }
 }
 
+// MIXED_F16X3: the same B rows split straight into interleaved FP16 hi/lo (hi of k-block q of a row
+// at 64 q, lo at 64 q + 32; rows of 2 L halves) with the scale sc[0]; work-group (x, row) takes
+// 256 consecutive kk2 of the row.
+__global__ __launch_bounds__(256) void t3_pack_split_kernel(int64_t o, int64_t v, int64_t Kp, int i, int j, int k0,
+                                                           const double* __restrict__ t2,
+                                                           const double* __restrict__ Cooov,
+                                                           const float* __restrict__ sc, _Float16* __restrict__ Bhl) {
+  const int64_t L = 2 * Kp, row = blockIdx.y, blk = row / v, z = row - blk * v;
+  const int e3 = (int)(blk / 3), sl = (int)(blk - 3 * e3), k = k0 + e3;
+  const int64_t kk2 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (kk2 >= L) return;
+  const bool h = kk2 >= Kp;
+  const int64_t kk = h ? kk2 - Kp : kk2;
+  int q, r;
+  if (sl == 0) { q = h ? i : j; r = k; }        // (j, k) | (i, k)
+  else if (sl == 1) { q = h ? i : k; r = j; }   // (k, j) | (i, j)
+  else { q = h ? j : k; r = i; }                // (k, i) | (j, i)
+  double val = 0.0;
+  if (kk < v) val = t2[(((int64_t)r * o + q) * v + z) * v + kk];
+  else if (kk < v + o) val = -Cooov[(((kk - v) * o + q) * o + r) * v + z];
+  const float x = (float)(val * (double)sc[0]);
+  const _Float16 hv = (_Float16)x;
+  const int64_t at = row * 2 * L + ((kk2 >> 5) << 6) + (kk2 & 31);
+  Bhl[at] = hv;
+  Bhl[at + 32] = (_Float16)(x - (float)hv);
+}
+
 // One work-group per (orbit {A >= B >= C} of 8^3 tiles, triple k0 + blockIdx.y): W of the orbit's
 // distinct tiles (one, three or six permutations of (A, B, C)) from Y1, Y2, Y3 into LDS (FP32),
 // then the energy sum of t3_energy_kernel over those tiles, every permuted W read from LDS. HBM
@@ -554,16 +581,24 @@ __global__ void t3_pair_blocks_kernel(int64_t o, int64_t v, int64_t nt, int i, i
   }
 }
 
-// e[0] += sum over the chunk's triples (i, j, k0 + x), in order, of weight * sum(partials[x][0 .. norb))
-__global__ void t3_accumulate_chunk_kernel(const double* __restrict__ partials, int64_t norb, int nk, int i, int j,
-                                           int k0, double* e) {
-  for (int x = 0; x < nk; ++x) {
-    double s = 0.0;
-    for (int64_t b = threadIdx.x; b < norb; b += blockDim.x) s += partials[(int64_t)x * norb + b];
-    s = t3_block_sum(s);
-    const int k = k0 + x;
-    if (threadIdx.x == 0) e[0] += (2.0 - (double)((i == j) + (i == k) + (j == k))) * s;   // cctriples.py:237
+// e[0] += sum over the chunk's triples (i, j, k0 + x) of weight * sum(partials[x][0 .. norb)), in a
+// fixed order: work-group g of T3_NSUM sums the weighted partials g, g + T3_NSUM, ... into sums[g],
+// then one work-group adds sums[0 .. T3_NSUM) to e[0].
+constexpr int T3_NSUM = 256;
+__global__ __launch_bounds__(256) void t3_chunk_sums_kernel(const double* __restrict__ partials, int64_t norb, int nk,
+                                                            int i, int j, int k0, double* __restrict__ sums) {
+  const int64_t n = (int64_t)nk * norb;
+  double s = 0.0;
+  for (int64_t t = blockIdx.x + (int64_t)T3_NSUM * threadIdx.x; t < n; t += (int64_t)T3_NSUM * blockDim.x) {
+    const int k = k0 + (int)(t / norb);
+    s += (2.0 - (double)((i == j) + (i == k) + (j == k))) * partials[t];   // cctriples.py:237
   }
+  s = t3_block_sum(s);
+  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(T3_NSUM) void t3_accumulate_chunk_kernel(const double* __restrict__ sums, double* e) {
+  const double s = t3_block_sum(sums[threadIdx.x]);
+  if (threadIdx.x == 0) e[0] += s;
 }
 
 // orbits {A >= B >= C} of the nt^3 tiles as (A nt + B) nt + C
@@ -620,15 +655,18 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
   const bool tiled = v % 8 == 0;
   const int64_t ldy = tiled ? v : (v + 3) / 4 * 4;
   const int64_t nA = (int64_t)i1 * vv * Kp, nBr = 3 * KC * v, np = x3_partials();
-  CCX_CHECK(vv * ldy * KC < (int64_t(1) << 40) && ceil_div(vv, 256) < (int64_t(1) << 24), ERR_ARG,
+  CCX_CHECK(vv * ldy * KC < (int64_t(1) << 40) && ceil_div(vv, 256) < (int64_t(1) << 24) && nBr < 65536, ERR_ARG,
             "too many virtual orbitals for the (T) scratch");
   DevBuf<f16> Ahl(nA * 2);   // A split, [i1 v^2][2 Kp]
-  DevBuf<float> sc(16 + 2 * np);   // sc[0] = scale of A, sc[1] = scale of the chunk's B, then partials
+  DevBuf<float> sc(16 + 3 * np);   // sc[0] = scale of A, sc[1] = scale of B, then partials
   // The entries of A over every p are exactly those of Covvv and of t2 (plus zero padding), so its
-  // scale is that of their union: the same for every range of i, whatever rows are built.
+  // scale is that of their union: the same for every range of i, whatever rows are built. Those
+  // of every chunk's B rows are entries of t2 and of -Cooov: B takes the scale of that union.
   x3_absmax<double>(c.Covvv, o * vv * v, sc.p + 16, s);
   x3_absmax<double>(t2, o * o * vv, sc.p + 16 + np, s);
+  x3_absmax<double>(c.Cooov, o * o * o * v, sc.p + 16 + 2 * np, s);
   x3_scale_union(sc.p + 16, (int)(2 * np), sc.p, s);
+  x3_scale_union(sc.p + 16 + np, (int)(2 * np), sc.p + 1, s);
   if (nA > 0) {   // A split; its FP64 form lives only until then
     DevBuf<double> A64(nA);
     t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv, t2,
@@ -641,11 +679,10 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
   const int64_t norb = (int64_t)orb.size();
   DevBuf<int> dorb(norb);
   CCX_HIP(hipMemcpy(dorb.p, orb.data(), (size_t)norb * sizeof(int), hipMemcpyHostToDevice));
-  DevBuf<double> Bp(nBr * L);        // K-concatenated B rows of a chunk
-  DevBuf<f16> Bhl(nBr * 2 * L);      // the same split, rows of 2 L halves
+  DevBuf<f16> Bhl(nBr * 2 * L);      // the K-concatenated B rows of a chunk, split: rows of 2 L halves
   const int64_t ys = vv * ldy;          // one triple's Y block (see t3_orbit_energy_kernel)
   DevBuf<float> Y(3 * KC * ys);        // Y1, Y2, Y3 of the chunk's triples
-  DevBuf<double> part(KC * norb);
+  DevBuf<double> part(KC * norb + T3_NSUM);   // per (orbit, triple), then the chunk's T3_NSUM sums
   DevBuf<double> PB((1 + 2 * KC) * nt * nt * 64 * 2);   // the chunk's disconnected blocks
   float *Y1 = Y.p, *Y2 = Y1 + KC * ys, *Y3 = Y2 + KC * ys;
   if (!accumulate) {
@@ -658,12 +695,9 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
       for (int k0 = 0; k0 <= j; k0 += (int)KC) {
         const int nk = (int)std::min<int64_t>(KC, j + 1 - k0);
         const int64_t nrows = 3 * (int64_t)nk * v;
-        t3_pack_kcat_kernel<<<(int)std::min<int64_t>(4096, ceil_div(nrows * L, 256)), 256, 0, s>>>(
-            o, v, Kp, i, j, k0, nk, t2, c.Cooov, Bp.p);
+        t3_pack_split_kernel<<<dim3((unsigned)ceil_div(L, 256), (unsigned)nrows), 256, 0, s>>>(
+            o, v, Kp, i, j, k0, t2, c.Cooov, sc.p + 1, Bhl.p);
         CCX_HIP(hipGetLastError());
-        x3_absmax<double>(Bp.p, nrows * L, sc.p + 16, s);
-        x3_scale(sc.p + 16, 1.f, nullptr, 0.f, sc.p + 1, s);
-        x3_split_rows<double>(Bp.p, nrows, L, L, sc.p + 1, Bhl.p, Bhl.p + 32, L, s, true);
         X3Gemm g;   // C_e[m][n] = sum_kk [A | A2_e swapped][m][kk] B_e[n][kk]
         g.M = vv; g.N = v; g.Kp = 2 * Kp; g.K1 = Kp; g.lda = 2 * Kp; g.ldb = 2 * L; g.ldc = ldy;
         g.kb = 32; g.ska = 64; g.skb = 64; g.sw = v; g.nbatch = nk; g.ytile = tiled ? v : 0;
@@ -693,7 +727,9 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
           t3_orbit_energy_kernel<false><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
               o, v, ldy, i, j, k0, nk, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, PB.p, part.p);
         CCX_HIP(hipGetLastError());
-        t3_accumulate_chunk_kernel<<<1, 256, 0, s>>>(part.p, norb, nk, i, j, k0, e_dev);
+        t3_chunk_sums_kernel<<<T3_NSUM, 256, 0, s>>>(part.p, norb, nk, i, j, k0, part.p + KC * norb);
+        CCX_HIP(hipGetLastError());
+        t3_accumulate_chunk_kernel<<<1, T3_NSUM, 0, s>>>(part.p + KC * norb, e_dev);
         CCX_HIP(hipGetLastError());
       }
   CCX_HIP(hipStreamSynchronize(s));
