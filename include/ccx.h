@@ -164,7 +164,7 @@ int ccx_density_energy(ccx_ctx* ctx, const double* t1, const double* t2, const d
  * whose slot products are FP16x3 split products with FP32 accumulation and output (E(T) within
  * ~1e-9 relative of FP64 for tens of virtuals and more, ~5e-8 for two or three). Synchronises
  * `stream`. Device scratch for the duration of the call:
- * o v^2 (v + o) + 25 v^3 doubles in FP64; in CCX_MODE_MIXED_F16X3 o v^2 (v + o) doubles while the
+ * o v^2 (v + o) + 24 v^3 doubles in FP64; in CCX_MODE_MIXED_F16X3 o v^2 (v + o) doubles while the
  * operand is split (half that afterwards, as FP16 hi/lo) + 12 v^3 + 34 v^2 doubles. */
 int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, double* e_dev,
                 void* stream);

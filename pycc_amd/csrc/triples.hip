@@ -1,14 +1,13 @@
 // (T) correction of closed-shell CCSD(T) (pycc/cctriples.py:177-240, t_tjl): FP64, except in the
-// MIXED_F16X3 mode, whose slot GEMMs are FP16x3 with FP32 output and which sums the energy of each
-// orbit of tiles from LDS, never storing W (triples_energy_x3 below).
+// MIXED_F16X3 mode, whose slot GEMMs are FP16x3 with FP32 output (triples_energy_x3 below).
 //
 // For every occupied triple i >= j >= k the connected numerator W[a][b][c] (cctriples.py:27-72,
 // t3c_ijk without the denominator) comes from GEMMs against A_p = [Covvv[p] | t2[p]^T] (built once
 // per call): the six slot products, each holding two of the twelve t3c_ijk terms, pair up along K
 // into three GEMMs per triple (Y1, Y2, Y3 below), batched over up to T3_KC values of k for fixed
-// (i, j); a tiled kernel adds their three v^3 blocks into W's [a][b][c] order through 16^3 LDS
-// tiles. A tiled kernel then adds the disconnected part (t3d_ijk,
-// cctriples.py:108-146: <ij|ab> t1_kc + t2_ijab f_kc and their permutations) to form V and reduces
+// (i, j). One kernel per chunk then forms W from the three blocks, orbit by orbit of 8^3 tiles, in
+// LDS (W is never stored), adds the disconnected part (t3d_ijk, cctriples.py:108-146:
+// <ij|ab> t1_kc + t2_ijab f_kc and their permutations) to form V and reduces
 //
 //   E_ijk = sum_abc V_abc (4 W_abc + W_bca + W_cab - 2 W_acb - 2 W_bac - 2 W_cba) / D_ijk^abc,
 //   E(T)  = sum_{i>=j>=k} (2 - d_ij - d_ik - d_jk) E_ijk,
@@ -43,51 +42,6 @@ __device__ __forceinline__ double t3_block_sum(double x) {   // blockDim.x <= 10
 
 __global__ void t3_zero_kernel(double* e) {
   if (threadIdx.x == 0) e[0] = 0.0;
-}
-
-// ---- 16^3 tiles of v^3 arrays: output (a, b, c) in [org_a, org_a + 16) x ... ----------------------
-// A thread owns (b_l, c_l) = (tid / 16, tid % 16) and the 16 elements a_l = 0..15 of its column.
-constexpr int TT = 16, TP = TT + 1;   // tile edge; LDS row pitch (padded against bank conflicts)
-
-// Staging of one 16^3 box of a source array S[g0][g1][off + g2] whose dimension k is indexed by the
-// output dimension Dk (0 = a, 1 = b, 2 = c). A thread loads the 16 elements (l0 = 0..15, l1 = tid / 16,
-// l2 = tid % 16) -- 128-byte rows along the source's contiguous dimension -- into registers; they go
-// to LDS as sh[l0][l1][l2] and are read back in the output's order. Loads of the next box are issued
-// before the current box is read back, so they overlap the LDS work.
-template <int D0, int D1, int D2>
-__device__ __forceinline__ void t3_load(const double* __restrict__ S, int64_t s0, int64_t s1, int64_t off,
-                                        int64_t v, const int64_t* org, double* r) {
-  const int l1 = threadIdx.x / TT, l2 = threadIdx.x % TT;
-  const int64_t g1 = org[D1] + l1, g2 = org[D2] + l2;
-  const bool in = g1 < v && g2 < v;
-  const double* p = S + org[D0] * s0 + g1 * s1 + off + g2;
-  const int64_t n0 = v - org[D0];
-#pragma unroll
-  for (int l0 = 0; l0 < TT; ++l0) r[l0] = (in && l0 < n0) ? p[l0 * s0] : 0.0;
-}
-
-__device__ __forceinline__ void t3_store(double (*sh)[TT][TP], const double* r) {
-  const int l1 = threadIdx.x / TT, l2 = threadIdx.x % TT;
-#pragma unroll
-  for (int l0 = 0; l0 < TT; ++l0) sh[l0][l1][l2] = r[l0];
-}
-
-// acc[a_l] += coef * (staged box at the output position (a_l, b_l, c_l))
-template <int D0, int D1, int D2>
-__device__ __forceinline__ void t3_accumulate(double (*sh)[TT][TP], double coef, double* acc) {
-  const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
-#pragma unroll
-  for (int al = 0; al < TT; ++al) {
-    const int L[3] = {al, bl, cl};
-    acc[al] += coef * sh[L[D0]][L[D1]][L[D2]];
-  }
-}
-
-__device__ __forceinline__ void t3_tile_origin(int64_t v, int64_t* org, int64_t t = blockIdx.x) {
-  const int64_t nt = (v + TT - 1) / TT;
-  org[0] = t / (nt * nt) * TT;
-  org[1] = t / nt % nt * TT;
-  org[2] = t % nt * TT;
 }
 
 // A[p][(x, y)][kk] = Covvv[p][x][y][kk] for kk < v, t2[p][kk - v][x][y] for v <= kk < v + o, 0 above:
@@ -125,117 +79,6 @@ struct DevBuf {
 
 // k values per chunk of a fixed (i, j)
 constexpr int T3_KC = 8;
-
-// W[a][b][c] = Y1[a][b][c] + Y2[a][c][b] + Y3[b][c][a] (the K-paired slot products below, FP64,
-// rows [x][y] of v elements)
-__global__ __launch_bounds__(256) void t3_assemble3_kernel(int64_t v, const double* __restrict__ Y1,
-                                                           const double* __restrict__ Y2,
-                                                           const double* __restrict__ Y3, double* __restrict__ W) {
-  __shared__ double sh[TT][TT][TP];
-  int64_t org[3];
-  t3_tile_origin(v, org);
-  const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
-  const int64_t vv = v * v;
-  double acc[TT], r[TT];
-#pragma unroll
-  for (int al = 0; al < TT; ++al) acc[al] = 0.0;
-  t3_load<0, 1, 2>(Y1, vv, v, 0, v, org, r);
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<0, 2, 1>(Y2, vv, v, 0, v, org, r);
-  t3_accumulate<0, 1, 2>(sh, 1.0, acc);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<1, 2, 0>(Y3, vv, v, 0, v, org, r);
-  t3_accumulate<0, 2, 1>(sh, 1.0, acc);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_accumulate<1, 2, 0>(sh, 1.0, acc);
-  const int64_t b = org[1] + bl, c = org[2] + cl;
-  if (b < v && c < v)
-    for (int al = 0; al < TT; ++al) {
-      const int64_t a = org[0] + al;
-      if (a < v) W[(a * v + b) * v + c] = acc[al];
-    }
-}
-
-// partials[block] = sum over the block's tile of V_abc (4 W_abc + W_bca + W_cab - 2 (W_acb + W_bac +
-// W_cba)) / D_ijk^abc with V = W + the disconnected numerator of t3d_ijk
-__global__ __launch_bounds__(256) void t3_energy_kernel(int64_t o, int64_t v, int i, int j, int k,
-                                                        const double* __restrict__ W,
-                                                        const double* __restrict__ F, int64_t nmo,
-                                                        const double* __restrict__ t1,
-                                                        const double* __restrict__ t2,
-                                                        const double* __restrict__ Ioovv,
-                                                        const int* __restrict__ tile_of_block,
-                                                        double* __restrict__ partials) {
-  __shared__ double sh[TT][TT][TP];
-  int64_t org[3];
-  t3_tile_origin(v, org, tile_of_block[blockIdx.x]);
-  const int bl = threadIdx.x / TT, cl = threadIdx.x % TT;
-  double q[TT], w[TT], r[TT];
-#pragma unroll
-  for (int al = 0; al < TT; ++al) q[al] = w[al] = 0.0;
-  const int64_t vv = v * v;
-  t3_load<0, 1, 2>(W, vv, v, 0, v, org, r);     // W_abc
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<1, 2, 0>(W, vv, v, 0, v, org, r);     // W_bca = W[b][c][a]
-  t3_accumulate<0, 1, 2>(sh, 1.0, w);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<2, 0, 1>(W, vv, v, 0, v, org, r);     // W_cab
-  t3_accumulate<1, 2, 0>(sh, 1.0, q);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<0, 2, 1>(W, vv, v, 0, v, org, r);     // W_acb
-  t3_accumulate<2, 0, 1>(sh, 1.0, q);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<1, 0, 2>(W, vv, v, 0, v, org, r);     // W_bac
-  t3_accumulate<0, 2, 1>(sh, -2.0, q);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_load<2, 1, 0>(W, vv, v, 0, v, org, r);     // W_cba
-  t3_accumulate<1, 0, 2>(sh, -2.0, q);
-  __syncthreads();
-  t3_store(sh, r);
-  __syncthreads();
-  t3_accumulate<2, 1, 0>(sh, -2.0, q);
-  const int64_t b = org[1] + bl, c = org[2] + cl;
-  double s = 0.0;
-  if (b < v && c < v) {
-    const double* Iij = Ioovv + ((int64_t)i * o + j) * vv;
-    const double* Iik = Ioovv + ((int64_t)i * o + k) * vv;
-    const double* Ijk = Ioovv + ((int64_t)j * o + k) * vv;
-    const double* Tij = t2 + ((int64_t)i * o + j) * vv;
-    const double* Tik = t2 + ((int64_t)i * o + k) * vv;
-    const double* Tjk = t2 + ((int64_t)j * o + k) * vv;
-    const double* Fi = F + (int64_t)i * nmo + o;   // f_ia row
-    const double* Fj = F + (int64_t)j * nmo + o;
-    const double* Fk = F + (int64_t)k * nmo + o;
-    const int64_t ob = o + b, oc = o + c;
-    const double dbc = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k] - F[ob * nmo + ob] - F[oc * nmo + oc];
-    const double ijbc = Ijk[b * v + c], tjbc = Tjk[b * v + c];
-    const double t1kc = t1[k * v + c], t1jb = t1[j * v + b], fkc = Fk[c], fjb = Fj[b];
-    for (int al = 0; al < TT; ++al) {
-      const int64_t a = org[0] + al;
-      if (a >= v) break;
-      const double vabc = w[al] + Iij[a * v + b] * t1kc + Iik[a * v + c] * t1jb + ijbc * t1[i * v + a] +
-                          Tij[a * v + b] * fkc + Tik[a * v + c] * fjb + tjbc * Fi[a];
-      const int64_t oa = o + a;
-      s += vabc * (4.0 * w[al] + q[al]) / (dbc - F[oa * nmo + oa]);
-    }
-  }
-  s = t3_block_sum(s);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
-}
 
 // ---- slot products paired along K ----------------------------------------------------------------
 // The slot products X_pqr[x][y][z] = sum_kk A_p[(x, y)][kk] B_(q,r)[z][kk] hold the twelve t3c_ijk
@@ -302,7 +145,7 @@ __global__ __launch_bounds__(256) void t3_pack_split_kernel(int64_t o, int64_t v
 
 // One work-group per (orbit {A >= B >= C} of 8^3 tiles, triple k0 + blockIdx.y): W of the orbit's
 // distinct tiles (one, three or six permutations of (A, B, C)) from Y1, Y2, Y3 into LDS (FP32),
-// then the energy sum of t3_energy_kernel over those tiles, every permuted W read from LDS. HBM
+// then the energy sum over those tiles, every permuted W read from LDS. HBM
 // traffic is Y read once; W is never written. A work-group needs 36 KiB of LDS, so that four are
 // resident on a CU and the loads of one overlap the sums of the others.
 constexpr int OE = 8;                // tile edge
@@ -336,23 +179,28 @@ __device__ __forceinline__ int t3_slot_tile(int x, int n, int A, int B, int C) {
   return src == 0 ? A : (src == 1 ? B : C);
 }
 
-// 1/x: the hardware estimate (relative error below 2^-23) and one Newton step (below 2^-45)
+// 1/x: the hardware estimate (relative error below 2^-23) and NS Newton steps (one: below 2^-45)
+template <int NS>
 __device__ __forceinline__ double t3_rcp(double x) {
-  const double r = __builtin_amdgcn_rcp(x);
-  return fma(fma(-x, r, 1.0), r, r);
+  double r = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int n = 0; n < NS; ++n) r = fma(fma(-x, r, 1.0), r, r);
+  return r;
 }
 
-// FULL: Y in the tiled layout of x3_gemm_kcat's ytile (v a multiple of 8, every tile whole); else
-// Y is [x][y][z] with rows of ldy floats and the tiles at the edge are partial.
+// TY: the type of Y and of W in LDS (float in MIXED_F16X3, double in the FP64 path). FULL (float
+// only): Y in the tiled layout of x3_gemm_kcat's ytile (v a multiple of 8, every tile whole); else
+// Y is [x][y][z] with rows of ldy elements and the tiles at the edge are partial.
 // The disconnected blocks come from PB (t3_pair_blocks_kernel) and t1, F. One work-group per (orbit,
 // triple) pair t = orbit + norb e of the 1-D grid, taken in an order that gives each XCD a
 // contiguous range of t (consecutive orbits share tiles, so their blocks meet in that XCD's L2).
-template <bool FULL>
+template <typename TY, bool FULL>
 __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
-    int64_t o, int64_t v, int64_t ldy, int i, int j, int k0, int nk, const float* __restrict__ Y1,
-    const float* __restrict__ Y2, const float* __restrict__ Y3, const int* __restrict__ orbits, int64_t norb,
+    int64_t o, int64_t v, int64_t ldy, int i, int j, int k0, int nk, const TY* __restrict__ Y1,
+    const TY* __restrict__ Y2, const TY* __restrict__ Y3, const int* __restrict__ orbits, int64_t norb,
     const double* __restrict__ F, int64_t nmo, const double* __restrict__ t1, const double* __restrict__ PB,
     double* __restrict__ partials) {
+  static_assert(!FULL || sizeof(TY) == 4, "the tiled Y layout is FP32");
   // for the tile (P0, P1, P2) of slot x, the slots of (P1, P2, P0), (P2, P0, P1), (P0, P2, P1),
   // (P1, P0, P2) and (P2, P1, P0)
   constexpr int t3_cyc[6] = {3, 5, 1, 4, 0, 2};
@@ -360,9 +208,9 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   constexpr int t3_sw23[6] = {1, 0, 3, 2, 5, 4};
   constexpr int t3_sw12[6] = {2, 4, 0, 5, 1, 3};
   constexpr int t3_rev[6] = {5, 3, 4, 1, 2, 0};
-  typedef float f4v __attribute__((ext_vector_type(4)));
+  typedef TY f4v __attribute__((ext_vector_type(4)));
   typedef double d2v __attribute__((ext_vector_type(2)));
-  __shared__ float sW[6 * OTW];
+  __shared__ TY sW[6 * OTW];
   // the disconnected factors of the tiles, in pairs: sP[x][0][a][b] = (<ij|ab>, t2_ijab),
   // sP[x][1][a][c] = (<ik|ac>, t2_ikac), sP[x][2][b][c] = (<jk|bc>, t2_jkbc); sTF[n][s][.] = (t1,
   // f row) of the occupied orbital (i, j, k)[n] over tile (A, B, C)[s] and sE[s][.] = eps over it
@@ -396,21 +244,21 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   int kept = 0;
 #pragma unroll
   for (int x = 0; x < 6; ++x) kept |= (rep[x] == x) << x;
-  const float* y1 = Y1 + e * ys;
-  const float* y2 = Y2 + e * ys;
-  const float* y3 = Y3 + e * ys;
+  const TY* y1 = Y1 + e * ys;
+  const TY* y2 = Y2 + e * ys;
+  const TY* y3 = Y3 + e * ys;
   // Box loads: the threads of half hf = t / 128 take the boxes of the slots 2 p + hf; thread (x, z,
   // yg) = (t / 16 % 8, t / 2 % 8, t % 2) the four elements y = 4 yg + u, u = 0..3, of the row x and
   // column z (in the tiled layout the 16 bytes at 32 z + 16 yg of the box's 256-byte run for x).
   const int hf = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);   // wave-uniform
   const int bx = (threadIdx.x >> 4) & 7, bz = (threadIdx.x >> 1) & 7, yg = threadIdx.x & 1;
   const int lbox = bx * (int)vv + 8 * bz + 4 * yg;   // the lane's offset in a whole box (v^2 < 2^28)
-  auto box = [&](const float* Y, int gx, int gy, int gz) -> f4v {   // tile origins in tiles
+  auto box = [&](const TY* Y, int gx, int gy, int gz) -> f4v {   // tile origins in tiles
     if constexpr (FULL) {
       return *(const f4v*)(Y + ((OE * (int64_t)gx * v + OE * gy) * v + 64 * gz) + lbox);
     } else {
       const int64_t x = OE * gx + bx, z = OE * gz + bz;
-      f4v val = f4v{0.f, 0.f, 0.f, 0.f};
+      f4v val = f4v{0, 0, 0, 0};
       if (x < v && z < v) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -472,7 +320,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   for (int p = 0; p < 3; ++p) {
     const int x = 2 * p + hf;
     if ((kept >> x) & 1) {
-      float* w = sW + x * OTW;
+      TY* w = sW + x * OTW;
 #pragma unroll
       for (int u = 0; u < 4; ++u) w[a1 ^ t3_ay(u)] = ra[p][u];
     }
@@ -492,7 +340,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   for (int p = 0; p < 3; ++p) {
     const int x = 2 * p + hf;
     if ((kept >> x) & 1) {
-      float* w = sW + x * OTW;
+      TY* w = sW + x * OTW;
 #pragma unroll
       for (int u = 0; u < 4; ++u) w[a2 ^ t3_az(u)] += rb[p][u];
     }
@@ -502,7 +350,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   for (int p = 0; p < 3; ++p) {
     const int x = 2 * p + hf;
     if ((kept >> x) & 1) {
-      float* w = sW + x * OTW;
+      TY* w = sW + x * OTW;
 #pragma unroll
       for (int u = 0; u < 4; ++u) w[a3 ^ t3_az(u)] += rc[p][u];
     }
@@ -519,7 +367,7 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
   const int e3 = t3_ax(al0) ^ t3_ay(cl) ^ t3_az(bl);   // (a, c, b)
   const int e4 = t3_ax(bl) ^ t3_ay(al0) ^ t3_az(cl);   // (b, a, c)
   const int e5 = t3_ax(cl) ^ t3_ay(bl) ^ t3_az(al0);   // (c, b, a)
-  auto rd = [&](int off) { return (double)*(const float*)((const char*)sW + off); };
+  auto rd = [&](int off) { return (double)*(const TY*)((const char*)sW + off); };
   const double dijk = F[i * nmo + i] + F[j * nmo + j] + F[k * nmo + k];
   double s = 0.0;
 #pragma unroll
@@ -527,10 +375,12 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
     if (rep[x] != x) continue;   // block-uniform
     const int64_t a0 = OE * (int64_t)T[x][0], b = OE * (int64_t)T[x][1] + bl, c = OE * (int64_t)T[x][2] + cl;
     if (FULL || (b < v && c < v)) {
-      // byte offsets in sW of the six permuted reads: slot (bits 11 and up) | word address * 4
-      const int q0 = (x * OTW + e0) * 4, q1 = (rep[t3_cyc[x]] * OTW + e1) * 4;
-      const int q2 = (rep[t3_cyc2[x]] * OTW + e2) * 4, q3 = (rep[t3_sw23[x]] * OTW + e3) * 4;
-      const int q4 = (rep[t3_sw12[x]] * OTW + e4) * 4, q5 = (rep[t3_rev[x]] * OTW + e5) * 4;
+      // byte offsets in sW of the six permuted reads: slot (the bits above the tile) | element
+      // address * sizeof(TY)
+      constexpr int SZ = (int)sizeof(TY);
+      const int q0 = (x * OTW + e0) * SZ, q1 = (rep[t3_cyc[x]] * OTW + e1) * SZ;
+      const int q2 = (rep[t3_cyc2[x]] * OTW + e2) * SZ, q3 = (rep[t3_sw23[x]] * OTW + e3) * SZ;
+      const int q4 = (rep[t3_sw12[x]] * OTW + e4) * SZ, q5 = (rep[t3_rev[x]] * OTW + e5) * SZ;
       const int s0 = t3_slot_src(x, 0), s1 = t3_slot_src(x, 1), s2 = t3_slot_src(x, 2);
       const double dbc = dijk - sE[s1][bl] - sE[s2][cl];
       const d2v pbc = sP[x][2][bl][cl], tfb = sTF[1][s1][bl], tfc = sTF[2][s2][cl];
@@ -539,13 +389,14 @@ __global__ __launch_bounds__(OT_THREADS) void t3_orbit_energy_kernel(
         const int al = al0 + 4 * ii;
         if (!FULL && a0 + al >= v) break;   // wave-uniform
         // 4 W_abc + W_bca + W_cab - 2 (W_acb + W_bac + W_cba)
-        const double w = rd(q0 ^ 4 * t3_ax(4 * ii));
-        const double z = fma(-2.0, rd(q3 ^ 4 * t3_ax(4 * ii)) + rd(q4 ^ 4 * t3_ay(4 * ii)) + rd(q5 ^ 4 * t3_az(4 * ii)),
-                             fma(4.0, w, rd(q1 ^ 4 * t3_az(4 * ii)) + rd(q2 ^ 4 * t3_ay(4 * ii))));
+        const double w = rd(q0 ^ SZ * t3_ax(4 * ii));
+        const double z = fma(-2.0, rd(q3 ^ SZ * t3_ax(4 * ii)) + rd(q4 ^ SZ * t3_ay(4 * ii)) + rd(q5 ^ SZ * t3_az(4 * ii)),
+                             fma(4.0, w, rd(q1 ^ SZ * t3_az(4 * ii)) + rd(q2 ^ SZ * t3_ay(4 * ii))));
         const d2v pab = sP[x][0][al][bl], pac = sP[x][1][al][cl], tfa = sTF[0][s0][al];
         const double vabc = w + pab.x * tfc.x + pac.x * tfb.x + pbc.x * tfa.x + pab.y * tfc.y + pac.y * tfb.y +
                             pbc.y * tfa.y;
-        s += vabc * z * t3_rcp(dbc - sE[s0][al]);
+        // FP32 W: one Newton step (relative error below 2^-45); FP64: two (to the last bits)
+        s += vabc * z * t3_rcp<sizeof(TY) == 8 ? 2 : 1>(dbc - sE[s0][al]);
       }
     }
   }
@@ -608,38 +459,6 @@ std::vector<int> t3_orbits(int64_t nt) {
     for (int64_t B = 0; B <= A; ++B)
       for (int64_t C = 0; C <= B; ++C) out.push_back((int)((A * nt + B) * nt + C));
   return out;
-}
-
-// Work-group -> tile table of the energy kernel. The energy of a tile (A, B, C) reads W at the six
-// permutations of (A, B, C), so the tiles of one orbit {(A, B, C) permuted} are given to consecutive
-// work-groups of one XCD (work-group w runs on XCD w mod 8): the orbit's W tiles are fetched from HBM
-// about once and the other members read them from that XCD's L2.
-std::vector<int> t3_orbit_order(int64_t nt) {
-  std::vector<int> seq;
-  seq.reserve((size_t)(nt * nt * nt));
-  for (int64_t A = 0; A < nt; ++A)
-    for (int64_t B = 0; B <= A; ++B)
-      for (int64_t C = 0; C <= B; ++C) {
-        const int64_t P[6][3] = {{A, B, C}, {A, C, B}, {B, A, C}, {B, C, A}, {C, A, B}, {C, B, A}};
-        int got[6], n = 0;
-        for (const auto& p : P) {
-          const int t = (int)((p[0] * nt + p[1]) * nt + p[2]);
-          if (std::find(got, got + n, t) == got + n) got[n++] = t;
-        }
-        seq.insert(seq.end(), got, got + n);
-      }
-  const int64_t n = (int64_t)seq.size(), q = n / 8;
-  std::vector<int> out((size_t)n);
-  for (int64_t w = 0; w < n; ++w) out[(size_t)w] = w < 8 * q ? seq[(size_t)((w % 8) * q + w / 8)] : seq[(size_t)w];
-  return out;
-}
-
-// e[0] += weight * sum(partials[0 .. n)), in a fixed order
-__global__ void t3_accumulate_kernel(const double* __restrict__ partials, int64_t n, double weight, double* e) {
-  double s = 0.0;
-  for (int64_t b = threadIdx.x; b < n; b += blockDim.x) s += partials[b];
-  s = t3_block_sum(s);
-  if (threadIdx.x == 0) e[0] += weight * s;
 }
 
 }  // namespace
@@ -721,10 +540,10 @@ static void triples_energy_x3(Ctx& c, const double* F, const double* t1, const d
           CCX_HIP(hipGetLastError());
         }
         if (tiled)
-          t3_orbit_energy_kernel<true><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
+          t3_orbit_energy_kernel<float, true><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
               o, v, ldy, i, j, k0, nk, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, PB.p, part.p);
         else
-          t3_orbit_energy_kernel<false><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
+          t3_orbit_energy_kernel<float, false><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
               o, v, ldy, i, j, k0, nk, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, PB.p, part.p);
         CCX_HIP(hipGetLastError());
         t3_chunk_sums_kernel<<<T3_NSUM, 256, 0, s>>>(part.p, norb, nk, i, j, k0, part.p + KC * norb);
@@ -742,8 +561,9 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   CCX_CHECK(0 <= i0 && i0 <= i1 && i1 <= c.o, ERR_ARG, "triples: need 0 <= i0 <= i1 <= no");
   const int64_t o = c.o, v = c.v, nmo = c.nmo, vv = v * v, vvv = vv * v;
   CCX_CHECK(c.Covvv && c.Cooov && c.Ioovv, ERR_STATE, "the context holds no FP64 <ov|vv>/<oo|ov> blocks");
-  const int64_t nt = ceil_div(v, TT), tiles = nt * nt * nt;
-  CCX_CHECK(tiles < (int64_t(1) << 31), ERR_ARG, "too many virtual orbitals for the (T) tiles");
+  const int64_t nt8 = ceil_div(v, OE);
+  CCX_CHECK(nt8 * nt8 * nt8 < (int64_t(1) << 31) && (nt8 + 2) * (nt8 + 1) * nt8 / 6 * T3_KC < (int64_t(1) << 31),
+            ERR_ARG, "too many virtual orbitals for the (T) tiles");
   // In MIXED_F16X3 the slot GEMMs are FP16x3 split products (interleaved hi/lo operands, K padded
   // to 32, FP32 accumulation and output, two slot products per GEMM); every other mode runs them
   // in FP64.
@@ -758,20 +578,22 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   }
   // For fixed (i, j) the k values run in chunks of KC. The six slot products pair up along K into
   // the three products Y1, Y2, Y3 of triples_energy_x3 (FP64 here); per chunk they are three
-  // batched GEMMs (plain_gemm_kcat, a batch entry per k), then per triple W is assembled from them
-  // and its energy summed. Scratch: Y1, Y2, Y3 (KC v^3 each), W (v^3), the chunk's B rows
-  // (3 KC v x 2 Kp), tile partials and A (i1 v^2 x Kp).
-  // (every region starts on a 256-byte boundary: the GEMM loads its operands 16 bytes at a time)
+  // batched GEMMs (plain_gemm_kcat, a batch entry per k), then the chunk's disconnected blocks and
+  // one orbit-energy launch (W formed in LDS from Y, never stored). Scratch: Y1, Y2, Y3 (KC v^3
+  // each), the chunk's B rows (3 KC v x 2 Kp), its disconnected blocks, the orbit partials and A
+  // (i1 v^2 x Kp). (every region starts on a 256-byte boundary: the GEMM loads its operands 16
+  // bytes at a time)
   auto r32 = [](int64_t n) { return (n + 31) / 32 * 32; };
-  const int64_t L = 2 * Kp, nY = r32(KC * vvv), nW = r32(vvv), nB = r32(3 * KC * v * L), nP = r32(tiles);
-  DevBuf<int> order(tiles);
-  {
-    const std::vector<int> h = t3_orbit_order(nt);
-    CCX_HIP(hipMemcpy(order.p, h.data(), (size_t)tiles * sizeof(int), hipMemcpyHostToDevice));
-  }
-  DevBuf<double> buf(3 * nY + nW + nB + nP + nA);
-  double *Y1 = buf.p, *Y2 = Y1 + nY, *Y3 = Y2 + nY, *W = Y3 + nY, *B = W + nW, *partials = B + nB;
-  double* A = partials + nP;
+  const int64_t ot = ceil_div(v, OE);
+  const std::vector<int> orb = t3_orbits(ot);
+  const int64_t norb = (int64_t)orb.size();
+  DevBuf<int> dorb(norb);
+  CCX_HIP(hipMemcpy(dorb.p, orb.data(), (size_t)norb * sizeof(int), hipMemcpyHostToDevice));
+  const int64_t L = 2 * Kp, nY = r32(KC * vvv), nB = r32(3 * KC * v * L), nP = r32(KC * norb + T3_NSUM);
+  const int64_t nPB = r32((1 + 2 * KC) * ot * ot * 64 * 2);
+  DevBuf<double> buf(3 * nY + nB + nP + nPB + nA);
+  double *Y1 = buf.p, *Y2 = Y1 + nY, *Y3 = Y2 + nY, *B = Y3 + nY, *part = B + nB, *PB = part + nP;
+  double* A = PB + nPB;
   if (nA > 0) {
     t3_operand_a_kernel<<<(int)std::min<int64_t>(65536, ceil_div(nA, 256)), 256, 0, s>>>(o, v, Kp, i1, c.Covvv, t2, A);
     CCX_HIP(hipGetLastError());
@@ -799,17 +621,17 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
         plain_gemm_kcat(g, s);
         g.A = Ap(j); g.B = B + 2 * v * L; g.C = Y3;                                // [A_j | A_k^s]
         plain_gemm_kcat(g, s);
-        for (int e = 0; e < nk; ++e) {
-          const int k = k0 + e;
-          t3_assemble3_kernel<<<(int)tiles, 256, 0, s>>>(v, Y1 + e * vvv, Y2 + e * vvv, Y3 + e * vvv, W);
-          CCX_HIP(hipGetLastError());
-          t3_energy_kernel<<<(int)tiles, 256, 0, s>>>(o, v, i, j, k, W, F, nmo, t1, t2, c.Ioovv, order.p,
-                                                      partials);
-          CCX_HIP(hipGetLastError());
-          const double weight = 2.0 - (double)((i == j) + (i == k) + (j == k));   // cctriples.py:237
-          t3_accumulate_kernel<<<1, 256, 0, s>>>(partials, tiles, weight, e_dev);
-          CCX_HIP(hipGetLastError());
-        }
+        const int64_t npb = (1 + 2 * (int64_t)nk) * ot * ot * 64;
+        t3_pair_blocks_kernel<<<(int)std::min<int64_t>(8192, ceil_div(npb, 256)), 256, 0, s>>>(
+            o, v, ot, i, j, k0, nk, c.Ioovv, t2, PB);
+        CCX_HIP(hipGetLastError());
+        t3_orbit_energy_kernel<double, false><<<(unsigned)(norb * nk), OT_THREADS, 0, s>>>(
+            o, v, v, i, j, k0, nk, Y1, Y2, Y3, dorb.p, norb, F, nmo, t1, PB, part);
+        CCX_HIP(hipGetLastError());
+        t3_chunk_sums_kernel<<<T3_NSUM, 256, 0, s>>>(part, norb, nk, i, j, k0, part + KC * norb);
+        CCX_HIP(hipGetLastError());
+        t3_accumulate_chunk_kernel<<<1, T3_NSUM, 0, s>>>(part + KC * norb, e_dev);
+        CCX_HIP(hipGetLastError());
       }
   CCX_HIP(hipStreamSynchronize(s));
 }
