@@ -88,9 +88,9 @@ def test_triples_matches_oracle(no, nv, seed):
 # o >= 9: several k chunks per (i, j); v = 50: 4^3 tiles of 16 (orbits of one, three and six tiles,
 # the last tile partly past v); v = 3, 2: one partial tile, rows shorter than a 4-float vector;
 # v = 8, 24, 48: whole 8^3 tiles, Y in the GEMM's 8 x 8-block layout (the path of v a multiple of
-# 8; one tile, then orbits of one, three and six tiles)
+# 8; one tile, then orbits of one, three and six tiles); v = 9, 17: a last tile of one element
 @pytest.mark.parametrize("no,nv,seed", [(4, 21, 7), (9, 13, 8), (6, 50, 10), (2, 3, 11), (3, 2, 12),
-                                        (4, 48, 13), (3, 8, 14), (10, 24, 15)])
+                                        (4, 48, 13), (3, 8, 14), (10, 24, 15), (5, 9, 16), (3, 17, 17)])
 def test_triples_density_fitted_and_mixed_contexts(no, nv, seed):
     _need_gpu()
     import pycc_amd
