@@ -26,7 +26,7 @@ class cchbar(object):
 
     def __init__(self, ccwfn, store_vvvv=False):
         time_init = time.time()
-        ccwfn._require_dp("cchbar")
+        ccwfn._require_dp("cchbar" + (" with the stored v^4 block" if store_vvvv else ""), mixed_ok=not store_vvvv)
         self.ccwfn = ccwfn
         self.contract = ccwfn.contract
         self.o, self.v = ccwfn.o, ccwfn.v

@@ -25,7 +25,7 @@ class cclambda(object):
     """Lambda amplitudes of a ``CCwfn`` on its HBAR (cclambda.py:42-200)."""
 
     def __init__(self, ccwfn, hbar):
-        ccwfn._require_dp("cclambda")
+        ccwfn._require_dp("cclambda", mixed_ok=True)
         self.ccwfn = ccwfn
         self.hbar = hbar
         self.contract = ccwfn.contract

@@ -269,10 +269,14 @@ class CCwfn(object):
                           self._stream())
 
     # ---- backward: HBAR, Lambda residual, vector-Jacobian product ----------------------------
-    def _require_dp(self, what):
-        if self.precision != 'DP':
-            raise PyCCError("%s runs in FP64: build the CCwfn with precision='DP' (this one is %s)"
-                            % (what, self.precision))
+    def _require_dp(self, what, mixed_ok=False):
+        """HBAR, the Lambda residual and the vJp also run in a MIXED_F16X3 context (``mixed_ok``;
+        their large contractions as FP16x3 split products, HBAR held in FP64, its v^4 block never
+        stored); the Jacobian and the densities run in a DP context only."""
+        if self.precision == 'DP' or (mixed_ok and self.precision == 'MIXED_F16X3'):
+            return
+        raise PyCCError("%s runs in FP64: build the CCwfn with precision='DP'%s (this one is %s)"
+                        % (what, " (or 'MIXED_F16X3')" if mixed_ok else "", self.precision))
 
     def _hbar_for(self, F, t1, t2, stored=False, owner=None):
         """Make the context's HBAR the one at (F, t1, t2), building it unless it already is.
@@ -280,7 +284,7 @@ class CCwfn(object):
         The context holds one HBAR. It is identified by the tensors it was built from (held here, so
         their memory cannot be reused) and their version counters, so an in-place change of t1, t2
         or F forces a rebuild. A stored HBAR also serves factored requests."""
-        self._require_dp("HBAR")
+        self._require_dp("HBAR" + (" with the stored v^4 block" if stored else ""), mixed_ok=not stored)
         key = tuple((x.data_ptr(), x._version, tuple(x.shape)) for x in (F, t1, t2))
         cur = getattr(self, "_hbar_src", None)
         # the owner (a cchbar, which refers back to this object) is held weakly: no reference cycle
@@ -327,6 +331,7 @@ class CCwfn(object):
         """(j1, j2) = the derivative of ``residuals(F, t1, t2)`` along (x1, x2): the CCSD Jacobian
         applied on HBAR (``CCderiv._ccsd_jacobian``, ccderiv.py:149-180, doubles symmetrised). Only the
         pair-symmetric part of x2 acts; F is held constant."""
+        self._require_dp("the Jacobian (jvp)")
         torch = self._torch
         F = self.H.F if F is None else self._as_dev(F, (self.nmo, self.nmo))
         t1 = self._as_dev(t1, (self.no, self.nv))

@@ -158,6 +158,10 @@ struct Ctx {
   void destroy();
 };
 
+// The packed ladder of the MIXED_F16X3 forward applied to l2 (tau = l2, (ab) columns only, FP32 into
+// CP / CM), for the Lambda residual (split16.hip).
+void ladder_x3_l2(Ctx& c, const double* l2, hipStream_t s);
+
 void set_integrals_dense(Ctx& c, const double* fock, const double* eri, hipStream_t s);
 void set_integrals_df(Ctx& c, const double* fock, const double* B, int64_t naux, hipStream_t s);
 void residual(Ctx& c, const double* fock, const double* t1, const double* t2, double* r1, double* r2,

@@ -33,8 +33,13 @@ TV tvs(const void* p, const char* lab, std::vector<int64_t> ext, std::vector<int
 // dimensions in storage order (e.g. a [m][e][n][f] buffer seen as "menf").
 inline TV relabel(TV v, const char* lab) { v.lab = lab; return v; }
 
+// split16: an FP64 contraction whose GEMM has at least X3_ROUTE_FLOPS flops (and no batch index,
+// and a full tile grid) runs as an FP16x3 split product (FP32 accumulation, about 2^-22 relative
+// error per product) instead of in FP64; smaller ones, and every FP32 one, stay as they are.
+constexpr double X3_ROUTE_FLOPS = 1e11;
 template <typename T>
-void contract(T alpha, const TV& A, const TV& B, T beta, const TV& C, Workspace& ws, hipStream_t s);
+void contract(T alpha, const TV& A, const TV& B, T beta, const TV& C, Workspace& ws, hipStream_t s,
+              bool split16 = false);
 
 // C = alpha * A + beta * C elementwise, with A's labels a permutation of C's (up to rank 6).
 template <typename T>

@@ -1,5 +1,7 @@
 // Einsum-style contraction planner and strided elementwise kernels (see contract.h).
 #include "contract.h"
+#include "split16.h"
+#include <type_traits>
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -115,7 +117,7 @@ TV slice(const TV& v, char c, int64_t i) {
 }  // namespace
 
 template <typename T>
-void contract(T alpha, const TV& A_, const TV& B_, T beta, const TV& C, Workspace& ws, hipStream_t s) {
+void contract(T alpha, const TV& A_, const TV& B_, T beta, const TV& C, Workspace& ws, hipStream_t s, bool split16) {
   // Keep C's contiguous index in the N class (B operand): the epilogue stores 16 consecutive n
   // per MFMA row, so this makes the stores coalesced.
   bool swap = false;
@@ -173,7 +175,7 @@ void contract(T alpha, const TV& A_, const TV& B_, T beta, const TV& C, Workspac
   auto loop_over = [&](char c) {
     int64_t e = ext_of(c);
     for (int64_t i = 0; i < e; ++i)
-      contract<T>(alpha, slice<T>(A, c, i), slice<T>(B, c, i), beta, slice<T>(C, c, i), ws, s);
+      contract<T>(alpha, slice<T>(A, c, i), slice<T>(B, c, i), beta, slice<T>(C, c, i), ws, s, split16);
   };
   // An M (N) class with a third index and no batch index: that index becomes the batch, with
   // stride 0 in B (A), which lacks it. Pick the index whose removal leaves a fusable class.
@@ -213,7 +215,7 @@ void contract(T alpha, const TV& A_, const TV& B_, T beta, const TV& C, Workspac
     char c = smallest(ok);
     int64_t e = ext_of(c);
     for (int64_t i = 0; i < e; ++i)
-      contract<T>(alpha, slice<T>(A, c, i), slice<T>(B, c, i), i == 0 ? beta : T(1), C, ws, s);
+      contract<T>(alpha, slice<T>(A, c, i), slice<T>(B, c, i), i == 0 ? beta : T(1), C, ws, s, split16);
     return;
   }
   if (gm.size() > 2) { loop_over(smallest(om)); return; }
@@ -237,6 +239,9 @@ void contract(T alpha, const TV& A_, const TV& B_, T beta, const TV& C, Workspac
   p.A = (const T*)A.p;
   p.B = (const T*)B.p;
   p.C = (T*)C.p;
+  if constexpr (std::is_same<T, double>::value) {
+    if (split16 && p.batch == 1 && 2.0 * p.M * p.N * p.K >= X3_ROUTE_FLOPS && x3_contract(p, ws, s)) return;
+  }
   tc_gemm<T>(p, ws, s);
 }
 
@@ -305,8 +310,8 @@ void scale(T beta, const TV& C, hipStream_t s) {
   launch_perm<T>(d, T(0), (const T*)nullptr, beta, (T*)C.p, s);
 }
 
-template void contract<double>(double, const TV&, const TV&, double, const TV&, Workspace&, hipStream_t);
-template void contract<float>(float, const TV&, const TV&, float, const TV&, Workspace&, hipStream_t);
+template void contract<double>(double, const TV&, const TV&, double, const TV&, Workspace&, hipStream_t, bool);
+template void contract<float>(float, const TV&, const TV&, float, const TV&, Workspace&, hipStream_t, bool);
 template void axpby<double>(double, const TV&, double, const TV&, hipStream_t);
 template void axpby<float>(float, const TV&, float, const TV&, hipStream_t);
 template void scale<double>(double, const TV&, hipStream_t);

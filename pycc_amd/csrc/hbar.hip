@@ -70,22 +70,23 @@ __global__ __launch_bounds__(256) void pair_sym_kernel(int64_t o, int64_t v, dou
 
 // Y[i][j][a][b] = CP[(ij)][(ab)] + s(i,j) s(a,b) CM[(ij)][(ab)]: the full ladder product
 // sum_ef X_ijef <ab|ef> of a pair-symmetric X from its packed GEMM outputs.
-__global__ void ladder_unpack_kernel(int64_t o, int64_t v, const double* __restrict__ CP,
-                                     const double* __restrict__ CM, double* __restrict__ Y) {
+template <typename TC>
+__global__ void ladder_unpack_kernel(int64_t o, int64_t v, const TC* __restrict__ CP,
+                                     const TC* __restrict__ CM, double* __restrict__ Y) {
   const int64_t npP = v * (v + 1) / 2, npM = v * (v - 1) / 2, ov = o * v;
   const int64_t ldP = npP + ov, ldM = npM + ov;
   const int64_t ij = blockIdx.y, i = ij / o, j = ij % o;
   const int64_t il = i < j ? i : j, ih = i < j ? j : i;
-  const double* cpr = CP + pairP(il, ih) * ldP;
-  const double* cmr = i != j ? CM + pairM(il, ih) * ldM : nullptr;
+  const TC* cpr = CP + pairP(il, ih) * ldP;
+  const TC* cmr = i != j ? CM + pairM(il, ih) * ldM : nullptr;
   const bool sij = i < j;
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < v * v;
        x += (int64_t)gridDim.x * blockDim.x) {
     const int64_t a = x / v, b = x % v;
     const int64_t al = a < b ? a : b, ah = a < b ? b : a;
-    double r = cpr[pairP(al, ah)];
+    double r = (double)cpr[pairP(al, ah)];
     if (cmr && a != b) {
-      double m = cmr[pairM(al, ah)];
+      double m = (double)cmr[pairM(al, ah)];
       r += (sij == (a < b)) ? m : -m;
     }
     Y[ij * v * v + x] = r;
@@ -165,7 +166,12 @@ static void alloc_hbar(Ctx& c, bool stored) {
 
 void hbar_build(Ctx& c, const double* F, const double* t1, const double* t2, bool stored, hipStream_t s) {
   CCX_CHECK(c.have_ints, ERR_STATE, "HBAR requested before the integrals were set");
-  CCX_CHECK(c.mode == MODE_DP, ERR_STATE, "HBAR and the Lambda equations run in a DP context");
+  // DP: every contraction in FP64. MIXED_F16X3: the large ones as FP16x3 split products
+  // (x3_contract), HBAR still held in FP64; the v^4 block is never stored there (the packed ladder
+  // is split FP16).
+  CCX_CHECK(c.mode == MODE_DP || (c.mode == MODE_MIXED_F16X3 && !stored), ERR_STATE,
+            "HBAR and the Lambda equations run in a DP context, or in a MIXED_F16X3 one without the "
+            "stored v^4 block");
   CCX_CHECK(F && t1 && t2, ERR_ARG, "null argument");
   CCX_CHECK(c.o * c.o < 65536, ERR_ARG, "too many occupied orbitals");
   alloc_work(c);
@@ -176,8 +182,9 @@ void hbar_build(Ctx& c, const double* F, const double* t1, const double* t2, boo
   const std::vector<int64_t> OV{o, v}, OO{o, o}, VV{v, v}, OOVV{o, o, v, v}, OVOV{o, v, o, v};
   const std::vector<int64_t> OOOV{o, o, o, v}, OVVV{o, v, v, v}, OOOO{o, o, o, o}, VVVV{v, v, v, v};
   const std::vector<int64_t> VOVV{v, o, v, v}, OVVO{o, v, v, o}, VVVO{v, v, v, o}, OVOO{o, v, o, o};
+  const bool x3 = c.mode == MODE_MIXED_F16X3;
   auto cd = [&](double a, const TV& A, const TV& B, double b, const TV& C) {
-    contract<double>(a, A, B, b, C, ws, s);
+    contract<double>(a, A, B, b, C, ws, s, x3);
   };
   CCX_HIP(hipMemcpyAsync(c.Ht1, t1, ov * sizeof(double), hipMemcpyDeviceToDevice, s));
   CCX_HIP(hipMemcpyAsync(c.Ht2, t2, ov * ov * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -347,8 +354,9 @@ void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, dou
   const std::vector<int64_t> OOOV{o, o, o, v}, OOOO{o, o, o, o}, OVVV{o, v, v, v};
   const std::vector<int64_t> VOVV{v, o, v, v}, OVVO{o, v, v, o}, VVVO{v, v, v, o}, OVOO{o, v, o, o};
   const std::vector<int64_t> OOVO{o, o, v, o};
+  const bool x3 = c.mode == MODE_MIXED_F16X3;
   auto cd = [&](double a, const TV& A, const TV& B, double b, const TV& C) {
-    contract<double>(a, A, B, b, C, ws, s);
+    contract<double>(a, A, B, b, C, ws, s, x3);
   };
   auto L1 = [&](const char* l) { return tv(l1, l, OV); };
   auto L2 = [&](const char* l) { return tv(l2, l, OOVV); };
@@ -418,14 +426,19 @@ void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, dou
     {
       // the packed ladder GEMM of the forward residual applied to l2, (ab) columns only
       const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
-      tau_pack<double>(o, v, nullptr, l2, (double*)c.Tp, (double*)c.Tm, s);
-      cd(1.0, tv(c.Tp, "xk", {noP, npP}), tvs(c.LP, "ky", {npP, npP}, {npP + ov, 1}), 0.0,
-         tvs(c.CP, "xy", {noP, npP}, {npP + ov, 1}));
-      if (noM > 0)
-        cd(1.0, tv(c.Tm, "xk", {noM, npM}), tvs(c.LM, "ky", {npM, npM}, {npM + ov, 1}), 0.0,
-           tvs(c.CM, "xy", {noM, npM}, {npM + ov, 1}));
       dim3 grid((unsigned)std::min<int64_t>(ceil_div(v * v, 256), 64), (unsigned)(o * o));
-      ladder_unpack_kernel<<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
+      if (x3) {   // FP16x3 against the context's split ladder rows, FP32 output
+        ladder_x3_l2(c, l2, s);
+        ladder_unpack_kernel<float><<<grid, 256, 0, s>>>(o, v, (const float*)c.CP, (const float*)c.CM, c.LY);
+      } else {
+        tau_pack<double>(o, v, nullptr, l2, (double*)c.Tp, (double*)c.Tm, s);
+        cd(1.0, tv(c.Tp, "xk", {noP, npP}), tvs(c.LP, "ky", {npP, npP}, {npP + ov, 1}), 0.0,
+           tvs(c.CP, "xy", {noP, npP}, {npP + ov, 1}));
+        if (noM > 0)
+          cd(1.0, tv(c.Tm, "xk", {noM, npM}), tvs(c.LM, "ky", {npM, npM}, {npM + ov, 1}), 0.0,
+             tvs(c.CM, "xy", {noM, npM}, {npM + ov, 1}));
+        ladder_unpack_kernel<double><<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
+      }
       CCX_HIP(hipGetLastError());
     }
     cd(-1.0, Vf, CVVV("mbea"), 1.0, Y);                        // <em|ab> = (mb|ea)
@@ -487,6 +500,7 @@ void lambda_residual(Ctx& c, const double* l1, const double* l2, double* r1, dou
 // x2 must be pair-symmetric.
 void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2, hipStream_t s) {
   CCX_CHECK(c.have_hbar, ERR_STATE, "the Jacobian needs an HBAR (ccx_hbar_build)");
+  CCX_CHECK(c.mode == MODE_DP, ERR_STATE, "the Jacobian runs in a DP context");
   CCX_CHECK(x1 && x2 && j1 && j2, ERR_ARG, "null argument");
   const int64_t o = c.o, v = c.v, ov = o * v;
   Workspace& ws = c.ws;
@@ -607,7 +621,7 @@ void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2
     if (noM > 0)
       cd(1.0, tv(c.Tm, "xk", {noM, npM}), tv(c.LM, "ky", {npM, npM + ov}), 0.0, tv(c.CM, "xy", {noM, npM + ov}));
     dim3 grid((unsigned)std::min<int64_t>(ceil_div(v * v, 256), 64), (unsigned)(o * o));
-    ladder_unpack_kernel<<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
+    ladder_unpack_kernel<double><<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
     CCX_HIP(hipGetLastError());
     // Zf[m][i][j][b] = Z_ijef <mb|ef> from the (m, b) columns of the same GEMM
     const TV Zf = tv(c.LQ, "mijb", OOOV);
@@ -858,7 +872,7 @@ void density_energy(Ctx& c, const double* t1, const double* t2, const double* l2
       contract<double>(1.0, tv(c.Tm, "xk", {noM, npM}), tvs(c.LM, "ky", {npM, npM}, {npM + ov, 1}), 0.0,
                        tvs(c.CM, "xy", {noM, npM}, {npM + ov, 1}), ws, s);
     dim3 grid((unsigned)std::min<int64_t>(ceil_div(v * v, 256), 64), (unsigned)(o * o));
-    ladder_unpack_kernel<<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
+    ladder_unpack_kernel<double><<<grid, 256, 0, s>>>(o, v, (const double*)c.CP, (const double*)c.CM, c.LY);
     CCX_HIP(hipGetLastError());
     build_tau(o, v, 1.0, 1.0, t1, t2, c.tau, s);
     dot(ov * ov, c.tau, c.LY, 0.5);

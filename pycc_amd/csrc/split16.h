@@ -48,6 +48,12 @@ struct Workspace;
 template <typename E>   // __bf16 or _Float16
 void x3_gemm(const X3Gemm& p, hipStream_t s, Workspace* ws = nullptr);
 
+// An FP64 GEMM of the contraction planner (batch 1) as an FP16x3 split product: A and B^T are
+// split (one power-of-two scale per operand, from its largest magnitude) into scratch from ws,
+// multiplied with FP32 accumulation and added into C (alpha, beta) in FP64. Returns false, doing
+// nothing, when the tile grid is under one full round (the caller then runs the FP64 GEMM).
+bool x3_contract(const GemmParams<double>& p, Workspace& ws, hipStream_t s);
+
 // nbatch FP16x3 GEMMs C_b = alpha / (sa sb) [A | A2_b swapped] [B_b] with K = Kp concatenated
 // from two operands (fields Ah2 .. sC above): the connected (T) numerators, two slot products per
 // GEMM. Output tiles of 384 x 192, 256 x 256 or 256 x 192, whichever pads M x N least.

@@ -918,8 +918,8 @@ __global__ __launch_bounds__(256) void tau_pack_x3_kernel(int64_t o, int64_t v, 
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npP; c += (int64_t)gridDim.x * blockDim.x) {
     int64_t e, f;
     decode_pair(c, e, f);
-    const double tef = blk[e * v + f] + t1[i * v + e] * t1[j * v + f];
-    const double tfe = blk[f * v + e] + t1[i * v + f] * t1[j * v + e];
+    const double tef = blk[e * v + f] + (t1 ? t1[i * v + e] * t1[j * v + f] : 0.0);   // t1 null: tau = t2
+    const double tfe = blk[f * v + e] + (t1 ? t1[i * v + f] * t1[j * v + e] : 0.0);
     split_store<E>((float)((e < f ? tef + tfe : tef) * s), Tph, Tpl, il_at(rowP * KpP + c));
     if (hasM && e < f) split_store<E>((float)((tef - tfe) * s), Tmh, Tml, il_at(rowM * KpM + pairM(e, f)));
   }
@@ -1016,6 +1016,120 @@ void x3_gemm(const X3Gemm& p0, hipStream_t s, Workspace* ws) {
 }
 template void x3_gemm<__bf16>(const X3Gemm&, hipStream_t, Workspace*);
 template void x3_gemm<_Float16>(const X3Gemm&, hipStream_t, Workspace*);
+
+// ---- FP64 contractions as FP16x3 split products (contract(..., split16 = true)) -------------------
+namespace {
+
+__device__ __forceinline__ int64_t idx2_at(const Idx2& d, int64_t i) { return (i / d.n2) * d.s1 + (i % d.n2) * d.s2; }
+
+// The R x K view X(r, k) = src[idx2_at(dr, r) + idx2_at(dk, k)] in 64 x 64 tiles; a tile's loads run
+// along r when r is the contiguous index of src (rfast), else along k.
+// Pass 1 (part != null): per-work-group max |X| into part[blockIdx.x + gridDim.x blockIdx.y].
+// Pass 2: hl[r][2 Kp] = interleaved FP16 hi/lo of X(r, k) sc (zero for K <= k < Kp), through LDS.
+__global__ __launch_bounds__(256) void view_split_kernel(const double* __restrict__ src, int64_t R, int64_t K,
+                                                         int64_t Kp, Idx2 dr, Idx2 dk, bool rfast,
+                                                         const float* __restrict__ sc, float* __restrict__ part,
+                                                         f16* __restrict__ hl) {
+  __shared__ float tile[64][65];
+  const double scale = part ? 1.0 : (double)sc[0];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4
+  const int64_t ntr = (R + 63) / 64, ntk = (Kp + 63) / 64;
+  double mx = 0.0;
+  for (int64_t tr = blockIdx.y; tr < ntr; tr += gridDim.y)
+    for (int64_t tk = blockIdx.x; tk < ntk; tk += gridDim.x) {
+      const int64_t r0 = tr * 64, k0 = tk * 64;
+      for (int j = ty; j < 64; j += 4) {
+        const int rl = rfast ? tx : j, kl = rfast ? j : tx;
+        const int64_t r = r0 + rl, k = k0 + kl;
+        double x = 0.0;
+        if (r < R && k < K) x = src[idx2_at(dr, r) + idx2_at(dk, k)];
+        if (part) mx = fmax(mx, fabs(x));
+        else tile[rl][kl] = (float)(x * scale);
+      }
+      if (part) continue;
+      __syncthreads();
+      for (int j = ty; j < 64; j += 4) {
+        const int64_t r = r0 + j, k = k0 + tx;
+        if (r < R && k < Kp) {
+          const float x = tile[j][tx];
+          const f16 h = (f16)x;
+          const int64_t at = r * 2 * Kp + ((k >> 5) << 6) + (k & 31);
+          hl[at] = h;
+          hl[at + 32] = (f16)(x - (float)h);
+        }
+      }
+      __syncthreads();
+    }
+  if (part) {
+    const float m = block_max((float)mx);
+    if (threadIdx.x == 0) part[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = m;
+  }
+}
+
+// C[idx2_at(cm, m) + idx2_at(cn, n)] = beta C + S[m][n] (S from the split GEMM, alpha applied)
+__global__ __launch_bounds__(256) void scatter_add_kernel(const float* __restrict__ S, int64_t M, int64_t N, Idx2 cm,
+                                                          Idx2 cn, double beta, double* __restrict__ C) {
+  for (int64_t m = blockIdx.y; m < M; m += gridDim.y) {
+    const int64_t cb = idx2_at(cm, m);
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < N; n += (int64_t)gridDim.x * blockDim.x) {
+      double* c = C + cb + idx2_at(cn, n);
+      const double val = (double)S[m * N + n];
+      *c = beta == 0.0 ? val : beta * *c + val;
+    }
+  }
+}
+
+}  // namespace
+
+bool x3_contract(const GemmParams<double>& p, Workspace& ws, hipStream_t s) {
+  const int64_t M = p.M, N = p.N, K = p.K, Kp = (K + 31) / 32 * 32;
+  // A full round of 256 x 256 tiles at least (smaller grids would need split-K partials)
+  if (p.batch != 1 || K < 32 || ceil_div(M, 256) * ceil_div(N, 256) < 256) return false;
+  // A(m, k) = A[am(m) + ak(k)], B(k, n) = B[bk(k) + bn(n)]: B^T rows are (n, k). C is written by
+  // the GEMM in FP64 when it is a matrix C[m ldc + n] and x3_gemm's grid has 512 tiles (its direct
+  // FP64 epilogue), else through an FP32 copy scattered into C.
+  const bool big = M >= 256, wide = big && N >= 8192;
+  const int64_t tiles = M <= 64 ? ceil_div(N, 128) : ceil_div(M, big ? 256 : 128) * ceil_div(N, wide ? 320 : (big ? 256 : 128));
+  const bool direct = p.cn.s2 == 1 && p.cn.n2 >= N && p.cm.n2 >= M && p.cm.s2 >= N && tiles >= 512;
+  auto r256 = [](int64_t b) { return (b + 255) / 256 * 256; };
+  constexpr int PB = 256;   // partial maxima per operand (a 16 x 16 grid of the absmax pass)
+  const int64_t bA = r256(M * 2 * Kp * 2), bB = r256(N * 2 * Kp * 2), bC = direct ? 0 : r256(M * N * 4);
+  char* base = (char*)ws.get((size_t)(bA + bB + bC + 2 * PB * 4 + 256));
+  f16* Ahl = (f16*)base;
+  f16* Bhl = (f16*)(base + bA);
+  float* S = (float*)(base + bA + bB);
+  float* part = (float*)(base + bA + bB + bC);
+  float* sc = part + 2 * PB;   // sc[0] scale of A, sc[1] scale of B
+  const dim3 gmax(16, 16);
+  auto split = [&](const double* src, int64_t R, Idx2 dr, Idx2 dk, float* pmax, float* scl, f16* hl) {
+    const bool rfast = dr.s2 == 1 && dk.s2 != 1;
+    view_split_kernel<<<gmax, 256, 0, s>>>(src, R, K, Kp, dr, dk, rfast, nullptr, pmax, nullptr);
+    CCX_HIP(hipGetLastError());
+    x3_scale_kernel<<<1, 256, 0, s>>>(pmax, PB, 1.f, pmax, 0, 0.f, scl);
+    CCX_HIP(hipGetLastError());
+    const dim3 g((unsigned)std::min<int64_t>(ceil_div(Kp, 64), 1024), (unsigned)std::min<int64_t>(ceil_div(R, 64), 65535));
+    view_split_kernel<<<g, 256, 0, s>>>(src, R, K, Kp, dr, dk, rfast, scl, nullptr, hl);
+    CCX_HIP(hipGetLastError());
+  };
+  split(p.A, M, p.am, p.ak, part, sc, Ahl);
+  split(p.B, N, p.bn, p.bk, part + PB, sc + 1, Bhl);
+  X3Gemm g;
+  g.M = M; g.N = N; g.Kp = Kp; g.lda = 2 * Kp; g.ldb = 2 * Kp;
+  g.kb = 32; g.ska = 64; g.skb = 64;
+  g.Ah = Ahl; g.Al = Ahl + 32; g.Bh = Bhl; g.Bl = Bhl + 32;
+  g.sa = sc; g.sb = sc + 1; g.alpha = (float)p.alpha;
+  if (direct) {
+    g.C64 = p.C; g.ldc = p.cm.s2; g.beta = (float)p.beta;
+    x3_gemm<_Float16>(g, s);
+  } else {
+    g.C = S; g.ldc = N; g.beta = 0.f;
+    x3_gemm<_Float16>(g, s);
+    const dim3 gs((unsigned)std::min<int64_t>(ceil_div(N, 256), 64), (unsigned)std::min<int64_t>(M, 65535));
+    scatter_add_kernel<<<gs, 256, 0, s>>>(S, M, N, p.cm, p.cn, p.beta, p.C);
+    CCX_HIP(hipGetLastError());
+  }
+  return true;
+}
 
 void x3_gemm_kcat(const X3Gemm& p, hipStream_t s) {
   if (p.M == 0 || p.N == 0 || p.nbatch == 0) return;
@@ -1423,29 +1537,47 @@ void gemm_f16x3(bool ta, bool tb, int64_t M, int64_t N, int64_t K, float alpha, 
 
 // The packed ladder (ccwfn.py:931 and Z, :715) as FP16x3 GEMMs: scaled tau rows times the
 // per-row-scaled ladder matrices, FP32 outputs CP / CM.
-void ladder_x3(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
-  const int64_t o = c.o, v = c.v, ov = c.ov();
+// The packed ladder GEMMs (tau+- against the split <ab|ef> rows): with Z the (ab) and the ov Z
+// columns, else the (ab) columns only; the outputs keep their rows of npP + ov (npM + ov) floats.
+static void ladder_x3_gemms(Ctx& c, const float* sa, bool with_z, hipStream_t s) {
+  const int64_t ov = c.ov();
   const int64_t npP = c.npP(), npM = c.npM(), noP = c.noP(), noM = c.noM();
   const int64_t KpP = bf16_kpad(npP), KpM = bf16_kpad(npM);
-  // packed tau+- are bounded by 2 max|t2| + 2 max|t1|^2 (scale xs[4] from x3_amp_scales)
-  tau_pack_x3(o, v, KpP, KpM, t1, t2, c.xs + 4, (f16*)c.Tph, (f16*)c.Tpl, (f16*)c.Tmh, (f16*)c.Tml, s);
   X3Gemm g;
   // all operands interleaved: rows of 2 Kp, kb = 32, ska = skb = 64
-  g.M = noP; g.N = npP + ov; g.Kp = KpP; g.lda = 2 * KpP; g.ldb = 2 * KpP; g.ldc = npP + ov;
+  g.M = noP; g.N = npP + (with_z ? ov : 0); g.Kp = KpP; g.lda = 2 * KpP; g.ldb = 2 * KpP; g.ldc = npP + ov;
   g.kb = 32; g.ska = 64; g.skb = 64;
   g.Ah = c.Tph; g.Al = c.Tpl; g.Bh = c.LPh; g.Bl = c.LPl; g.C = (float*)c.CP;
-  g.sa = c.xs + 4; g.sbn = c.lsP;
-  int k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noP * (npP + ov) * npP);
+  g.sa = sa; g.sbn = c.lsP;
+  int k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noP * g.N * npP);
   x3_gemm<_Float16>(g, s);
   c.tend(s, k);
   if (noM > 0) {
     X3Gemm h = g;
-    h.M = noM; h.N = npM + ov; h.Kp = KpM; h.lda = 2 * KpM; h.ldb = 2 * KpM; h.ldc = npM + ov;
+    h.M = noM; h.N = npM + (with_z ? ov : 0); h.Kp = KpM; h.lda = 2 * KpM; h.ldb = 2 * KpM; h.ldc = npM + ov;
     h.Ah = c.Tmh; h.Al = c.Tml; h.Bh = c.LMh; h.Bl = c.LMl; h.C = (float*)c.CM; h.sbn = c.lsM;
-    k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noM * (npM + ov) * npM);
+    k = c.tbegin(s, Ctx::PH_LADDER, 2.0 * noM * h.N * npM);
     x3_gemm<_Float16>(h, s);
     c.tend(s, k);
   }
+}
+
+void ladder_x3(Ctx& c, const double* t1, const double* t2, hipStream_t s) {
+  const int64_t o = c.o, v = c.v;
+  const int64_t KpP = bf16_kpad(c.npP()), KpM = bf16_kpad(c.npM());
+  // packed tau+- are bounded by 2 max|t2| + 2 max|t1|^2 (scale xs[4] from x3_amp_scales)
+  tau_pack_x3(o, v, KpP, KpM, t1, t2, c.xs + 4, (f16*)c.Tph, (f16*)c.Tpl, (f16*)c.Tmh, (f16*)c.Tml, s);
+  ladder_x3_gemms(c, c.xs + 4, true, s);
+}
+
+void ladder_x3_l2(Ctx& c, const double* l2, hipStream_t s) {
+  const int64_t o = c.o, v = c.v;
+  const int64_t KpP = bf16_kpad(c.npP()), KpM = bf16_kpad(c.npM());
+  // l2's packed +- combinations are bounded by 2 max|l2|: scale xs[16]
+  x3_absmax<double>(l2, o * o * v * v, c.xpart, s);
+  x3_scale(c.xpart, 2.f, nullptr, 0.f, c.xs + 16, s);
+  tau_pack_x3(o, v, KpP, KpM, nullptr, l2, c.xs + 16, (f16*)c.Tph, (f16*)c.Tpl, (f16*)c.Tmh, (f16*)c.Tml, s);
+  ladder_x3_gemms(c, c.xs + 16, false, s);
 }
 
 }  // namespace ccx
