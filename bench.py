@@ -64,7 +64,8 @@ def _ms(fn):
 
 
 def backward_pass(cc, store_vvvv=False):
-    """One FP64 backward of the converged solve: HBAR at (F, t1, t2), then the gradient of the CCSD
+    """One backward of the converged solve (FP64 in a DP context; in a MIXED_F16X3 context its large
+    contractions as FP16x3 split products, HBAR held in FP64): HBAR at (F, t1, t2), then the gradient of the CCSD
     Lagrangian E + l1.r1 + 1/2 l2.r2 at the guess l1 = 2 t1, l2 = 2(2 t2 - t2^T) (the Lambda
     residual), and a second Lambda residual on the same HBAR (the cost of one Lambda iteration)."""
     import torch
@@ -163,7 +164,7 @@ def run_leg(sd, no, precision, warmup, steps, converge, maxiter, e_conv, r_conv,
     }
     if backward and out["converged"]:
         out["backward"] = backward_pass(cc)
-        log("[%s] FP64 backward: HBAR %.0f ms, Lambda residual %.0f ms, %.1f GB" % (
+        log("[%s] backward: HBAR %.0f ms, Lambda residual %.0f ms, %.1f GB" % (
             precision, out["backward"]["hbar_build_ms"], out["backward"]["lambda_residual_ms"],
             out["backward"]["device_GB"]))
     t1c, t2c = cc.t1.clone(), cc.t2.clone()
@@ -293,7 +294,7 @@ def main():
     ap.add_argument("--no-converge", action="store_true", help="skip the untimed convergence legs")
     ap.add_argument("--no-dp", action="store_true", help="skip the FP64 leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-backward", action="store_true", help="skip the FP64 backward pass")
+    ap.add_argument("--no-backward", action="store_true", help="skip the backward passes (FP64 and mixed)")
     ap.add_argument("--secondary", default="MIXED,MIXED_BF16X3",
                     help="comma-separated further mixed forms timed and converged after the FP64 leg "
                          "(empty: none)")
@@ -339,7 +340,8 @@ def main():
     converge = not args.no_converge
     mixed_thr = (args.e_conv_mixed, args.r_conv_mixed)
     mixed, mt1, mt2 = run_leg(sd, o, args.precision, args.warmup, args.steps, converge, args.maxiter,
-                              *(mixed_thr if args.precision != "DP" else (args.e_conv, args.r_conv_dp)))
+                              *(mixed_thr if args.precision != "DP" else (args.e_conv, args.r_conv_dp)),
+                              backward=not args.no_backward and args.precision == "MIXED_F16X3")
     mt1, mt2 = mt1.cpu(), mt2.cpu()
     dp = None
     if not args.no_dp and args.precision != "DP":
@@ -408,6 +410,14 @@ def main():
         b["workload"] = ("FP64 Lagrangian backward (factored HBAR + Lambda residual = vJp with the energy "
                          "source) at the converged FP64 amplitudes, o=%d v=%d" % (o, v))
         res["backward"] = b
+    if "backward" in mixed:
+        b = mixed["backward"]
+        b["workload"] = ("MIXED_F16X3 Lagrangian backward (factored HBAR + Lambda residual, the contractions "
+                         "of >= 1e11 flops and the ladder as FP16x3 split products) at the converged mixed "
+                         "amplitudes, o=%d v=%d" % (o, v))
+        if "backward" in res:
+            b["speedup_vs_fp64"] = {k: res["backward"][k] / b[k] for k in ("hbar_build_ms", "lambda_residual_ms")}
+        res["backward_mixed"] = b
     res["mixed"] = {"iterations": mixed["iterations"], "energy": mixed["energy"], "converged": mixed["converged"],
                     "time_to_solution_s": mixed["solve_s"], "e_conv": mixed["e_conv"], "r_conv": mixed["r_conv"],
                     "setup_s": mixed["setup_s"], "device_GB": mixed["device_GB"]}

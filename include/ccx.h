@@ -94,7 +94,12 @@ int ccx_diis_reset(ccx_ctx* ctx, const double* t1, const double* t2, int max_dii
 int ccx_diis_add(ccx_ctx* ctx, const double* t1, const double* t2, void* stream);
 int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out, void* stream);
 
-/* ---- backward: HBAR, the Lambda residual and the vector-Jacobian product (FP64, DP contexts) ----
+/* ---- backward: HBAR, the Lambda residual and the vector-Jacobian product ----------------------
+ *
+ * DP contexts run them in FP64. CCX_MODE_MIXED_F16X3 contexts run them too, without
+ * CCX_HBAR_STORED: HBAR is held in FP64, and its contractions of at least 1e11 flops and the Lambda
+ * ladder run as FP16x3 split products (results within 1e-6 relative of DP). ccx_jvp and
+ * ccx_densities need a DP context.
  *
  * ccx_hbar_build = cchbar(ccwfn) (pycc/cchbar.py:54-98, spatial CCSD builders :128-823) at the given
  *   fock, t1, t2 (copied; the inputs may change afterwards). flags = 0 holds the blocks of size
