@@ -1083,8 +1083,12 @@ __global__ __launch_bounds__(256) void scatter_add_kernel(const float* __restric
 
 bool x3_contract(const GemmParams<double>& p, Workspace& ws, hipStream_t s) {
   const int64_t M = p.M, N = p.N, K = p.K, Kp = (K + 31) / 32 * 32;
-  // A full round of 256 x 256 tiles at least (smaller grids would need split-K partials)
-  if (p.batch != 1 || K < 32 || ceil_div(M, 256) * ceil_div(N, 256) < 256) return false;
+  // A full round of 256 x 256 tiles at least (smaller grids would need split-K partials), and
+  // M N / (M + N) >= 128: splitting costs about 16 bytes of traffic per element of A and B, which the
+  // faster product repays only when each element takes part in enough products (a thin GEMM, e.g.
+  // o x v^3 times v, is faster left in FP64)
+  if (p.batch != 1 || K < 32 || ceil_div(M, 256) * ceil_div(N, 256) < 256 || (double)M * N < 128.0 * (M + N))
+    return false;
   // A(m, k) = A[am(m) + ak(k)], B(k, n) = B[bk(k) + bn(n)]: B^T rows are (n, k). C is written by
   // the GEMM in FP64 when it is a matrix C[m ldc + n] and x3_gemm's grid has 512 tiles (its direct
   // FP64 epilogue), else through an FP32 copy scattered into C.
