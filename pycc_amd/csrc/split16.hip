@@ -1020,7 +1020,14 @@ template void x3_gemm<_Float16>(const X3Gemm&, hipStream_t, Workspace*);
 // ---- FP64 contractions as FP16x3 split products (contract(..., split16 = true)) -------------------
 namespace {
 
-__device__ __forceinline__ int64_t idx2_at(const Idx2& d, int64_t i) { return (i / d.n2) * d.s1 + (i % d.n2) * d.s2; }
+__device__ __forceinline__ int64_t idx2_at(const Idx2& d, int64_t i) {
+  if (i < d.n2) return i * d.s2;
+  if ((i | d.n2) < (int64_t(1) << 31)) {   // 32-bit division when it fits
+    const int q = (int)i / (int)d.n2;
+    return q * d.s1 + ((int)i - q * (int)d.n2) * d.s2;
+  }
+  return (i / d.n2) * d.s1 + (i % d.n2) * d.s2;
+}
 
 // The R x K view X(r, k) = src[idx2_at(dr, r) + idx2_at(dk, k)] in 64 x 64 tiles; a tile's loads run
 // along r when r is the contiguous index of src (rfast), else along k.
@@ -1031,6 +1038,7 @@ __global__ __launch_bounds__(256) void view_split_kernel(const double* __restric
                                                          const float* __restrict__ sc, float* __restrict__ part,
                                                          f16* __restrict__ hl) {
   __shared__ float tile[64][65];
+  __shared__ int64_t roff[64], koff[64];   // the tile's row and column offsets in src (-1: outside)
   const double scale = part ? 1.0 : (double)sc[0];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4
   const int64_t ntr = (R + 63) / 64, ntk = (Kp + 63) / 64;
@@ -1038,11 +1046,14 @@ __global__ __launch_bounds__(256) void view_split_kernel(const double* __restric
   for (int64_t tr = blockIdx.y; tr < ntr; tr += gridDim.y)
     for (int64_t tk = blockIdx.x; tk < ntk; tk += gridDim.x) {
       const int64_t r0 = tr * 64, k0 = tk * 64;
+      __syncthreads();   // the previous tile's offsets and LDS tile are no longer read
+      if (ty == 0) roff[tx] = r0 + tx < R ? idx2_at(dr, r0 + tx) : -1;
+      else if (ty == 1) koff[tx] = k0 + tx < K ? idx2_at(dk, k0 + tx) : -1;
+      __syncthreads();
       for (int j = ty; j < 64; j += 4) {
         const int rl = rfast ? tx : j, kl = rfast ? j : tx;
-        const int64_t r = r0 + rl, k = k0 + kl;
-        double x = 0.0;
-        if (r < R && k < K) x = src[idx2_at(dr, r) + idx2_at(dk, k)];
+        const int64_t ro = roff[rl], ko = koff[kl];
+        const double x = (ro >= 0 && ko >= 0) ? src[ro + ko] : 0.0;
         if (part) mx = fmax(mx, fabs(x));
         else tile[rl][kl] = (float)(x * scale);
       }
@@ -1058,7 +1069,6 @@ __global__ __launch_bounds__(256) void view_split_kernel(const double* __restric
           hl[at + 32] = (f16)(x - (float)h);
         }
       }
-      __syncthreads();
     }
   if (part) {
     const float m = block_max((float)mx);
