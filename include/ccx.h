@@ -98,7 +98,8 @@ int ccx_diis_extrapolate(ccx_ctx* ctx, double* t1, double* t2, double* coef_out,
  *
  * DP contexts run them in FP64. CCX_MODE_MIXED_F16X3 contexts run them too, without
  * CCX_HBAR_STORED: HBAR is held in FP64, and its contractions of at least 1e11 flops and the Lambda
- * ladder run as FP16x3 split products (results within 1e-6 relative of DP). ccx_jvp and
+ * ladder run as FP16x3 split products (the Lambda residual within 5e-7 relative of DP at o=16,
+ * v=256). ccx_jvp and
  * ccx_densities need a DP context.
  *
  * ccx_hbar_build = cchbar(ccwfn) (pycc/cchbar.py:54-98, spatial CCSD builders :128-823) at the given

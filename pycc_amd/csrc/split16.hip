@@ -1104,7 +1104,8 @@ bool x3_contract(const GemmParams<double>& p, Workspace& ws, hipStream_t s) {
   // FP64 epilogue), else through an FP32 copy scattered into C.
   const bool big = M >= 256, wide = big && N >= 8192;
   const int64_t tiles = M <= 64 ? ceil_div(N, 128) : ceil_div(M, big ? 256 : 128) * ceil_div(N, wide ? 320 : (big ? 256 : 128));
-  const bool direct = p.cn.s2 == 1 && p.cn.n2 >= N && p.cm.n2 >= M && p.cm.s2 >= N && tiles >= 512;
+  const bool direct = p.cn.s2 == 1 && p.cn.n2 >= N && p.cm.n2 >= M && p.cm.s2 >= N && tiles >= 512 &&
+                      (double)(float)p.beta == p.beta;   // the epilogue applies beta in FP32
   auto r256 = [](int64_t b) { return (b + 255) / 256 * 256; };
   constexpr int PB = 256;   // partial maxima per operand (a 16 x 16 grid of the absmax pass)
   const int64_t bA = r256(M * 2 * Kp * 2), bB = r256(N * 2 * Kp * 2), bC = direct ? 0 : r256(M * N * 4);
