@@ -183,6 +183,17 @@ int ccx_triples(ccx_ctx* ctx, const double* fock, const double* t1, const double
 int ccx_triples_range(ccx_ctx* ctx, const double* fock, const double* t1, const double* t2, int i0, int i1,
                       double* e_dev, void* stream);
 
+/* t3[x][y][z] /= D for x, y, z < n (t3 a contiguous n^3 device array), the orbital-energy
+ * denominator of the WithDenom=True branches of the batched T3 builders: pycc/cctriples.py:64-70
+ * (t3c_ijk) and :139-145 (t3d_ijk) with occ_fixed = 1, fixed occupied orbitals (p, q, r) = (i, j, k)
+ * and running virtuals e0 + a, e0 + b, e0 + c:  D = f_pp + f_qq + f_rr - (f_aa + f_bb + f_cc);
+ * :97-103 (t3c_abc) and :165-171 (t3d_abc) with occ_fixed = 0, fixed virtuals (a, b, c) and
+ * running occupied orbitals: D = (f_ii + f_jj + f_kk) - (f_pp + f_qq + f_rr). Orbital indices are
+ * rows of the nmo x nmo device matrix `fock`, whose diagonal is read; each sum of three is added
+ * left to right, as the reference's broadcasts add them. Asynchronous on `stream`. */
+int ccx_t3_denominator(const double* fock, int64_t nmo, int p, int q, int r, int e0, int n, int occ_fixed,
+                       double* t3, void* stream);
+
 /* Measurement hooks (no counterpart in the reference, which has only wall-clock prints,
  * pycc/utils.py:229-238): HIP events recorded on the residual's stream around the ladder GEMMs,
  * the ring GEMMs and each whole residual.

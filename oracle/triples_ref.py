@@ -2,6 +2,7 @@
 
 * ``pycc/cctriples.py:27-72``   ``t3c_ijk``: connected T3 numerator for fixed i, j, k
 * ``pycc/cctriples.py:108-146`` ``t3d_ijk``: disconnected numerator (<ij|ab> t1_kc + t2_ijab f_kc, ...)
+* ``pycc/cctriples.py:64-70, 97-103`` ``denominator``: D_ijkabc over an i, j, k or an a, b, c batch
 * ``pycc/cctriples.py:177-240`` ``t_tjl``: Lee-Rendell sum over i >= j >= k and a >= b >= c with
   the X3 / Y3 / Z3 intermediates and the (1 + d_ab + d_ac + d_bc) and (2 - d_ij - d_ik - d_jk)
   weights
@@ -9,7 +10,8 @@
 The a, b, c loops of the reference are written as masks over whole v^3 arrays. Pinned by the
 reference's own (T) energies for water (``pycc/tests/test_005_ccsd_t_energy.py:33,44``, STO-3G and
 cc-pVDZ, frozen core) through the psi4-free front end, and by ``t_full`` below (the textbook
-full-loop form over all i, j, k) on tiny systems.
+full-loop form over all i, j, k) on tiny systems; the a, b, c batch of ``denominator`` equals its
+i, j, k batch element for element (tests/test_triples_oracle.py).
 """
 from __future__ import annotations
 
@@ -48,6 +50,16 @@ def t3d_ijk(i, j, k, t1, t2, Woovv, Fov):
     return t3
 
 
+def denominator(Fo, Fv, i=None, j=None, k=None, a=None, b=None, c=None):
+    """D_ijkabc = (f_ii + f_jj + f_kk) - (f_aa + f_bb + f_cc) over one batch: fixed (i, j, k) and all
+    a, b, c (cctriples.py:64-70, shape (nv, nv, nv)), or fixed (a, b, c) and all i, j, k (:97-103,
+    shape (no, no, no)); each sum of three added left to right, as the reference's broadcasts add it.
+    t_tjl and t_full divide by the first form."""
+    if i is not None:
+        return (Fo[i] + Fo[j] + Fo[k]) - (Fv.reshape(-1, 1, 1) + Fv.reshape(-1, 1) + Fv)
+    return (Fo.reshape(-1, 1, 1) + Fo.reshape(-1, 1) + Fo) - (Fv[a] + Fv[b] + Fv[c])
+
+
 def _blocks(F, ERI, no, nfzc):
     nmo = F.shape[0]
     o = slice(nfzc, nfzc + no)
@@ -76,7 +88,7 @@ def t_tjl(F, ERI, t1, t2, no, nfzc=0):
                       + W3.swapaxes(0, 2) * V3.swapaxes(0, 2))
                 Y3 = V3 + V3.swapaxes(0, 1).swapaxes(1, 2) + V3.swapaxes(0, 1).swapaxes(0, 2)
                 Z3 = V3.swapaxes(1, 2) + V3.swapaxes(0, 1) + V3.swapaxes(0, 2)
-                denom = Fo[i] + Fo[j] + Fo[k] - (Fv.reshape(-1, 1, 1) + Fv.reshape(-1, 1) + Fv)
+                denom = denominator(Fo, Fv, i=i, j=j, k=k)
                 Wcyc = W3 + W3.transpose(1, 2, 0) + W3.transpose(2, 0, 1)      # W_abc + W_bca + W_cab
                 Wacy = W3.transpose(0, 2, 1) + W3.transpose(1, 0, 2) + W3.transpose(2, 1, 0)
                 term = ((Y3 - 2.0 * Z3) * Wcyc + (Z3 - 2.0 * Y3) * Wacy + 3.0 * X3) / denom
@@ -96,7 +108,7 @@ def t_full(F, ERI, t1, t2, no, nfzc=0):
             for k in range(no):
                 W = t3c_ijk(i, j, k, t2, Wvvvo, Wovoo)
                 V = W + t3d_ijk(i, j, k, t1, t2, Woovv, Fov)
-                D = Fo[i] + Fo[j] + Fo[k] - (Fv.reshape(-1, 1, 1) + Fv.reshape(-1, 1) + Fv)
+                D = denominator(Fo, Fv, i=i, j=j, k=k)
                 Q = 4.0 * W + W.transpose(1, 2, 0) + W.transpose(2, 0, 1)
                 ET += float((Q * (V - V.transpose(2, 1, 0)) / (3.0 * D)).sum())
     return ET

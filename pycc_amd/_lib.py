@@ -50,6 +50,8 @@ SIGNATURES = {
     "ccx_density_energy": (ctypes.c_int, [_vp] * 13),
     "ccx_triples": (ctypes.c_int, [_vp] * 6),
     "ccx_triples_range": (ctypes.c_int, [_vp] * 4 + [ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "ccx_t3_denominator": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "ccx_timing_start": (ctypes.c_int, [_vp]),
     "ccx_timing_read": (ctypes.c_int, [_vp, c_dp, ctypes.c_int]),
     "ccx_device_bytes": (ctypes.c_int64, [_vp]),

@@ -297,6 +297,11 @@ int ccx_triples_range(ccx_ctx* ctx, const double* fock, const double* t1, const 
   });
 }
 
+int ccx_t3_denominator(const double* fock, int64_t nmo, int p, int q, int r, int e0, int n, int occ_fixed,
+                       double* t3, void* stream) {
+  return guarded([&] { ccx::t3_denominator(fock, nmo, p, q, r, e0, n, occ_fixed != 0, t3, st(stream)); });
+}
+
 int ccx_timing_start(ccx_ctx* ctx) {
   return guarded([&] {
     CCX_CHECK(ctx, ccx::ERR_ARG, "null context");

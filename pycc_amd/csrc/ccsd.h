@@ -191,6 +191,10 @@ void jacobian(Ctx& c, const double* x1, const double* x2, double* j1, double* j2
 // *e_dev (accumulate) or written there (triples.hip); synchronises the stream
 void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2, int i0, int i1, bool accumulate,
                     double* e_dev, hipStream_t s);
+// t3[x][y][z] (n^3, contiguous) divided by the orbital-energy denominator of the fixed triple (p, q, r) and the
+// running orbitals e0 + x, e0 + y, e0 + z (occ_fixed: D = f_pp + f_qq + f_rr - f_xx - f_yy - f_zz, else -D)
+void t3_denominator(const double* F, int64_t nmo, int p, int q, int r, int e0, int n, bool occ_fixed, double* t3,
+                    hipStream_t s);
 void timing_start(Ctx& c);
 void timing_read(Ctx& c, double* out, int n);
 void diis_reset(Ctx& c, const double* t1, const double* t2, int max_diis, int cap, hipStream_t s);

@@ -636,4 +636,39 @@ void triples_energy(Ctx& c, const double* F, const double* t1, const double* t2,
   CCX_HIP(hipStreamSynchronize(s));
 }
 
+namespace {
+
+// t3[x][y][z] /= D for one fixed index triple (p, q, r) and the running (x, y, z) of one n^3 batch:
+// D = (f_pp + f_qq + f_rr) - (f_xx + f_yy + f_zz) with running orbitals e0 + x, ... when the fixed
+// triple is occupied (the i, j, k batches of cctriples.py:64-70, :139-145), its negative when it is
+// virtual (the a, b, c batches, :97-103, :165-171); both sums are added left to right, as there.
+// Grid (ceil(n / 256), n, n): z = blockIdx.x 256 + threadIdx.x, y = blockIdx.y, x = blockIdx.z.
+__global__ void __launch_bounds__(256) t3_denominator_kernel(const double* __restrict__ F, int64_t nmo, int p,
+                                                             int q, int r, int e0, int n, int occ_fixed,
+                                                             double* __restrict__ t3) {
+  const int z = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, x = blockIdx.z;
+  if (z >= n) return;
+  const int64_t d1 = nmo + 1;
+  const double sf = (F[p * d1] + F[q * d1]) + F[r * d1];
+  const double sr = (F[(e0 + x) * d1] + F[(e0 + y) * d1]) + F[(e0 + z) * d1];
+  double* e = t3 + ((int64_t)x * n + y) * n + z;
+  *e = *e / (occ_fixed ? sf - sr : sr - sf);
+}
+
+}  // namespace
+
+void t3_denominator(const double* F, int64_t nmo, int p, int q, int r, int e0, int n, bool occ_fixed, double* t3,
+                    hipStream_t s) {
+  CCX_CHECK(F != nullptr && nmo >= 1 && n >= 0, ERR_ARG, "t3 denominator: null fock or negative extent");
+  CCX_CHECK(p >= 0 && q >= 0 && r >= 0 && p < nmo && q < nmo && r < nmo, ERR_ARG,
+            "t3 denominator: fixed orbital outside the Fock matrix");
+  CCX_CHECK(e0 >= 0 && (int64_t)e0 + n <= nmo, ERR_ARG, "t3 denominator: running orbitals outside the Fock matrix");
+  CCX_CHECK(n <= 65535, ERR_ARG, "t3 denominator: more than 65535 running orbitals");
+  if (n == 0) return;
+  CCX_CHECK(t3 != nullptr, ERR_ARG, "t3 denominator: null t3");
+  const dim3 g((unsigned)ceil_div(n, 256), (unsigned)n, (unsigned)n);
+  t3_denominator_kernel<<<g, 256, 0, s>>>(F, nmo, p, q, r, e0, n, occ_fixed ? 1 : 0, t3);
+  CCX_HIP(hipGetLastError());
+}
+
 }  // namespace ccx

@@ -71,3 +71,19 @@ def test_full_abc_rewrite_equals_triangular_sum(no, nv, seed):
                 E += (2.0 - ((i == j) + (i == k) + (j == k))) * float((V * Q / D).sum())
     ref = t_tjl(F, H.ERI, t1, t2, no)
     assert abs(E - ref) < 1e-13 * max(1.0, abs(ref)), (E, ref)
+
+
+def test_denominator_batches_agree():
+    """The a, b, c batch of the oracle's denominator (cctriples.py:97-103) equals its i, j, k batch
+    (:64-70, the form t_tjl and t_full divide by) element for element, bit for bit."""
+    from oracle.triples_ref import denominator
+    rng = np.random.default_rng(7)
+    no, nv = 3, 5
+    Fo, Fv = -1.0 - rng.random(no), 0.5 + rng.random(nv)
+    ijk = np.stack([np.stack([np.stack([denominator(Fo, Fv, i=i, j=j, k=k) for k in range(no)])
+                              for j in range(no)]) for i in range(no)])            # [i][j][k][a][b][c]
+    for a in range(nv):
+        for b in range(nv):
+            for c in range(nv):
+                assert np.array_equal(denominator(Fo, Fv, a=a, b=b, c=c), ijk[:, :, :, a, b, c])
+    assert np.all(ijk < 0)
