@@ -50,6 +50,13 @@ def test_invalid_arguments_return_error_codes_without_a_gpu():
     assert lib.ccx_triples_range(None, None, None, None, 0, 1, None, None) == -1
     assert b"null context" in lib.ccx_last_error()
     assert lib.ccx_contract(b"ab,bc", 1.0, None, None, None, None, None, None, 0.0, None, None, None, None) == -1
+    assert lib.ccx_t3_denominator(None, 5, 0, 0, 0, 2, 3, 1, None, None) == -1
+    assert b"null fock" in lib.ccx_last_error()
+    fake = ctypes.c_void_p(16)                                          # never dereferenced: rejected first
+    assert lib.ccx_t3_denominator(fake, 5, 0, 0, 5, 2, 3, 1, fake, None) == -1
+    assert b"fixed orbital outside" in lib.ccx_last_error()
+    assert lib.ccx_t3_denominator(fake, 5, 0, 1, 1, 3, 3, 1, fake, None) == -1
+    assert b"running orbitals outside" in lib.ccx_last_error()
 
 
 def test_device_manager_validates_keywords():
