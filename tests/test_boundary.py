@@ -124,3 +124,28 @@ def test_pycc_binding_refuses_bad_operands_before_calling_the_library():
     with pytest.raises(ValueError, match="no libccx mode"):
         cb.CCXContext(2, 3, "HALF", None, None, lib=_RecordingLib())
     assert set(cb.CCXContext.MODES) == {"DP", "SP", "F16X3"}
+
+
+def test_t3_builders_validate_arguments_without_a_gpu():
+    """pycc_amd.cctriples' batched builders refuse what the device path cannot take (host tensors, a
+    contraction callable other than the device backend, non-slice o / v, out-of-range indices) before
+    any library call."""
+    import numpy as np
+    import torch
+    from pycc_amd import PyCCError, cctriples
+    from pycc_amd.device import ContractionBackend
+    backend = ContractionBackend(torch.device("cuda:0"))               # constructing it touches no GPU
+    no, nv = 2, 3
+    o, v = slice(0, no), slice(no, no + nv)
+    F = torch.zeros((no + nv, no + nv), dtype=torch.float64)
+    t2 = torch.zeros((no, no, nv, nv), dtype=torch.float64)
+    W = torch.zeros((nv, nv, nv, no), dtype=torch.float64)
+    U = torch.zeros((no, nv, no, no), dtype=torch.float64)
+    with pytest.raises(PyCCError, match="device contraction backend"):
+        cctriples.t3c_ijk(o, v, 0, 0, 0, t2, W, U, F, np.einsum)
+    with pytest.raises(PyCCError, match="slices"):
+        cctriples.t3c_ijk([0, 1], v, 0, 0, 0, t2, W, U, F, backend)
+    with pytest.raises(PyCCError, match="float64 tensors on cuda:0"):      # host tensors
+        cctriples.t3c_abc(o, v, 0, 0, 0, t2, W, U, F, backend)
+    with pytest.raises(PyCCError, match="float64 tensors on cuda:0"):
+        cctriples.t3d_ijk(o, v, 0, 0, 0, t2[0].float(), t2, t2, F, backend)
